@@ -1,0 +1,182 @@
+"""Python host wrapper over the C ABI (include/mpm_c_abi.h), used by the tests and the benchmark.
+
+`Solver` owns one device context (one HIP stream + device buffers) and mirrors the C entry points
+one to one; arrays are numpy float32 in the planar layout of the C ABI (x has shape (3, n), B and F
+have shape (9, n) row-major, scalars have shape (n,)).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import LevelSetPrim, Params, Stats
+
+
+class MPMError(RuntimeError):
+    pass
+
+
+def _fp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _planar(a, rows, n, name):
+    if a is None:
+        return None
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(rows, n) if rows > 1
+                               else np.asarray(a, dtype=np.float32).reshape(n))
+    return arr
+
+
+class Solver:
+    """One simulation context on one GPU (HybridSolver of HybridSolver.h:27-95, over the C ABI)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.mpm_create(ctypes.byref(h), int(device))
+        if rc != 0:
+            raise MPMError(f"mpm_create failed ({rc}): {self.lib.mpm_last_error(None).decode()}")
+        self.h = h
+        self.n = 0
+        self.res = None
+
+    # ---------------------------------------------------------------- plumbing
+    def _check(self, rc, what):
+        if rc != 0:
+            raise MPMError(f"{what} failed ({rc}): {self.lib.mpm_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.mpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------------- setup
+    def set_grid(self, min_bound, max_bound, res):
+        mn = (ctypes.c_double * 3)(*[float(v) for v in min_bound])
+        mx = (ctypes.c_double * 3)(*[float(v) for v in max_bound])
+        rs = (ctypes.c_int32 * 3)(*[int(v) for v in res])
+        self._check(self.lib.mpm_set_grid(self.h, mn, mx, rs), "mpm_set_grid")
+        self.res = tuple(int(v) for v in res)
+
+    def set_params(self, params):
+        self._check(self.lib.mpm_set_params(self.h, ctypes.byref(params)), "mpm_set_params")
+
+    def set_levelset(self, prims):
+        arr = (LevelSetPrim * max(1, len(prims)))()
+        for i, p in enumerate(prims):
+            arr[i] = p
+        self._check(self.lib.mpm_set_levelset(self.h, arr, len(prims)), "mpm_set_levelset")
+
+    def set_particles(self, x, mass, v=None, B=None, F=None, plastic=None, volume=None, phi=None):
+        x = np.asarray(x, dtype=np.float32)
+        n = x.shape[-1]
+        arrs = dict(x=_planar(x, 3, n, "x"), v=_planar(v, 3, n, "v"), B=_planar(B, 9, n, "B"),
+                    F=_planar(F, 9, n, "F"), plastic=_planar(plastic, 1, n, "plastic"),
+                    mass=_planar(mass, 1, n, "mass"), volume=_planar(volume, 1, n, "volume"),
+                    phi=_planar(phi, 1, n, "phi"))
+        self._keep = arrs
+        rc = self.lib.mpm_set_particles(self.h, n, _fp(arrs["x"]), _fp(arrs["v"]), _fp(arrs["B"]), _fp(arrs["F"]),
+                                        _fp(arrs["plastic"]), _fp(arrs["mass"]), _fp(arrs["volume"]),
+                                        _fp(arrs["phi"]))
+        self._check(rc, "mpm_set_particles")
+        self.n = n
+
+    # ---------------------------------------------------------------- stepping
+    def estimate_volumes(self):
+        self._check(self.lib.mpm_estimate_volumes(self.h), "mpm_estimate_volumes")
+
+    def substep(self, dt):
+        self._check(self.lib.mpm_substep(self.h, float(dt)), "mpm_substep")
+
+    def run(self, dt, nsteps):
+        self._check(self.lib.mpm_run(self.h, float(dt), int(nsteps)), "mpm_run")
+
+    def stage(self, stage, dt):
+        self._check(self.lib.mpm_stage(self.h, int(stage), float(dt)), "mpm_stage")
+
+    def synchronize(self):
+        self._check(self.lib.mpm_synchronize(self.h), "mpm_synchronize")
+
+    # ---------------------------------------------------------------- readback
+    def get_particles(self):
+        n = self.n
+        out = dict(x=np.empty((3, n), np.float32), v=np.empty((3, n), np.float32), B=np.empty((9, n), np.float32),
+                   F=np.empty((9, n), np.float32), plastic=np.empty(n, np.float32), volume=np.empty(n, np.float32))
+        rc = self.lib.mpm_get_particles(self.h, _fp(out["x"]), _fp(out["v"]), _fp(out["B"]), _fp(out["F"]),
+                                        _fp(out["plastic"]), _fp(out["volume"]))
+        self._check(rc, "mpm_get_particles")
+        return out
+
+    def get_sorted(self):
+        n = self.n
+        ncell = int(np.prod(self.res))
+        keys = np.empty(n, np.uint32)
+        ids = np.empty(n, np.int32)
+        bins = np.empty(ncell + 1, np.int32)
+        rc = self.lib.mpm_get_sorted(self.h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                     ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                     bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        self._check(rc, "mpm_get_sorted")
+        return keys, ids, bins
+
+    def get_grid(self):
+        ng = int(np.prod(self.res))
+        m = np.empty(ng, np.float32)
+        mom = np.empty((3, ng), np.float32)
+        v = np.empty((3, ng), np.float32)
+        vbf = np.empty((3, ng), np.float32)
+        self._check(self.lib.mpm_get_grid(self.h, _fp(m), _fp(mom), _fp(v), _fp(vbf)), "mpm_get_grid")
+        return dict(m=m, mom=mom, v=v, vbf=vbf)
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.mpm_get_stats(self.h, ctypes.byref(s)), "mpm_get_stats")
+        return {name: getattr(s, name) for name, _ in Stats._fields_}
+
+    def clear_status(self):
+        self._check(self.lib.mpm_clear_status(self.h), "mpm_clear_status")
+
+    def time_substeps(self, dt, nsteps):
+        total = ctypes.c_float()
+        stages = (ctypes.c_float * _lib.NUM_STAGES)()
+        rc = self.lib.mpm_time_substeps(self.h, float(dt), int(nsteps), ctypes.byref(total), stages)
+        self._check(rc, "mpm_time_substeps")
+        return total.value, list(stages)
+
+
+def generate_block(seed, lo, hi, cells, ppa):
+    """Seeded stratified-jitter particle block (mpm_generate_block); returns x with shape (3, n)."""
+    lib = _lib.load()
+    n = int(cells[0]) * int(cells[1]) * int(cells[2]) * int(ppa) ** 3
+    x = np.empty((3, n), np.float32)
+    lo_ = (ctypes.c_double * 3)(*[float(v) for v in lo])
+    hi_ = (ctypes.c_double * 3)(*[float(v) for v in hi])
+    c_ = (ctypes.c_int32 * 3)(*[int(v) for v in cells])
+    rc = lib.mpm_generate_block(ctypes.c_uint64(int(seed)), lo_, hi_, c_, int(ppa), n, _fp(x))
+    if rc != 0:
+        raise MPMError(f"mpm_generate_block failed ({rc})")
+    return x
+
+
+def levelset(kind, a=(0, 0, 0, 0), b=(0, 0, 0, 0)):
+    p = LevelSetPrim()
+    p.type = int(kind)
+    for i in range(4):
+        p.a[i] = float(a[i])
+        p.b[i] = float(b[i])
+    return p

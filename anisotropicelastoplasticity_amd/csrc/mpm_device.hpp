@@ -1,0 +1,350 @@
+// mpm_device.hpp -- device math for the gfx950 MPM kernels: B-spline weights, 3x3 SVD (FP32 with an
+// FP64 redo for near-degenerate input), constitutive stress, return mapping and level sets.
+//
+// Written for CDNA4 wave64 execution: everything is branch-light scalar code that lives in VGPRs;
+// no shared state. Reference behaviour is cited per function.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mpm_c_abi.h"
+
+namespace mpm {
+
+// ------------------------------------------------------------------ parameters on the device
+struct DevParams {
+  int order;             // 2 or 3
+  int material;          // MPM_SNOW / MPM_SAND
+  int predicted;         // stress at F^ = (I + dt grad v) F (reference) or at F^n
+  int friction_model;
+  float gravity[3];
+  float lambda0, mu0;    // Lame parameters (HybridSolver.cpp:264-265)
+  float theta_c, theta_s, xi;
+  float h0, h1, h2, h3;  // DP hardening, degrees
+  int per_particle_phi;
+  float friction;
+  float damping;
+  float svd_ratio;
+  float apex_radius;
+  float dinv;            // D^-1 = 3/h^2 (cubic) or 4/h^2 (quadratic), h = min spacing (HybridSolver.cpp:175-177)
+  float gmin[3], h[3], inv_h[3];
+  int res[3];
+  int tiles[3];          // ceil(res/4)
+  int nls;
+  mpm_levelset ls[16];
+};
+
+// ------------------------------------------------------------------ B-splines
+// Cubic (interpolation.cpp:9-33) and quadratic B-spline value and derivative at r (in cells).
+template <int ORDER, class T>
+__device__ __forceinline__ void spline(T r, T& w, T& dw) {
+  T ar = r < T(0) ? -r : r;
+  T sg = r < T(0) ? T(-1) : T(1);
+  if constexpr (ORDER == 3) {
+    if (ar < T(1)) {
+      w = T(0.5) * ar * ar * ar - ar * ar + T(2) / T(3);
+      dw = sg * (T(1.5) * ar * ar - T(2) * ar);
+    } else if (ar < T(2)) {
+      T t = T(2) - ar;
+      w = t * t * t / T(6);
+      dw = -sg * T(0.5) * t * t;
+    } else {
+      w = T(0);
+      dw = T(0);
+    }
+  } else {
+    if (ar < T(0.5)) {
+      w = T(0.75) - ar * ar;
+      dw = T(-2) * r;
+    } else if (ar < T(1.5)) {
+      T t = T(1.5) - ar;
+      w = T(0.5) * t * t;
+      dw = -sg * t;
+    } else {
+      w = T(0);
+      dw = T(0);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ 3x3 helpers (row-major)
+template <class T>
+__device__ __forceinline__ void mat_mul(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+template <class T>
+__device__ __forceinline__ T mat_det(const T* A) {
+  return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+// One-sided (Hestenes) Jacobi SVD of a 3x3 matrix: A = U diag(S) V^T with S >= 0 in decreasing
+// order (the convention of Eigen's JacobiSVD used at HybridSolver.cpp:308,620). Columns of W = A V
+// are orthogonalised by plane rotations until every pair is orthogonal to working precision or
+// max_sweeps is reached; S = column norms, U = normalised columns. Returns false if not converged.
+template <class T>
+__device__ __forceinline__ bool svd3(const T* A, T* U, T* S, T* V, int max_sweeps) {
+  T w[9], v[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { w[i] = A[i]; v[i] = T(0); }
+  v[0] = v[4] = v[8] = T(1);
+  const T eps = sizeof(T) == 4 ? T(1.5e-7) : T(4e-16);
+  bool converged = false;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    bool rotated = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int p = k == 2 ? 1 : 0;
+      const int q = k == 0 ? 1 : 2;
+      T alpha = w[p] * w[p] + w[3 + p] * w[3 + p] + w[6 + p] * w[6 + p];
+      T beta = w[q] * w[q] + w[3 + q] * w[3 + q] + w[6 + q] * w[6 + q];
+      T gamma = w[p] * w[q] + w[3 + p] * w[3 + q] + w[6 + p] * w[6 + q];
+      T ag = gamma < T(0) ? -gamma : gamma;
+      if (ag > eps * sqrt(alpha * beta)) {
+        rotated = true;
+        T zeta = (beta - alpha) / (T(2) * gamma);
+        T az = zeta < T(0) ? -zeta : zeta;
+        T t = (zeta >= T(0) ? T(1) : T(-1)) / (az + sqrt(T(1) + zeta * zeta));
+        T c = T(1) / sqrt(T(1) + t * t);
+        T s = c * t;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T wp = w[3 * r + p], wq = w[3 * r + q];
+          w[3 * r + p] = c * wp - s * wq;
+          w[3 * r + q] = s * wp + c * wq;
+          T vp = v[3 * r + p], vq = v[3 * r + q];
+          v[3 * r + p] = c * vp - s * vq;
+          v[3 * r + q] = s * vp + c * vq;
+        }
+      }
+    }
+    if (!rotated) { converged = true; break; }
+  }
+  T s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = sqrt(w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c]);
+  // sort columns by decreasing singular value
+  int o0 = 0, o1 = 1, o2 = 2;
+  if (s[o0] < s[o1]) { int t = o0; o0 = o1; o1 = t; }
+  if (s[o1] < s[o2]) { int t = o1; o1 = o2; o2 = t; }
+  if (s[o0] < s[o1]) { int t = o0; o0 = o1; o1 = t; }
+  const int ord[3] = {o0, o1, o2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int oc = ord[c];
+    T sc = oc == 0 ? s[0] : (oc == 1 ? s[1] : s[2]);
+    S[c] = sc;
+    T inv = sc > T(0) ? T(1) / sc : T(0);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      T wv = oc == 0 ? w[3 * r] : (oc == 1 ? w[3 * r + 1] : w[3 * r + 2]);
+      T vv = oc == 0 ? v[3 * r] : (oc == 1 ? v[3 * r + 1] : v[3 * r + 2]);
+      U[3 * r + c] = wv * inv;
+      V[3 * r + c] = vv;
+    }
+  }
+  if (!(S[2] > T(0))) {  // complete U with u0 x u1
+    U[2] = U[3] * U[7] - U[6] * U[4];
+    U[5] = U[6] * U[1] - U[0] * U[7];
+    U[8] = U[0] * U[4] - U[3] * U[1];
+  }
+  return converged;
+}
+
+// FP32 SVD with an FP64 redo when the input is near-degenerate (sigma_min < ratio * sigma_max),
+// not converged or not finite. Counts redos in *fallback (per thread, summed by the caller).
+__device__ __forceinline__ void svd3_guarded(const float* A, float* U, float* S, float* V, float ratio, int& fallback) {
+  bool ok = svd3<float>(A, U, S, V, 8);
+  bool finite = isfinite(S[0]) && isfinite(S[2]);
+  if (!ok || !finite || S[2] < ratio * S[0]) {
+    double Ad[9], Ud[9], Sd[3], Vd[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ad[i] = A[i];
+    svd3<double>(Ad, Ud, Sd, Vd, 30);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { U[i] = float(Ud[i]); V[i] = float(Vd[i]); }
+    S[0] = float(Sd[0]); S[1] = float(Sd[1]); S[2] = float(Sd[2]);
+    fallback += 1;
+  }
+}
+
+// Stress matrix V_p * P(Fh) * G^T (HybridSolver.cpp:279-339) with P the first Piola-Kirchhoff
+// stress of fixed-corotated snow with hardening (:281-287, 314-325) or of Hencky sand (:326-339).
+__device__ __forceinline__ void stress_matrix(const DevParams& P, const float* Fh, const float* G, float Jp, float vol,
+                                              float* out, int& fallback) {
+  float U[9], S[3], V[9];
+  svd3_guarded(Fh, U, S, V, P.svd_ratio, fallback);
+  float Pk[9];
+  if (P.material == MPM_SNOW) {
+    float sc = expf(P.xi * (1.0f - Jp));
+    float lambda = P.lambda0 * sc, mu = P.mu0 * sc;
+    float J = mat_det(Fh);
+    // J F^-T = cofactor matrix
+    float C[9];
+    C[0] = Fh[4] * Fh[8] - Fh[5] * Fh[7];
+    C[1] = Fh[5] * Fh[6] - Fh[3] * Fh[8];
+    C[2] = Fh[3] * Fh[7] - Fh[4] * Fh[6];
+    C[3] = Fh[2] * Fh[7] - Fh[1] * Fh[8];
+    C[4] = Fh[0] * Fh[8] - Fh[2] * Fh[6];
+    C[5] = Fh[1] * Fh[6] - Fh[0] * Fh[7];
+    C[6] = Fh[1] * Fh[5] - Fh[2] * Fh[4];
+    C[7] = Fh[2] * Fh[3] - Fh[0] * Fh[5];
+    C[8] = Fh[0] * Fh[4] - Fh[1] * Fh[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float R = U[3 * r] * V[3 * c] + U[3 * r + 1] * V[3 * c + 1] + U[3 * r + 2] * V[3 * c + 2];
+        Pk[3 * r + c] = 2.0f * mu * (Fh[3 * r + c] - R) + lambda * (J - 1.0f) * C[3 * r + c];
+      }
+  } else {
+    float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
+    float tr = l0 + l1 + l2;
+    float d0 = (2.0f * P.mu0 * l0 + P.lambda0 * tr) / S[0];
+    float d1 = (2.0f * P.mu0 * l1 + P.lambda0 * tr) / S[1];
+    float d2 = (2.0f * P.mu0 * l2 + P.lambda0 * tr) / S[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Pk[3 * r + c] = U[3 * r] * d0 * V[3 * c] + U[3 * r + 1] * d1 * V[3 * c + 1] + U[3 * r + 2] * d2 * V[3 * c + 2];
+  }
+  // out = vol * Pk * G^T
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      out[3 * r + c] = vol * (Pk[3 * r] * G[3 * c] + Pk[3 * r + 1] * G[3 * c + 1] + Pk[3 * r + 2] * G[3 * c + 2]);
+}
+
+// Drucker-Prager projection of principal stretches (HybridSolver.cpp:634-673).
+template <class T>
+__device__ __forceinline__ void dp_project(T lambda, T mu, T phi0, T h1, T h2, T h3, T q, const T* Sin, T* Sout, T& dq) {
+  const T pi = T(3.14159265358979323846);
+  T fa = (phi0 + (h1 * q - h3) * exp(-h2 * q)) * pi / T(180);
+  T sf = sin(fa);
+  T alpha = sqrt(T(2) / T(3)) * T(2) * sf / (T(3) - sf);
+  T l0 = log(Sin[0]), l1 = log(Sin[1]), l2 = log(Sin[2]);
+  T tr = l0 + l1 + l2;
+  T e0 = l0 - tr / T(3), e1 = l1 - tr / T(3), e2 = l2 - tr / T(3);
+  T en = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  T dg = en + (T(3) * lambda + T(2) * mu) / T(2) / mu * tr * alpha;
+  dq = T(0);
+  Sout[0] = Sin[0]; Sout[1] = Sin[1]; Sout[2] = Sin[2];
+  if (dg <= T(0)) {
+  } else if (en == T(0) || tr > T(0)) {
+    Sout[0] = Sout[1] = Sout[2] = T(1);
+    dq = sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+  } else {
+    T k = dg / en;
+    Sout[0] = exp(l0 - k * e0);
+    Sout[1] = exp(l1 - k * e1);
+    Sout[2] = exp(l2 - k * e2);
+    dq = dg;
+  }
+}
+
+// Return mapping of one particle (HybridSolver.cpp:616-681). Fh is replaced by the elastic part
+// U Sigma V^T; plastic holds J_P (snow, whose update det(F_P') = J_P * prod(Sigma)/prod(Sigma') is
+// the determinant of F_P' = V Sigma'^-1 U^T F^ F_P, :676-677) or plasticAmount q (sand).
+__device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float& plastic, float phi_p, int& fb_svd,
+                                           int& fb_apex) {
+  float U[9], S[3], V[9];
+  svd3_guarded(Fh, U, S, V, P.svd_ratio, fb_svd);
+  float Sn[3];
+  if (P.material == MPM_SNOW) {
+    float lo = 1.0f - P.theta_c, hi = 1.0f + P.theta_s;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Sn[k] = S[k] > hi ? hi : (S[k] < lo ? lo : S[k]);
+    plastic = plastic * ((S[0] / Sn[0]) * (S[1] / Sn[1]) * (S[2] / Sn[2]));
+  } else {
+    float phi0 = P.per_particle_phi ? phi_p : P.h0;
+    float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
+    float en2 = l0 * l0 + l1 * l1 + l2 * l2;
+    float dq;
+    if (en2 < P.apex_radius * P.apex_radius) {
+      double Sd[3] = {S[0], S[1], S[2]}, So[3], dqd;
+      dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, Sd, So, dqd);
+      Sn[0] = float(So[0]); Sn[1] = float(So[1]); Sn[2] = float(So[2]);
+      dq = float(dqd);
+      fb_apex += 1;
+    } else {
+      dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, S, Sn, dq);
+    }
+    plastic += dq;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Fh[3 * r + c] = U[3 * r] * Sn[0] * V[3 * c] + U[3 * r + 1] * Sn[1] * V[3 * c + 1] + U[3 * r + 2] * Sn[2] * V[3 * c + 2];
+}
+
+// ------------------------------------------------------------------ level sets (LevelSet.cpp:8-42)
+__device__ __forceinline__ void levelset(const DevParams& P, const float* x, float& phi, float* n) {
+  phi = 3.0e38f;
+  n[0] = n[1] = n[2] = 0.0f;
+  for (int k = 0; k < P.nls; ++k) {
+    const mpm_levelset& pr = P.ls[k];
+    float p, nn[3] = {0.0f, 0.0f, 0.0f};
+    if (pr.type == MPM_LS_PLANE) {
+      float len = sqrtf(pr.a[0] * pr.a[0] + pr.a[1] * pr.a[1] + pr.a[2] * pr.a[2]);
+      nn[0] = pr.a[0] / len; nn[1] = pr.a[1] / len; nn[2] = pr.a[2] / len;
+      p = nn[0] * x[0] + nn[1] * x[1] + nn[2] * x[2] - pr.a[3];
+    } else if (pr.type == MPM_LS_WALL2GROUND) {
+      float wx = pr.a[0], wy = pr.a[1], gz = pr.a[2];
+      p = fminf(fminf(x[2] - gz, wx - x[0]), wy - x[1]);
+      float dz = fabsf(x[2] - gz), dx = fabsf(wx - x[0]), dy = fabsf(wy - x[1]);
+      if (dz <= dx && dz <= dy) nn[2] = 1.0f;
+      else if (dy <= dx) nn[1] = -1.0f;
+      else nn[0] = -1.0f;
+    } else if (pr.type == MPM_LS_BOX) {
+      p = 3.0e38f;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        float dlo = x[ax] - pr.a[ax], dhi = pr.b[ax] - x[ax];
+        if (dlo < p) { p = dlo; nn[0] = nn[1] = nn[2] = 0.0f; nn[ax] = 1.0f; }
+        if (dhi < p) { p = dhi; nn[0] = nn[1] = nn[2] = 0.0f; nn[ax] = -1.0f; }
+      }
+    } else {
+      float d0 = x[0] - pr.a[0], d1 = x[1] - pr.a[1], d2 = x[2] - pr.a[2];
+      float len = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+      p = len - pr.a[3];
+      if (len > 0.0f) { nn[0] = d0 / len; nn[1] = d1 / len; nn[2] = d2 / len; }
+      else nn[1] = 1.0f;
+    }
+    if (p < phi) { phi = p; n[0] = nn[0]; n[1] = nn[1]; n[2] = nn[2]; }
+  }
+}
+
+// Grid collision of one node (HybridSolver.cpp:478-507); v in/out, vbf = velocity before friction.
+__device__ __forceinline__ void collide(const DevParams& P, const float* xnode, float* v, float* vbf) {
+  vbf[0] = v[0]; vbf[1] = v[1]; vbf[2] = v[2];
+  if (P.nls == 0) return;
+  float phi, n[3];
+  levelset(P, xnode, phi, n);
+  if (!(phi <= 0.0f)) return;
+  if (P.friction_model == MPM_FRICTION_STICKY) {
+    v[0] = v[1] = v[2] = 0.0f;
+    vbf[0] = vbf[1] = vbf[2] = 0.0f;
+    return;
+  }
+  float vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+  if (!(vn < 0.0f)) return;
+  float vt[3] = {v[0] - vn * n[0], v[1] - vn * n[1], v[2] - vn * n[2]};
+  float vtn = sqrtf(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+  vbf[0] = vt[0]; vbf[1] = vt[1]; vbf[2] = vt[2];
+  float mu = P.friction;
+  if (P.friction_model == MPM_FRICTION_REFERENCE) {
+    if (vtn < -mu * vn) { v[0] = v[1] = v[2] = 0.0f; }
+    else { v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2]; }
+  } else {
+    if (vtn <= -mu * vn) { v[0] = v[1] = v[2] = 0.0f; }
+    else {
+      float k = mu * vn / vtn;
+      v[0] = vt[0] + k * vt[0]; v[1] = vt[1] + k * vt[1]; v[2] = vt[2] + k * vt[2];
+    }
+  }
+}
+
+}  // namespace mpm

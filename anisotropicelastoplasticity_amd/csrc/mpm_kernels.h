@@ -1,0 +1,127 @@
+// mpm_kernels.h -- layout constants, kernel argument blocks and host launchers of the gfx950 MPM
+// substep (internal to libmpm_hip.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpm_device.hpp"
+
+namespace mpm {
+
+// Particle state planes (each `cap` floats; PID holds int32 bits).
+enum Plane : int {
+  PX = 0,     // x0 x1 x2
+  PV = 3,     // v0 v1 v2
+  PB = 6,     // B row-major (rows = affineMomenta_1..3)
+  PF = 15,    // F_E row-major
+  PPL = 24,   // J_P (snow) or plasticAmount q (sand)
+  PM = 25,    // mass
+  PVOL = 26,  // volume
+  PPHI = 27,  // per-particle friction angle (degrees)
+  PID = 28,   // caller index (int bits)
+  NPLANES = 29
+};
+
+constexpr int SORT_IPT = 16;
+constexpr int SORT_TILE = 256 * SORT_IPT;  // keys per sort block
+constexpr int TILE = 4;                    // cells per tile edge
+constexpr int FOOT = TILE + 3;             // footprint nodes per edge (cell window c-1 .. c+2)
+constexpr int FOOT3 = FOOT * FOOT * FOOT;
+constexpr int PART_STRIDE = 7 * FOOT3;     // m, p(3), f(3) per tile slot
+
+enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_NUM = 2 };
+enum P2GMode : int { P2G_MOM_STRESS = 0, P2G_MOM = 1, P2G_FORCE = 2 };
+enum GridMode : int { GRID_FUSED = 0, GRID_NORMALIZE = 1, GRID_FORCE_UPDATE = 2 };
+
+struct SortBufs {
+  uint32_t *k1, *k2;
+  int *v1, *v2;
+  uint32_t* hist;     // [3][256]
+  uint32_t* tickets;  // [4]
+  uint32_t* status;   // [3][nblocks][256]
+  int* counters;      // zeroed by the histogram kernel
+};
+
+struct ResetArgs {
+  uint32_t* zero_words;
+  int64_t n_zero;
+  const uint32_t* prev_keys;
+  int n;
+  int* cell_start;
+  int* cell_end;
+  const int* counters;
+  const int* active_tiles;
+  int* tile_slot;
+  const int* active_blocks;
+  int* block_slot;
+  int64_t max_tiles;
+};
+
+struct P2GArgs {
+  DevParams P;
+  const float* state;
+  int64_t cap;
+  int n;
+  const int* perm;
+  const int* cell_start;
+  const int* cell_end;
+  const int* counters;
+  const int* active_tiles;
+  float* partials;
+  const float* gv;  // grid velocity (force pass)
+  int64_t ng;
+  float dt;
+  unsigned long long* fallback_svd;
+};
+
+struct GridArgs {
+  DevParams P;
+  const int* counters;
+  const int* active_blocks;
+  const int* tile_slot;
+  const float* partials;
+  float* gm;
+  float* gmom;
+  float* gv;
+  float* gvbf;
+  int64_t ng;
+  float dt;
+};
+
+struct G2PArgs {
+  DevParams P;
+  const float* sin;
+  float* sout;
+  int64_t cap;
+  int n;
+  const int* perm;
+  const float* gv;
+  const float* gvbf;
+  int64_t ng;
+  float dt;
+  const float* thr;
+  uint32_t* keys;
+  int* status;
+  unsigned long long* fallback_svd;
+  unsigned long long* fallback_apex;
+};
+
+void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const float* thr, uint32_t* keys,
+                 int* status);
+void launch_reset(hipStream_t s, const ResetArgs& r);
+void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n);
+void launch_segments(hipStream_t s, const DevParams& P, const uint32_t* ks, const int* vin, int* vout, int n,
+                     const float* st, int64_t cap, int* cell_start, int* cell_end, int* tile_slot, int* active_tiles,
+                     int* block_slot, int* active_blocks, int* counters);
+void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid);
+void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
+void launch_g2p(hipStream_t s, const G2PArgs& a);
+void launch_volumes(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, const float* gm);
+void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out);
+void launch_grid_export(hipStream_t s, const DevParams& P, const float* src, int64_t ng, int nplanes, const int* block_slot,
+                        float* out);
+void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
+                        unsigned long long* out, int grid);
+void launch_fill_int(hipStream_t s, int* p, int64_t n, int v);
+
+}  // namespace mpm

@@ -1,0 +1,824 @@
+// mpm_kernels.hip -- gfx950 kernels of the explicit MPM substep and their host launchers.
+//
+// Data layout in HBM (one context = one scene):
+//   particles   two ping-pong buffers of NPLANES float planes of `cap` floats each (SoA):
+//               x(3) v(3) B(9) F(9) plastic m V phi id(int bits). After the first substep the
+//               planes are kept physically sorted by cell key, so every particle pass streams them.
+//   keys        uint32 cell key per particle in the current physical order (written by G2P).
+//   sort        K1/K2 keys and V1/V2 index ping-pong buffers, per-pass lookback status words.
+//   grid        dense planes m, mom(3), v(3), vbf(3) over res0*res1*res2 nodes; only the 4x4x4 node
+//               blocks touched by particles are written each substep.
+//   cells       cell_start/cell_end per cell (reset sparsely from the previous substep's keys).
+//   tiles       4x4x4-cell tiles: slot per tile (-1 = inactive), active list, per-slot P2G partials
+//               over the tile's 7x7x7 node footprint.
+//
+// Substep (stress at F^n): reset -> hist -> 3 radix passes -> segments -> p2g -> grid -> g2p.
+// Reference-compatible substep (stress at predicted F^): p2g(momentum) -> grid(normalise) ->
+// p2g(force, gathers grad v) -> grid(force update) replace the p2g -> grid pair.
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.hpp"
+#include "mpm_kernels.h"
+
+namespace mpm {
+
+// ================================================================== small helpers
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Indices read from device buffers are kept inside [0, n) so that an inconsistent buffer can only
+// produce wrong values, never an out-of-bounds access.
+__device__ __forceinline__ int clamp_index(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
+// Exact cell index along one axis: the number of thresholds thr[c] <= x, where thr[c] is the
+// smallest float f with (double(f) - min)/h >= c. This equals static_cast<int>((x - min)/h) of the
+// reference evaluated in double (HybridSolver.cpp:33-36) for x >= min, using FP32 compares only.
+__device__ __forceinline__ int axis_cell(const float* thr, int res, float x, float gmin, float inv_h, int& oob) {
+  if (!(x >= thr[0])) { oob = 1; return 0; }
+  if (x >= thr[res]) { oob = 1; return res - 1; }
+  int c = (int)floorf((x - gmin) * inv_h);
+  c = c < 0 ? 0 : (c > res - 1 ? res - 1 : c);
+  while (c > 0 && x < thr[c]) --c;
+  while (c < res - 1 && x >= thr[c + 1]) ++c;
+  return c;
+}
+
+__device__ __forceinline__ uint32_t cell_key(const DevParams& P, const float* thr, float x0, float x1, float x2, int& oob) {
+  int c0 = axis_cell(thr, P.res[0], x0, P.gmin[0], P.inv_h[0], oob);
+  int c1 = axis_cell(thr + (P.res[0] + 1), P.res[1], x1, P.gmin[1], P.inv_h[1], oob);
+  int c2 = axis_cell(thr + (P.res[0] + 1) + (P.res[1] + 1), P.res[2], x2, P.gmin[2], P.inv_h[2], oob);
+  return (uint32_t)(c2 * P.res[0] * P.res[1] + c1 * P.res[0] + c0);
+}
+
+// ================================================================== keys of the initial order
+__global__ __launch_bounds__(256) void k_keys(DevParams P, const float* __restrict__ st, int64_t cap, int n,
+                                              const float* __restrict__ thr, uint32_t* __restrict__ keys,
+                                              int* __restrict__ status) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int oob = 0;
+  if (i < n) keys[i] = cell_key(P, thr, st[PX * cap + i], st[(PX + 1) * cap + i], st[(PX + 2) * cap + i], oob);
+  if (__ballot(oob) && lane_id() == 0) atomicOr(status, MPM_STATUS_OUT_OF_DOMAIN);
+}
+
+// ================================================================== radix sort (stable LSD, 8-bit digits)
+// Per substep: one histogram launch for all three digit positions, then one launch per digit.
+// A pass block ranks its 4096 keys stably with wave ballots (per-wave, per-digit running counts in
+// LDS), publishes its per-digit counts and finds its global offsets by decoupled look-back over the
+// preceding blocks (blocks take tickets in start order, so a block only waits on running blocks).
+constexpr uint32_t FLAG_AGG = 1u << 30, FLAG_PREFIX = 2u << 30, VALUE_MASK = (1u << 30) - 1u;
+
+__global__ __launch_bounds__(256) void k_reset(uint32_t* __restrict__ zero_words, int64_t n_zero,
+                                               const uint32_t* __restrict__ prev_keys, int n,
+                                               int* __restrict__ cell_start, int* __restrict__ cell_end,
+                                               const int* __restrict__ counters, const int* __restrict__ active_tiles,
+                                               int* __restrict__ tile_slot, const int* __restrict__ active_blocks,
+                                               int* __restrict__ block_slot) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t0; i < n_zero; i += stride) zero_words[i] = 0u;
+  for (int64_t j = t0; j < n; j += stride) {
+    uint32_t k = prev_keys[j];
+    if (j == 0 || prev_keys[j - 1] != k) { cell_start[k] = 0; cell_end[k] = 0; }
+  }
+  int nt = counters[CNT_TILES], nb = counters[CNT_BLOCKS];
+  for (int64_t j = t0; j < nt; j += stride) tile_slot[active_tiles[j]] = -1;
+  for (int64_t j = t0; j < nb; j += stride) block_slot[active_blocks[j]] = -1;
+}
+
+__global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ hist,
+                                                   int* __restrict__ counters) {
+  __shared__ uint32_t h[3][256];
+  for (int i = threadIdx.x; i < 768; i += 256) (&h[0][0])[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * SORT_TILE;
+  for (int k = 0; k < SORT_IPT; ++k) {
+    int i = base + k * 256 + threadIdx.x;
+    if (i < n) {
+      uint32_t key = keys[i];
+      atomicAdd(&h[0][key & 255u], 1u);
+      atomicAdd(&h[1][(key >> 8) & 255u], 1u);
+      atomicAdd(&h[2][(key >> 16) & 255u], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 768; i += 256) {
+    uint32_t c = (&h[0][0])[i];
+    if (c) atomicAdd(&hist[i], c);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
+                                                   uint32_t* __restrict__ kout, int* __restrict__ vout, int n, int pass,
+                                                   const uint32_t* __restrict__ hist, uint32_t* __restrict__ tickets,
+                                                   uint32_t* __restrict__ status) {
+  __shared__ uint32_t s_wh[4][256];
+  __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_scan[256];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = (int)atomicAdd(&tickets[pass], 1u);
+  for (int i = tid; i < 1024; i += 256) (&s_wh[0][0])[i] = 0u;
+  __syncthreads();
+  const int b = s_tile;
+  if (b >= (int)gridDim.x) return;  // tickets are zeroed per substep; uniform across the block
+  const int base = b * SORT_TILE + w * (64 * SORT_IPT);
+  const int shift = 8 * pass;
+  uint32_t kr[SORT_IPT];
+  int vr[SORT_IPT];
+  uint32_t rr[SORT_IPT];
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int i = 0; i < SORT_IPT; ++i) {
+    int pos = base + i * 64 + lane;
+    bool valid = pos < n;
+    uint32_t k = valid ? kin[pos] : 0u;
+    int v = valid ? (vin ? vin[pos] : pos) : 0;
+    uint32_t d = (k >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      bool set = (d >> bit) & 1u;
+      uint64_t bb = __ballot(valid && set);
+      peers &= set ? bb : ~bb;
+    }
+    uint32_t prev = s_wh[w][d];
+    if (valid && (peers >> lane) == 1ull) s_wh[w][d] = prev + (uint32_t)__popcll(peers);
+    kr[i] = k;
+    vr[i] = v;
+    rr[i] = prev + (uint32_t)__popcll(peers & lt);
+  }
+  __syncthreads();
+  {
+    const int d = tid;
+    uint32_t t0 = s_wh[0][d], t1 = s_wh[1][d], t2 = s_wh[2][d], t3 = s_wh[3][d];
+    s_wh[0][d] = 0u;
+    s_wh[1][d] = t0;
+    s_wh[2][d] = t0 + t1;
+    s_wh[3][d] = t0 + t1 + t2;
+    uint32_t total = t0 + t1 + t2 + t3;
+    uint32_t* my = status + (size_t)b * 256 + d;
+    uint32_t excl = 0u;
+    if (b == 0) {
+      __hip_atomic_store(my, FLAG_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(my, FLAG_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int bb = b - 1; bb >= 0; --bb) {
+        uint32_t s;
+        do {
+          s = __hip_atomic_load(status + (size_t)bb * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } while ((s & ~VALUE_MASK) == 0u);
+        excl += s & VALUE_MASK;
+        if ((s & ~VALUE_MASK) == FLAG_PREFIX) break;
+      }
+      __hip_atomic_store(my, FLAG_PREFIX | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // exclusive scan of the global digit histogram of this pass
+    s_scan[d] = hist[pass * 256 + d];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      uint32_t add = d >= off ? s_scan[d - off] : 0u;
+      __syncthreads();
+      s_scan[d] += add;
+      __syncthreads();
+    }
+    s_off[d] = (s_scan[d] - hist[pass * 256 + d]) + excl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SORT_IPT; ++i) {
+    int pos = base + i * 64 + lane;
+    if (pos < n) {
+      uint32_t d = (kr[i] >> shift) & 255u;
+      uint32_t dst = s_off[d] + s_wh[w][d] + rr[i];
+      if (dst < (uint32_t)n) {  // always true for a consistent histogram; keeps a bad one in bounds
+        kout[dst] = kr[i];
+        vout[dst] = vr[i];
+      }
+    }
+  }
+}
+
+// ================================================================== segments, bins, tiles
+// Thread per sorted position. Within each run of equal keys, the particles are re-ordered by their
+// caller id, so that the final order is that of std::stable_sort of the keys in caller order (the
+// order in which the reference visits particles within a cell). The run start also writes the
+// cell's bin bounds and claims the cell's tile and the 27 node blocks around it.
+__global__ __launch_bounds__(256) void k_segments(DevParams P, const uint32_t* __restrict__ ks, const int* __restrict__ vin,
+                                                  int* __restrict__ vout, int n, const float* __restrict__ st,
+                                                  int64_t cap, int* __restrict__ cell_start, int* __restrict__ cell_end,
+                                                  int* __restrict__ tile_slot, int* __restrict__ active_tiles,
+                                                  int* __restrict__ block_slot, int* __restrict__ active_blocks,
+                                                  int* __restrict__ counters) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int* ids = reinterpret_cast<const int*>(st + (size_t)PID * cap);
+  uint32_t k = ks[j];
+  int s = j;
+  while (s > 0 && ks[s - 1] == k) --s;
+  int e = j + 1;
+  while (e < n && ks[e] == k) ++e;
+  int my = clamp_index(vin[j], n);
+  int myid = ids[my];
+  int rank = 0;
+  for (int a = s; a < e; ++a) rank += ids[clamp_index(vin[a], n)] < myid ? 1 : 0;
+  if (s + rank < e) vout[s + rank] = my;
+  if (j != s) return;
+  cell_start[k] = s;
+  cell_end[k] = e;
+  int r01 = P.res[0] * P.res[1];
+  int c2 = (int)(k / (uint32_t)r01), rem = (int)(k % (uint32_t)r01);
+  int c1 = rem / P.res[0], c0 = rem % P.res[0];
+  int t0 = c0 >> 2, t1 = c1 >> 2, t2 = c2 >> 2;
+  int tid = (t2 * P.tiles[1] + t1) * P.tiles[0] + t0;
+  if (atomicCAS(&tile_slot[tid], -1, -2) == -1) {
+    int slot = atomicAdd(&counters[CNT_TILES], 1);
+    active_tiles[slot] = tid;
+    tile_slot[tid] = slot;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int b0 = t0 + dx, b1 = t1 + dy, b2 = t2 + dz;
+          if (b0 < 0 || b1 < 0 || b2 < 0 || b0 >= P.tiles[0] || b1 >= P.tiles[1] || b2 >= P.tiles[2]) continue;
+          int bid = (b2 * P.tiles[1] + b1) * P.tiles[0] + b0;
+          if (atomicCAS(&block_slot[bid], -1, -2) == -1) {
+            int bs = atomicAdd(&counters[CNT_BLOCKS], 1);
+            active_blocks[bs] = bid;
+            block_slot[bid] = bs;
+          }
+        }
+  }
+}
+
+// ================================================================== P2G
+// One workgroup (4 waves) per active 4x4x4-cell tile. Wave w owns cell layer z = w; lane l owns
+// cell (l>>2 & 3, l>>4) of that layer and, for accumulation, the node slab zo = l & 3 of the cell's
+// 4x4x4 node window (nodes c-1 .. c+2, a superset of the quadratic stencil and exactly the cubic one).
+// The cell's particles are staged into LDS four at a time (lane (cell, r) stages rank 4k + r) with
+// their payload computed once per particle; each lane then accumulates its 16 window nodes of its
+// cell's particles in registers, in ascending rank. Lanes write to the tile's LDS node array in
+// fixed phases (every address is written by at most one lane per instruction; waves take turns),
+// and the array is stored as the tile's partial. Node values are therefore deterministic sums in a
+// fixed order, without float atomics on global memory.
+constexpr int PAY = 28;  // payload floats per staged particle (7 x float4)
+
+template <int ORDER>
+__device__ __forceinline__ void axis_weights(float f, float* w, float* dw) {
+#pragma unroll
+  for (int o = 0; o < 4; ++o) spline<ORDER>(f + 1.0f - (float)o, w[o], dw[o]);
+}
+
+template <int ORDER, int MODE>
+__global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
+  __shared__ float4 s_pay[4][64][PAY / 4];
+  __shared__ float s_acc[4][FOOT3];
+  __shared__ int s_max;
+  const DevParams& P = a.P;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int cxy = lane >> 2, sub = lane & 3;
+  const int cx = cxy & 3, cy = cxy >> 2;
+  const int ntile = a.counters[CNT_TILES];
+  constexpr int NQ = (MODE == P2G_FORCE) ? 3 : 4;
+  for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
+    const int tile = a.active_tiles[slot];
+    const int t0 = tile % P.tiles[0], t1 = (tile / P.tiles[0]) % P.tiles[1], t2 = tile / (P.tiles[0] * P.tiles[1]);
+    for (int i = tid; i < 4 * FOOT3; i += 256) (&s_acc[0][0])[i] = 0.0f;
+    if (tid == 0) s_max = 0;
+    __syncthreads();
+    const int g0 = 4 * t0 + cx, g1 = 4 * t1 + cy, g2 = 4 * t2 + w;
+    int start = 0, cnt = 0;
+    if (g0 < P.res[0] && g1 < P.res[1] && g2 < P.res[2]) {
+      int key = (g2 * P.res[1] + g1) * P.res[0] + g0;
+      start = a.cell_start[key];
+      cnt = a.cell_end[key] - start;
+    }
+    if (sub == 0 && cnt > 0) atomicMax(&s_max, cnt);
+    float acc[16][NQ];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0f;
+    __syncthreads();
+    const int nch = (s_max + 3) >> 2;
+    for (int ch = 0; ch < nch; ++ch) {
+      // ---- stage: lane (cell, sub) prepares rank 4*ch + sub of its cell
+      {
+        float pay[PAY];
+#pragma unroll
+        for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
+        const int rank = 4 * ch + sub;
+        if (rank < cnt) {
+          const int pi = clamp_index(a.perm[start + rank], a.n);
+          const float* st = a.state;
+          const int64_t cap = a.cap;
+          float xp[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) xp[d] = st[(PX + d) * cap + pi];
+          const int cc[3] = {g0, g1, g2};
+          float f[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) f[d] = (xp[d] - P.gmin[d]) * P.inv_h[d] - (float)cc[d];
+          pay[0] = f[0]; pay[1] = f[1]; pay[2] = f[2];
+          const float mp = st[PM * cap + pi];
+          if constexpr (MODE != P2G_FORCE) {
+            // momentum w*(q0 + Q o): Q = m D^-1 B diag(h), q0 = m v - Q (1 + f)  (HybridSolver.cpp:134,169-170)
+            float Q[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) Q[3 * r + c] = mp * P.dinv * st[(PB + 3 * r + c) * cap + pi] * P.h[c];
+            pay[3] = mp;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              pay[4 + r] = mp * st[(PV + r) * cap + pi] - (Q[3 * r] * (1.0f + f[0]) + Q[3 * r + 1] * (1.0f + f[1]) + Q[3 * r + 2] * (1.0f + f[2]));
+#pragma unroll
+            for (int i = 0; i < 9; ++i) pay[7 + i] = Q[i];
+          }
+          if constexpr (MODE != P2G_MOM) {
+            // stress matrix V P(F^) F^n^T (HybridSolver.cpp:306-339), columns scaled by 1/h and by dt
+            // when the impulse is folded into the momentum.
+            float Fn[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+            float Fh[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Fh[i] = Fn[i];
+            if constexpr (MODE == P2G_FORCE) {
+              // F^ = F + dt (sum_i v_i grad w_ip^T) F from the post-P2G grid velocity (:269-306)
+              float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
+              axis_weights<ORDER>(f[0], wx, dx);
+              axis_weights<ORDER>(f[1], wy, dy);
+              axis_weights<ORDER>(f[2], wz, dz);
+              float G[9];
+#pragma unroll
+              for (int i = 0; i < 9; ++i) G[i] = 0.0f;
+              for (int oz = 0; oz < 4; ++oz) {
+                int n2 = g2 - 1 + oz;
+                if (n2 < 0 || n2 >= P.res[2]) continue;
+                for (int oy = 0; oy < 4; ++oy) {
+                  int n1 = g1 - 1 + oy;
+                  if (n1 < 0 || n1 >= P.res[1]) continue;
+                  for (int ox = 0; ox < 4; ++ox) {
+                    int n0 = g0 - 1 + ox;
+                    if (n0 < 0 || n0 >= P.res[0]) continue;
+                    float gw[3] = {dx[ox] * wy[oy] * wz[oz] * P.inv_h[0], wx[ox] * dy[oy] * wz[oz] * P.inv_h[1],
+                                   wx[ox] * wy[oy] * dz[oz] * P.inv_h[2]};
+                    int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                      float vr = a.gv[r * a.ng + node];
+#pragma unroll
+                      for (int c = 0; c < 3; ++c) G[3 * r + c] += vr * gw[c];
+                    }
+                  }
+                }
+              }
+              float GF[9];
+              mat_mul(G, Fn, GF);
+#pragma unroll
+              for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + a.dt * GF[i];
+            }
+            float Jp = st[PPL * cap + pi];
+            float vol = st[PVOL * cap + pi];
+            float S[9];
+            int fb = 0;
+            stress_matrix(P, Fh, Fn, Jp, vol, S, fb);
+            if (fb) atomicAdd(a.fallback_svd, (unsigned long long)fb);
+            const float scale = (MODE == P2G_MOM_STRESS) ? a.dt : 1.0f;
+            const int off = (MODE == P2G_FORCE) ? 3 : 16;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) pay[off + 3 * r + c] = scale * S[3 * r + c] * P.inv_h[c];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < PAY / 4; ++i) s_pay[w][lane][i] = make_float4(pay[4 * i], pay[4 * i + 1], pay[4 * i + 2], pay[4 * i + 3]);
+      }
+      __syncthreads();
+      // ---- accumulate: lane (cell, zo) over the staged particles of its cell
+      const int zo = sub;
+      for (int r2 = 0; r2 < 4; ++r2) {
+        if (4 * ch + r2 >= cnt) break;
+        float pay[PAY];
+#pragma unroll
+        for (int i = 0; i < PAY / 4; ++i) {
+          float4 t = s_pay[w][cxy * 4 + r2][i];
+          pay[4 * i] = t.x; pay[4 * i + 1] = t.y; pay[4 * i + 2] = t.z; pay[4 * i + 3] = t.w;
+        }
+        float wx[4], dwx[4], wy[4], dwy[4];
+        axis_weights<ORDER>(pay[0], wx, dwx);
+        axis_weights<ORDER>(pay[1], wy, dwy);
+        float wz, dwz;
+        spline<ORDER>(pay[2] + 1.0f - (float)zo, wz, dwz);
+        const float* S = (MODE == P2G_FORCE) ? &pay[3] : &pay[16];
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+          const float wyz = wy[oy] * wz, dyz = dwy[oy] * wz, ydz = wy[oy] * dwz;
+          float e[3], dv[3], av[3], mw = 0.0f;
+          if constexpr (MODE != P2G_FORCE) {
+            mw = pay[3] * wyz;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              float qrow = pay[4 + r] + pay[7 + 3 * r + 1] * (float)oy + pay[7 + 3 * r + 2] * (float)zo;
+              e[r] = wyz * qrow;
+              dv[r] = wyz * pay[7 + 3 * r];
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) { e[r] = 0.0f; dv[r] = 0.0f; }
+          }
+          if constexpr (MODE != P2G_MOM) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              e[r] -= S[3 * r + 1] * dyz + S[3 * r + 2] * ydz;
+              av[r] = S[3 * r] * wyz;
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) av[r] = 0.0f;
+          }
+#pragma unroll
+          for (int ox = 0; ox < 4; ++ox) {
+            const int ni = oy * 4 + ox;
+            const float wxo = wx[ox] * (float)ox;
+            if constexpr (MODE != P2G_FORCE) {
+              acc[ni][0] += wx[ox] * mw;
+#pragma unroll
+              for (int r = 0; r < 3; ++r) acc[ni][1 + r] += wx[ox] * e[r] + wxo * dv[r] - dwx[ox] * av[r];
+            } else {
+#pragma unroll
+              for (int r = 0; r < 3; ++r) acc[ni][r] += wx[ox] * e[r] - dwx[ox] * av[r];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // ---- write the lane sums into the tile's node array: waves in turn, 16 conflict-free phases each
+    for (int turn = 0; turn < 4; ++turn) {
+      if (w == turn) {
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy)
+#pragma unroll
+          for (int ox = 0; ox < 4; ++ox) {
+            const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * (w + sub);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) atomicAdd(&s_acc[q][ln], acc[oy * 4 + ox][q]);
+          }
+      }
+      __syncthreads();
+    }
+    float* part = a.partials + (size_t)slot * PART_STRIDE + (MODE == P2G_FORCE ? 4 * FOOT3 : 0);
+    for (int i = tid; i < NQ * FOOT3; i += 256) part[i] = (&s_acc[0][0])[i];
+    __syncthreads();
+  }
+}
+
+// ================================================================== grid update
+// One 64-thread workgroup per active 4x4x4 node block; each node sums the partials of the (at most
+// 2 per axis) tiles whose footprint covers it, in ascending tile order, then applies
+// v = p/m (HybridSolver.cpp:233-240), v += dt (f/m + g) (:457, :725-737) and collision (:460-511).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_grid(GridArgs a) {
+  const DevParams& P = a.P;
+  const int nblk = a.counters[CNT_BLOCKS];
+  const int lx = threadIdx.x & 3, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x >> 4;
+  for (int s = blockIdx.x; s < nblk; s += gridDim.x) {
+    const int blk = a.active_blocks[s];
+    const int b0 = blk % P.tiles[0], b1 = (blk / P.tiles[0]) % P.tiles[1], b2 = blk / (P.tiles[0] * P.tiles[1]);
+    const int n0 = 4 * b0 + lx, n1 = 4 * b1 + ly, n2 = 4 * b2 + lz;
+    if (n0 >= P.res[0] || n1 >= P.res[1] || n2 >= P.res[2]) continue;
+    const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+    const int nn[3] = {n0, n1, n2};
+    int tlo[3], thi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int lo = (nn[d] - 5 + 3) / 4;  // ceil((n-5)/4) for n >= 2; clamped below
+      if (nn[d] - 5 < 0) lo = 0;
+      int hi = (nn[d] + 1) / 4;
+      thi[d] = hi < P.tiles[d] - 1 ? hi : P.tiles[d] - 1;
+      tlo[d] = lo;
+    }
+    float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int qoff = (MODE == GRID_FORCE_UPDATE) ? 4 : 0;
+    const int nq = (MODE == GRID_FORCE_UPDATE) ? 3 : 4;
+    for (int t2 = tlo[2]; t2 <= thi[2]; ++t2)
+      for (int t1 = tlo[1]; t1 <= thi[1]; ++t1)
+        for (int t0 = tlo[0]; t0 <= thi[0]; ++t0) {
+          const int sl = a.tile_slot[(t2 * P.tiles[1] + t1) * P.tiles[0] + t0];
+          if (sl < 0) continue;
+          const int ln = (n0 - (4 * t0 - 1)) + FOOT * (n1 - (4 * t1 - 1)) + FOOT * FOOT * (n2 - (4 * t2 - 1));
+          const float* part = a.partials + (size_t)sl * PART_STRIDE + qoff * FOOT3 + ln;
+          for (int q = 0; q < nq; ++q) sum[q] += part[q * FOOT3];
+        }
+    float v[3], vbf[3];
+    if constexpr (MODE == GRID_FORCE_UPDATE) {
+      const float m = a.gm[node];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        v[d] = a.gv[d * a.ng + node];
+        if (m > 0.0f) v[d] += a.dt * (sum[d] / m + P.gravity[d]);
+      }
+    } else {
+      const float m = sum[0];
+      a.gm[node] = m;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.gmom[d * a.ng + node] = sum[1 + d];
+        v[d] = m > 0.0f ? sum[1 + d] / m : 0.0f;
+        if (MODE == GRID_FUSED && m > 0.0f) v[d] += a.dt * P.gravity[d];
+      }
+    }
+    if constexpr (MODE == GRID_NORMALIZE) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) a.gv[d * a.ng + node] = v[d];
+    } else {
+      const float xn[3] = {P.gmin[0] + (float)n0 * P.h[0], P.gmin[1] + (float)n1 * P.h[1], P.gmin[2] + (float)n2 * P.h[2]};
+      collide(P, xn, v, vbf);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.gv[d * a.ng + node] = v[d];
+        a.gvbf[d * a.ng + node] = vbf[d];
+      }
+    }
+  }
+}
+
+// ================================================================== G2P + F update + return map
+// Thread per sorted position j: gathers the stencil of grid v (post-collision) and v_BF
+// (pre-friction), computes v_p, APIC B (HybridSolver.cpp:739-825), x += dt sum w v_BF (:942-950),
+// F^ = (I + dt sum v_BF grad w^T) F (:553-577) and the return map (:612-681), and writes the particle
+// to slot j of the other buffer, so that the buffers stay physically sorted by cell.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
+  const DevParams& P = a.P;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
+  if (j < a.n) {
+    const int pi = clamp_index(a.perm[j], a.n);
+    const float* st = a.sin;
+    const int64_t cap = a.cap;
+    float xp[3], g[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = st[(PX + d) * cap + pi];
+      g[d] = (xp[d] - P.gmin[d]) * P.inv_h[d];
+    }
+    int base[3];
+    float w[3][4], dw[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      base[d] = (ORDER == 2) ? (int)floorf(g[d] - 0.5f) : (int)floorf(g[d]) - 1;
+#pragma unroll
+      for (int o = 0; o < ORDER + 1; ++o) spline<ORDER>(g[d] - (float)(base[d] + o), w[d][o], dw[d][o]);
+    }
+    float vp[3] = {0.0f, 0.0f, 0.0f}, vb[3] = {0.0f, 0.0f, 0.0f};
+    float Bn[9], Gv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { Bn[i] = 0.0f; Gv[i] = 0.0f; }
+    for (int oz = 0; oz < ORDER + 1; ++oz) {
+      const int n2 = base[2] + oz;
+      if (n2 < 0 || n2 >= P.res[2]) continue;
+      for (int oy = 0; oy < ORDER + 1; ++oy) {
+        const int n1 = base[1] + oy;
+        if (n1 < 0 || n1 >= P.res[1]) continue;
+#pragma unroll
+        for (int ox = 0; ox < ORDER + 1; ++ox) {
+          const int n0 = base[0] + ox;
+          if (n0 < 0 || n0 >= P.res[0]) continue;
+          const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+          const float wyz = w[1][oy] * w[2][oz];
+          const float wt = w[0][ox] * wyz;
+          const float gw[3] = {dw[0][ox] * wyz * P.inv_h[0], w[0][ox] * dw[1][oy] * w[2][oz] * P.inv_h[1],
+                               w[0][ox] * w[1][oy] * dw[2][oz] * P.inv_h[2]};
+          const float dxn[3] = {((float)n0 - g[0]) * P.h[0], ((float)n1 - g[1]) * P.h[1], ((float)n2 - g[2]) * P.h[2]};
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float vi = a.gv[r * a.ng + node];
+            const float bi = a.gvbf[r * a.ng + node];
+            const float wv = wt * vi;
+            vp[r] += wv;
+            vb[r] += wt * bi;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              Bn[3 * r + c] += wv * dxn[c];
+              Gv[3 * r + c] += bi * gw[c];
+            }
+          }
+        }
+      }
+    }
+    if (P.damping != 0.0f) {
+      float C[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) C[i] = Bn[i];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float sym = 0.5f * (C[3 * r + c] + C[3 * c + r]);
+          Bn[3 * r + c] = (C[3 * r + c] - sym) + (1.0f - P.damping) * sym;
+        }
+    }
+    float Fn[9], Fh[9], GF[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+    mat_mul(Gv, Fn, GF);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + a.dt * GF[i];
+    float plastic = st[PPL * cap + pi];
+    const float phi_p = st[PPHI * cap + pi];
+    return_map(P, Fh, plastic, phi_p, fb_svd, fb_apex);
+    float xn[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xn[d] = xp[d] + a.dt * vb[d];
+      if (fabsf(xn[d] - xp[d]) > P.h[d]) cfl = 1;
+    }
+    float* so = a.sout;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      so[(PX + d) * cap + j] = xn[d];
+      so[(PV + d) * cap + j] = vp[d];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      so[(PB + i) * cap + j] = Bn[i];
+      so[(PF + i) * cap + j] = Fh[i];
+    }
+    so[PPL * cap + j] = plastic;
+    so[PM * cap + j] = st[PM * cap + pi];
+    so[PVOL * cap + j] = st[PVOL * cap + pi];
+    so[PPHI * cap + j] = phi_p;
+    reinterpret_cast<int*>(so + (size_t)PID * cap)[j] = reinterpret_cast<const int*>(st + (size_t)PID * cap)[pi];
+    a.keys[j] = cell_key(P, a.thr, xn[0], xn[1], xn[2], oob);
+    bad = !(isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]) && isfinite(vp[0]) && isfinite(vp[1]) &&
+            isfinite(vp[2]) && isfinite(Fh[0]) && isfinite(Fh[4]) && isfinite(Fh[8]) && isfinite(plastic));
+  }
+  const int flags = (oob ? MPM_STATUS_OUT_OF_DOMAIN : 0) | (bad ? MPM_STATUS_NONFINITE : 0) | (cfl ? MPM_STATUS_CFL : 0);
+  if (__ballot(flags != 0)) {
+    if (flags) atomicOr(a.status, flags);
+  }
+  if (fb_svd) atomicAdd(a.fallback_svd, (unsigned long long)fb_svd);
+  if (fb_apex) atomicAdd(a.fallback_apex, (unsigned long long)fb_apex);
+}
+
+// ================================================================== auxiliary kernels
+// rho_p = sum_i w_ip m_i / h^3, V_p = m_p / rho_p (HybridSolver.cpp:242-249), at the current positions.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_volumes(DevParams P, float* __restrict__ st, int64_t cap, int n,
+                                                 const float* __restrict__ gm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g[3];
+  int base[3];
+  float w[3][4], dw[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    g[d] = (st[(PX + d) * cap + i] - P.gmin[d]) * P.inv_h[d];
+    base[d] = (ORDER == 2) ? (int)floorf(g[d] - 0.5f) : (int)floorf(g[d]) - 1;
+#pragma unroll
+    for (int o = 0; o < ORDER + 1; ++o) spline<ORDER>(g[d] - (float)(base[d] + o), w[d][o], dw[d][o]);
+  }
+  float rho = 0.0f;
+  for (int oz = 0; oz < ORDER + 1; ++oz)
+    for (int oy = 0; oy < ORDER + 1; ++oy)
+      for (int ox = 0; ox < ORDER + 1; ++ox) {
+        int n0 = base[0] + ox, n1 = base[1] + oy, n2 = base[2] + oz;
+        if (n0 < 0 || n1 < 0 || n2 < 0 || n0 >= P.res[0] || n1 >= P.res[1] || n2 >= P.res[2]) continue;
+        rho += w[0][ox] * w[1][oy] * w[2][oz] * gm[((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0];
+      }
+  rho /= P.h[0] * P.h[1] * P.h[2];
+  st[PVOL * cap + i] = st[PM * cap + i] / rho;
+}
+
+// Copy planes [p0, p0+np) of the device order into caller order (out[plane][id] = st[plane][i]).
+__global__ __launch_bounds__(256) void k_unpermute(const float* __restrict__ st, int64_t cap, int n, int p0, int np,
+                                                   float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = reinterpret_cast<const int*>(st + (size_t)PID * cap)[i];
+  if (id < 0 || id >= n) return;
+  for (int p = 0; p < np; ++p) out[(size_t)p * n + id] = st[(p0 + p) * cap + i];
+}
+
+// Dense copy of grid planes with nodes outside active blocks set to zero.
+__global__ __launch_bounds__(256) void k_grid_export(DevParams P, const float* __restrict__ src, int64_t ng, int nplanes,
+                                                     const int* __restrict__ block_slot, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng) return;
+  const int n0 = (int)(i % P.res[0]), n1 = (int)((i / P.res[0]) % P.res[1]), n2 = (int)(i / ((int64_t)P.res[0] * P.res[1]));
+  const int b = ((n2 >> 2) * P.tiles[1] + (n1 >> 2)) * P.tiles[0] + (n0 >> 2);
+  const bool on = block_slot[b] >= 0;
+  for (int p = 0; p < nplanes; ++p) out[p * ng + i] = on ? src[p * ng + i] : 0.0f;
+}
+
+// Number of nodes with m > 0 in the active blocks.
+__global__ __launch_bounds__(64) void k_count_nodes(DevParams P, const float* __restrict__ gm, const int* __restrict__ counters,
+                                                    const int* __restrict__ active_blocks, unsigned long long* __restrict__ out) {
+  const int nblk = counters[CNT_BLOCKS];
+  const int lx = threadIdx.x & 3, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x >> 4;
+  unsigned long long c = 0;
+  for (int s = blockIdx.x; s < nblk; s += gridDim.x) {
+    const int blk = active_blocks[s];
+    const int b0 = blk % P.tiles[0], b1 = (blk / P.tiles[0]) % P.tiles[1], b2 = blk / (P.tiles[0] * P.tiles[1]);
+    const int n0 = 4 * b0 + lx, n1 = 4 * b1 + ly, n2 = 4 * b2 + lz;
+    if (n0 >= P.res[0] || n1 >= P.res[1] || n2 >= P.res[2]) continue;
+    if (gm[((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0] > 0.0f) ++c;
+  }
+  if (c) atomicAdd(out, c);
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ================================================================== launchers
+static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const float* thr, uint32_t* keys,
+                 int* status) {
+  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, thr, keys, status);
+}
+
+void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n) {
+  const int nb = blocks_for(n, SORT_TILE);
+  hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(256), 0, s, keys_in, n, b.hist, b.counters);
+  hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, keys_in, (const int*)nullptr, b.k1, b.v1, n, 0, b.hist,
+                     b.tickets, b.status);
+  hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k1, (const int*)b.v1, b.k2, b.v2, n, 1,
+                     b.hist, b.tickets, b.status + (size_t)nb * 256);
+  hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k2, (const int*)b.v2, b.k1, b.v1, n, 2,
+                     b.hist, b.tickets, b.status + (size_t)2 * nb * 256);
+}
+
+void launch_reset(hipStream_t s, const ResetArgs& r) {
+  int64_t work = r.n_zero > r.n ? r.n_zero : r.n;
+  if (work < r.max_tiles) work = r.max_tiles;
+  int grid = blocks_for(work, 256);
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, r.zero_words, r.n_zero, r.prev_keys, r.n, r.cell_start,
+                     r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot);
+}
+
+void launch_segments(hipStream_t s, const DevParams& P, const uint32_t* ks, const int* vin, int* vout, int n,
+                     const float* st, int64_t cap, int* cell_start, int* cell_end, int* tile_slot, int* active_tiles,
+                     int* block_slot, int* active_blocks, int* counters) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, vin, vout, n, st, cap, cell_start,
+                       cell_end, tile_slot, active_tiles, block_slot, active_blocks, counters);
+}
+
+void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid) {
+  const int o = a.P.order;
+  if (mode == P2G_MOM_STRESS) {
+    if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM_STRESS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM_STRESS>), dim3(grid), dim3(256), 0, s, a);
+  } else if (mode == P2G_MOM) {
+    if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM>), dim3(grid), dim3(256), 0, s, a);
+  } else {
+    if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_FORCE>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_FORCE>), dim3(grid), dim3(256), 0, s, a);
+  }
+}
+
+void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid) {
+  if (mode == GRID_FUSED) hipLaunchKernelGGL(k_grid<GRID_FUSED>, dim3(grid), dim3(64), 0, s, a);
+  else if (mode == GRID_NORMALIZE) hipLaunchKernelGGL(k_grid<GRID_NORMALIZE>, dim3(grid), dim3(64), 0, s, a);
+  else hipLaunchKernelGGL(k_grid<GRID_FORCE_UPDATE>, dim3(grid), dim3(64), 0, s, a);
+}
+
+void launch_g2p(hipStream_t s, const G2PArgs& a) {
+  if (a.n <= 0) return;
+  if (a.P.order == 2) hipLaunchKernelGGL(k_g2p<2>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_g2p<3>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+}
+
+void launch_volumes(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, const float* gm) {
+  if (n <= 0) return;
+  if (P.order == 2) hipLaunchKernelGGL(k_volumes<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, gm);
+  else hipLaunchKernelGGL(k_volumes<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, gm);
+}
+
+void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out) {
+  if (n > 0) hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, cap, n, p0, np, out);
+}
+
+void launch_grid_export(hipStream_t s, const DevParams& P, const float* src, int64_t ng, int nplanes, const int* block_slot,
+                        float* out) {
+  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(ng, 256)), dim3(256), 0, s, P, src, ng, nplanes, block_slot, out);
+}
+
+void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
+                        unsigned long long* out, int grid) {
+  hipLaunchKernelGGL(k_count_nodes, dim3(grid), dim3(64), 0, s, P, gm, counters, active_blocks, out);
+}
+
+void launch_fill_int(hipStream_t s, int* p, int64_t n, int v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace mpm

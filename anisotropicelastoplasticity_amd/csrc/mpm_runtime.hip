@@ -1,0 +1,810 @@
+// mpm_runtime.hip -- context, device buffers, stage sequencing, HIP-graph replay and the C ABI of
+// libmpm_hip.so (declared in include/mpm_c_abi.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_c_abi.h"
+#include "mpm_kernels.h"
+
+using namespace mpm;
+
+namespace {
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+}  // namespace
+
+struct mpm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // grid
+  bool have_grid = false;
+  double gmin[3] = {0, 0, 0}, gmax[3] = {1, 1, 1}, h[3] = {1, 1, 1};
+  int res[3] = {0, 0, 0};
+  int64_t ng = 0;
+  int tiles[3] = {0, 0, 0};
+  int64_t ntiles = 0;
+  mpm_params params{};
+  std::vector<mpm_levelset> ls;
+  DevParams dp{};
+
+  float* thr = nullptr;      // per-axis cell thresholds
+  float* grid = nullptr;     // m, mom(3), v(3), vbf(3)
+  int* cell_start = nullptr;
+  int* cell_end = nullptr;
+  int* tile_slot = nullptr;
+  int* active_tiles = nullptr;
+  int* block_slot = nullptr;
+  int* active_blocks = nullptr;
+  float* partials = nullptr;
+  int64_t partial_slots = 0;
+
+  // particles
+  int n = 0;
+  int64_t cap = 0;
+  float* state[2] = {nullptr, nullptr};
+  int cur = 0;
+  int sort_buf = 0;
+  uint32_t* keys = nullptr;
+  uint32_t *k1 = nullptr, *k2 = nullptr;
+  int *v1 = nullptr, *v2 = nullptr;
+  uint32_t* sortws = nullptr;
+  int64_t sortws_words = 0;
+  int nb_sort = 0;
+  bool keys_valid = false;
+
+  // counters and statistics
+  int* counters = nullptr;
+  int* status = nullptr;
+  unsigned long long* fallbacks = nullptr;  // [0] svd, [1] apex, [2] node count scratch
+  int64_t substeps = 0;
+  int next_stage = MPM_STAGE_SORT;
+
+  // graph of two substeps, per starting buffer
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  float graph_dt[2] = {0.0f, 0.0f};
+
+  int p2g_grid = 1024;
+  int grid_grid = 2048;
+};
+
+#define CK(call)                                                                     \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+      return MPM_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+static int fail(mpm_ctx* ctx, int code, const std::string& msg) {
+  ctx->err = msg;
+  return code;
+}
+
+template <class T>
+static int dalloc(mpm_ctx* ctx, T*& p, size_t count) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  if (count == 0) return MPM_OK;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  p = static_cast<T*>(q);
+  return MPM_OK;
+}
+
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+static void drop_graphs(mpm_ctx* ctx) {
+  for (int i = 0; i < 2; ++i)
+    if (ctx->graph[i]) {
+      (void)hipGraphExecDestroy(ctx->graph[i]);
+      ctx->graph[i] = nullptr;
+    }
+}
+
+// ------------------------------------------------------------------ parameters
+static void build_devparams(mpm_ctx* ctx) {
+  DevParams& d = ctx->dp;
+  const mpm_params& p = ctx->params;
+  d.order = p.kernel_order == MPM_KERNEL_QUADRATIC ? 2 : 3;
+  d.material = p.material;
+  d.predicted = p.stress_at_predicted_F;
+  d.friction_model = p.friction_model;
+  for (int a = 0; a < 3; ++a) d.gravity[a] = p.gravity[a];
+  double E = p.youngs_modulus, nu = p.poisson_ratio;
+  d.lambda0 = float(E * nu / (1.0 + nu) / (1.0 - 2.0 * nu));
+  d.mu0 = float(E / 2.0 / (1.0 + nu));
+  d.theta_c = p.critical_compression;
+  d.theta_s = p.critical_stretch;
+  d.xi = p.hardening;
+  d.h0 = p.dp_h[0];
+  d.h1 = p.dp_h[1];
+  d.h2 = p.dp_h[2];
+  d.h3 = p.dp_h[3];
+  d.per_particle_phi = p.per_particle_phi;
+  d.friction = p.friction;
+  d.damping = p.apic_damping;
+  d.svd_ratio = p.fp64_svd_ratio;
+  d.apex_radius = p.fp64_apex_radius;
+  double hmin = std::min(ctx->h[0], std::min(ctx->h[1], ctx->h[2]));
+  d.dinv = float((d.order == 3 ? 3.0 : 4.0) / hmin / hmin);
+  for (int a = 0; a < 3; ++a) {
+    d.gmin[a] = float(ctx->gmin[a]);
+    d.h[a] = float(ctx->h[a]);
+    d.inv_h[a] = float(1.0 / ctx->h[a]);
+    d.res[a] = ctx->res[a];
+    d.tiles[a] = ctx->tiles[a];
+  }
+  d.nls = (int)ctx->ls.size();
+  for (int k = 0; k < d.nls; ++k) d.ls[k] = ctx->ls[k];
+  drop_graphs(ctx);
+}
+
+// smallest float f with (double(f) - gmin)/h >= c, evaluated exactly as HybridSolver.cpp:33-36
+static float threshold(double gmin, double h, int c) {
+  auto ok = [&](float f) { return (static_cast<double>(f) - gmin) / h >= static_cast<double>(c); };
+  float f = static_cast<float>(gmin + c * h);
+  while (ok(f)) f = std::nextafter(f, -INFINITY);
+  while (!ok(f)) f = std::nextafter(f, INFINITY);
+  return f;
+}
+
+// ------------------------------------------------------------------ substep stages
+static int stage_sort(mpm_ctx* ctx) {
+  hipStream_t s = ctx->stream;
+  float* st = ctx->state[ctx->cur];
+  if (!ctx->keys_valid) launch_keys(s, ctx->dp, st, ctx->cap, ctx->n, ctx->thr, ctx->keys, ctx->status);
+  ResetArgs r;
+  r.zero_words = ctx->sortws;
+  r.n_zero = ctx->sortws_words;
+  r.prev_keys = ctx->k1;
+  r.n = ctx->n;
+  r.cell_start = ctx->cell_start;
+  r.cell_end = ctx->cell_end;
+  r.counters = ctx->counters;
+  r.active_tiles = ctx->active_tiles;
+  r.tile_slot = ctx->tile_slot;
+  r.active_blocks = ctx->active_blocks;
+  r.block_slot = ctx->block_slot;
+  r.max_tiles = ctx->ntiles;
+  launch_reset(s, r);
+  SortBufs b;
+  b.k1 = ctx->k1;
+  b.k2 = ctx->k2;
+  b.v1 = ctx->v1;
+  b.v2 = ctx->v2;
+  b.hist = ctx->sortws;
+  b.tickets = ctx->sortws + 768;
+  b.status = ctx->sortws + 772;
+  b.counters = ctx->counters;
+  launch_sort(s, b, ctx->keys, ctx->n);
+  launch_segments(s, ctx->dp, ctx->k1, ctx->v1, ctx->v2, ctx->n, st, ctx->cap, ctx->cell_start, ctx->cell_end,
+                  ctx->tile_slot, ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters);
+  ctx->sort_buf = ctx->cur;
+  return MPM_OK;
+}
+
+static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
+  P2GArgs a;
+  a.P = ctx->dp;
+  a.state = ctx->state[ctx->cur];
+  a.cap = ctx->cap;
+  a.n = ctx->n;
+  a.perm = ctx->v2;
+  a.cell_start = ctx->cell_start;
+  a.cell_end = ctx->cell_end;
+  a.counters = ctx->counters;
+  a.active_tiles = ctx->active_tiles;
+  a.partials = ctx->partials;
+  a.gv = ctx->grid + 4 * ctx->ng;
+  a.ng = ctx->ng;
+  a.dt = dt;
+  a.fallback_svd = ctx->fallbacks;
+  return a;
+}
+
+static GridArgs grid_args(mpm_ctx* ctx, float dt) {
+  GridArgs g;
+  g.P = ctx->dp;
+  g.counters = ctx->counters;
+  g.active_blocks = ctx->active_blocks;
+  g.tile_slot = ctx->tile_slot;
+  g.partials = ctx->partials;
+  g.gm = ctx->grid;
+  g.gmom = ctx->grid + ctx->ng;
+  g.gv = ctx->grid + 4 * ctx->ng;
+  g.gvbf = ctx->grid + 7 * ctx->ng;
+  g.ng = ctx->ng;
+  g.dt = dt;
+  return g;
+}
+
+static int stage_p2g(mpm_ctx* ctx, float dt) {
+  hipStream_t s = ctx->stream;
+  P2GArgs a = p2g_args(ctx, dt);
+  if (!ctx->dp.predicted) {
+    launch_p2g(s, a, P2G_MOM_STRESS, ctx->p2g_grid);
+  } else {
+    launch_p2g(s, a, P2G_MOM, ctx->p2g_grid);
+    launch_grid(s, grid_args(ctx, dt), GRID_NORMALIZE, ctx->grid_grid);
+    launch_p2g(s, a, P2G_FORCE, ctx->p2g_grid);
+  }
+  return MPM_OK;
+}
+
+static int stage_grid(mpm_ctx* ctx, float dt) {
+  launch_grid(ctx->stream, grid_args(ctx, dt), ctx->dp.predicted ? GRID_FORCE_UPDATE : GRID_FUSED, ctx->grid_grid);
+  return MPM_OK;
+}
+
+static int stage_g2p(mpm_ctx* ctx, float dt) {
+  G2PArgs a;
+  a.P = ctx->dp;
+  a.sin = ctx->state[ctx->cur];
+  a.sout = ctx->state[1 - ctx->cur];
+  a.cap = ctx->cap;
+  a.n = ctx->n;
+  a.perm = ctx->v2;
+  a.gv = ctx->grid + 4 * ctx->ng;
+  a.gvbf = ctx->grid + 7 * ctx->ng;
+  a.ng = ctx->ng;
+  a.dt = dt;
+  a.thr = ctx->thr;
+  a.keys = ctx->keys;
+  a.status = ctx->status;
+  a.fallback_svd = ctx->fallbacks;
+  a.fallback_apex = ctx->fallbacks + 1;
+  launch_g2p(ctx->stream, a);
+  ctx->cur = 1 - ctx->cur;
+  ctx->keys_valid = true;
+  ctx->substeps += 1;
+  return MPM_OK;
+}
+
+static int run_stage(mpm_ctx* ctx, int stage, float dt) {
+  switch (stage) {
+    case MPM_STAGE_SORT: return stage_sort(ctx);
+    case MPM_STAGE_P2G: return stage_p2g(ctx, dt);
+    case MPM_STAGE_GRID: return stage_grid(ctx, dt);
+    case MPM_STAGE_G2P: return stage_g2p(ctx, dt);
+  }
+  return fail(ctx, MPM_ERR_INVALID, "unknown stage");
+}
+
+static int ready(mpm_ctx* ctx) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid has not been called");
+  if (ctx->n <= 0) return fail(ctx, MPM_ERR_STATE, "mpm_set_particles has not been called");
+  return MPM_OK;
+}
+
+static int one_substep(mpm_ctx* ctx, float dt) {
+  for (int st = MPM_STAGE_SORT; st <= MPM_STAGE_G2P; ++st) {
+    int rc = run_stage(ctx, st, dt);
+    if (rc) return rc;
+  }
+  CK(hipGetLastError());
+  return MPM_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+void mpm_params_default(mpm_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->kernel_order = MPM_KERNEL_CUBIC;
+  p->material = MPM_SAND;
+  p->stress_at_predicted_F = 1;
+  p->friction_model = MPM_FRICTION_REFERENCE;
+  p->gravity[0] = 0.0f;
+  p->gravity[1] = 0.0f;
+  p->gravity[2] = -9.8f;
+  p->youngs_modulus = 1.4e5f;
+  p->poisson_ratio = 0.2f;
+  p->critical_compression = 2.5e-2f;
+  p->critical_stretch = 7.5e-3f;
+  p->hardening = 10.0f;
+  p->dp_h[0] = 35.0f;
+  p->dp_h[1] = 9.0f;
+  p->dp_h[2] = 0.2f;
+  p->dp_h[3] = 10.0f;
+  p->per_particle_phi = 0;
+  p->friction = 0.2f;
+  p->apic_damping = 0.0f;
+  p->fp64_svd_ratio = 1e-3f;
+  p->fp64_apex_radius = 1e-4f;
+}
+
+int mpm_create(mpm_ctx** out, int32_t device) {
+  if (!out) return MPM_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    g_create_error = "no HIP device available";
+    return MPM_ERR_HIP;
+  }
+  if (device < 0 || device >= count) {
+    g_create_error = "device index out of range";
+    return MPM_ERR_INVALID;
+  }
+  mpm_ctx* ctx = new (std::nothrow) mpm_ctx();
+  if (!ctx) return MPM_ERR_NOMEM;
+  ctx->device = device;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_error = std::string("stream creation failed: ") + hipGetErrorString(e);
+    delete ctx;
+    return MPM_ERR_HIP;
+  }
+  mpm_params_default(&ctx->params);
+  int rc = dalloc(ctx, ctx->counters, CNT_NUM);
+  if (!rc) rc = dalloc(ctx, ctx->status, 1);
+  if (!rc) rc = dalloc(ctx, ctx->fallbacks, 4);
+  if (rc) {
+    g_create_error = ctx->err;
+    mpm_destroy(ctx);
+    return rc;
+  }
+  (void)hipMemsetAsync(ctx->counters, 0, CNT_NUM * sizeof(int), ctx->stream);
+  (void)hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream);
+  (void)hipMemsetAsync(ctx->fallbacks, 0, 4 * sizeof(unsigned long long), ctx->stream);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->p2g_grid = std::max(256, prop.multiProcessorCount * 4);
+    ctx->grid_grid = std::max(512, prop.multiProcessorCount * 8);
+  }
+  *out = ctx;
+  return MPM_OK;
+}
+
+int mpm_destroy(mpm_ctx* ctx) {
+  if (!ctx) return MPM_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graphs(ctx);
+  dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_end);
+  dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
+  dfree(ctx->partials);
+  dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys);
+  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->sortws);
+  dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MPM_OK;
+}
+
+const char* mpm_last_error(const mpm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+static int alloc_partials(mpm_ctx* ctx) {
+  if (!ctx->have_grid || ctx->n <= 0) return MPM_OK;
+  int64_t slots = std::min<int64_t>(ctx->ntiles, ctx->n);
+  if (slots == ctx->partial_slots && ctx->partials) return MPM_OK;
+  int rc = dalloc(ctx, ctx->partials, (size_t)slots * PART_STRIDE);
+  if (rc) return rc;
+  ctx->partial_slots = slots;
+  return MPM_OK;
+}
+
+int mpm_set_grid(mpm_ctx* ctx, const double min_bound[3], const double max_bound[3], const int32_t res[3]) {
+  if (!ctx || !min_bound || !max_bound || !res) return MPM_ERR_INVALID;
+  for (int a = 0; a < 3; ++a) {
+    if (!(max_bound[a] > min_bound[a])) return fail(ctx, MPM_ERR_INVALID, "maxBound must be bigger than minBound");
+    if (res[a] < 1) return fail(ctx, MPM_ERR_INVALID, "resolution must be positive");
+  }
+  int64_t ng = (int64_t)res[0] * res[1] * res[2];
+  if (ng > (1ll << 24)) return fail(ctx, MPM_ERR_UNSUPPORTED, "grids above 2^24 cells need more radix passes");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  for (int a = 0; a < 3; ++a) {
+    ctx->gmin[a] = min_bound[a];
+    ctx->gmax[a] = max_bound[a];
+    ctx->res[a] = res[a];
+    ctx->h[a] = (max_bound[a] - min_bound[a]) / res[a];  // RegularGrid.cpp:144-146
+    ctx->tiles[a] = (res[a] + TILE - 1) / TILE;
+  }
+  ctx->ng = ng;
+  ctx->ntiles = (int64_t)ctx->tiles[0] * ctx->tiles[1] * ctx->tiles[2];
+  int rc = dalloc(ctx, ctx->grid, (size_t)10 * ng);
+  if (!rc) rc = dalloc(ctx, ctx->cell_start, ng);
+  if (!rc) rc = dalloc(ctx, ctx->cell_end, ng);
+  if (!rc) rc = dalloc(ctx, ctx->tile_slot, ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, ctx->active_tiles, ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, ctx->block_slot, ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, ctx->active_blocks, ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, ctx->thr, (size_t)(res[0] + res[1] + res[2] + 3));
+  if (rc) return rc;
+  CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * 10 * ng, ctx->stream));
+  CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * ng, ctx->stream));
+  CK(hipMemsetAsync(ctx->cell_end, 0, sizeof(int) * ng, ctx->stream));
+  launch_fill_int(ctx->stream, ctx->tile_slot, ctx->ntiles, -1);
+  launch_fill_int(ctx->stream, ctx->block_slot, ctx->ntiles, -1);
+  CK(hipMemsetAsync(ctx->counters, 0, CNT_NUM * sizeof(int), ctx->stream));
+  std::vector<float> thr;
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c <= res[a]; ++c) thr.push_back(threshold(ctx->gmin[a], ctx->h[a], c));
+  CK(hipMemcpyAsync(ctx->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->have_grid = true;
+  ctx->keys_valid = false;
+  ctx->partial_slots = 0;
+  if (ctx->k1) CK(hipMemset(ctx->k1, 0, sizeof(uint32_t) * ctx->cap));
+  rc = alloc_partials(ctx);
+  if (rc) return rc;
+  build_devparams(ctx);
+  return MPM_OK;
+}
+
+int mpm_set_params(mpm_ctx* ctx, const mpm_params* p) {
+  if (!ctx || !p) return MPM_ERR_INVALID;
+  if (p->kernel_order != MPM_KERNEL_QUADRATIC && p->kernel_order != MPM_KERNEL_CUBIC)
+    return fail(ctx, MPM_ERR_INVALID, "kernel_order must be 2 or 3");
+  if (p->material != MPM_SNOW && p->material != MPM_SAND) return fail(ctx, MPM_ERR_INVALID, "unknown material");
+  if (p->friction_model < 0 || p->friction_model > 2) return fail(ctx, MPM_ERR_INVALID, "unknown friction model");
+  if (!(p->poisson_ratio < 0.5f) || !(p->youngs_modulus > 0.0f)) return fail(ctx, MPM_ERR_INVALID, "bad elastic moduli");
+  ctx->params = *p;
+  build_devparams(ctx);
+  return MPM_OK;
+}
+
+int mpm_set_levelset(mpm_ctx* ctx, const mpm_levelset* prims, int32_t n) {
+  if (!ctx || n < 0 || n > 16 || (n > 0 && !prims)) return ctx ? fail(ctx, MPM_ERR_INVALID, "0 <= n <= 16 primitives") : MPM_ERR_INVALID;
+  for (int i = 0; i < n; ++i)
+    if (prims[i].type < MPM_LS_PLANE || prims[i].type > MPM_LS_SPHERE) return fail(ctx, MPM_ERR_INVALID, "unknown level-set type");
+  ctx->ls.assign(prims, prims + n);
+  build_devparams(ctx);
+  return MPM_OK;
+}
+
+int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, const float* B, const float* F,
+                      const float* plastic, const float* mass, const float* volume, const float* phi) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid must precede mpm_set_particles");
+  if (n <= 0 || !x || !mass) return fail(ctx, MPM_ERR_INVALID, "n > 0, x and mass are required");
+  if (ctx->params.per_particle_phi && !phi) return fail(ctx, MPM_ERR_INVALID, "per_particle_phi needs phi");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  if (n != ctx->n) {
+    int rc = dalloc(ctx, ctx->state[0], (size_t)NPLANES * n);
+    if (!rc) rc = dalloc(ctx, ctx->state[1], (size_t)NPLANES * n);
+    if (!rc) rc = dalloc(ctx, ctx->keys, n);
+    if (!rc) rc = dalloc(ctx, ctx->k1, n);
+    if (!rc) rc = dalloc(ctx, ctx->k2, n);
+    if (!rc) rc = dalloc(ctx, ctx->v1, n);
+    if (!rc) rc = dalloc(ctx, ctx->v2, n);
+    ctx->nb_sort = (n + SORT_TILE - 1) / SORT_TILE;
+    ctx->sortws_words = 772 + (int64_t)3 * ctx->nb_sort * 256;
+    if (!rc) rc = dalloc(ctx, ctx->sortws, ctx->sortws_words);
+    if (rc) return rc;
+    CK(hipMemsetAsync(ctx->state[1], 0, sizeof(float) * NPLANES * (size_t)n, ctx->stream));
+    CK(hipMemsetAsync(ctx->keys, 0, sizeof(uint32_t) * n, ctx->stream));
+    CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
+    CK(hipMemsetAsync(ctx->v1, 0, sizeof(int) * n, ctx->stream));
+    CK(hipMemsetAsync(ctx->v2, 0, sizeof(int) * n, ctx->stream));
+    CK(hipMemsetAsync(ctx->sortws, 0, sizeof(uint32_t) * ctx->sortws_words, ctx->stream));
+    ctx->n = n;
+    ctx->cap = n;
+    rc = alloc_partials(ctx);
+    if (rc) return rc;
+  }
+  const int64_t cap = ctx->cap;
+  std::vector<float> h((size_t)NPLANES * cap, 0.0f);
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      h[(PX + a) * cap + i] = x[(size_t)a * n + i];
+      h[(PV + a) * cap + i] = v ? v[(size_t)a * n + i] : 0.0f;
+    }
+    for (int c = 0; c < 9; ++c) {
+      h[(PB + c) * cap + i] = B ? B[(size_t)c * n + i] : 0.0f;
+      h[(PF + c) * cap + i] = F ? F[(size_t)c * n + i] : ((c % 4 == 0) ? 1.0f : 0.0f);
+    }
+    h[PPL * cap + i] = plastic ? plastic[i] : (ctx->params.material == MPM_SNOW ? 1.0f : 0.0f);
+    h[PM * cap + i] = mass[i];
+    h[PVOL * cap + i] = volume ? volume[i] : 0.0f;
+    h[PPHI * cap + i] = phi ? phi[i] : 0.0f;
+    int id = i;
+    std::memcpy(&h[PID * cap + i], &id, sizeof(int));
+  }
+  CK(hipMemcpyAsync(ctx->state[0], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * n, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->cur = 0;
+  ctx->sort_buf = 0;
+  ctx->keys_valid = false;
+  ctx->substeps = 0;
+  ctx->next_stage = MPM_STAGE_SORT;
+  return MPM_OK;
+}
+
+int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* F, float* plastic, float* volume) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  CK(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  float* tmp = nullptr;
+  rc = dalloc(ctx, tmp, (size_t)9 * n);
+  if (rc) return rc;
+  struct Req { float* dst; int p0; int np; };
+  const Req reqs[6] = {{x, PX, 3}, {v, PV, 3}, {B, PB, 9}, {F, PF, 9}, {plastic, PPL, 1}, {volume, PVOL, 1}};
+  hipError_t e = hipSuccess;
+  for (const Req& r : reqs) {
+    if (!r.dst) continue;
+    launch_unpermute(ctx->stream, ctx->state[ctx->cur], ctx->cap, n, r.p0, r.np, tmp);
+    e = hipMemcpyAsync(r.dst, tmp, sizeof(float) * r.np * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) break;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(tmp);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_get_particles: ") + hipGetErrorString(e));
+  return MPM_OK;
+}
+
+int mpm_estimate_volumes(mpm_ctx* ctx) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  rc = stage_sort(ctx);
+  if (rc) return rc;
+  P2GArgs a = p2g_args(ctx, 0.0f);
+  launch_p2g(ctx->stream, a, P2G_MOM, ctx->p2g_grid);
+  launch_grid(ctx->stream, grid_args(ctx, 0.0f), GRID_NORMALIZE, ctx->grid_grid);
+  launch_volumes(ctx->stream, ctx->dp, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->grid);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (stage != ctx->next_stage) return fail(ctx, MPM_ERR_STATE, "stages must be issued in order SORT, P2G, GRID, G2P");
+  CK(hipSetDevice(ctx->device));
+  rc = run_stage(ctx, stage, dt);
+  if (rc) return rc;
+  CK(hipGetLastError());
+  ctx->next_stage = (stage + 1) % MPM_NUM_STAGES;
+  return MPM_OK;
+}
+
+int mpm_substep(mpm_ctx* ctx, float dt) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  return one_substep(ctx, dt);
+}
+
+int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(ctx, MPM_ERR_INVALID, "nsteps < 0");
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  if (nsteps > 0 && !ctx->keys_valid) {
+    rc = one_substep(ctx, dt);
+    if (rc) return rc;
+    --nsteps;
+  }
+  const int pairs = nsteps / 2;
+  if (pairs > 0) {
+    const int c0 = ctx->cur;
+    if (!ctx->graph[c0] || ctx->graph_dt[c0] != dt) {
+      if (ctx->graph[c0]) {
+        (void)hipGraphExecDestroy(ctx->graph[c0]);
+        ctx->graph[c0] = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      const int64_t steps0 = ctx->substeps;
+      CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int r1 = one_substep(ctx, dt);
+      int r2 = r1 ? r1 : one_substep(ctx, dt);
+      hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+      ctx->substeps = steps0;
+      if (r2) return r2;
+      if (ec != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+      hipError_t ei = hipGraphInstantiate(&ctx->graph[c0], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      ctx->graph_dt[c0] = dt;
+      ctx->cur = c0;  // capture did not execute anything
+    }
+    for (int i = 0; i < pairs; ++i) CK(hipGraphLaunch(ctx->graph[c0], ctx->stream));
+    ctx->substeps += 2 * pairs;
+    ctx->sort_buf = 1 - c0;
+    ctx->cur = c0;
+  }
+  if (nsteps % 2) {
+    rc = one_substep(ctx, dt);
+    if (rc) return rc;
+  }
+  CK(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  CK(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  CK(hipStreamSynchronize(ctx->stream));
+  if (keys) CK(hipMemcpy(keys, ctx->k1, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  if (ids) {
+    std::vector<int> perm(n), idp(n);
+    CK(hipMemcpy(perm.data(), ctx->v2, sizeof(int) * n, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(idp.data(), ctx->state[ctx->sort_buf] + (size_t)PID * ctx->cap, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; ++j) ids[j] = idp[perm[j]];
+  }
+  if (bins) {
+    const int64_t ng = ctx->ng;
+    std::vector<int> cs(ng), ce(ng);
+    CK(hipMemcpy(cs.data(), ctx->cell_start, sizeof(int) * ng, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(ce.data(), ctx->cell_end, sizeof(int) * ng, hipMemcpyDeviceToHost));
+    bins[ng] = n;
+    for (int64_t c = ng - 1; c >= 0; --c) bins[c] = (ce[c] > cs[c]) ? cs[c] : bins[c + 1];
+  }
+  return MPM_OK;
+}
+
+int mpm_get_grid(mpm_ctx* ctx, float* m, float* mom, float* v, float* vbf) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "no grid");
+  CK(hipSetDevice(ctx->device));
+  const int64_t ng = ctx->ng;
+  float* tmp = nullptr;
+  int rc = dalloc(ctx, tmp, (size_t)3 * ng);
+  if (rc) return rc;
+  struct Req { float* dst; int p0; int np; };
+  const Req reqs[4] = {{m, 0, 1}, {mom, 1, 3}, {v, 4, 3}, {vbf, 7, 3}};
+  hipError_t e = hipSuccess;
+  for (const Req& r : reqs) {
+    if (!r.dst) continue;
+    launch_grid_export(ctx->stream, ctx->dp, ctx->grid + r.p0 * ng, ng, r.np, ctx->block_slot, tmp);
+    e = hipMemcpyAsync(r.dst, tmp, sizeof(float) * r.np * ng, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) break;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(tmp);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_get_grid: ") + hipGetErrorString(e));
+  return MPM_OK;
+}
+
+int mpm_get_stats(mpm_ctx* ctx, mpm_stats* s) {
+  if (!ctx || !s) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  std::memset(s, 0, sizeof(*s));
+  s->substeps = ctx->substeps;
+  s->n_particles = ctx->n;
+  int cnt[CNT_NUM] = {0, 0};
+  int status = 0;
+  unsigned long long fb[4] = {0, 0, 0, 0};
+  if (ctx->have_grid && ctx->n > 0) {
+    CK(hipMemsetAsync(ctx->fallbacks + 2, 0, sizeof(unsigned long long), ctx->stream));
+    launch_count_nodes(ctx->stream, ctx->dp, ctx->grid, ctx->counters, ctx->active_blocks, ctx->fallbacks + 2, ctx->grid_grid);
+  }
+  CK(hipMemcpyAsync(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipMemcpyAsync(&status, ctx->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipMemcpyAsync(fb, ctx->fallbacks, sizeof(fb), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  s->status = status;
+  s->n_active_tiles = cnt[CNT_TILES];
+  s->n_active_blocks = cnt[CNT_BLOCKS];
+  s->n_active_nodes = (int64_t)fb[2];
+  s->fp64_svd_count = (int64_t)fb[0];
+  s->fp64_apex_count = (int64_t)fb[1];
+  return MPM_OK;
+}
+
+int mpm_clear_status(mpm_ctx* ctx) {
+  if (!ctx) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream));
+  CK(hipMemsetAsync(ctx->fallbacks, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+int mpm_synchronize(mpm_ctx* ctx) {
+  if (!ctx) return MPM_ERR_INVALID;
+  CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  hipEvent_t ev[MPM_NUM_STAGES + 1];
+  for (auto& e : ev) CK(hipEventCreate(&e));
+  hipEvent_t t0, t1;
+  CK(hipEventCreate(&t0));
+  CK(hipEventCreate(&t1));
+  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0};
+  CK(hipEventRecord(t0, ctx->stream));
+  for (int i = 0; i < nsteps; ++i) {
+    CK(hipEventRecord(ev[0], ctx->stream));
+    for (int st = 0; st < MPM_NUM_STAGES; ++st) {
+      rc = run_stage(ctx, st, dt);
+      if (rc) return rc;
+      CK(hipEventRecord(ev[st + 1], ctx->stream));
+    }
+    CK(hipEventSynchronize(ev[MPM_NUM_STAGES]));
+    for (int st = 0; st < MPM_NUM_STAGES; ++st) {
+      float ms = 0;
+      CK(hipEventElapsedTime(&ms, ev[st], ev[st + 1]));
+      acc[st] += ms;
+    }
+  }
+  CK(hipEventRecord(t1, ctx->stream));
+  CK(hipEventSynchronize(t1));
+  float tot = 0;
+  CK(hipEventElapsedTime(&tot, t0, t1));
+  if (total_ms) *total_ms = tot;
+  if (stage_ms)
+    for (int st = 0; st < MPM_NUM_STAGES; ++st) stage_ms[st] = acc[st];
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  CK(hipGetLastError());
+  return MPM_OK;
+}
+
+// ------------------------------------------------------------------ generator (host only)
+static inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int mpm_generate_block(uint64_t seed, const double lo[3], const double hi[3], const int32_t cells[3], int32_t ppa,
+                       int64_t n, float* x) {
+  if (!lo || !hi || !cells || !x || ppa < 1) return MPM_ERR_INVALID;
+  const int64_t pp = (int64_t)ppa * ppa * ppa;
+  const int64_t expect = (int64_t)cells[0] * cells[1] * cells[2] * pp;
+  if (cells[0] < 1 || cells[1] < 1 || cells[2] < 1 || n != expect) return MPM_ERR_INVALID;
+  const uint64_t sk = seed * 0x632BE59BD9B4E019ull;
+  for (int k = 0; k < cells[2]; ++k)
+    for (int j = 0; j < cells[1]; ++j)
+      for (int i = 0; i < cells[0]; ++i) {
+        const int64_t cell = ((int64_t)k * cells[1] + j) * cells[0] + i;
+        const int c[3] = {i, j, k};
+        for (int sz = 0; sz < ppa; ++sz)
+          for (int sy = 0; sy < ppa; ++sy)
+            for (int sx = 0; sx < ppa; ++sx) {
+              const int64_t id = cell * pp + ((int64_t)sz * ppa + sy) * ppa + sx;
+              const int s3[3] = {sx, sy, sz};
+              for (int a = 0; a < 3; ++a) {
+                const uint64_t r = splitmix64(sk ^ (uint64_t)(4 * id + a));
+                const double u = (double)(r >> 40) * (1.0 / 16777216.0);
+                const double t = (c[a] + (s3[a] + u) / ppa) / cells[a];
+                x[(size_t)a * n + id] = (float)(lo[a] + (hi[a] - lo[a]) * t);
+              }
+            }
+      }
+  return MPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+"""Seeded scene builders for the benchmark configurations and the parity cases.
+
+Each builder returns a `Scene`: simulation parameters, grid, level-set primitives and the initial
+particle arrays in the planar layout of the C ABI. Positions come from the seeded stratified-jitter
+generator of the C ABI (mpm_generate_block), so identical seeds give identical inputs to the device
+and to any CPU restatement. These replace the unseeded factories of the reference
+(ParticleSystem.cpp:119-401); material defaults are those of the configurations named below.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .api import generate_block, levelset
+
+
+@dataclass
+class Scene:
+    name: str
+    params: object
+    gmin: tuple
+    gmax: tuple
+    res: tuple
+    prims: list
+    particles: dict
+    dt: float
+    substeps: int
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.particles["x"].shape[1]
+
+    def load_into(self, solver):
+        solver.set_grid(self.gmin, self.gmax, self.res)
+        solver.set_params(self.params)
+        solver.set_levelset(self.prims)
+        p = self.particles
+        solver.set_particles(p["x"], p["mass"], v=p["v"], B=p.get("B"), F=p.get("F"), plastic=p["plastic"],
+                             volume=p["volume"], phi=p.get("phi"))
+
+
+def _particles(x, rho, vol_total, v0, material, phi=None):
+    n = x.shape[1]
+    vp = vol_total / n
+    v = np.empty((3, n), np.float32)
+    v[:] = np.asarray(v0, np.float32).reshape(3, 1)
+    F = np.zeros((9, n), np.float32)
+    F[0] = F[4] = F[8] = 1.0
+    return dict(x=x, v=v, B=np.zeros((9, n), np.float32), F=F,
+                plastic=np.full(n, 1.0 if material == _lib.SNOW else 0.0, np.float32),
+                mass=np.full(n, rho * vp, np.float32), volume=np.full(n, vp, np.float32),
+                phi=phi)
+
+
+def snow_params(order=_lib.KERNEL_QUADRATIC, gravity=(0.0, -9.8, 0.0)):
+    p = _lib.default_params()
+    p.kernel_order = order
+    p.material = _lib.SNOW
+    p.stress_at_predicted_F = 0
+    p.friction_model = _lib.FRICTION_STICKY
+    for i in range(3):
+        p.gravity[i] = gravity[i]
+    p.youngs_modulus = 1.4e5
+    p.poisson_ratio = 0.2
+    p.critical_compression = 2.5e-2
+    p.critical_stretch = 7.5e-3
+    p.hardening = 10.0
+    return p
+
+
+def sand_params(order=_lib.KERNEL_QUADRATIC, E=3.5e5, nu=0.3, phi_deg=30.0, mu=0.4, gravity=(0.0, -9.8, 0.0)):
+    p = _lib.default_params()
+    p.kernel_order = order
+    p.material = _lib.SAND
+    p.stress_at_predicted_F = 0
+    p.friction_model = _lib.FRICTION_COULOMB
+    p.friction = mu
+    for i in range(3):
+        p.gravity[i] = gravity[i]
+    p.youngs_modulus = E
+    p.poisson_ratio = nu
+    p.dp_h[0] = phi_deg  # constant friction angle: h1 = h3 = 0
+    p.dp_h[1] = 0.0
+    p.dp_h[2] = 0.0
+    p.dp_h[3] = 0.0
+    return p
+
+
+def config1_snow_block(seed=1, y_offset=0.0, substeps=200):
+    """Plain snow block: grid 64^3 over the unit cube, 32,768 particles (8 per cell) in
+    [0.3,0.55]^3, v0 = (0,-1,0), fixed-corotated snow E=1.4e5 nu=0.2 rho=400, theta_c=2.5e-2,
+    theta_s=7.5e-3, xi=10, gravity -9.8 y, dt=1e-4, sticky ground plane y=0.05.
+    `y_offset` lowers the block (a variant that reaches the ground within the run)."""
+    lo = (0.3, 0.3 - y_offset, 0.3)
+    hi = (0.55, 0.55 - y_offset, 0.55)
+    x = generate_block(seed, lo, hi, (16, 16, 16), 2)
+    parts = _particles(x, 400.0, 0.25 ** 3, (0.0, -1.0, 0.0), _lib.SNOW)
+    prims = [levelset(_lib.LS_PLANE, (0.0, 1.0, 0.0, 0.05))]
+    return Scene("config1_snow_block", snow_params(), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (64, 64, 64), prims, parts,
+                 1e-4, substeps, dict(ppc=8))
+
+
+def config2_sand_column(seed=2, substeps=1000, cells=(32, 128, 32)):
+    """Drucker-Prager sand column collapse: grid 128^3, 2^20 particles in [0.4,0.6]x[0.05,0.65]x
+    [0.4,0.6] (a 32x128x32 jittered lattice with 8 particles per lattice cell, about 20.8 per grid
+    cell), E=3.5e5 nu=0.3 rho=2200, constant friction angle 30 deg, cohesion 0; box walls
+    [0.05,0.95]^3 plus a static sphere (r=0.1, centre (0.5,0.15,0.7)), Coulomb friction mu=0.4,
+    dt=5e-5."""
+    lo, hi = (0.4, 0.05, 0.4), (0.6, 0.65, 0.6)
+    x = generate_block(seed, lo, hi, cells, 2)
+    vol = 0.2 * 0.6 * 0.2
+    parts = _particles(x, 2200.0, vol, (0.0, 0.0, 0.0), _lib.SAND)
+    prims = [levelset(_lib.LS_BOX, (0.05, 0.05, 0.05, 0.0), (0.95, 0.95, 0.95, 0.0)),
+             levelset(_lib.LS_SPHERE, (0.5, 0.15, 0.7, 0.1))]
+    n = x.shape[1]
+    return Scene("config2_sand_column", sand_params(), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (128, 128, 128), prims, parts,
+                 5e-5, substeps, dict(ppc=n / (vol * 128 ** 3)))
+
+
+def small_scene(material, order, predicted, friction_model, seed=7, res=(24, 24, 24), cells=(6, 5, 6), ppa=2,
+                lo=(0.3, 0.2, 0.35), hi=(0.55, 0.42, 0.6), v0=(0.3, -1.5, 0.2), z_up=False, prims=None,
+                dt=2e-4, E=None):
+    """A small scene for per-stage parity: any material / kernel / stress mode / friction model."""
+    grav = (0.0, 0.0, -9.8) if z_up else (0.0, -9.8, 0.0)
+    if material == _lib.SNOW:
+        p = snow_params(order, grav)
+        rho = 400.0
+    else:
+        p = sand_params(order, gravity=grav)
+        rho = 2200.0
+    if E is not None:
+        p.youngs_modulus = E
+    p.stress_at_predicted_F = int(predicted)
+    p.friction_model = friction_model
+    if friction_model == _lib.FRICTION_REFERENCE:
+        p.friction = 0.2
+    x = generate_block(seed, lo, hi, cells, ppa)
+    vol = float(np.prod(np.subtract(hi, lo)))
+    parts = _particles(x, rho, vol, v0, material)
+    if prims is None:
+        if z_up:
+            prims = [levelset(_lib.LS_PLANE, (0.0, 0.0, 1.0, 0.2))]
+        else:
+            prims = [levelset(_lib.LS_PLANE, (0.0, 1.0, 0.0, 0.18))]
+    return Scene("small", p, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), res, prims, parts, dt, 1)

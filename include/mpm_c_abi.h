@@ -1,0 +1,222 @@
+/*
+ * mpm_c_abi.h -- C ABI of the MI355X (gfx950) explicit MPM substep library (libmpm_hip.so).
+ *
+ * This is the drop-in boundary for the hot path of the reference solver:
+ * the body of the `while (t <= maxt)` loop of HybridSolver::solve
+ * (AnisotropicElastoplasticity/HybridSolver.cpp:867-989) and the private methods it calls.
+ * The reference has no C ABI of its own: its callers hold raw pointers to ParticleSystem /
+ * RegularGrid / LagrangianMesh objects and call HybridSolver methods (HybridSolver.h:83-93).
+ * The C++ host classes in include/mpm/ keep that API and forward to the functions below;
+ * each function names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status (MPM_OK = 0). No C++ exception crosses the ABI.
+ *     mpm_last_error() returns a message for the last failure on a context.
+ *     (The reference reports errors with cerr and exit()/while(1), e.g. RegularGrid.cpp:129-134.)
+ *   - The caller owns host arrays; they are copied in and out. The context owns device memory.
+ *   - Particle arrays are structure-of-arrays planes of n floats: x[3n] = {x0[n], x1[n], x2[n]};
+ *     3x3 matrices are 9 row-major planes: M[(3*a+b)*n + p] = M_ab of particle p.
+ *     (The reference's MatrixX3d members are column-major n x 3, i.e. already one plane per
+ *     component; affineMomenta_1..3 are the rows of B; ParticleSystem.h:27-35.)
+ *   - Particle order seen by the caller never changes: getters return arrays in the order given
+ *     to mpm_set_particles, whatever order the device keeps them in.
+ *   - One context per host thread. Calls are asynchronous on the context's HIP stream;
+ *     getters synchronise.
+ *   - Device state is FP32. FP64 is used on the device only in the 3x3 SVD of near-degenerate
+ *     deformation gradients and in the Drucker-Prager projection near the cone apex.
+ */
+#ifndef MPM_C_ABI_H
+#define MPM_C_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MPM_OK 0
+#define MPM_ERR_INVALID 1     /* bad argument or size */
+#define MPM_ERR_HIP 2         /* a HIP runtime call failed */
+#define MPM_ERR_STATE 3       /* call order (e.g. substep before set_particles) */
+#define MPM_ERR_NOMEM 4       /* device allocation failed */
+#define MPM_ERR_UNSUPPORTED 5 /* option not implemented */
+
+/* ---- enums ---- */
+/* Interpolation kernel. Reference: cubic B-spline (interpolation.cpp:9-33), D^-1 = 3/h^2
+ * (HybridSolver.cpp:175-177). Quadratic B-spline with D^-1 = 4/h^2 is the standard APIC choice. */
+#define MPM_KERNEL_QUADRATIC 2
+#define MPM_KERNEL_CUBIC 3
+
+/* Material. Reference enum MaterialType {SNOW, SAND} (HybridSolver.h:21-25). */
+#define MPM_SNOW 0
+#define MPM_SAND 1
+
+/* Grid collision friction model (HybridSolver.cpp:460-511).
+ * REFERENCE: the reference as written: on approach, v_BF = v_t and v = 0 if |v_t| < -mu v_n else
+ *            v = v_t (the Coulomb term at :500-501 is lost to a stray ';').
+ * COULOMB:   the intended rule: v = v_t + mu v_n v_t/|v_t| (or 0), v_BF = v_t.
+ * STICKY:    every node inside the level set gets v = v_BF = 0. */
+#define MPM_FRICTION_REFERENCE 0
+#define MPM_FRICTION_COULOMB 1
+#define MPM_FRICTION_STICKY 2
+
+/* Level-set primitives; the composite level set is the minimum over primitives and the normal
+ * is taken from the primitive that attains it. Replaces the std::function pair LevelSet/DLevelSet
+ * (LevelSet.h:5-6), which cannot cross to the GPU.
+ *   PLANE       phi = n.x - d                  a = {n0,n1,n2,d}      (groundLevelSet: n=(0,0,1), d=groundZ,
+ *                                                                   LevelSet.cpp:8-16)
+ *   WALL2GROUND phi = min(x2-g, wx-x0, wy-x1), normal of the nearest plane by |distance|
+ *                                              a = {wx, wy, g, 0}    (LevelSet.cpp:18-42)
+ *   BOX         material inside [lo,hi]: phi = min_i min(x_i-lo_i, hi_i-x_i), inward face normal
+ *                                              a = {lo0,lo1,lo2,0} b = {hi0,hi1,hi2,0}
+ *   SPHERE      solid obstacle: phi = |x-c| - r, normal (x-c)/|x-c|
+ *                                              a = {c0,c1,c2,r} */
+#define MPM_LS_PLANE 0
+#define MPM_LS_WALL2GROUND 1
+#define MPM_LS_BOX 2
+#define MPM_LS_SPHERE 3
+
+/* Stages of one substep, in order (HybridSolver.cpp:867-989 reordered so that P2G opens the
+ * substep; the reference's initial P2G at :854 becomes the first substep's P2G). */
+#define MPM_STAGE_SORT 0  /* cell keys -> stable radix sort -> bins                         */
+#define MPM_STAGE_P2G 1   /* particleToGrid_ :113-250 (+ computeGridForces_ :252-368)        */
+#define MPM_STAGE_GRID 2  /* updateGridVelocities_ :725-737, gravity :457, collision :460-551 */
+#define MPM_STAGE_G2P 3   /* :739-825, advection :942-950, :553-609, updatePlasticity_ :612-681 */
+#define MPM_NUM_STAGES 4
+
+/* Status flags reported by mpm_get_stats (sticky until mpm_clear_status). */
+#define MPM_STATUS_NONFINITE 1      /* a particle state value became NaN/Inf */
+#define MPM_STATUS_OUT_OF_DOMAIN 2  /* a particle cell was clamped to the grid */
+#define MPM_STATUS_CFL 4            /* a particle moved more than one cell in a substep */
+#define MPM_STATUS_TILE_OVERFLOW 8  /* a tile held more particles than the P2G staging limit */
+
+/* Simulation parameters. Defaults (mpm_params_default) reproduce the reference's hard-coded
+ * values; every deviation the configurations need is one switch here, not a second code path. */
+typedef struct mpm_params {
+  int32_t kernel_order;          /* MPM_KERNEL_*; reference = CUBIC */
+  int32_t material;              /* MPM_SNOW / MPM_SAND; reference solve() hard-codes SAND (:873,955,959) */
+  int32_t stress_at_predicted_F; /* 1: stress at F^ = (I + dt grad v)F from post-P2G grid velocity, times
+                                    F^T (reference, :269-306,321-338); 0: stress at F^n (standard explicit) */
+  int32_t friction_model;        /* MPM_FRICTION_*; reference = REFERENCE */
+  float gravity[3];              /* reference: (0,0,-9.8) as a force m*g (:457) */
+  float youngs_modulus;          /* E (ParticleSystem::youngsModulus) */
+  float poisson_ratio;           /* nu */
+  float critical_compression;    /* snow theta_c (ParticleSystem.cpp:173-176) */
+  float critical_stretch;        /* snow theta_s */
+  float hardening;               /* snow xi, lambda,mu scale exp(xi(1-J_P)) (:267,285-286) */
+  float dp_h[4];                 /* sand friction angle in degrees: phi(q) = h0 + (h1 q - h3) exp(-h2 q)
+                                    (:641-647); reference (35, 9, 0.2, 10); constant phi <=> h1 = h3 = 0 */
+  int32_t per_particle_phi;      /* 1: a per-particle friction angle (degrees) replaces h0 */
+  float friction;                /* grid collision friction coefficient; reference 0.2 (:465) */
+  float apic_damping;            /* C <- skew + (1-d) sym (:808-824); reference d = 0 for particles */
+  float fp64_svd_ratio;          /* redo the SVD in FP64 when sigma_min < ratio * sigma_max */
+  float fp64_apex_radius;        /* run the DP projection in FP64 when |eps_trial| < radius */
+  int32_t reserved[4];
+} mpm_params;
+
+typedef struct mpm_levelset {
+  int32_t type; /* MPM_LS_* */
+  float a[4];
+  float b[4];
+} mpm_levelset;
+
+typedef struct mpm_stats {
+  int64_t substeps;          /* substeps run since mpm_set_particles */
+  int32_t status;            /* OR of MPM_STATUS_* */
+  int32_t n_particles;
+  int32_t n_active_tiles;    /* 4x4x4-cell tiles holding particles, last substep */
+  int32_t n_active_blocks;   /* 4x4x4-node grid blocks updated, last substep */
+  int64_t n_active_nodes;    /* nodes with m > 0, last substep (counted on request) */
+  int64_t fp64_svd_count;    /* SVDs redone in FP64, cumulative */
+  int64_t fp64_apex_count;   /* DP projections run in FP64, cumulative */
+} mpm_stats;
+
+typedef struct mpm_ctx mpm_ctx; /* opaque: one HIP stream + device buffers */
+
+/* Fill p with the reference's defaults: cubic kernel, sand, stress at predicted F, reference
+ * friction with mu = 0.2, gravity (0,0,-9.8), snow (1.4e5, 0.2, 2.5e-2, 7.5e-3, 10) from
+ * ParticleSystem.cpp:173-176 and HybridSolver.cpp:267, DP hardening (35, 9, 0.2, 10) from
+ * HybridSolver.cpp:641-644. */
+void mpm_params_default(mpm_params* p);
+
+/* Create a context on HIP device `device`. Replaces HybridSolver::HybridSolver (HybridSolver.h:83-84). */
+int mpm_create(mpm_ctx** out, int32_t device);
+/* Release all device memory and the stream. */
+int mpm_destroy(mpm_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed mpm_create when ctx is NULL). */
+const char* mpm_last_error(const mpm_ctx* ctx);
+
+/* Node-centred grid: node (i,j,k) sits at min + (i,j,k)*h with h = (max-min)/res, linear index
+ * k*res0*res1 + j*res0 + i. Mirrors RegularGrid::RegularGrid (RegularGrid.cpp:117-162),
+ * RegularGrid::toIndex (:164-168) and HybridSolver::setRegularGrid (HybridSolver.h:87). */
+int mpm_set_grid(mpm_ctx* ctx, const double min_bound[3], const double max_bound[3], const int32_t res[3]);
+
+/* Replaces the constants hard-coded in HybridSolver.cpp (:264-267, :465, :641-644, :457). */
+int mpm_set_params(mpm_ctx* ctx, const mpm_params* params);
+
+/* Replaces HybridSolver::setLevelSet(LevelSet, DLevelSet) (HybridSolver.h:89). n <= 16. */
+int mpm_set_levelset(mpm_ctx* ctx, const mpm_levelset* prims, int32_t n);
+
+/* Mirrors the ParticleSystem constructor (ParticleSystem.cpp:81-117) and
+ * HybridSolver::setParticleSystem (HybridSolver.h:86). plastic = J_P = det F_P for snow (the only
+ * use of F_P, HybridSolver.cpp:283) and plasticAmount q for sand (:646,665,672). phi (degrees)
+ * may be NULL unless params.per_particle_phi. B may be NULL (zero) -- the reference leaves it
+ * uninitialised (ParticleSystem.cpp:113-115). */
+int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, const float* B,
+                      const float* F, const float* plastic, const float* mass, const float* volume,
+                      const float* phi);
+
+/* Copy particle state out in caller order; any pointer may be NULL. */
+int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* F, float* plastic,
+                      float* volume);
+
+/* The reference's first P2G with evaluateVolumesAndDensities = true (HybridSolver.cpp:242-249):
+ * rho_p = sum_i w_ip m_i / h^3, V_p = m_p / rho_p, at the current positions. */
+int mpm_estimate_volumes(mpm_ctx* ctx);
+
+/* One explicit substep with fixed dt (HybridSolver.cpp:873-987). */
+int mpm_substep(mpm_ctx* ctx, float dt);
+/* nsteps substeps with fixed dt, replayed from one captured HIP graph. */
+int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps);
+/* Run a single stage (MPM_STAGE_*) of the current substep; stages must be issued in order
+ * SORT, P2G, GRID, G2P. Used by per-stage parity tests. */
+int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt);
+
+/* Result of the last SORT stage: keys in sorted order (cell index k*r0*r1 + j*r0 + i of
+ * trunc((x-min)/h)), the caller-order particle id at each sorted position, and bin offsets
+ * (n_cells+1 exclusive prefix of the per-cell counts). Equals std::stable_sort of the keys by
+ * caller index. Any pointer may be NULL. */
+int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins);
+
+/* Dense grid arrays after the last P2G / GRID stage. mom = momentum accumulated by P2G
+ * (plus dt * stress force when stress is evaluated at F^n); v, vbf = velocity after the grid
+ * update and collision, and before friction (HybridSolver.cpp:463,492). Nodes outside the
+ * blocks touched by particles read 0. Any pointer may be NULL; m has n_nodes floats, others 3*n_nodes
+ * (planar). */
+int mpm_get_grid(mpm_ctx* ctx, float* m, float* mom, float* v, float* vbf);
+
+int mpm_get_stats(mpm_ctx* ctx, mpm_stats* stats);
+int mpm_clear_status(mpm_ctx* ctx);
+int mpm_synchronize(mpm_ctx* ctx);
+
+/* Timing helper for benchmarks: run nsteps substeps, recording HIP events on the context stream
+ * around every stage; stage_ms[MPM_NUM_STAGES] receives the summed stage times and *total_ms the
+ * whole-span time. */
+int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms);
+
+/* Seeded jittered particle generator (splitmix64 counter hash). The box [lo,hi) is cut into a
+ * lattice of cells[0] x cells[1] x cells[2] cells; each cell holds ppa^3 particles on a stratified
+ * sub-lattice, each jittered uniformly within its sub-cell. Particle id
+ * = ((k*cells1 + j)*cells0 + i)*ppa^3 + (sk*ppa + sj)*ppa + si; jitter component c of particle id
+ * is u = (splitmix64(seed*0x632BE59BD9B4E019 ^ (4*id + c)) >> 40) * 2^-24 and the coordinate is
+ * lo + (hi-lo) * (cell + (sub + u)/ppa) / cells, evaluated in double and rounded to float.
+ * n must equal cells0*cells1*cells2*ppa^3. Replaces the unseeded samplers of the reference
+ * factories (ParticleSystem.cpp:119-401). */
+int mpm_generate_block(uint64_t seed, const double lo[3], const double hi[3], const int32_t cells[3],
+                       int32_t ppa, int64_t n, float* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_C_ABI_H */

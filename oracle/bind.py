@@ -1,0 +1,210 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libmpm_oracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Arrays cross as float64 in the planar layout of the C ABI (x: (3, n), B/F: (9, n) row-major).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmpm_oracle.so")
+
+_d = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        from . import build as _b
+        _b.build()
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "orc_create": (vp, [ctypes.c_int, vp, _d, _d, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]),
+        "orc_destroy": (None, [vp]),
+        "orc_set_levelset": (ctypes.c_int, [vp, vp, ctypes.c_int]),
+        "orc_set_particles": (ctypes.c_int, [vp, ctypes.c_int, _d, _d, _d, _d, _d, _d, _d, _d]),
+        "orc_get_particles": (ctypes.c_int, [vp, _d, _d, _d, _d, _d, _d]),
+        "orc_stage": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
+        "orc_substep": (ctypes.c_int, [vp, ctypes.c_double]),
+        "orc_run": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
+        "orc_estimate_volumes": (ctypes.c_int, [vp]),
+        "orc_get_grid": (ctypes.c_int, [vp, _d, _d, _d, _d, _d]),
+        "orc_cell_keys": (None, [ctypes.c_int, _d, _d, _d, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint32)]),
+        "orc_stable_sort": (None, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+        "orc_svd3": (None, [_d, _d, _d, _d]),
+        "orc_qr3": (None, [_d, _d, _d]),
+        "orc_spline": (None, [ctypes.c_int, ctypes.c_int, _d, _d, _d]),
+        "orc_piola": (None, [vp, _d, ctypes.c_double, _d]),
+        "orc_dp_project": (None, [vp, ctypes.c_double, ctypes.c_double, _d, _d, _d]),
+        "orc_levelset": (None, [vp, ctypes.c_int, _d, _d, _d]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_d)
+
+
+def _f64(a, shape=None):
+    if a is None:
+        return None
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    return arr if shape is None else arr.reshape(shape)
+
+
+class Oracle:
+    """A CPU restatement of the substep on one scene (precision 64 or 32)."""
+
+    def __init__(self, params, gmin, gmax, res, prims=(), precision=64, reference_forms=False):
+        self.lib = load()
+        mn = (ctypes.c_double * 3)(*gmin)
+        mx = (ctypes.c_double * 3)(*gmax)
+        rs = (ctypes.c_int32 * 3)(*res)
+        self._params = params
+        self.h = self.lib.orc_create(precision, ctypes.addressof(params), mn, mx, rs, int(reference_forms))
+        self.res = tuple(res)
+        self.n = 0
+        if prims:
+            self.set_levelset(prims)
+
+    def set_levelset(self, prims):
+        from anisotropicelastoplasticity_amd._lib import LevelSetPrim
+        arr = (LevelSetPrim * len(prims))(*prims)
+        self._prims = arr
+        self.lib.orc_set_levelset(self.h, ctypes.addressof(arr), len(prims))
+
+    def set_particles(self, x, mass, v=None, B=None, F=None, plastic=None, volume=None, phi=None):
+        x = _f64(x)
+        n = x.shape[-1]
+        keep = [x, _f64(v), _f64(B), _f64(F), _f64(plastic), _f64(mass), _f64(volume), _f64(phi)]
+        self._keep = keep
+        self.lib.orc_set_particles(self.h, n, *[_dp(a) for a in keep])
+        self.n = n
+
+    def load_scene(self, scene):
+        p = scene.particles
+        self.set_particles(p["x"], p["mass"], v=p["v"], B=p.get("B"), F=p.get("F"), plastic=p["plastic"],
+                           volume=p["volume"], phi=p.get("phi"))
+
+    def stage(self, st, dt):
+        self.lib.orc_stage(self.h, int(st), float(dt))
+
+    def substep(self, dt):
+        self.lib.orc_substep(self.h, float(dt))
+
+    def run(self, dt, n):
+        self.lib.orc_run(self.h, float(dt), int(n))
+
+    def estimate_volumes(self):
+        self.lib.orc_estimate_volumes(self.h)
+
+    def get_particles(self):
+        n = self.n
+        out = dict(x=np.empty((3, n)), v=np.empty((3, n)), B=np.empty((9, n)), F=np.empty((9, n)),
+                   plastic=np.empty(n), volume=np.empty(n))
+        self.lib.orc_get_particles(self.h, _dp(out["x"]), _dp(out["v"]), _dp(out["B"]), _dp(out["F"]),
+                                   _dp(out["plastic"]), _dp(out["volume"]))
+        return out
+
+    def get_grid(self):
+        ng = int(np.prod(self.res))
+        out = dict(m=np.empty(ng), p=np.empty((3, ng)), f=np.empty((3, ng)), v=np.empty((3, ng)),
+                   vbf=np.empty((3, ng)))
+        self.lib.orc_get_grid(self.h, _dp(out["m"]), _dp(out["p"]), _dp(out["f"]), _dp(out["v"]), _dp(out["vbf"]))
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cell_keys(x, gmin, h, res):
+    lib = load()
+    x = _f64(x)
+    n = x.shape[-1]
+    keys = np.empty(n, np.uint32)
+    lib.orc_cell_keys(n, _dp(x), (ctypes.c_double * 3)(*gmin), (ctypes.c_double * 3)(*h),
+                      (ctypes.c_int32 * 3)(*res), keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return keys
+
+
+def stable_sort(keys, ncells):
+    lib = load()
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = keys.shape[0]
+    perm = np.empty(n, np.int32)
+    bins = np.empty(ncells + 1, np.int32)
+    lib.orc_stable_sort(n, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                        perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ncells,
+                        bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return perm, bins
+
+
+def svd3(A):
+    lib = load()
+    A = _f64(A, (9,))
+    U, S, V = np.empty(9), np.empty(3), np.empty(9)
+    lib.orc_svd3(_dp(A), _dp(U), _dp(S), _dp(V))
+    return U.reshape(3, 3), S, V.reshape(3, 3)
+
+
+def qr3(D):
+    lib = load()
+    D = _f64(D, (9,))
+    Q, R = np.empty(9), np.empty(9)
+    lib.orc_qr3(_dp(D), _dp(Q), _dp(R))
+    return Q.reshape(3, 3), R.reshape(3, 3)
+
+
+def spline(order, xs):
+    lib = load()
+    xs = _f64(xs)
+    w, dw = np.empty_like(xs), np.empty_like(xs)
+    lib.orc_spline(int(order), xs.size, _dp(xs), _dp(w), _dp(dw))
+    return w, dw
+
+
+def piola(params, F, Jp=1.0):
+    lib = load()
+    F = _f64(F, (9,))
+    out = np.empty(9)
+    lib.orc_piola(ctypes.addressof(params), _dp(F), float(Jp), _dp(out))
+    return out.reshape(3, 3)
+
+
+def dp_project(params, phi0, q, S):
+    lib = load()
+    S = _f64(S, (3,))
+    out = np.empty(3)
+    dq = np.empty(1)
+    lib.orc_dp_project(ctypes.addressof(params), float(phi0), float(q), _dp(S), _dp(out), _dp(dq))
+    return out, float(dq[0])
+
+
+def levelset_eval(prims, x):
+    from anisotropicelastoplasticity_amd._lib import LevelSetPrim
+    lib = load()
+    arr = (LevelSetPrim * len(prims))(*prims)
+    x = _f64(x, (3,))
+    phi = np.empty(1)
+    n = np.empty(3)
+    lib.orc_levelset(ctypes.addressof(arr), len(prims), _dp(x), _dp(phi), _dp(n))
+    return float(phi[0]), n

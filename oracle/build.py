@@ -1,0 +1,35 @@
+"""Build the CPU oracle (test infrastructure only) into oracle/_build/libmpm_oracle.so.
+
+The oracle is a C++ restatement of the reference substep (mpm_oracle.cpp). The reference itself
+cannot be compiled here: it needs Eigen 3 and libigl, neither of which is present, so there is no
+oracle/_ref build.
+
+Flags: -O3 with a portable x86-64-v3 target so that the library built here also runs on the host
+CPU of the GPU machine. OpenMP is not used: the reference solver is single-threaded.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT_DIR = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT_DIR, "libmpm_oracle.so")
+
+
+def build(verbose=False):
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared",
+           os.path.join(HERE, "mpm_oracle.cpp"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the oracle")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every entry point include/mpm_c_abi.h declares (no GPU calls),
+and the reference defaults are the reference's hard-coded constants."""
+import ctypes
+import os
+import re
+
+from anisotropicelastoplasticity_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(mpm_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(hip_lib):
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(hip_lib, n), n
+    bound = {s[0] for s in _lib.SIGNATURES}
+    assert names == bound
+
+
+def test_struct_sizes_match_header():
+    # sizes follow from the C declarations in include/mpm_c_abi.h
+    assert ctypes.sizeof(_lib.Params) == 4 * (4 + 3 + 5 + 4 + 1 + 4 + 4)
+    assert ctypes.sizeof(_lib.LevelSetPrim) == 36
+    assert ctypes.sizeof(_lib.Stats) == 8 + 4 * 4 + 8 * 3
+
+
+def test_reference_defaults():
+    p = _lib.default_params()
+    assert p.kernel_order == _lib.KERNEL_CUBIC            # interpolation.cpp:9-33
+    assert p.material == _lib.SAND                        # HybridSolver.cpp:873
+    assert p.stress_at_predicted_F == 1                   # HybridSolver.cpp:269-306
+    assert p.friction_model == _lib.FRICTION_REFERENCE    # HybridSolver.cpp:494-501
+    assert abs(p.friction - 0.2) < 1e-7                   # HybridSolver.cpp:465
+    assert tuple(p.gravity) == (0.0, 0.0, -9.800000190734863)  # HybridSolver.cpp:457
+    assert abs(p.youngs_modulus - 1.4e5) < 1e-2 and abs(p.poisson_ratio - 0.2) < 1e-7
+    assert abs(p.critical_compression - 2.5e-2) < 1e-9 and abs(p.critical_stretch - 7.5e-3) < 1e-9
+    assert p.hardening == 10.0                            # HybridSolver.cpp:267
+    assert [round(v, 6) for v in p.dp_h] == [35.0, 9.0, 0.2, 10.0]  # HybridSolver.cpp:641-644
+
+
+def test_create_without_gpu_fails_cleanly(hip_lib):
+    # On a machine without a GPU mpm_create reports an error instead of crashing; with a GPU it
+    # succeeds. Either way no kernel is launched.
+    h = ctypes.c_void_p()
+    rc = hip_lib.mpm_create(ctypes.byref(h), 0)
+    if rc == 0:
+        hip_lib.mpm_destroy(h)
+    else:
+        assert hip_lib.mpm_last_error(None)
