@@ -245,13 +245,49 @@ __device__ __forceinline__ void dp_project(T lambda, T mu, T phi0, T h1, T h2, T
   }
 }
 
+// Volume-weighted Kirchhoff stress K = V_p P(F) F^T of F = U diag(S) V^T, stored as the six entries
+// (xx, yy, zz, xy, xz, yz) of a symmetric matrix. With P of HybridSolver.cpp:314-339:
+//   sand: P F^T = U diag(2 mu ln s_i + lambda sum ln s) U^T
+//   snow: P F^T = 2 mu (F F^T - R F^T) + lambda (J-1) J I = U diag(2 mu (s_i^2 - s_i) + lambda (J-1) J) U^T
+// (R = U V^T, J = det F), with lambda, mu scaled by exp(xi (1 - J_P)) for snow (:281-287).
+__device__ __forceinline__ void kirchhoff(const DevParams& P, const float* U, const float* S, float J, float Jp,
+                                          float vol, float* K) {
+  float d[3];
+  if (P.material == MPM_SNOW) {
+    float sc = expf(P.xi * (1.0f - Jp));
+    float lambda = P.lambda0 * sc, mu = P.mu0 * sc;
+    float iso = lambda * (J - 1.0f) * J;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = vol * (2.0f * mu * (S[k] * S[k] - S[k]) + iso);
+  } else {
+    float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
+    float tr = P.lambda0 * (l0 + l1 + l2);
+    d[0] = vol * (2.0f * P.mu0 * l0 + tr);
+    d[1] = vol * (2.0f * P.mu0 * l1 + tr);
+    d[2] = vol * (2.0f * P.mu0 * l2 + tr);
+  }
+  const int R[6] = {0, 1, 2, 0, 0, 1}, C[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int e = 0; e < 6; ++e)
+    K[e] = U[3 * R[e]] * d[0] * U[3 * C[e]] + U[3 * R[e] + 1] * d[1] * U[3 * C[e] + 1] + U[3 * R[e] + 2] * d[2] * U[3 * C[e] + 2];
+}
+
+// K of a given F (used once after the caller sets the state; afterwards G2P produces K).
+__device__ __forceinline__ void kirchhoff_of_F(const DevParams& P, const float* F, float Jp, float vol, float* K, int& fb) {
+  float U[9], S[3], V[9];
+  svd3_guarded(F, U, S, V, P.svd_ratio, fb);
+  kirchhoff(P, U, S, mat_det(F), Jp, vol, K);
+}
+
 // Return mapping of one particle (HybridSolver.cpp:616-681). Fh is replaced by the elastic part
 // U Sigma V^T; plastic holds J_P (snow, whose update det(F_P') = J_P * prod(Sigma)/prod(Sigma') is
-// the determinant of F_P' = V Sigma'^-1 U^T F^ F_P, :676-677) or plasticAmount q (sand).
-__device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float& plastic, float phi_p, int& fb_svd,
-                                           int& fb_apex) {
+// the determinant of F_P' = V Sigma'^-1 U^T F^ F_P, :676-677) or plasticAmount q (sand). K receives
+// the Kirchhoff stress V tau of the new elastic state (same SVD).
+__device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float& plastic, float phi_p, float vol,
+                                           float* K, int& fb_svd, int& fb_apex) {
   float U[9], S[3], V[9];
   svd3_guarded(Fh, U, S, V, P.svd_ratio, fb_svd);
+  const float sgn = mat_det(U) * mat_det(V) < 0.0f ? -1.0f : 1.0f;
   float Sn[3];
   if (P.material == MPM_SNOW) {
     float lo = 1.0f - P.theta_c, hi = 1.0f + P.theta_s;
@@ -260,17 +296,18 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
     plastic = plastic * ((S[0] / Sn[0]) * (S[1] / Sn[1]) * (S[2] / Sn[2]));
   } else {
     float phi0 = P.per_particle_phi ? phi_p : P.h0;
-    float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
-    float en2 = l0 * l0 + l1 * l1 + l2 * l2;
     float dq;
-    if (en2 < P.apex_radius * P.apex_radius) {
-      double Sd[3] = {S[0], S[1], S[2]}, So[3], dqd;
-      dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, Sd, So, dqd);
-      Sn[0] = float(So[0]); Sn[1] = float(So[1]); Sn[2] = float(So[2]);
-      dq = float(dqd);
-      fb_apex += 1;
-    } else {
-      dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, S, Sn, dq);
+    dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, S, Sn, dq);
+    if (dq > 0.0f) {
+      // a projection happened; close to the cone apex redo it in FP64
+      float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
+      if (l0 * l0 + l1 * l1 + l2 * l2 < P.apex_radius * P.apex_radius) {
+        double Sd[3] = {S[0], S[1], S[2]}, So[3], dqd;
+        dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, Sd, So, dqd);
+        Sn[0] = float(So[0]); Sn[1] = float(So[1]); Sn[2] = float(So[2]);
+        dq = float(dqd);
+        fb_apex += 1;
+      }
     }
     plastic += dq;
   }
@@ -278,6 +315,7 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) Fh[3 * r + c] = U[3 * r] * Sn[0] * V[3 * c] + U[3 * r + 1] * Sn[1] * V[3 * c + 1] + U[3 * r + 2] * Sn[2] * V[3 * c + 2];
+  kirchhoff(P, U, Sn, sgn * Sn[0] * Sn[1] * Sn[2], plastic, vol, K);
 }
 
 // ------------------------------------------------------------------ level sets (LevelSet.cpp:8-42)

@@ -19,8 +19,12 @@ enum Plane : int {
   PVOL = 26,  // volume
   PPHI = 27,  // per-particle friction angle (degrees)
   PID = 28,   // caller index (int bits)
-  NPLANES = 29
+  PK = 29,    // volume-weighted Kirchhoff stress V tau: xx yy zz xy xz yz (stress at F^n only)
+  NPLANES = 35
 };
+
+// Grid planes: m (1), momentum (3), then per node two float4 (v0 v1 v2 vbf0 | vbf1 vbf2 0 0).
+constexpr int GRID_M = 0, GRID_MOM = 1, GRID_VEL = 4, GRID_FLOATS = 12;
 
 constexpr int SORT_IPT = 16;
 constexpr int SORT_TILE = 256 * SORT_IPT;  // keys per sort block
@@ -62,14 +66,14 @@ struct P2GArgs {
   const float* state;
   int64_t cap;
   int n;
-  const int* perm;
+  const int* perm;  // caller ids in sorted order
+  const int* pos;   // caller id -> physical index in `state`
   const int* cell_start;
   const int* cell_end;
   const int* counters;
   const int* active_tiles;
   float* partials;
-  const float* gv;  // grid velocity (force pass)
-  int64_t ng;
+  const float4* gvel;  // node velocities (force pass)
   float dt;
   unsigned long long* fallback_svd;
 };
@@ -82,8 +86,7 @@ struct GridArgs {
   const float* partials;
   float* gm;
   float* gmom;
-  float* gv;
-  float* gvbf;
+  float4* gvel;
   int64_t ng;
   float dt;
 };
@@ -94,32 +97,34 @@ struct G2PArgs {
   float* sout;
   int64_t cap;
   int n;
-  const int* perm;
-  const float* gv;
-  const float* gvbf;
-  int64_t ng;
+  const int* perm;     // caller ids in sorted order
+  const int* pos_in;   // caller id -> physical index in sin
+  int* pos_out;        // caller id -> physical index in sout
+  const float4* gvel;
   float dt;
+  int write_stress;    // store V tau of the new state (stress at F^n)
   const float* thr;
-  uint32_t* keys;
+  uint32_t* keys;      // next cell key, indexed by caller id
   int* status;
   unsigned long long* fallback_svd;
   unsigned long long* fallback_apex;
 };
 
-void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const float* thr, uint32_t* keys,
-                 int* status);
+void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const int* pos, const float* thr,
+                 uint32_t* keys, int* status);
+void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
 void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n);
-void launch_segments(hipStream_t s, const DevParams& P, const uint32_t* ks, const int* vin, int* vout, int n,
-                     const float* st, int64_t cap, int* cell_start, int* cell_end, int* tile_slot, int* active_tiles,
-                     int* block_slot, int* active_blocks, int* counters);
+void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int* cell_start, int* cell_end,
+                   int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters);
 void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
 void launch_g2p(hipStream_t s, const G2PArgs& a);
 void launch_volumes(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, const float* gm);
 void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out);
-void launch_grid_export(hipStream_t s, const DevParams& P, const float* src, int64_t ng, int nplanes, const int* block_slot,
+void launch_grid_export(hipStream_t s, const DevParams& P, const float* grid, int64_t ng, int which, const int* block_slot,
                         float* out);
+void launch_iota(hipStream_t s, int* p, int n);
 void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
                         unsigned long long* out, int grid);
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v);

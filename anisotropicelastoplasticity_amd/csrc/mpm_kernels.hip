@@ -49,14 +49,34 @@ __device__ __forceinline__ uint32_t cell_key(const DevParams& P, const float* th
   return (uint32_t)(c2 * P.res[0] * P.res[1] + c1 * P.res[0] + c0);
 }
 
-// ================================================================== keys of the initial order
+// ================================================================== keys of the caller order
+// keys[id] = cell key of caller particle id (physical index pos[id]); the radix sort consumes the
+// keys in caller order, so that equal keys keep caller order as in the reference.
 __global__ __launch_bounds__(256) void k_keys(DevParams P, const float* __restrict__ st, int64_t cap, int n,
-                                              const float* __restrict__ thr, uint32_t* __restrict__ keys,
-                                              int* __restrict__ status) {
+                                              const int* __restrict__ pos, const float* __restrict__ thr,
+                                              uint32_t* __restrict__ keys, int* __restrict__ status) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int oob = 0;
-  if (i < n) keys[i] = cell_key(P, thr, st[PX * cap + i], st[(PX + 1) * cap + i], st[(PX + 2) * cap + i], oob);
+  if (i < n) {
+    int pi = clamp_index(pos[i], n);
+    keys[i] = cell_key(P, thr, st[PX * cap + pi], st[(PX + 1) * cap + pi], st[(PX + 2) * cap + pi], oob);
+  }
   if (__ballot(oob) && lane_id() == 0) atomicOr(status, MPM_STATUS_OUT_OF_DOMAIN);
+}
+
+// Volume-weighted Kirchhoff stress of the current F (after the caller sets particles or parameters).
+__global__ __launch_bounds__(256) void k_stress_init(DevParams P, float* __restrict__ st, int64_t cap, int n,
+                                                     unsigned long long* __restrict__ fallback) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float F[9], K[6];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) F[c] = st[(PF + c) * cap + i];
+  int fb = 0;
+  kirchhoff_of_F(P, F, st[PPL * cap + i], st[PVOL * cap + i], K, fb);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) st[(PK + e) * cap + i] = K[e];
+  if (fb) atomicAdd(fallback, (unsigned long long)fb);
 }
 
 // ================================================================== radix sort (stable LSD, 8-bit digits)
@@ -84,20 +104,35 @@ __global__ __launch_bounds__(256) void k_reset(uint32_t* __restrict__ zero_words
   for (int64_t j = t0; j < nb; j += stride) block_slot[active_blocks[j]] = -1;
 }
 
+// Lanes of a wave that share a digit are grouped with 8 ballots; one lane per group adds the count.
+__device__ __forceinline__ uint64_t digit_peers(bool valid, uint32_t d) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit) {
+    bool set = (d >> bit) & 1u;
+    uint64_t bb = __ballot(valid && set);
+    peers &= set ? bb : ~bb;
+  }
+  return peers;
+}
+
 __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ hist,
                                                    int* __restrict__ counters) {
   __shared__ uint32_t h[3][256];
   for (int i = threadIdx.x; i < 768; i += 256) (&h[0][0])[i] = 0u;
   if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
   __syncthreads();
-  int base = blockIdx.x * SORT_TILE;
+  const int base = blockIdx.x * SORT_TILE;
+  const uint32_t lane = lane_id();
   for (int k = 0; k < SORT_IPT; ++k) {
-    int i = base + k * 256 + threadIdx.x;
-    if (i < n) {
-      uint32_t key = keys[i];
-      atomicAdd(&h[0][key & 255u], 1u);
-      atomicAdd(&h[1][(key >> 8) & 255u], 1u);
-      atomicAdd(&h[2][(key >> 16) & 255u], 1u);
+    const int i = base + k * 256 + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys[i] : 0u;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const uint32_t d = (key >> (8 * p)) & 255u;
+      const uint64_t peers = digit_peers(valid, d);
+      if (valid && (peers & ((1ull << lane) - 1ull)) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
     }
   }
   __syncthreads();
@@ -134,13 +169,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
     uint32_t k = valid ? kin[pos] : 0u;
     int v = valid ? (vin ? vin[pos] : pos) : 0;
     uint32_t d = (k >> shift) & 255u;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {
-      bool set = (d >> bit) & 1u;
-      uint64_t bb = __ballot(valid && set);
-      peers &= set ? bb : ~bb;
-    }
+    const uint64_t peers = digit_peers(valid, d);
     uint32_t prev = s_wh[w][d];
     if (valid && (peers >> lane) == 1ull) s_wh[w][d] = prev + (uint32_t)__popcll(peers);
     kr[i] = k;
@@ -198,50 +227,38 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
   }
 }
 
-// ================================================================== segments, bins, tiles
-// Thread per sorted position. Within each run of equal keys, the particles are re-ordered by their
-// caller id, so that the final order is that of std::stable_sort of the keys in caller order (the
-// order in which the reference visits particles within a cell). The run start also writes the
-// cell's bin bounds and claims the cell's tile and the 27 node blocks around it.
-__global__ __launch_bounds__(256) void k_segments(DevParams P, const uint32_t* __restrict__ ks, const int* __restrict__ vin,
-                                                  int* __restrict__ vout, int n, const float* __restrict__ st,
-                                                  int64_t cap, int* __restrict__ cell_start, int* __restrict__ cell_end,
-                                                  int* __restrict__ tile_slot, int* __restrict__ active_tiles,
-                                                  int* __restrict__ block_slot, int* __restrict__ active_blocks,
-                                                  int* __restrict__ counters) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+// ================================================================== bins and tiles
+// Thread per sorted position: the first position of each run of equal keys writes the cell's bin
+// start, the last its end; the run start also claims the cell's tile and the 27 node blocks around
+// it (slot numbers depend on claim order, the results do not: the grid sums tiles in coordinate order).
+__global__ __launch_bounds__(256) void k_bounds(DevParams P, const uint32_t* __restrict__ ks, int n,
+                                                int* __restrict__ cell_start, int* __restrict__ cell_end,
+                                                int* __restrict__ tile_slot, int* __restrict__ active_tiles,
+                                                int* __restrict__ block_slot, int* __restrict__ active_blocks,
+                                                int* __restrict__ counters) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const int* ids = reinterpret_cast<const int*>(st + (size_t)PID * cap);
-  uint32_t k = ks[j];
-  int s = j;
-  while (s > 0 && ks[s - 1] == k) --s;
-  int e = j + 1;
-  while (e < n && ks[e] == k) ++e;
-  int my = clamp_index(vin[j], n);
-  int myid = ids[my];
-  int rank = 0;
-  for (int a = s; a < e; ++a) rank += ids[clamp_index(vin[a], n)] < myid ? 1 : 0;
-  if (s + rank < e) vout[s + rank] = my;
-  if (j != s) return;
-  cell_start[k] = s;
-  cell_end[k] = e;
-  int r01 = P.res[0] * P.res[1];
-  int c2 = (int)(k / (uint32_t)r01), rem = (int)(k % (uint32_t)r01);
-  int c1 = rem / P.res[0], c0 = rem % P.res[0];
-  int t0 = c0 >> 2, t1 = c1 >> 2, t2 = c2 >> 2;
-  int tid = (t2 * P.tiles[1] + t1) * P.tiles[0] + t0;
+  const uint32_t k = ks[j];
+  if (j == n - 1 || ks[j + 1] != k) cell_end[k] = j + 1;
+  if (j != 0 && ks[j - 1] == k) return;
+  cell_start[k] = j;
+  const int r01 = P.res[0] * P.res[1];
+  const int c2 = (int)(k / (uint32_t)r01), rem = (int)(k % (uint32_t)r01);
+  const int c1 = rem / P.res[0], c0 = rem % P.res[0];
+  const int t0 = c0 >> 2, t1 = c1 >> 2, t2 = c2 >> 2;
+  const int tid = (t2 * P.tiles[1] + t1) * P.tiles[0] + t0;
   if (atomicCAS(&tile_slot[tid], -1, -2) == -1) {
-    int slot = atomicAdd(&counters[CNT_TILES], 1);
+    const int slot = atomicAdd(&counters[CNT_TILES], 1);
     active_tiles[slot] = tid;
     tile_slot[tid] = slot;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
-          int b0 = t0 + dx, b1 = t1 + dy, b2 = t2 + dz;
+          const int b0 = t0 + dx, b1 = t1 + dy, b2 = t2 + dz;
           if (b0 < 0 || b1 < 0 || b2 < 0 || b0 >= P.tiles[0] || b1 >= P.tiles[1] || b2 >= P.tiles[2]) continue;
-          int bid = (b2 * P.tiles[1] + b1) * P.tiles[0] + b0;
+          const int bid = (b2 * P.tiles[1] + b1) * P.tiles[0] + b0;
           if (atomicCAS(&block_slot[bid], -1, -2) == -1) {
-            int bs = atomicAdd(&counters[CNT_BLOCKS], 1);
+            const int bs = atomicAdd(&counters[CNT_BLOCKS], 1);
             active_blocks[bs] = bid;
             block_slot[bid] = bs;
           }
@@ -307,7 +324,7 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
         for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
         const int rank = 4 * ch + sub;
         if (rank < cnt) {
-          const int pi = clamp_index(a.perm[start + rank], a.n);
+          const int pi = clamp_index(a.pos[clamp_index(a.perm[start + rank], a.n)], a.n);
           const float* st = a.state;
           const int64_t cap = a.cap;
           float xp[3];
@@ -333,9 +350,18 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) pay[7 + i] = Q[i];
           }
-          if constexpr (MODE != P2G_MOM) {
-            // stress matrix V P(F^) F^n^T (HybridSolver.cpp:306-339), columns scaled by 1/h and by dt
-            // when the impulse is folded into the momentum.
+          if constexpr (MODE == P2G_MOM_STRESS) {
+            // stress at F^n: the Kirchhoff stress V P(F) F^T written by the previous G2P (or by
+            // k_stress_init), columns scaled by dt/h so that the impulse folds into the momentum
+            const int R[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};  // symmetric 3x3 from (xx yy zz xy xz yz)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) pay[16 + 3 * r + c] = a.dt * st[(PK + R[3 * r + c]) * cap + pi] * P.inv_h[c];
+          }
+          if constexpr (MODE == P2G_FORCE) {
+            // stress matrix V P(F^) F^n^T at the predicted F^ (HybridSolver.cpp:269-339), columns
+            // scaled by 1/h
             float Fn[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
@@ -363,12 +389,12 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
                     float gw[3] = {dx[ox] * wy[oy] * wz[oz] * P.inv_h[0], wx[ox] * dy[oy] * wz[oz] * P.inv_h[1],
                                    wx[ox] * wy[oy] * dz[oz] * P.inv_h[2]};
                     int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+                    const float4 gv4 = a.gvel[2 * node];
+                    const float vr[3] = {gv4.x, gv4.y, gv4.z};
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                      float vr = a.gv[r * a.ng + node];
+                    for (int r = 0; r < 3; ++r)
 #pragma unroll
-                      for (int c = 0; c < 3; ++c) G[3 * r + c] += vr * gw[c];
-                    }
+                      for (int c = 0; c < 3; ++c) G[3 * r + c] += vr[r] * gw[c];
                   }
                 }
               }
@@ -383,12 +409,10 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
             int fb = 0;
             stress_matrix(P, Fh, Fn, Jp, vol, S, fb);
             if (fb) atomicAdd(a.fallback_svd, (unsigned long long)fb);
-            const float scale = (MODE == P2G_MOM_STRESS) ? a.dt : 1.0f;
-            const int off = (MODE == P2G_FORCE) ? 3 : 16;
 #pragma unroll
             for (int r = 0; r < 3; ++r)
 #pragma unroll
-              for (int c = 0; c < 3; ++c) pay[off + 3 * r + c] = scale * S[3 * r + c] * P.inv_h[c];
+              for (int c = 0; c < 3; ++c) pay[3 + 3 * r + c] = S[3 * r + c] * P.inv_h[c];
           }
         }
 #pragma unroll
@@ -514,11 +538,11 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
     float v[3], vbf[3];
     if constexpr (MODE == GRID_FORCE_UPDATE) {
       const float m = a.gm[node];
+      const float4 g4 = a.gvel[2 * node];
+      v[0] = g4.x; v[1] = g4.y; v[2] = g4.z;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        v[d] = a.gv[d * a.ng + node];
+      for (int d = 0; d < 3; ++d)
         if (m > 0.0f) v[d] += a.dt * (sum[d] / m + P.gravity[d]);
-      }
     } else {
       const float m = sum[0];
       a.gm[node] = m;
@@ -530,32 +554,30 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
       }
     }
     if constexpr (MODE == GRID_NORMALIZE) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) a.gv[d * a.ng + node] = v[d];
+      vbf[0] = v[0]; vbf[1] = v[1]; vbf[2] = v[2];
     } else {
       const float xn[3] = {P.gmin[0] + (float)n0 * P.h[0], P.gmin[1] + (float)n1 * P.h[1], P.gmin[2] + (float)n2 * P.h[2]};
       collide(P, xn, v, vbf);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        a.gv[d * a.ng + node] = v[d];
-        a.gvbf[d * a.ng + node] = vbf[d];
-      }
     }
+    a.gvel[2 * node] = make_float4(v[0], v[1], v[2], vbf[0]);
+    a.gvel[2 * node + 1] = make_float4(vbf[1], vbf[2], 0.0f, 0.0f);
   }
 }
 
 // ================================================================== G2P + F update + return map
-// Thread per sorted position j: gathers the stencil of grid v (post-collision) and v_BF
-// (pre-friction), computes v_p, APIC B (HybridSolver.cpp:739-825), x += dt sum w v_BF (:942-950),
-// F^ = (I + dt sum v_BF grad w^T) F (:553-577) and the return map (:612-681), and writes the particle
-// to slot j of the other buffer, so that the buffers stay physically sorted by cell.
+// Thread per sorted position j (caller id pid = perm[j]): gathers the stencil of grid v
+// (post-collision) and v_BF (pre-friction), computes v_p, APIC B (HybridSolver.cpp:739-825),
+// x += dt sum w v_BF (:942-950), F^ = (I + dt sum v_BF grad w^T) F (:553-577) and the return map
+// (:612-681), and writes the particle to slot j of the other buffer, so that the buffers stay
+// physically sorted by cell. It also records where pid now lives and pid's next cell key.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   const DevParams& P = a.P;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
   if (j < a.n) {
-    const int pi = clamp_index(a.perm[j], a.n);
+    const int pid = clamp_index(a.perm[j], a.n);
+    const int pi = clamp_index(a.pos_in[pid], a.n);
     const float* st = a.sin;
     const int64_t cap = a.cap;
     float xp[3], g[3];
@@ -582,27 +604,32 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
       for (int oy = 0; oy < ORDER + 1; ++oy) {
         const int n1 = base[1] + oy;
         if (n1 < 0 || n1 >= P.res[1]) continue;
+        const float wyz = w[1][oy] * w[2][oz];
+        const float dyz = dw[1][oy] * w[2][oz] * P.inv_h[1];
+        const float ydz = w[1][oy] * dw[2][oz] * P.inv_h[2];
+        const float dy = ((float)n1 - g[1]) * P.h[1];
+        const float dz = ((float)n2 - g[2]) * P.h[2];
 #pragma unroll
         for (int ox = 0; ox < ORDER + 1; ++ox) {
           const int n0 = base[0] + ox;
           if (n0 < 0 || n0 >= P.res[0]) continue;
           const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
-          const float wyz = w[1][oy] * w[2][oz];
+          const float4 q0 = a.gvel[2 * node];
+          const float4 q1 = a.gvel[2 * node + 1];
+          const float vi[3] = {q0.x, q0.y, q0.z};
+          const float bi[3] = {q0.w, q1.x, q1.y};
           const float wt = w[0][ox] * wyz;
-          const float gw[3] = {dw[0][ox] * wyz * P.inv_h[0], w[0][ox] * dw[1][oy] * w[2][oz] * P.inv_h[1],
-                               w[0][ox] * w[1][oy] * dw[2][oz] * P.inv_h[2]};
-          const float dxn[3] = {((float)n0 - g[0]) * P.h[0], ((float)n1 - g[1]) * P.h[1], ((float)n2 - g[2]) * P.h[2]};
+          const float gw[3] = {dw[0][ox] * wyz * P.inv_h[0], w[0][ox] * dyz, w[0][ox] * ydz};
+          const float dxn[3] = {((float)n0 - g[0]) * P.h[0], dy, dz};
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            const float vi = a.gv[r * a.ng + node];
-            const float bi = a.gvbf[r * a.ng + node];
-            const float wv = wt * vi;
+            const float wv = wt * vi[r];
             vp[r] += wv;
-            vb[r] += wt * bi;
+            vb[r] += wt * bi[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               Bn[3 * r + c] += wv * dxn[c];
-              Gv[3 * r + c] += bi * gw[c];
+              Gv[3 * r + c] += bi[r] * gw[c];
             }
           }
         }
@@ -628,7 +655,9 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + a.dt * GF[i];
     float plastic = st[PPL * cap + pi];
     const float phi_p = st[PPHI * cap + pi];
-    return_map(P, Fh, plastic, phi_p, fb_svd, fb_apex);
+    const float vol = st[PVOL * cap + pi];
+    float K[6];
+    return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex);
     float xn[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -646,12 +675,17 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
       so[(PB + i) * cap + j] = Bn[i];
       so[(PF + i) * cap + j] = Fh[i];
     }
+    if (a.write_stress) {
+#pragma unroll
+      for (int e = 0; e < 6; ++e) so[(PK + e) * cap + j] = K[e];
+    }
     so[PPL * cap + j] = plastic;
     so[PM * cap + j] = st[PM * cap + pi];
-    so[PVOL * cap + j] = st[PVOL * cap + pi];
+    so[PVOL * cap + j] = vol;
     so[PPHI * cap + j] = phi_p;
-    reinterpret_cast<int*>(so + (size_t)PID * cap)[j] = reinterpret_cast<const int*>(st + (size_t)PID * cap)[pi];
-    a.keys[j] = cell_key(P, a.thr, xn[0], xn[1], xn[2], oob);
+    reinterpret_cast<int*>(so + (size_t)PID * cap)[j] = pid;
+    a.pos_out[pid] = j;
+    a.keys[pid] = cell_key(P, a.thr, xn[0], xn[1], xn[2], oob);
     bad = !(isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]) && isfinite(vp[0]) && isfinite(vp[1]) &&
             isfinite(vp[2]) && isfinite(Fh[0]) && isfinite(Fh[4]) && isfinite(Fh[8]) && isfinite(plastic));
   }
@@ -702,15 +736,28 @@ __global__ __launch_bounds__(256) void k_unpermute(const float* __restrict__ st,
   for (int p = 0; p < np; ++p) out[(size_t)p * n + id] = st[(p0 + p) * cap + i];
 }
 
-// Dense copy of grid planes with nodes outside active blocks set to zero.
-__global__ __launch_bounds__(256) void k_grid_export(DevParams P, const float* __restrict__ src, int64_t ng, int nplanes,
+// Dense planar copy of one grid quantity (0 = m, 1 = momentum, 2 = v, 3 = v_BF), with nodes
+// outside the active blocks set to zero.
+__global__ __launch_bounds__(256) void k_grid_export(DevParams P, const float* __restrict__ grid, int64_t ng, int which,
                                                      const int* __restrict__ block_slot, float* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ng) return;
   const int n0 = (int)(i % P.res[0]), n1 = (int)((i / P.res[0]) % P.res[1]), n2 = (int)(i / ((int64_t)P.res[0] * P.res[1]));
   const int b = ((n2 >> 2) * P.tiles[1] + (n1 >> 2)) * P.tiles[0] + (n0 >> 2);
   const bool on = block_slot[b] >= 0;
-  for (int p = 0; p < nplanes; ++p) out[p * ng + i] = on ? src[p * ng + i] : 0.0f;
+  if (which == 0) {
+    out[i] = on ? grid[GRID_M * ng + i] : 0.0f;
+  } else if (which == 1) {
+    for (int d = 0; d < 3; ++d) out[d * ng + i] = on ? grid[(GRID_MOM + d) * ng + i] : 0.0f;
+  } else {
+    const float* vel = grid + GRID_VEL * ng + 8 * i;
+    for (int d = 0; d < 3; ++d) out[d * ng + i] = on ? vel[which == 2 ? d : 3 + d] : 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_iota(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
 }
 
 // Number of nodes with m > 0 in the active blocks.
@@ -737,9 +784,13 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, int64_t n
 // ================================================================== launchers
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const float* thr, uint32_t* keys,
-                 int* status) {
-  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, thr, keys, status);
+void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const int* pos, const float* thr,
+                 uint32_t* keys, int* status) {
+  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, pos, thr, keys, status);
+}
+
+void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, unsigned long long* fallback) {
+  if (n > 0) hipLaunchKernelGGL(k_stress_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, fallback);
 }
 
 void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n) {
@@ -763,12 +814,11 @@ void launch_reset(hipStream_t s, const ResetArgs& r) {
                      r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot);
 }
 
-void launch_segments(hipStream_t s, const DevParams& P, const uint32_t* ks, const int* vin, int* vout, int n,
-                     const float* st, int64_t cap, int* cell_start, int* cell_end, int* tile_slot, int* active_tiles,
-                     int* block_slot, int* active_blocks, int* counters) {
+void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int* cell_start, int* cell_end,
+                   int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters) {
   if (n > 0)
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, vin, vout, n, st, cap, cell_start,
-                       cell_end, tile_slot, active_tiles, block_slot, active_blocks, counters);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, n, cell_start, cell_end, tile_slot,
+                       active_tiles, block_slot, active_blocks, counters);
 }
 
 void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid) {
@@ -807,9 +857,13 @@ void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0
   if (n > 0) hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, cap, n, p0, np, out);
 }
 
-void launch_grid_export(hipStream_t s, const DevParams& P, const float* src, int64_t ng, int nplanes, const int* block_slot,
+void launch_grid_export(hipStream_t s, const DevParams& P, const float* grid, int64_t ng, int which, const int* block_slot,
                         float* out) {
-  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(ng, 256)), dim3(256), 0, s, P, src, ng, nplanes, block_slot, out);
+  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(ng, 256)), dim3(256), 0, s, P, grid, ng, which, block_slot, out);
+}
+
+void launch_iota(hipStream_t s, int* p, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
 }
 
 void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,

@@ -57,13 +57,15 @@ struct mpm_ctx {
   float* state[2] = {nullptr, nullptr};
   int cur = 0;
   int sort_buf = 0;
-  uint32_t* keys = nullptr;
+  uint32_t* keys = nullptr;  // next cell key by caller id
+  int* pos[2] = {nullptr, nullptr};  // caller id -> physical index in state[b]
   uint32_t *k1 = nullptr, *k2 = nullptr;
   int *v1 = nullptr, *v2 = nullptr;
   uint32_t* sortws = nullptr;
   int64_t sortws_words = 0;
   int nb_sort = 0;
   bool keys_valid = false;
+  bool stress_valid = false;  // PK planes of state[cur] hold V tau of the current F
 
   // counters and statistics
   int* counters = nullptr;
@@ -157,6 +159,7 @@ static void build_devparams(mpm_ctx* ctx) {
   }
   d.nls = (int)ctx->ls.size();
   for (int k = 0; k < d.nls; ++k) d.ls[k] = ctx->ls[k];
+  ctx->stress_valid = false;
   drop_graphs(ctx);
 }
 
@@ -173,7 +176,7 @@ static float threshold(double gmin, double h, int c) {
 static int stage_sort(mpm_ctx* ctx) {
   hipStream_t s = ctx->stream;
   float* st = ctx->state[ctx->cur];
-  if (!ctx->keys_valid) launch_keys(s, ctx->dp, st, ctx->cap, ctx->n, ctx->thr, ctx->keys, ctx->status);
+  if (!ctx->keys_valid) launch_keys(s, ctx->dp, st, ctx->cap, ctx->n, ctx->pos[ctx->cur], ctx->thr, ctx->keys, ctx->status);
   ResetArgs r;
   r.zero_words = ctx->sortws;
   r.n_zero = ctx->sortws_words;
@@ -198,8 +201,8 @@ static int stage_sort(mpm_ctx* ctx) {
   b.status = ctx->sortws + 772;
   b.counters = ctx->counters;
   launch_sort(s, b, ctx->keys, ctx->n);
-  launch_segments(s, ctx->dp, ctx->k1, ctx->v1, ctx->v2, ctx->n, st, ctx->cap, ctx->cell_start, ctx->cell_end,
-                  ctx->tile_slot, ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters);
+  launch_bounds(s, ctx->dp, ctx->k1, ctx->n, ctx->cell_start, ctx->cell_end, ctx->tile_slot, ctx->active_tiles,
+                ctx->block_slot, ctx->active_blocks, ctx->counters);
   ctx->sort_buf = ctx->cur;
   return MPM_OK;
 }
@@ -210,14 +213,14 @@ static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   a.state = ctx->state[ctx->cur];
   a.cap = ctx->cap;
   a.n = ctx->n;
-  a.perm = ctx->v2;
+  a.perm = ctx->v1;
+  a.pos = ctx->pos[ctx->cur];
   a.cell_start = ctx->cell_start;
   a.cell_end = ctx->cell_end;
   a.counters = ctx->counters;
   a.active_tiles = ctx->active_tiles;
   a.partials = ctx->partials;
-  a.gv = ctx->grid + 4 * ctx->ng;
-  a.ng = ctx->ng;
+  a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
   a.fallback_svd = ctx->fallbacks;
   return a;
@@ -230,10 +233,9 @@ static GridArgs grid_args(mpm_ctx* ctx, float dt) {
   g.active_blocks = ctx->active_blocks;
   g.tile_slot = ctx->tile_slot;
   g.partials = ctx->partials;
-  g.gm = ctx->grid;
-  g.gmom = ctx->grid + ctx->ng;
-  g.gv = ctx->grid + 4 * ctx->ng;
-  g.gvbf = ctx->grid + 7 * ctx->ng;
+  g.gm = ctx->grid + GRID_M * ctx->ng;
+  g.gmom = ctx->grid + GRID_MOM * ctx->ng;
+  g.gvel = reinterpret_cast<float4*>(ctx->grid + GRID_VEL * ctx->ng);
   g.ng = ctx->ng;
   g.dt = dt;
   return g;
@@ -243,6 +245,10 @@ static int stage_p2g(mpm_ctx* ctx, float dt) {
   hipStream_t s = ctx->stream;
   P2GArgs a = p2g_args(ctx, dt);
   if (!ctx->dp.predicted) {
+    if (!ctx->stress_valid) {
+      launch_stress_init(s, ctx->dp, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
+      ctx->stress_valid = true;
+    }
     launch_p2g(s, a, P2G_MOM_STRESS, ctx->p2g_grid);
   } else {
     launch_p2g(s, a, P2G_MOM, ctx->p2g_grid);
@@ -264,11 +270,12 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.sout = ctx->state[1 - ctx->cur];
   a.cap = ctx->cap;
   a.n = ctx->n;
-  a.perm = ctx->v2;
-  a.gv = ctx->grid + 4 * ctx->ng;
-  a.gvbf = ctx->grid + 7 * ctx->ng;
-  a.ng = ctx->ng;
+  a.perm = ctx->v1;
+  a.pos_in = ctx->pos[ctx->cur];
+  a.pos_out = ctx->pos[1 - ctx->cur];
+  a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
+  a.write_stress = ctx->dp.predicted ? 0 : 1;
   a.thr = ctx->thr;
   a.keys = ctx->keys;
   a.status = ctx->status;
@@ -277,6 +284,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   launch_g2p(ctx->stream, a);
   ctx->cur = 1 - ctx->cur;
   ctx->keys_valid = true;
+  ctx->stress_valid = !ctx->dp.predicted;
   ctx->substeps += 1;
   return MPM_OK;
 }
@@ -386,7 +394,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_end);
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
-  dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys);
+  dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
   dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->sortws);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -425,7 +433,7 @@ int mpm_set_grid(mpm_ctx* ctx, const double min_bound[3], const double max_bound
   }
   ctx->ng = ng;
   ctx->ntiles = (int64_t)ctx->tiles[0] * ctx->tiles[1] * ctx->tiles[2];
-  int rc = dalloc(ctx, ctx->grid, (size_t)10 * ng);
+  int rc = dalloc(ctx, ctx->grid, (size_t)GRID_FLOATS * ng);
   if (!rc) rc = dalloc(ctx, ctx->cell_start, ng);
   if (!rc) rc = dalloc(ctx, ctx->cell_end, ng);
   if (!rc) rc = dalloc(ctx, ctx->tile_slot, ctx->ntiles);
@@ -434,7 +442,7 @@ int mpm_set_grid(mpm_ctx* ctx, const double min_bound[3], const double max_bound
   if (!rc) rc = dalloc(ctx, ctx->active_blocks, ctx->ntiles);
   if (!rc) rc = dalloc(ctx, ctx->thr, (size_t)(res[0] + res[1] + res[2] + 3));
   if (rc) return rc;
-  CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * 10 * ng, ctx->stream));
+  CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * GRID_FLOATS * ng, ctx->stream));
   CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * ng, ctx->stream));
   CK(hipMemsetAsync(ctx->cell_end, 0, sizeof(int) * ng, ctx->stream));
   launch_fill_int(ctx->stream, ctx->tile_slot, ctx->ntiles, -1);
@@ -489,6 +497,8 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
     int rc = dalloc(ctx, ctx->state[0], (size_t)NPLANES * n);
     if (!rc) rc = dalloc(ctx, ctx->state[1], (size_t)NPLANES * n);
     if (!rc) rc = dalloc(ctx, ctx->keys, n);
+    if (!rc) rc = dalloc(ctx, ctx->pos[0], n);
+    if (!rc) rc = dalloc(ctx, ctx->pos[1], n);
     if (!rc) rc = dalloc(ctx, ctx->k1, n);
     if (!rc) rc = dalloc(ctx, ctx->k2, n);
     if (!rc) rc = dalloc(ctx, ctx->v1, n);
@@ -502,6 +512,7 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
     CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->v1, 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->v2, 0, sizeof(int) * n, ctx->stream));
+    CK(hipMemsetAsync(ctx->pos[1], 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->sortws, 0, sizeof(uint32_t) * ctx->sortws_words, ctx->stream));
     ctx->n = n;
     ctx->cap = n;
@@ -528,7 +539,9 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
   }
   CK(hipMemcpyAsync(ctx->state[0], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * n, ctx->stream));
+  launch_iota(ctx->stream, ctx->pos[0], n);
   CK(hipStreamSynchronize(ctx->stream));
+  ctx->stress_valid = false;
   ctx->cur = 0;
   ctx->sort_buf = 0;
   ctx->keys_valid = false;
@@ -571,6 +584,7 @@ int mpm_estimate_volumes(mpm_ctx* ctx) {
   launch_p2g(ctx->stream, a, P2G_MOM, ctx->p2g_grid);
   launch_grid(ctx->stream, grid_args(ctx, 0.0f), GRID_NORMALIZE, ctx->grid_grid);
   launch_volumes(ctx->stream, ctx->dp, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->grid);
+  ctx->stress_valid = false;
   CK(hipGetLastError());
   CK(hipStreamSynchronize(ctx->stream));
   return MPM_OK;
@@ -602,7 +616,7 @@ int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps) {
   if (nsteps < 0) return fail(ctx, MPM_ERR_INVALID, "nsteps < 0");
   if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
   CK(hipSetDevice(ctx->device));
-  if (nsteps > 0 && !ctx->keys_valid) {
+  if (nsteps > 0 && (!ctx->keys_valid || (!ctx->dp.predicted && !ctx->stress_valid))) {
     rc = one_substep(ctx, dt);
     if (rc) return rc;
     --nsteps;
@@ -650,12 +664,7 @@ int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins) {
   const int n = ctx->n;
   CK(hipStreamSynchronize(ctx->stream));
   if (keys) CK(hipMemcpy(keys, ctx->k1, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-  if (ids) {
-    std::vector<int> perm(n), idp(n);
-    CK(hipMemcpy(perm.data(), ctx->v2, sizeof(int) * n, hipMemcpyDeviceToHost));
-    CK(hipMemcpy(idp.data(), ctx->state[ctx->sort_buf] + (size_t)PID * ctx->cap, sizeof(int) * n, hipMemcpyDeviceToHost));
-    for (int j = 0; j < n; ++j) ids[j] = idp[perm[j]];
-  }
+  if (ids) CK(hipMemcpy(ids, ctx->v1, sizeof(int) * n, hipMemcpyDeviceToHost));
   if (bins) {
     const int64_t ng = ctx->ng;
     std::vector<int> cs(ng), ce(ng);
@@ -675,12 +684,12 @@ int mpm_get_grid(mpm_ctx* ctx, float* m, float* mom, float* v, float* vbf) {
   float* tmp = nullptr;
   int rc = dalloc(ctx, tmp, (size_t)3 * ng);
   if (rc) return rc;
-  struct Req { float* dst; int p0; int np; };
-  const Req reqs[4] = {{m, 0, 1}, {mom, 1, 3}, {v, 4, 3}, {vbf, 7, 3}};
+  struct Req { float* dst; int which; int np; };
+  const Req reqs[4] = {{m, 0, 1}, {mom, 1, 3}, {v, 2, 3}, {vbf, 3, 3}};
   hipError_t e = hipSuccess;
   for (const Req& r : reqs) {
     if (!r.dst) continue;
-    launch_grid_export(ctx->stream, ctx->dp, ctx->grid + r.p0 * ng, ng, r.np, ctx->block_slot, tmp);
+    launch_grid_export(ctx->stream, ctx->dp, ctx->grid, ng, r.which, ctx->block_slot, tmp);
     e = hipMemcpyAsync(r.dst, tmp, sizeof(float) * r.np * ng, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) break;
   }

@@ -1,0 +1,149 @@
+"""Benchmark: particle-substeps/s of the explicit MPM substep on one MI355X.
+
+Workload: Drucker-Prager sand column collapse, 1,048,576 particles on a 128^3 grid, dt = 5e-5,
+quadratic B-spline APIC, stress at F^n, box walls + sphere with Coulomb friction, exact return
+mapping. Inputs are synthetic (seeded jittered lattice), resident in HBM before timing starts.
+
+Prints one JSON line. `roofline` is for the dominant kernel (its own algorithmic bytes over its
+HIP-event-timed duration); `substep_roofline` is the whole substep against 208 B/particle +
+56 B/active node. `cpu_baseline` is the CPU oracle (a single-threaded FP64 restatement of the
+reference substep) timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-substeps/sec, surrogate vs exact return map at ≤1e-3 rel pos err"
+
+# Algorithmic bytes of each stage (see the kernel descriptions in mpm_kernels.hip):
+#   P2G  reads x, v, B, F, m, V (26 floats) per particle, writes m and momentum (16 B) per active node
+#   GRID reads m, momentum (16 B), writes v and v_BF (24 B) per active node
+#   G2P  reads x, F, plastic (13 floats), writes x, v, B, F, plastic (25 floats) per particle,
+#        reads v and v_BF (24 B) per active node
+#   SORT reads the 4-byte key and writes key + index (8 B) per particle and digit pass (3 passes)
+STAGES = ["sort", "p2g", "grid", "g2p"]
+ALG_PER_PARTICLE = {"sort": 3 * 12, "p2g": 104, "grid": 0, "g2p": 152}
+ALG_PER_NODE = {"sort": 0, "p2g": 16, "grid": 40, "g2p": 24}
+SUBSTEP_PER_PARTICLE, SUBSTEP_PER_NODE = 208, 56
+KERNEL_NAMES = {"p2g": "k_p2g<2, P2G_MOM_STRESS>", "g2p": "k_g2p<2>", "grid": "k_grid<GRID_FUSED>",
+                "sort": "k_sort_pass (x3) + hist + reset + segments"}
+
+
+def load_traffic(kernel):
+    """HBM traffic per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc_config2.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(sample_steps=3):
+    """Time the FP64 oracle on this host: config 2 at full size, 1 warm-up + `sample_steps` substeps."""
+    from anisotropicelastoplasticity_amd import scenes
+    from oracle.bind import Oracle
+    sc = scenes.config2_sand_column()
+    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
+    o.load_scene(sc)
+    o.substep(sc.dt)
+    t0 = time.perf_counter()
+    o.run(sc.dt, sample_steps)
+    t = time.perf_counter() - t0
+    return dict(value=sc.n * sample_steps / t, unit="particle-substeps/s", cores=1, kind="port",
+                sample=f"config2 full size ({sc.n} particles, 128^3), 1 warm-up + {sample_steps} timed substeps, "
+                       "FP64 oracle (C++ restatement of the reference substep, -O3, one thread)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--stage-steps", type=int, default=50, help="substeps timed stage by stage with HIP events")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    from anisotropicelastoplasticity_amd import Solver, scenes
+    sc = scenes.config2_sand_column()
+    n = sc.n
+    with Solver(0) as s:
+        sc.load_into(s)
+        s.run(sc.dt, args.warmup)
+        s.synchronize()
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+        except Exception:
+            pass
+        t0 = time.perf_counter()
+        s.run(sc.dt, args.steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        # per-stage HIP-event timing on the context stream (eager launches)
+        total_ms, stage_ms = s.time_substeps(sc.dt, args.stage_steps)
+        st = s.stats()
+
+    value = n * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+    n_nodes = st["n_active_nodes"]
+    per_stage_ms = {k: stage_ms[i] / args.stage_steps for i, k in enumerate(STAGES)}
+    dom = max(("p2g", "grid", "g2p"), key=lambda k: per_stage_ms[k])
+    alg = ALG_PER_PARTICLE[dom] * n + ALG_PER_NODE[dom] * n_nodes
+    achieved = alg / (per_stage_ms[dom] * 1e-3) / 1e9
+    traffic = load_traffic(dom)
+    b_alg = SUBSTEP_PER_PARTICLE * n + SUBSTEP_PER_NODE * n_nodes
+    sub_ach = b_alg / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice)",
+        "config": {"workload": "config2_sand_column", "particles": n, "grid": list(sc.res), "dt": sc.dt,
+                   "kernel": "quadratic B-spline APIC", "return_map": "exact (Drucker-Prager, FP32 + FP64 fallback)",
+                   "stress": "at F^n", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": per_stage_ms[dom]},
+        "substep_roofline": {"achieved": sub_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": sub_ach / HBM_PEAK_GBS, "alg_bytes_per_substep": b_alg},
+        "stage_ms": per_stage_ms,
+        "active_nodes": n_nodes,
+        "status": st["status"],
+        "fp64_svd_count": st["fp64_svd_count"],
+        "fp64_apex_count": st["fp64_apex_count"],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
