@@ -86,6 +86,7 @@ SIGNATURES = [
     ("mpm_clear_status", ctypes.c_int, [_ctx]),
     ("mpm_synchronize", ctypes.c_int, [_ctx]),
     ("mpm_time_substeps", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _f, _f]),
+    ("mpm_bench_copy", ctypes.c_int, [_ctx, ctypes.c_int64, ctypes.c_int32, _f]),
     ("mpm_generate_block", ctypes.c_int,
      [ctypes.c_uint64, _d, _d, _i32, ctypes.c_int32, ctypes.c_int64, _f]),
 ]

@@ -151,6 +151,11 @@ class Solver:
     def clear_status(self):
         self._check(self.lib.mpm_clear_status(self.h), "mpm_clear_status")
 
+    def bench_copy(self, nbytes, reps):
+        ms = ctypes.c_float()
+        self._check(self.lib.mpm_bench_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)), "mpm_bench_copy")
+        return ms.value
+
     def time_substeps(self, dt, nsteps):
         total = ctypes.c_float()
         stages = (ctypes.c_float * _lib.NUM_STAGES)()

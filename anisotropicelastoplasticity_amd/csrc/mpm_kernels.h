@@ -27,7 +27,9 @@ enum Plane : int {
 constexpr int GRID_M = 0, GRID_MOM = 1, GRID_VEL = 4, GRID_FLOATS = 12;
 
 constexpr int SORT_IPT = 16;
-constexpr int SORT_TILE = 256 * SORT_IPT;  // keys per sort block
+constexpr int SORT_TILE = 256 * SORT_IPT;  // keys per sort pass block
+constexpr int HIST_IPT = 16;
+constexpr int HIST_TILE = 256 * HIST_IPT;  // keys per histogram block
 constexpr int TILE = 4;                    // cells per tile edge
 constexpr int FOOT = TILE + 3;             // footprint nodes per edge (cell window c-1 .. c+2)
 constexpr int FOOT3 = FOOT * FOOT * FOOT;
@@ -115,8 +117,8 @@ void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap
 void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
 void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n);
-void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int* cell_start, int* cell_end,
-                   int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters);
+void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
+                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters);
 void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
 void launch_g2p(hipStream_t s, const G2PArgs& a);
@@ -128,5 +130,6 @@ void launch_iota(hipStream_t s, int* p, int n);
 void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
                         unsigned long long* out, int grid);
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v);
+void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4);
 
 }  // namespace mpm

@@ -122,17 +122,23 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ 
   for (int i = threadIdx.x; i < 768; i += 256) (&h[0][0])[i] = 0u;
   if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
   __syncthreads();
-  const int base = blockIdx.x * SORT_TILE;
-  const uint32_t lane = lane_id();
-  for (int k = 0; k < SORT_IPT; ++k) {
+  const int base = blockIdx.x * HIST_TILE;
+  for (int k = 0; k < HIST_IPT; ++k) {
     const int i = base + k * 256 + threadIdx.x;
     const bool valid = i < n;
     const uint32_t key = valid ? keys[i] : 0u;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0ull) continue;
+    const int first = __ffsll((long long)vmask) - 1;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
       const uint32_t d = (key >> (8 * p)) & 255u;
-      const uint64_t peers = digit_peers(valid, d);
-      if (valid && (peers & ((1ull << lane) - 1ull)) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
+      const uint32_t d0 = __shfl(d, first);
+      if (__ballot(valid && d == d0) == vmask) {  // one digit across the wave: one add
+        if ((int)lane_id() == first) atomicAdd(&h[p][d0], (uint32_t)__popcll(vmask));
+      } else if (valid) {
+        atomicAdd(&h[p][d], 1u);
+      }
     }
   }
   __syncthreads();
@@ -146,8 +152,11 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ kout, int* __restrict__ vout, int n, int pass,
                                                    const uint32_t* __restrict__ hist, uint32_t* __restrict__ tickets,
                                                    uint32_t* __restrict__ status) {
+  __shared__ uint32_t s_k[SORT_TILE];
+  __shared__ int s_v[SORT_TILE];
   __shared__ uint32_t s_wh[4][256];
   __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_loc[256];
   __shared__ uint32_t s_scan[256];
   __shared__ int s_tile;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -162,6 +171,8 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
   int vr[SORT_IPT];
   uint32_t rr[SORT_IPT];
   const uint64_t lt = (1ull << lane) - 1ull;
+  // stable rank of every key among the equal digits before it in the block (waves in order,
+  // 64-key chunks in order, lanes in order)
 #pragma unroll
   for (int i = 0; i < SORT_IPT; ++i) {
     int pos = base + i * 64 + lane;
@@ -184,7 +195,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
     s_wh[1][d] = t0;
     s_wh[2][d] = t0 + t1;
     s_wh[3][d] = t0 + t1 + t2;
-    uint32_t total = t0 + t1 + t2 + t3;
+    const uint32_t total = t0 + t1 + t2 + t3;
     uint32_t* my = status + (size_t)b * 256 + d;
     uint32_t excl = 0u;
     if (b == 0) {
@@ -192,36 +203,58 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
     } else {
       __hip_atomic_store(my, FLAG_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (int bb = b - 1; bb >= 0; --bb) {
-        uint32_t s;
+        uint32_t sv;
         do {
-          s = __hip_atomic_load(status + (size_t)bb * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } while ((s & ~VALUE_MASK) == 0u);
-        excl += s & VALUE_MASK;
-        if ((s & ~VALUE_MASK) == FLAG_PREFIX) break;
+          sv = __hip_atomic_load(status + (size_t)bb * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } while ((sv & ~VALUE_MASK) == 0u);
+        excl += sv & VALUE_MASK;
+        if ((sv & ~VALUE_MASK) == FLAG_PREFIX) break;
       }
       __hip_atomic_store(my, FLAG_PREFIX | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // exclusive scan of the global digit histogram of this pass
-    s_scan[d] = hist[pass * 256 + d];
+    // exclusive scans over the digits: global histogram of this pass, and this block's totals
+    const uint32_t gh = hist[pass * 256 + d];
+    s_scan[d] = gh;
+    s_loc[d] = total;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-      uint32_t add = d >= off ? s_scan[d - off] : 0u;
+      uint32_t a1 = d >= off ? s_scan[d - off] : 0u;
+      uint32_t a2 = d >= off ? s_loc[d - off] : 0u;
       __syncthreads();
-      s_scan[d] += add;
+      s_scan[d] += a1;
+      s_loc[d] += a2;
       __syncthreads();
     }
-    s_off[d] = (s_scan[d] - hist[pass * 256 + d]) + excl;
+    const uint32_t loc = s_loc[d] - total;
+    __syncthreads();
+    s_loc[d] = loc;
+    s_off[d] = (s_scan[d] - gh) + excl - loc;  // global destination = s_off[d] + block-local position
   }
   __syncthreads();
+  // place the block's keys in LDS in output order, then write contiguous runs
 #pragma unroll
   for (int i = 0; i < SORT_IPT; ++i) {
     int pos = base + i * 64 + lane;
     if (pos < n) {
       uint32_t d = (kr[i] >> shift) & 255u;
-      uint32_t dst = s_off[d] + s_wh[w][d] + rr[i];
+      uint32_t lp = s_loc[d] + s_wh[w][d] + rr[i];
+      if (lp < (uint32_t)SORT_TILE) {
+        s_k[lp] = kr[i];
+        s_v[lp] = vr[i];
+      }
+    }
+  }
+  __syncthreads();
+  const int cnt = min(SORT_TILE, n - b * SORT_TILE);
+#pragma unroll
+  for (int i = 0; i < SORT_IPT; ++i) {
+    const int t = i * 256 + tid;
+    if (t < cnt) {
+      const uint32_t k = s_k[t];
+      const uint32_t dst = s_off[(k >> shift) & 255u] + (uint32_t)t;
       if (dst < (uint32_t)n) {  // always true for a consistent histogram; keeps a bad one in bounds
-        kout[dst] = kr[i];
-        vout[dst] = vr[i];
+        kout[dst] = k;
+        vout[dst] = s_v[t];
       }
     }
   }
@@ -229,13 +262,10 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 
 // ================================================================== bins and tiles
 // Thread per sorted position: the first position of each run of equal keys writes the cell's bin
-// start, the last its end; the run start also claims the cell's tile and the 27 node blocks around
-// it (slot numbers depend on claim order, the results do not: the grid sums tiles in coordinate order).
+// start and flags the cell's tile (plain stores of one value), the last position writes the end.
 __global__ __launch_bounds__(256) void k_bounds(DevParams P, const uint32_t* __restrict__ ks, int n,
                                                 int* __restrict__ cell_start, int* __restrict__ cell_end,
-                                                int* __restrict__ tile_slot, int* __restrict__ active_tiles,
-                                                int* __restrict__ block_slot, int* __restrict__ active_blocks,
-                                                int* __restrict__ counters) {
+                                                int* __restrict__ tile_slot) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t k = ks[j];
@@ -245,23 +275,37 @@ __global__ __launch_bounds__(256) void k_bounds(DevParams P, const uint32_t* __r
   const int r01 = P.res[0] * P.res[1];
   const int c2 = (int)(k / (uint32_t)r01), rem = (int)(k % (uint32_t)r01);
   const int c1 = rem / P.res[0], c0 = rem % P.res[0];
-  const int t0 = c0 >> 2, t1 = c1 >> 2, t2 = c2 >> 2;
-  const int tid = (t2 * P.tiles[1] + t1) * P.tiles[0] + t0;
-  if (atomicCAS(&tile_slot[tid], -1, -2) == -1) {
-    const int slot = atomicAdd(&counters[CNT_TILES], 1);
-    active_tiles[slot] = tid;
-    tile_slot[tid] = slot;
+  tile_slot[((c2 >> 2) * P.tiles[1] + (c1 >> 2)) * P.tiles[0] + (c0 >> 2)] = -2;
+}
+
+// Thread per tile (MODE 0) or node block (MODE 1): a flagged entry (-2) receives the next slot of
+// the active list (one atomic per wave). A claimed tile flags the 27 node blocks around it. Slot
+// numbers depend on the order in which waves run; no result depends on them.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_claim(DevParams P, int64_t count, int* __restrict__ slot_of,
+                                               int* __restrict__ active, int* __restrict__ counter,
+                                               int* __restrict__ block_slot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool flagged = t < count && slot_of[t] == -2;
+  const uint64_t mask = __ballot(flagged);
+  if (mask == 0ull) return;
+  const int lane = (int)lane_id();
+  const int first = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == first) base = atomicAdd(counter, __popcll(mask));
+  base = __shfl(base, first);
+  if (!flagged) return;
+  const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+  slot_of[t] = slot;
+  active[slot] = (int)t;
+  if (MODE == 0) {
+    const int t0 = (int)(t % P.tiles[0]), t1 = (int)((t / P.tiles[0]) % P.tiles[1]), t2 = (int)(t / ((int64_t)P.tiles[0] * P.tiles[1]));
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
           const int b0 = t0 + dx, b1 = t1 + dy, b2 = t2 + dz;
           if (b0 < 0 || b1 < 0 || b2 < 0 || b0 >= P.tiles[0] || b1 >= P.tiles[1] || b2 >= P.tiles[2]) continue;
-          const int bid = (b2 * P.tiles[1] + b1) * P.tiles[0] + b0;
-          if (atomicCAS(&block_slot[bid], -1, -2) == -1) {
-            const int bs = atomicAdd(&counters[CNT_BLOCKS], 1);
-            active_blocks[bs] = bid;
-            block_slot[bid] = bs;
-          }
+          block_slot[(b2 * P.tiles[1] + b1) * P.tiles[0] + b0] = -2;
         }
   }
 }
@@ -776,6 +820,12 @@ __global__ __launch_bounds__(64) void k_count_nodes(DevParams P, const float* __
   if (c) atomicAdd(out, c);
 }
 
+// Plain copy (measurement reference for the PMC byte counters).
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n4) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, int64_t n, int v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -795,7 +845,7 @@ void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t ca
 
 void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n) {
   const int nb = blocks_for(n, SORT_TILE);
-  hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(256), 0, s, keys_in, n, b.hist, b.counters);
+  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, b.hist, b.counters);
   hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, keys_in, (const int*)nullptr, b.k1, b.v1, n, 0, b.hist,
                      b.tickets, b.status);
   hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k1, (const int*)b.v1, b.k2, b.v2, n, 1,
@@ -814,11 +864,14 @@ void launch_reset(hipStream_t s, const ResetArgs& r) {
                      r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot);
 }
 
-void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int* cell_start, int* cell_end,
-                   int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, n, cell_start, cell_end, tile_slot,
-                       active_tiles, block_slot, active_blocks, counters);
+void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
+                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, n, cell_start, cell_end, tile_slot);
+  hipLaunchKernelGGL(k_claim<0>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, P, ntiles, tile_slot, active_tiles,
+                     counters + CNT_TILES, block_slot);
+  hipLaunchKernelGGL(k_claim<1>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, P, ntiles, block_slot, active_blocks,
+                     counters + CNT_BLOCKS, (int*)nullptr);
 }
 
 void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid) {
@@ -869,6 +922,12 @@ void launch_iota(hipStream_t s, int* p, int n) {
 void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
                         unsigned long long* out, int grid) {
   hipLaunchKernelGGL(k_count_nodes, dim3(grid), dim3(64), 0, s, P, gm, counters, active_blocks, out);
+}
+
+void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4) {
+  int grid = blocks_for(n4, 256);
+  if (grid > 8192) grid = 8192;
+  if (n4 > 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, src, dst, n4);
 }
 
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v) {

@@ -201,8 +201,8 @@ static int stage_sort(mpm_ctx* ctx) {
   b.status = ctx->sortws + 772;
   b.counters = ctx->counters;
   launch_sort(s, b, ctx->keys, ctx->n);
-  launch_bounds(s, ctx->dp, ctx->k1, ctx->n, ctx->cell_start, ctx->cell_end, ctx->tile_slot, ctx->active_tiles,
-                ctx->block_slot, ctx->active_blocks, ctx->counters);
+  launch_bounds(s, ctx->dp, ctx->k1, ctx->n, ctx->ntiles, ctx->cell_start, ctx->cell_end, ctx->tile_slot,
+                ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters);
   ctx->sort_buf = ctx->cur;
   return MPM_OK;
 }
@@ -777,6 +777,31 @@ int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, f
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   CK(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_bench_copy(mpm_ctx* ctx, int64_t nbytes, int32_t reps, float* ms) {
+  if (!ctx || nbytes <= 0 || nbytes % 16 != 0 || reps < 1) return ctx ? fail(ctx, MPM_ERR_INVALID, "bad copy size") : MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  float4 *a = nullptr, *b = nullptr;
+  int rc = dalloc(ctx, a, (size_t)nbytes / 16);
+  if (!rc) rc = dalloc(ctx, b, (size_t)nbytes / 16);
+  if (rc) { dfree(a); dfree(b); return rc; }
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  CK(hipMemsetAsync(a, 0, (size_t)nbytes, ctx->stream));
+  CK(hipEventRecord(e0, ctx->stream));
+  for (int i = 0; i < reps; ++i) launch_copy(ctx->stream, a, b, nbytes / 16);
+  CK(hipEventRecord(e1, ctx->stream));
+  CK(hipEventSynchronize(e1));
+  float t = 0.0f;
+  CK(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(a);
+  dfree(b);
   return MPM_OK;
 }
 

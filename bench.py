@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--stage-steps", type=int, default=50, help="substeps timed stage by stage with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -101,6 +103,11 @@ def main():
         # per-stage HIP-event timing on the context stream (eager launches)
         total_ms, stage_ms = s.time_substeps(sc.dt, args.stage_steps)
         st = s.stats()
+        copy = None
+        if args.copy_calibration:
+            nbytes, reps = 256 << 20, 3
+            copy = {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "launches": reps,
+                    "kernel": "k_copy", "avg_ms": s.bench_copy(nbytes, reps)}
 
     value = n * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
@@ -140,6 +147,8 @@ def main():
         "fp64_svd_count": st["fp64_svd_count"],
         "fp64_apex_count": st["fp64_apex_count"],
     }
+    if copy is not None:
+        out["copy_calibration"] = copy
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

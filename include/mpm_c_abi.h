@@ -205,6 +205,11 @@ int mpm_synchronize(mpm_ctx* ctx);
  * whole-span time. */
 int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms);
 
+/* Measurement helper with no counterpart in the reference: copies nbytes (a multiple of 16) between
+ * two device buffers `reps` times with a plain float4 copy kernel (k_copy). Under rocprofv3 --pmc its
+ * FETCH_SIZE and WRITE_SIZE readings calibrate the counters against a known byte count. */
+int mpm_bench_copy(mpm_ctx* ctx, int64_t nbytes, int32_t reps, float* ms);
+
 /* Seeded jittered particle generator (splitmix64 counter hash). The box [lo,hi) is cut into a
  * lattice of cells[0] x cells[1] x cells[2] cells; each cell holds ppa^3 particles on a stratified
  * sub-lattice, each jittered uniformly within its sub-cell. Particle id

@@ -283,3 +283,15 @@ def test_fp64_svd_fallback():
     assert np.all(np.isfinite(q["F"]))
     qr = o.get_particles()
     assert rel(q["v"], qr["v"]) < 1e-4
+
+
+def test_cpp_host_api_example():
+    # the reference's calling pattern (ParticleSystem + RegularGrid + HybridSolver::solve) in C++
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "anisotropicelastoplasticity_amd",
+                       "sand_column_example")
+    say("[c++ host api] sand_column_example")
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    say(res.stdout.strip())
+    assert res.returncode == 0, res.stdout + res.stderr
