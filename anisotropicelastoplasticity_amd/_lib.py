@@ -55,6 +55,9 @@ class Stats(ctypes.Structure):
         ("n_active_nodes", ctypes.c_int64),
         ("fp64_svd_count", ctypes.c_int64),
         ("fp64_apex_count", ctypes.c_int64),
+        ("surrogate_candidates", ctypes.c_int64),
+        ("surrogate_accepted", ctypes.c_int64),
+        ("pairs_logged", ctypes.c_int64),
     ]
 
 
@@ -86,6 +89,17 @@ SIGNATURES = [
     ("mpm_clear_status", ctypes.c_int, [_ctx]),
     ("mpm_synchronize", ctypes.c_int, [_ctx]),
     ("mpm_time_substeps", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _f, _f]),
+    ("mpm_surrogate_nparam", ctypes.c_int32, []),
+    ("mpm_surrogate_init", ctypes.c_int, [_ctx, ctypes.c_uint64]),
+    ("mpm_surrogate_set_weights", ctypes.c_int, [_ctx, _f, ctypes.c_int32]),
+    ("mpm_surrogate_get_weights", ctypes.c_int, [_ctx, _f, ctypes.c_int32]),
+    ("mpm_surrogate_predict", ctypes.c_int, [_ctx, _f, _f, ctypes.c_int32]),
+    ("mpm_surrogate_gradient", ctypes.c_int, [_ctx, _f, _f, ctypes.c_int32, _f, _f]),
+    ("mpm_surrogate_train", ctypes.c_int,
+     [_ctx, _f, _f, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _f]),
+    ("mpm_set_pair_logging", ctypes.c_int, [_ctx, ctypes.c_int64]),
+    ("mpm_get_pairs", ctypes.c_int, [_ctx, _f, ctypes.c_int64, _P(ctypes.c_int64)]),
+    ("mpm_set_surrogate", ctypes.c_int, [_ctx, ctypes.c_int32, ctypes.c_float]),
     ("mpm_bench_copy", ctypes.c_int, [_ctx, ctypes.c_int64, ctypes.c_int32, _f]),
     ("mpm_generate_block", ctypes.c_int,
      [ctypes.c_uint64, _d, _d, _i32, ctypes.c_int32, ctypes.c_int64, _f]),

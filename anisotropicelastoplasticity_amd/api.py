@@ -151,6 +151,60 @@ class Solver:
     def clear_status(self):
         self._check(self.lib.mpm_clear_status(self.h), "mpm_clear_status")
 
+    # ---------------------------------------------------------------- learned return-map surrogate
+    def surrogate_nparam(self):
+        return int(self.lib.mpm_surrogate_nparam())
+
+    def surrogate_init(self, seed):
+        self._check(self.lib.mpm_surrogate_init(self.h, ctypes.c_uint64(int(seed))), "mpm_surrogate_init")
+
+    def surrogate_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.mpm_surrogate_set_weights(self.h, _fp(w), int(w.size)), "mpm_surrogate_set_weights")
+
+    def surrogate_get_weights(self):
+        w = np.empty(self.surrogate_nparam(), np.float32)
+        self._check(self.lib.mpm_surrogate_get_weights(self.h, _fp(w), int(w.size)), "mpm_surrogate_get_weights")
+        return w
+
+    def surrogate_predict(self, inputs):
+        inp = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 5)
+        H = np.empty((inp.shape[0], 3), np.float32)
+        self._check(self.lib.mpm_surrogate_predict(self.h, _fp(inp), _fp(H), inp.shape[0]), "mpm_surrogate_predict")
+        return H
+
+    def surrogate_gradient(self, inputs, H):
+        inp = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 5)
+        Ht = np.ascontiguousarray(H, dtype=np.float32).reshape(-1, 3)
+        g = np.empty(self.surrogate_nparam(), np.float32)
+        loss = ctypes.c_float()
+        rc = self.lib.mpm_surrogate_gradient(self.h, _fp(inp), _fp(Ht), inp.shape[0], _fp(g), ctypes.byref(loss))
+        self._check(rc, "mpm_surrogate_gradient")
+        return g, loss.value
+
+    def surrogate_train(self, inputs, H, epochs, batch, lr, seed=0):
+        inp = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 5)
+        Ht = np.ascontiguousarray(H, dtype=np.float32).reshape(-1, 3)
+        losses = np.empty(int(epochs), np.float32)
+        rc = self.lib.mpm_surrogate_train(self.h, _fp(inp), _fp(Ht), inp.shape[0], int(epochs), int(batch), float(lr),
+                                          ctypes.c_uint64(int(seed)), _fp(losses))
+        self._check(rc, "mpm_surrogate_train")
+        return losses
+
+    def set_pair_logging(self, capacity):
+        self._check(self.lib.mpm_set_pair_logging(self.h, int(capacity)), "mpm_set_pair_logging")
+
+    def get_pairs(self, max_pairs=None):
+        if max_pairs is None:
+            max_pairs = self.stats()["pairs_logged"]
+        out = np.empty((max(int(max_pairs), 1), 8), np.float32)
+        cnt = ctypes.c_int64()
+        self._check(self.lib.mpm_get_pairs(self.h, _fp(out), int(max_pairs), ctypes.byref(cnt)), "mpm_get_pairs")
+        return out[:cnt.value]
+
+    def set_surrogate(self, enable, gate_tol=1e-5):
+        self._check(self.lib.mpm_set_surrogate(self.h, int(bool(enable)), float(gate_tol)), "mpm_set_surrogate")
+
     def bench_copy(self, nbytes, reps):
         ms = ctypes.c_float()
         self._check(self.lib.mpm_bench_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)), "mpm_bench_copy")

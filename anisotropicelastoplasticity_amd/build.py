@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpm_hip.so")
-SOURCES = ["mpm_kernels.hip", "mpm_runtime.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_mlp.hip", "mpm_runtime.hip"]
 EXAMPLE = os.path.join(HERE, "sand_column_example")
 EXAMPLE_SRC = os.path.join(CSRC, "host", "sand_column_example.cpp")
 

@@ -77,6 +77,7 @@ __device__ __forceinline__ void mat_mul(const T* A, const T* B, T* C) {
 }
 template <class T>
 __device__ __forceinline__ T mat_det(const T* A) {
+#pragma clang fp contract(off)
   return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
 }
 
@@ -221,6 +222,7 @@ __device__ __forceinline__ void stress_matrix(const DevParams& P, const float* F
 // Drucker-Prager projection of principal stretches (HybridSolver.cpp:634-673).
 template <class T>
 __device__ __forceinline__ void dp_project(T lambda, T mu, T phi0, T h1, T h2, T h3, T q, const T* Sin, T* Sout, T& dq) {
+#pragma clang fp contract(off)
   const T pi = T(3.14159265358979323846);
   T fa = (phi0 + (h1 * q - h3) * exp(-h2 * q)) * pi / T(180);
   T sf = sin(fa);
@@ -252,6 +254,7 @@ __device__ __forceinline__ void dp_project(T lambda, T mu, T phi0, T h1, T h2, T
 // (R = U V^T, J = det F), with lambda, mu scaled by exp(xi (1 - J_P)) for snow (:281-287).
 __device__ __forceinline__ void kirchhoff(const DevParams& P, const float* U, const float* S, float J, float Jp,
                                           float vol, float* K) {
+#pragma clang fp contract(off)
   float d[3];
   if (P.material == MPM_SNOW) {
     float sc = expf(P.xi * (1.0f - Jp));
@@ -279,14 +282,76 @@ __device__ __forceinline__ void kirchhoff_of_F(const DevParams& P, const float* 
   kirchhoff(P, U, S, mat_det(F), Jp, vol, K);
 }
 
+// The sand return map after the SVD, as in return_map (used by the surrogate kernel's fallback so that
+// a rejected prediction gives bitwise the exact result): Sn, dq from S, then F_E and V tau.
+__device__ __forceinline__ void sand_finish(const DevParams& P, const float* U, const float* S, const float* V,
+                                            const float* Sn, float dq, float& plastic, float vol, float* Fe, float* K) {
+#pragma clang fp contract(off)
+  const float sgn = mat_det(U) * mat_det(V) < 0.0f ? -1.0f : 1.0f;
+  plastic += dq;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Fe[3 * r + c] = U[3 * r] * Sn[0] * V[3 * c] + U[3 * r + 1] * Sn[1] * V[3 * c + 1] + U[3 * r + 2] * Sn[2] * V[3 * c + 2];
+  kirchhoff(P, U, Sn, sgn * Sn[0] * Sn[1] * Sn[2], plastic, vol, K);
+}
+
+// Surrogate hooks of the return map: pair logging (exact path) or deferral of cone-branch
+// particles to the surrogate kernel.
+struct SurrogateIO {
+  int mode;          // 0 = exact, 1 = exact + log cone pairs, 2 = defer cone-branch particles
+  float* pair_log;   // [cap][8]: eps(3), phi, K/mu, H(3)
+  int64_t pair_cap;
+  unsigned long long* pair_count;
+  // filled when mode == 2 and the particle is deferred
+  bool deferred;
+  float sigma[3];
+  float U[9];
+  float V[9];
+};
+
+// Friction angle (degrees) and K/mu of one particle: phi(q) = h0 + (h1 q - h3) exp(-h2 q)
+// (HybridSolver.cpp:646-647), h0 replaced by the per-particle angle when configured.
+__device__ __forceinline__ float dp_phi(const DevParams& P, float phi_p, float q) {
+  return (P.per_particle_phi ? phi_p : P.h0) + (P.h1 * q - P.h3) * expf(-P.h2 * q);
+}
+__device__ __forceinline__ float bulk_over_shear(const DevParams& P) { return P.lambda0 / P.mu0 + 2.0f / 3.0f; }
+
+// FP32 Drucker-Prager projection as called by the return map; the surrogate's exact fallback calls
+// the same function on the same operands (no reassociation without fast-math), giving the same bits.
+__device__ __forceinline__ void sand_project(const DevParams& P, float phi0, float q, const float* S, float* Sn, float& dq) {
+  dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, q, S, Sn, dq);
+}
+
 // Return mapping of one particle (HybridSolver.cpp:616-681). Fh is replaced by the elastic part
 // U Sigma V^T; plastic holds J_P (snow, whose update det(F_P') = J_P * prod(Sigma)/prod(Sigma') is
 // the determinant of F_P' = V Sigma'^-1 U^T F^ F_P, :676-677) or plasticAmount q (sand). K receives
 // the Kirchhoff stress V tau of the new elastic state (same SVD).
 __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float& plastic, float phi_p, float vol,
-                                           float* K, int& fb_svd, int& fb_apex) {
+                                           float* K, int& fb_svd, int& fb_apex, SurrogateIO& io) {
+  // No FMA contraction in the return map: the surrogate kernel's exact fallback evaluates the same
+  // functions in another kernel and must reproduce these results bit for bit.
+#pragma clang fp contract(off)
   float U[9], S[3], V[9];
   svd3_guarded(Fh, U, S, V, P.svd_ratio, fb_svd);
+  if (io.mode == 2 && P.material == MPM_SAND) {
+    // cone branch (trial plastic, compressive): leave F^ as is and hand the particle to the surrogate
+    float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
+    float tr = l0 + l1 + l2;
+    float e0 = l0 - tr / 3.0f, e1 = l1 - tr / 3.0f, e2 = l2 - tr / 3.0f;
+    float en = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+    const float fa = dp_phi(P, phi_p, plastic) * 3.14159265358979323846f / 180.0f;
+    const float sf = sinf(fa);
+    const float alpha = sqrtf(2.0f / 3.0f) * 2.0f * sf / (3.0f - sf);
+    const float dg = en + (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0) * tr * alpha;
+    if (dg > 0.0f && en > 0.0f && !(tr > 0.0f) && l0 * l0 + l1 * l1 + l2 * l2 >= P.apex_radius * P.apex_radius) {
+      io.deferred = true;
+      io.sigma[0] = S[0]; io.sigma[1] = S[1]; io.sigma[2] = S[2];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) { io.U[i] = U[i]; io.V[i] = V[i]; }
+      return;
+    }
+  }
   const float sgn = mat_det(U) * mat_det(V) < 0.0f ? -1.0f : 1.0f;
   float Sn[3];
   if (P.material == MPM_SNOW) {
@@ -297,7 +362,7 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
   } else {
     float phi0 = P.per_particle_phi ? phi_p : P.h0;
     float dq;
-    dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, S, Sn, dq);
+    sand_project(P, phi0, plastic, S, Sn, dq);
     if (dq > 0.0f) {
       // a projection happened; close to the cone apex redo it in FP64
       float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
@@ -309,7 +374,23 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
         fb_apex += 1;
       }
     }
-    plastic += dq;
+    if (io.mode == 1 && dq > 0.0f) {
+      // log (trial, projected) cone pairs: the training data of the surrogate
+      float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
+      const float tr = l0 + l1 + l2;
+      if (tr <= 0.0f && Sn[0] != 1.0f) {
+        const unsigned long long idx = atomicAdd(io.pair_count, 1ull);
+        if ((int64_t)idx < io.pair_cap) {
+          float* rec = io.pair_log + 8 * idx;
+          rec[0] = l0; rec[1] = l1; rec[2] = l2;
+          rec[3] = dp_phi(P, phi_p, plastic);
+          rec[4] = bulk_over_shear(P);
+          rec[5] = logf(Sn[0]); rec[6] = logf(Sn[1]); rec[7] = logf(Sn[2]);
+        }
+      }
+    }
+    sand_finish(P, U, S, V, Sn, dq, plastic, vol, Fh, K);
+    return;
   }
 #pragma unroll
   for (int r = 0; r < 3; ++r)

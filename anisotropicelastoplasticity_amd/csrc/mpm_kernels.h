@@ -35,7 +35,7 @@ constexpr int FOOT = TILE + 3;             // footprint nodes per edge (cell win
 constexpr int FOOT3 = FOOT * FOOT * FOOT;
 constexpr int PART_STRIDE = 7 * FOOT3;     // m, p(3), f(3) per tile slot
 
-enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_NUM = 2 };
+enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_CAND = 2, CNT_NUM = 4 };
 enum P2GMode : int { P2G_MOM_STRESS = 0, P2G_MOM = 1, P2G_FORCE = 2 };
 enum GridMode : int { GRID_FUSED = 0, GRID_NORMALIZE = 1, GRID_FORCE_UPDATE = 2 };
 
@@ -110,6 +110,33 @@ struct G2PArgs {
   int* status;
   unsigned long long* fallback_svd;
   unsigned long long* fallback_apex;
+  // surrogate hooks: 0 = exact, 1 = exact + pair logging, 2 = defer cone-branch particles
+  int sur_mode;
+  float* pair_log;
+  int64_t pair_cap;
+  unsigned long long* pair_count;
+  float* cand;      // candidate records, 21 planes (sigma 3, U 9, V 9) of n floats
+  int* cand_slot;   // sorted slot of each candidate
+  int* cand_count;
+};
+
+// Candidate planes of the surrogate return map.
+constexpr int CAND_SIGMA = 0, CAND_U = 3, CAND_V = 12, CAND_PLANES = 21;
+
+struct SurrogateArgs {
+  DevParams P;
+  const uint16_t* wbf;   // BF16 forward weights
+  const float* master;   // FP32 master weights (biases)
+  float* st;             // particle state written by k_g2p (sorted slots)
+  int64_t cap;
+  int n;
+  const float* cand;
+  const int* cand_slot;
+  const int* cand_count;
+  float gate_tol;
+  int write_stress;
+  unsigned long long* accepted;
+  unsigned long long* candidates;  // cumulative candidate count (block 0 adds this substep's)
 };
 
 void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const int* pos, const float* thr,
@@ -131,5 +158,7 @@ void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, cons
                         unsigned long long* out, int grid);
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4);
+void surrogate_set_smem_limit();
+void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid);
 
 }  // namespace mpm

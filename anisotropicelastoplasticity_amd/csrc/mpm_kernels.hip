@@ -19,6 +19,7 @@
 
 #include "mpm_device.hpp"
 #include "mpm_kernels.h"
+#include "mpm_mlp_device.hpp"
 
 namespace mpm {
 
@@ -700,8 +701,37 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     float plastic = st[PPL * cap + pi];
     const float phi_p = st[PPHI * cap + pi];
     const float vol = st[PVOL * cap + pi];
-    float K[6];
-    return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex);
+    float K[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    SurrogateIO io;
+    io.mode = a.sur_mode;
+    io.pair_log = a.pair_log;
+    io.pair_cap = a.pair_cap;
+    io.pair_count = a.pair_count;
+    io.deferred = false;
+    return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
+    if (a.sur_mode == 2) {
+      // compact the deferred particles of this wave into the candidate list (one atomic per wave)
+      const uint64_t dm = __ballot(io.deferred);
+      if (dm) {
+        const int lane = (int)lane_id();
+        const int first = __ffsll((long long)dm) - 1;
+        int base = 0;
+        if (lane == first) base = atomicAdd(a.cand_count, __popcll(dm));
+        base = __shfl(base, first);
+        const int c = base + __popcll(dm & ((1ull << lane) - 1ull));
+        const int64_t n = a.n;
+        if (io.deferred && c < n) {
+          a.cand_slot[c] = j;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) a.cand[(CAND_SIGMA + k) * n + c] = io.sigma[k];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            a.cand[(CAND_U + k) * n + c] = io.U[k];
+            a.cand[(CAND_V + k) * n + c] = io.V[k];
+          }
+        }
+      }
+    }
     float xn[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -739,6 +769,115 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   }
   if (fb_svd) atomicAdd(a.fallback_svd, (unsigned long long)fb_svd);
   if (fb_apex) atomicAdd(a.fallback_apex, (unsigned long long)fb_apex);
+}
+
+// ================================================================== surrogate return map
+// Cone-branch particles deferred by k_g2p (sur_mode 2) are finished here. Each wave evaluates the MLP
+// for 16 candidates with BF16 MFMA (weights staged once per workgroup in LDS), then lanes 0..15
+// gate their prediction H of the projected Hencky strains:
+//   yield equality   | |dev H| + c alpha tr H | <= tol       (c = (3 lambda + 2 mu) / (2 mu))
+//   volume           | tr H - tr eps | <= tol                (the cone return is deviatoric)
+//   flow direction   the deviatoric correction eps - H is parallel to dev eps (orthogonal part <= tol)
+// An accepted prediction gives Sigma' = exp(H), dq = |eps - H|; a rejected one runs the exact
+// projection (HybridSolver.cpp:646-673) through the same code as k_g2p, so rejecting every
+// prediction reproduces the exact path bit for bit.
+__global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char smem[];
+  MLPShared& sh = *reinterpret_cast<MLPShared*>(smem);
+  uint16_t* acts = reinterpret_cast<uint16_t*>(smem + sizeof(MLPShared));
+  const DevParams& P = a.P;
+  // the counter is cleared by the sort stage; a G2P stage repeated without it cannot list more than n
+  const int count = min(*a.cand_count, a.n);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
+  if (blockIdx.x * 64 >= count) return;  // uniform per workgroup: nothing to do, skip the weight staging
+  stage_weights(sh, a.wbf, a.master);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint16_t* act = acts + w * 16 * LDA;
+  const int64_t cap = a.cap, n = a.n;
+  const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
+  for (int blk = blockIdx.x * 4 + w; blk * 16 < count; blk += gridDim.x * 4) {
+    const int row = blk * 16 + (lane & 15);
+    const bool valid = lane < 16 && row < count;
+    int j = 0;
+    float S[3] = {1.0f, 1.0f, 1.0f}, eps[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f, phi_p = 0.0f, phi = 0.0f;
+    if (valid) {
+      j = clamp_index(a.cand_slot[row], (int)n);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        S[k] = a.cand[(CAND_SIGMA + k) * n + row];
+        eps[k] = logf(S[k]);
+      }
+      q = a.st[PPL * cap + j];
+      phi_p = a.st[PPHI * cap + j];
+      phi = dp_phi(P, phi_p, q);
+    }
+    if (lane < 16) {
+      const float x[5] = {eps[0], eps[1], eps[2], phi, bulk_over_shear(P)};
+      mlp_input_row(act + lane * LDA, x);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    float y[3] = {0.0f, 0.0f, 0.0f};
+    wave_forward16(sh, act, y, nullptr, 16);
+    bool accepted = false;
+    if (valid) {
+      float H[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) H[k] = eps[k] + MLP_CORR_SCALE * y[k];
+      const float fa = phi * 3.14159265358979323846f / 180.0f;
+      const float sf = sinf(fa);
+      const float alpha = sqrtf(2.0f / 3.0f) * 2.0f * sf / (3.0f - sf);
+      const float trH = H[0] + H[1] + H[2], trE = eps[0] + eps[1] + eps[2];
+      float dH[3], dE[3], d[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        dH[k] = H[k] - trH / 3.0f;
+        dE[k] = eps[k] - trE / 3.0f;
+      }
+      const float nH = sqrtf(dH[0] * dH[0] + dH[1] * dH[1] + dH[2] * dH[2]);
+      const float nE = sqrtf(dE[0] * dE[0] + dE[1] * dE[1] + dE[2] * dE[2]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] = dE[k] - dH[k];  // deviatoric part of eps - H
+      const float along = (d[0] * dE[0] + d[1] * dE[1] + d[2] * dE[2]) / nE;
+      float ortho2 = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float o = d[k] - along * dE[k] / nE;
+        ortho2 += o * o;
+      }
+      const float tol = a.gate_tol;
+      accepted = fabsf(nH + c_yield * alpha * trH) <= tol && fabsf(trH - trE) <= tol && along > 0.0f &&
+                 sqrtf(ortho2) <= tol;
+      float U[9], V[9], Sn[3], dq;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        U[k] = a.cand[(CAND_U + k) * n + row];
+        V[k] = a.cand[(CAND_V + k) * n + row];
+      }
+      if (accepted) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Sn[k] = expf(H[k]);
+        dq = sqrtf((eps[0] - H[0]) * (eps[0] - H[0]) + (eps[1] - H[1]) * (eps[1] - H[1]) + (eps[2] - H[2]) * (eps[2] - H[2]));
+      } else {
+        const float phi0 = P.per_particle_phi ? phi_p : P.h0;
+        sand_project(P, phi0, q, S, Sn, dq);
+      }
+      float Fe[9], K[6];
+      const float vol = a.st[PVOL * cap + j];
+      sand_finish(P, U, S, V, Sn, dq, q, vol, Fe, K);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a.st[(PF + k) * cap + j] = Fe[k];
+      a.st[PPL * cap + j] = q;
+      if (a.write_stress) {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) a.st[(PK + e) * cap + j] = K[e];
+      }
+    }
+    const uint64_t am = __ballot(accepted);
+    if (lane == 0 && am) atomicAdd(a.accepted, (unsigned long long)__popcll(am));
+  }
 }
 
 // ================================================================== auxiliary kernels
@@ -928,6 +1067,17 @@ void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4) {
   int grid = blocks_for(n4, 256);
   if (grid > 8192) grid = 8192;
   if (n4 > 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, src, dst, n4);
+}
+
+static size_t surrogate_smem() { return sizeof(MLPShared) + 4 * 16 * LDA * sizeof(uint16_t); }
+
+void surrogate_set_smem_limit() {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_surrogate_apply),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)surrogate_smem());
+}
+
+void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid) {
+  hipLaunchKernelGGL(k_surrogate_apply, dim3(grid), dim3(256), surrogate_smem(), s, a);
 }
 
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v) {

@@ -12,6 +12,7 @@
 
 #include "../../include/mpm_c_abi.h"
 #include "mpm_kernels.h"
+#include "mpm_mlp.h"
 
 using namespace mpm;
 
@@ -80,6 +81,24 @@ struct mpm_ctx {
 
   int p2g_grid = 1024;
   int grid_grid = 2048;
+  int n_cu = 256;
+
+  // learned return-map surrogate
+  float* mlp_master = nullptr;
+  float* mlp_m1 = nullptr;
+  float* mlp_m2 = nullptr;
+  float* mlp_grad = nullptr;
+  uint16_t* mlp_wbf = nullptr;
+  uint16_t* mlp_wbft = nullptr;
+  int mlp_step = 0;
+  bool mlp_ready = false;
+  float* pair_log = nullptr;
+  int64_t pair_cap = 0;
+  unsigned long long* sur_counts = nullptr;  // [0] candidates, [1] accepted, [2] pairs logged
+  int sur_enabled = 0;
+  float gate_tol = 1e-5f;
+  float* cand = nullptr;
+  int* cand_slot = nullptr;
 };
 
 #define CK(call)                                                                     \
@@ -281,7 +300,32 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.status = ctx->status;
   a.fallback_svd = ctx->fallbacks;
   a.fallback_apex = ctx->fallbacks + 1;
+  const bool surrogate = ctx->sur_enabled && ctx->mlp_ready && ctx->cand && ctx->dp.material == MPM_SAND;
+  a.sur_mode = surrogate ? 2 : (ctx->pair_cap > 0 ? 1 : 0);
+  a.pair_log = ctx->pair_log;
+  a.pair_cap = ctx->pair_cap;
+  a.pair_count = ctx->sur_counts ? ctx->sur_counts + 2 : nullptr;
+  a.cand = ctx->cand;
+  a.cand_slot = ctx->cand_slot;
+  a.cand_count = ctx->counters + CNT_CAND;
   launch_g2p(ctx->stream, a);
+  if (surrogate) {
+    SurrogateArgs sa;
+    sa.P = ctx->dp;
+    sa.wbf = ctx->mlp_wbf;
+    sa.master = ctx->mlp_master;
+    sa.st = ctx->state[1 - ctx->cur];
+    sa.cap = ctx->cap;
+    sa.n = ctx->n;
+    sa.cand = ctx->cand;
+    sa.cand_slot = ctx->cand_slot;
+    sa.cand_count = ctx->counters + CNT_CAND;
+    sa.gate_tol = ctx->gate_tol;
+    sa.write_stress = a.write_stress;
+    sa.accepted = ctx->sur_counts + 1;
+    sa.candidates = ctx->sur_counts;
+    launch_surrogate_apply(ctx->stream, sa, ctx->n_cu);
+  }
   ctx->cur = 1 - ctx->cur;
   ctx->keys_valid = true;
   ctx->stress_valid = !ctx->dp.predicted;
@@ -370,6 +414,7 @@ int mpm_create(mpm_ctx** out, int32_t device) {
   int rc = dalloc(ctx, ctx->counters, CNT_NUM);
   if (!rc) rc = dalloc(ctx, ctx->status, 1);
   if (!rc) rc = dalloc(ctx, ctx->fallbacks, 4);
+  if (!rc) rc = dalloc(ctx, ctx->sur_counts, 4);
   if (rc) {
     g_create_error = ctx->err;
     mpm_destroy(ctx);
@@ -378,9 +423,11 @@ int mpm_create(mpm_ctx** out, int32_t device) {
   (void)hipMemsetAsync(ctx->counters, 0, CNT_NUM * sizeof(int), ctx->stream);
   (void)hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream);
   (void)hipMemsetAsync(ctx->fallbacks, 0, 4 * sizeof(unsigned long long), ctx->stream);
+  (void)hipMemsetAsync(ctx->sur_counts, 0, 4 * sizeof(unsigned long long), ctx->stream);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     ctx->p2g_grid = std::max(256, prop.multiProcessorCount * 4);
+    ctx->n_cu = std::max(1, prop.multiProcessorCount);
     ctx->grid_grid = std::max(512, prop.multiProcessorCount * 8);
   }
   *out = ctx;
@@ -397,6 +444,8 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
   dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->sortws);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
+  dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
+  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPM_OK;
@@ -506,6 +555,8 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
     ctx->nb_sort = (n + SORT_TILE - 1) / SORT_TILE;
     ctx->sortws_words = 772 + (int64_t)3 * ctx->nb_sort * 256;
     if (!rc) rc = dalloc(ctx, ctx->sortws, ctx->sortws_words);
+    if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
+    if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
     if (rc) return rc;
     CK(hipMemsetAsync(ctx->state[1], 0, sizeof(float) * NPLANES * (size_t)n, ctx->stream));
     CK(hipMemsetAsync(ctx->keys, 0, sizeof(uint32_t) * n, ctx->stream));
@@ -708,6 +759,7 @@ int mpm_get_stats(mpm_ctx* ctx, mpm_stats* s) {
   int cnt[CNT_NUM] = {0, 0};
   int status = 0;
   unsigned long long fb[4] = {0, 0, 0, 0};
+  unsigned long long sc[4] = {0, 0, 0, 0};
   if (ctx->have_grid && ctx->n > 0) {
     CK(hipMemsetAsync(ctx->fallbacks + 2, 0, sizeof(unsigned long long), ctx->stream));
     launch_count_nodes(ctx->stream, ctx->dp, ctx->grid, ctx->counters, ctx->active_blocks, ctx->fallbacks + 2, ctx->grid_grid);
@@ -715,6 +767,7 @@ int mpm_get_stats(mpm_ctx* ctx, mpm_stats* s) {
   CK(hipMemcpyAsync(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   CK(hipMemcpyAsync(&status, ctx->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CK(hipMemcpyAsync(fb, ctx->fallbacks, sizeof(fb), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipMemcpyAsync(sc, ctx->sur_counts, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
   s->status = status;
   s->n_active_tiles = cnt[CNT_TILES];
@@ -722,6 +775,9 @@ int mpm_get_stats(mpm_ctx* ctx, mpm_stats* s) {
   s->n_active_nodes = (int64_t)fb[2];
   s->fp64_svd_count = (int64_t)fb[0];
   s->fp64_apex_count = (int64_t)fb[1];
+  s->surrogate_candidates = (int64_t)sc[0];
+  s->surrogate_accepted = (int64_t)sc[1];
+  s->pairs_logged = (int64_t)std::min<unsigned long long>(sc[2], (unsigned long long)std::max<int64_t>(ctx->pair_cap, 0));
   return MPM_OK;
 }
 
@@ -730,6 +786,7 @@ int mpm_clear_status(mpm_ctx* ctx) {
   CK(hipSetDevice(ctx->device));
   CK(hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream));
   CK(hipMemsetAsync(ctx->fallbacks, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  CK(hipMemsetAsync(ctx->sur_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
   CK(hipStreamSynchronize(ctx->stream));
   return MPM_OK;
 }
@@ -805,7 +862,271 @@ int mpm_bench_copy(mpm_ctx* ctx, int64_t nbytes, int32_t reps, float* ms) {
   return MPM_OK;
 }
 
+// ------------------------------------------------------------------ learned return-map surrogate
+static inline uint64_t mix64(uint64_t z);
+
+static int ensure_mlp(mpm_ctx* ctx) {
+  if (ctx->mlp_master) return MPM_OK;
+  int rc = dalloc(ctx, ctx->mlp_master, MLP_NPARAM);
+  if (!rc) rc = dalloc(ctx, ctx->mlp_m1, MLP_NPARAM);
+  if (!rc) rc = dalloc(ctx, ctx->mlp_m2, MLP_NPARAM);
+  if (!rc) rc = dalloc(ctx, ctx->mlp_grad, MLP_NPARAM);
+  if (!rc) rc = dalloc(ctx, ctx->mlp_wbf, MLP_BF_SIZE);
+  if (!rc) rc = dalloc(ctx, ctx->mlp_wbft, MLP_T_SIZE);
+  if (rc) return rc;
+  CK(hipMemsetAsync(ctx->mlp_wbft, 0, sizeof(uint16_t) * MLP_T_SIZE, ctx->stream));
+  mlp_set_smem_limits();
+  surrogate_set_smem_limit();
+  return MPM_OK;
+}
+
+static int upload_weights(mpm_ctx* ctx, const float* w) {
+  CK(hipMemcpyAsync(ctx->mlp_master, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, ctx->stream));
+  CK(hipMemsetAsync(ctx->mlp_m1, 0, sizeof(float) * MLP_NPARAM, ctx->stream));
+  CK(hipMemsetAsync(ctx->mlp_m2, 0, sizeof(float) * MLP_NPARAM, ctx->stream));
+  launch_mlp_to_bf16(ctx->stream, ctx->mlp_master, ctx->mlp_wbf, ctx->mlp_wbft);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_step = 0;
+  ctx->mlp_ready = true;
+  return MPM_OK;
+}
+
+int32_t mpm_surrogate_nparam(void) { return MLP_NPARAM; }
+
+int mpm_surrogate_init(mpm_ctx* ctx, uint64_t seed) {
+  if (!ctx) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  int rc = ensure_mlp(ctx);
+  if (rc) return rc;
+  // uniform(-a, a) with a = sqrt(3 / fan_in) (unit variance for unit inputs); output layer scaled 0.1
+  std::vector<float> w(MLP_NPARAM, 0.0f);
+  for (int l = 0; l < 5; ++l) {
+    const int N = mlp_N(l), K = mlp_K(l), w0 = mlp_weight_offset(l);
+    const int fan_in = l == 0 ? MLP_IN : K;
+    const double a = std::sqrt(3.0 / fan_in) * (l == 4 ? 0.1 : 1.0);
+    for (int o = 0; o < N; ++o)
+      for (int k = 0; k < K; ++k) {
+        const int i = w0 + o * K + k;
+        if (!mlp_param_active(i)) continue;
+        const double u = (double)(mix64(seed * 0x9E3779B97F4A7C15ull ^ (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
+        w[i] = (float)((2.0 * u - 1.0) * a);
+      }
+  }
+  return upload_weights(ctx, w.data());
+}
+
+int mpm_surrogate_set_weights(mpm_ctx* ctx, const float* w, int32_t nparam) {
+  if (!ctx || !w || nparam != MLP_NPARAM) return ctx ? fail(ctx, MPM_ERR_INVALID, "nparam mismatch") : MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  int rc = ensure_mlp(ctx);
+  if (rc) return rc;
+  return upload_weights(ctx, w);
+}
+
+int mpm_surrogate_get_weights(mpm_ctx* ctx, float* w, int32_t nparam) {
+  if (!ctx || !w || nparam != MLP_NPARAM) return ctx ? fail(ctx, MPM_ERR_INVALID, "nparam mismatch") : MPM_ERR_INVALID;
+  if (!ctx->mlp_ready) return fail(ctx, MPM_ERR_STATE, "surrogate weights not set");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(w, ctx->mlp_master, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+  return MPM_OK;
+}
+
+// normalised targets y = (H - eps) / CORR_SCALE
+static void targets_of(const float* in, const float* H, int64_t n, std::vector<float>& y) {
+  y.resize((size_t)3 * n);
+  for (int64_t r = 0; r < n; ++r)
+    for (int c = 0; c < 3; ++c) y[3 * r + c] = (H[3 * r + c] - in[5 * r + c]) / MLP_CORR_SCALE;
+}
+
+int mpm_surrogate_predict(mpm_ctx* ctx, const float* in, float* H, int32_t n) {
+  if (!ctx || !in || !H || n < 0) return MPM_ERR_INVALID;
+  if (!ctx->mlp_ready) return fail(ctx, MPM_ERR_STATE, "surrogate weights not set");
+  if (n == 0) return MPM_OK;
+  CK(hipSetDevice(ctx->device));
+  float *din = nullptr, *dout = nullptr;
+  int rc = dalloc(ctx, din, (size_t)5 * n);
+  if (!rc) rc = dalloc(ctx, dout, (size_t)3 * n);
+  if (rc) { dfree(din); dfree(dout); return rc; }
+  hipError_t e = hipMemcpyAsync(din, in, sizeof(float) * 5 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, din, dout, n, nullptr,
+                       std::min(4 * ctx->n_cu, (n + 63) / 64));
+    e = hipMemcpyAsync(H, dout, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(din);
+  dfree(dout);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_surrogate_predict: ") + hipGetErrorString(e));
+  for (int64_t r = 0; r < n; ++r)
+    for (int c = 0; c < 3; ++c) H[3 * r + c] = in[5 * r + c] + MLP_CORR_SCALE * H[3 * r + c];
+  return MPM_OK;
+}
+
+// one optimisation pass machinery: forward (saving pre-activations), backward, ordered reduction
+struct TrainBufs {
+  float* in = nullptr;
+  float* tgt = nullptr;
+  float* y = nullptr;
+  float* z = nullptr;
+  float* gpart = nullptr;
+  float* lpart = nullptr;
+  float* lacc = nullptr;
+};
+
+static void free_train(TrainBufs& b) {
+  dfree(b.in); dfree(b.tgt); dfree(b.y); dfree(b.z); dfree(b.gpart); dfree(b.lpart); dfree(b.lacc);
+}
+
+// workgroups of the backward pass (each owns a partial-gradient slice of MLP_NPARAM floats)
+static int backward_grid(int rows) { return std::min((rows + 63) / 64, 1024); }
+
+static int alloc_train(mpm_ctx* ctx, TrainBufs& b, int64_t n, int batch, int nacc) {
+  const int groups = backward_grid(batch);
+  int rc = dalloc(ctx, b.in, (size_t)5 * n);
+  if (!rc) rc = dalloc(ctx, b.tgt, (size_t)3 * n);
+  if (!rc) rc = dalloc(ctx, b.y, (size_t)3 * batch);
+  if (!rc) rc = dalloc(ctx, b.z, (size_t)((batch + 15) / 16) * 4 * 16 * MLP_HIDDEN);
+  if (!rc) rc = dalloc(ctx, b.gpart, (size_t)groups * MLP_NPARAM);
+  if (!rc) rc = dalloc(ctx, b.lpart, (size_t)groups);
+  if (!rc) rc = dalloc(ctx, b.lacc, (size_t)nacc);
+  return rc;
+}
+
+static void train_batch(mpm_ctx* ctx, TrainBufs& b, int64_t r0, int nb, float* lacc) {
+  const int groups = backward_grid(nb);
+  launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, b.in + 5 * r0, b.y, nb, b.z,
+                     std::min((nb + 63) / 64, 4 * ctx->n_cu));
+  launch_mlp_backward(ctx->stream, ctx->mlp_wbft, b.in + 5 * r0, b.y, b.tgt + 3 * r0, b.z, nb, b.gpart, b.lpart, groups);
+  launch_mlp_reduce(ctx->stream, b.gpart, groups, ctx->mlp_grad, b.lpart, lacc);
+}
+
+int mpm_surrogate_gradient(mpm_ctx* ctx, const float* in, const float* H, int32_t n, float* grad, float* loss) {
+  if (!ctx || !in || !H || n <= 0) return MPM_ERR_INVALID;
+  if (!ctx->mlp_ready) return fail(ctx, MPM_ERR_STATE, "surrogate weights not set");
+  CK(hipSetDevice(ctx->device));
+  TrainBufs b;
+  int rc = alloc_train(ctx, b, n, n, 1);
+  if (rc) { free_train(b); return rc; }
+  std::vector<float> y;
+  targets_of(in, H, n, y);
+  hipError_t e = hipMemcpyAsync(b.in, in, sizeof(float) * 5 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.tgt, y.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(b.lacc, 0, sizeof(float), ctx->stream);
+  if (e == hipSuccess) {
+    train_batch(ctx, b, 0, n, b.lacc);
+    e = hipGetLastError();
+  }
+  float lsum = 0.0f;
+  if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, ctx->mlp_grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&lsum, b.lacc, sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_train(b);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_surrogate_gradient: ") + hipGetErrorString(e));
+  if (loss) *loss = lsum / (float)n;
+  return MPM_OK;
+}
+
+int mpm_surrogate_train(mpm_ctx* ctx, const float* in, const float* H, int32_t n, int32_t epochs, int32_t batch,
+                        float lr, uint64_t seed, float* epoch_loss) {
+  if (!ctx || !in || !H || n <= 0 || epochs < 1 || batch < 16) return ctx ? fail(ctx, MPM_ERR_INVALID, "bad training arguments") : MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  int rc = ensure_mlp(ctx);
+  if (rc) return rc;
+  if (!ctx->mlp_ready) {
+    rc = mpm_surrogate_init(ctx, seed);
+    if (rc) return rc;
+  }
+  // seeded shuffle (Fisher-Yates with a splitmix64 stream)
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  for (int i = n - 1; i > 0; --i) {
+    const int j = (int)(mix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(i + 1))) % (uint64_t)(i + 1));
+    std::swap(order[i], order[j]);
+  }
+  std::vector<float> ins((size_t)5 * n), hs((size_t)3 * n), y;
+  for (int r = 0; r < n; ++r) {
+    for (int c = 0; c < 5; ++c) ins[5 * (size_t)r + c] = in[5 * (size_t)order[r] + c];
+    for (int c = 0; c < 3; ++c) hs[3 * (size_t)r + c] = H[3 * (size_t)order[r] + c];
+  }
+  targets_of(ins.data(), hs.data(), n, y);
+  if (batch > n) batch = n;
+  TrainBufs b;
+  rc = alloc_train(ctx, b, n, batch, epochs);
+  if (rc) { free_train(b); return rc; }
+  hipError_t e = hipMemcpyAsync(b.in, ins.data(), sizeof(float) * 5 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.tgt, y.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(b.lacc, 0, sizeof(float) * epochs, ctx->stream);
+  for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
+    for (int64_t r0 = 0; r0 < n; r0 += batch) {
+      const int nb = (int)std::min<int64_t>(batch, n - r0);
+      train_batch(ctx, b, r0, nb, b.lacc + ep);
+      launch_mlp_adam(ctx->stream, ctx->mlp_master, ctx->mlp_m1, ctx->mlp_m2, ctx->mlp_grad, ctx->mlp_wbf, ctx->mlp_wbft,
+                      lr, ++ctx->mlp_step);
+    }
+    e = hipGetLastError();
+  }
+  std::vector<float> l(epochs, 0.0f);
+  if (e == hipSuccess) e = hipMemcpyAsync(l.data(), b.lacc, sizeof(float) * epochs, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_train(b);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_surrogate_train: ") + hipGetErrorString(e));
+  if (epoch_loss)
+    for (int ep = 0; ep < epochs; ++ep) epoch_loss[ep] = l[ep] / (float)n;
+  return MPM_OK;
+}
+
+int mpm_set_pair_logging(mpm_ctx* ctx, int64_t capacity) {
+  if (!ctx || capacity < 0) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  int rc = dalloc(ctx, ctx->pair_log, (size_t)8 * capacity);
+  if (rc) return rc;
+  ctx->pair_cap = capacity;
+  CK(hipMemsetAsync(ctx->sur_counts + 2, 0, sizeof(unsigned long long), ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+int mpm_get_pairs(mpm_ctx* ctx, float* pairs, int64_t max_pairs, int64_t* count) {
+  if (!ctx || max_pairs < 0) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  unsigned long long c = 0;
+  CK(hipMemcpy(&c, ctx->sur_counts + 2, sizeof(c), hipMemcpyDeviceToHost));
+  int64_t m = std::min<int64_t>((int64_t)c, ctx->pair_cap);
+  m = std::min<int64_t>(m, max_pairs);
+  if (count) *count = m;
+  if (pairs && m > 0) CK(hipMemcpy(pairs, ctx->pair_log, sizeof(float) * 8 * m, hipMemcpyDeviceToHost));
+  return MPM_OK;
+}
+
+int mpm_set_surrogate(mpm_ctx* ctx, int32_t enable, float gate_tol) {
+  if (!ctx || !(gate_tol >= 0.0f)) return MPM_ERR_INVALID;
+  if (enable && !ctx->mlp_ready) return fail(ctx, MPM_ERR_STATE, "surrogate weights not set");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  if (enable && ctx->n > 0 && !ctx->cand) {
+    int rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * ctx->n);
+    if (!rc) rc = dalloc(ctx, ctx->cand_slot, (size_t)ctx->n);
+    if (rc) return rc;
+  }
+  ctx->sur_enabled = enable ? 1 : 0;
+  ctx->gate_tol = gate_tol;
+  return MPM_OK;
+}
+
 // ------------------------------------------------------------------ generator (host only)
+static inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 static inline uint64_t splitmix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -130,6 +130,9 @@ typedef struct mpm_stats {
   int64_t n_active_nodes;    /* nodes with m > 0, last substep (counted on request) */
   int64_t fp64_svd_count;    /* SVDs redone in FP64, cumulative */
   int64_t fp64_apex_count;   /* DP projections run in FP64, cumulative */
+  int64_t surrogate_candidates; /* cone-branch particles handed to the surrogate, cumulative */
+  int64_t surrogate_accepted;   /* of which the prediction passed the gate, cumulative */
+  int64_t pairs_logged;         /* (trial, projected) pairs logged since mpm_set_pair_logging */
 } mpm_stats;
 
 typedef struct mpm_ctx mpm_ctx; /* opaque: one HIP stream + device buffers */
@@ -204,6 +207,33 @@ int mpm_synchronize(mpm_ctx* ctx);
  * around every stage; stage_ms[MPM_NUM_STAGES] receives the summed stage times and *total_ms the
  * whole-span time. */
 int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms);
+
+/* ---- learned return-map surrogate (no counterpart in the reference; replaces the cone branch of
+ * the Drucker-Prager projection, HybridSolver.cpp:667-673, behind a gate that falls back to it) ----
+ * A 5 -> 128 -> 128 -> 128 -> 128 -> 3 SiLU MLP with BF16 weights and activations and FP32
+ * accumulation (gfx950 MFMA). Input row: trial Hencky principal strains eps_1 >= eps_2 >= eps_3,
+ * friction angle (degrees), K/mu. Output: projected principal strains H. Weights are one flat FP32
+ * vector of mpm_surrogate_nparam() entries (the master copy; padding entries are kept at zero). */
+int32_t mpm_surrogate_nparam(void);
+int mpm_surrogate_init(mpm_ctx* ctx, uint64_t seed);
+int mpm_surrogate_set_weights(mpm_ctx* ctx, const float* w, int32_t nparam);
+int mpm_surrogate_get_weights(mpm_ctx* ctx, float* w, int32_t nparam);
+/* H (n x 3) predicted for inputs in (n x 5), row-major. */
+int mpm_surrogate_predict(mpm_ctx* ctx, const float* in, float* H, int32_t n);
+/* Gradient of L = sum (y - t)^2 / n over normalised outputs y = (H - eps)/1e-3, with the same kernels
+ * the trainer uses; *loss receives L. */
+int mpm_surrogate_gradient(mpm_ctx* ctx, const float* in, const float* H, int32_t n, float* grad, float* loss);
+/* Adam on the GPU over `epochs` passes of minibatches of `batch` rows in a seeded shuffled order;
+ * epoch_loss[epochs] receives the mean loss of each pass. */
+int mpm_surrogate_train(mpm_ctx* ctx, const float* in, const float* H, int32_t n, int32_t epochs, int32_t batch,
+                        float lr, uint64_t seed, float* epoch_loss);
+/* Log (trial, projected) cone-branch pairs emitted by the exact return map in later substeps:
+ * capacity pairs of 8 floats (eps(3), phi, K/mu, H(3)); 0 disables logging. */
+int mpm_set_pair_logging(mpm_ctx* ctx, int64_t capacity);
+int mpm_get_pairs(mpm_ctx* ctx, float* pairs, int64_t max_pairs, int64_t* count);
+/* Enable the surrogate in the substep: cone-branch particles take the network's H when it passes the
+ * gate (yield equality, volume and flow direction within gate_tol), else the exact projection. */
+int mpm_set_surrogate(mpm_ctx* ctx, int32_t enable, float gate_tol);
 
 /* Measurement helper with no counterpart in the reference: copies nbytes (a multiple of 16) between
  * two device buffers `reps` times with a plain float4 copy kernel (k_copy). Under rocprofv3 --pmc its
