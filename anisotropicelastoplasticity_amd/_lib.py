@@ -82,6 +82,9 @@ SIGNATURES = [
     ("mpm_estimate_volumes", ctypes.c_int, [_ctx]),
     ("mpm_substep", ctypes.c_int, [_ctx, ctypes.c_float]),
     ("mpm_run", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32]),
+    ("mpm_solve", ctypes.c_int, [_ctx, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("mpm_get_clock", ctypes.c_int,
+     [_ctx, _d, _d, _i32, _P(ctypes.c_int64)]),
     ("mpm_stage", ctypes.c_int, [_ctx, ctypes.c_int32, ctypes.c_float]),
     ("mpm_get_sorted", ctypes.c_int, [_ctx, _u32, _i32, _i32]),
     ("mpm_get_grid", ctypes.c_int, [_ctx, _f, _f, _f, _f]),
@@ -104,6 +107,9 @@ SIGNATURES = [
     ("mpm_generate_block", ctypes.c_int,
      [ctypes.c_uint64, _d, _d, _i32, ctypes.c_int32, ctypes.c_int64, _f]),
 ]
+
+# void (*)(mpm_ctx*, int32_t frame, double t, void* user)
+FRAME_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p)
 
 _lib = None
 

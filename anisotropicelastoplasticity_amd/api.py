@@ -106,6 +106,33 @@ class Solver:
     def run(self, dt, nsteps):
         self._check(self.lib.mpm_run(self.h, float(dt), int(nsteps)), "mpm_run")
 
+    def solve(self, cfl, maxt, on_frame=None):
+        """Adaptive time stepping of HybridSolver::solve (mpm_solve). `on_frame(frame, t)` runs after
+        the substep that completes each 1/60 s frame; exceptions it raises stop nothing but are
+        re-raised after mpm_solve returns."""
+        errors = []
+
+        def _cb(_ctx, frame, t, _user):
+            if on_frame is None or errors:
+                return
+            try:
+                on_frame(int(frame), float(t))
+            except BaseException as e:  # noqa: BLE001 - re-raised below
+                errors.append(e)
+
+        cb = _lib.FRAME_CALLBACK(_cb)
+        rc = self.lib.mpm_solve(self.h, float(cfl), float(maxt), ctypes.cast(cb, ctypes.c_void_p), None)
+        self._check(rc, "mpm_solve")
+        if errors:
+            raise errors[0]
+
+    def clock(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        frames, subs = ctypes.c_int32(), ctypes.c_int64()
+        self._check(self.lib.mpm_get_clock(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(frames),
+                                           ctypes.byref(subs)), "mpm_get_clock")
+        return dict(t=t.value, dt=dt.value, frames=frames.value, substeps=subs.value)
+
     def stage(self, stage, dt):
         self._check(self.lib.mpm_stage(self.h, int(stage), float(dt)), "mpm_stage")
 

@@ -364,9 +364,10 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
     float dq;
     sand_project(P, phi0, plastic, S, Sn, dq);
     if (dq > 0.0f) {
-      // a projection happened; close to the cone apex redo it in FP64
+      // a projection onto the cone happened (the tip branch, tr > 0, gives Sigma = 1 exactly);
+      // close to the cone apex redo it in FP64
       float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
-      if (l0 * l0 + l1 * l1 + l2 * l2 < P.apex_radius * P.apex_radius) {
+      if (!(l0 + l1 + l2 > 0.0f) && l0 * l0 + l1 * l1 + l2 * l2 < P.apex_radius * P.apex_radius) {
         double Sd[3] = {S[0], S[1], S[2]}, So[3], dqd;
         dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, Sd, So, dqd);
         Sn[0] = float(So[0]); Sn[1] = float(So[1]); Sn[2] = float(So[2]);

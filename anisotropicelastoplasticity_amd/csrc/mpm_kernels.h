@@ -39,6 +39,29 @@ enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_CAND = 2, CNT_NUM = 4 };
 enum P2GMode : int { P2G_MOM_STRESS = 0, P2G_MOM = 1, P2G_FORCE = 2 };
 enum GridMode : int { GRID_FUSED = 0, GRID_NORMALIZE = 1, GRID_FORCE_UPDATE = 2 };
 
+// Device clock of the adaptive time step (HybridSolver::solve, HybridSolver.cpp:860,878-892). The
+// substep kernels take a pointer to it (null for fixed-step runs): they read Dt from it and do
+// nothing once `skip` is set, which the last kernel of a substep does when the substep completed a
+// frame or the run, so that a replayed batch of substeps stops exactly there.
+struct Clock {
+  double t;          // simulated time, advanced by 1/60 at each frame (the reference's t)
+  double inner;      // time inside the current frame (innerFramet)
+  double dt;         // Dt of the current substep
+  double cfl, maxt, hmin;
+  float dt_f;        // Dt as the kernels use it
+  unsigned int vmax_bits;  // max |v| over nodes after the velocity update (float bits, atomicMax)
+  int frame;         // frames completed
+  int frame_now;     // the current substep completes a frame
+  int frame_done;    // a frame completed since the host last cleared it
+  int done;          // t > maxt
+  int skip;          // the rest of the replayed batch does nothing
+  int executed;      // substeps completed in the current batch
+  long long substeps;  // substeps completed in total
+};
+
+__device__ __forceinline__ bool clock_skip(const Clock* c) { return c != nullptr && c->skip != 0; }
+__device__ __forceinline__ float clock_dt(const Clock* c, float dt) { return c != nullptr ? c->dt_f : dt; }
+
 struct SortBufs {
   uint32_t *k1, *k2;
   int *v1, *v2;
@@ -46,6 +69,7 @@ struct SortBufs {
   uint32_t* tickets;  // [4]
   uint32_t* status;   // [3][nblocks][256]
   int* counters;      // zeroed by the histogram kernel
+  const Clock* clk;
 };
 
 struct ResetArgs {
@@ -61,6 +85,7 @@ struct ResetArgs {
   const int* active_blocks;
   int* block_slot;
   int64_t max_tiles;
+  const Clock* clk;
 };
 
 struct P2GArgs {
@@ -68,8 +93,7 @@ struct P2GArgs {
   const float* state;
   int64_t cap;
   int n;
-  const int* perm;  // caller ids in sorted order
-  const int* pos;   // caller id -> physical index in `state`
+  const int* src;   // sorted position -> physical index in `state`
   const int* cell_start;
   const int* cell_end;
   const int* counters;
@@ -78,6 +102,7 @@ struct P2GArgs {
   const float4* gvel;  // node velocities (force pass)
   float dt;
   unsigned long long* fallback_svd;
+  const Clock* clk;
 };
 
 struct GridArgs {
@@ -91,6 +116,8 @@ struct GridArgs {
   float4* gvel;
   int64_t ng;
   float dt;
+  const Clock* clk;
+  unsigned int* vmax_out;  // receives the max node speed after the velocity update (atomicMax of float bits)
 };
 
 struct G2PArgs {
@@ -100,7 +127,7 @@ struct G2PArgs {
   int64_t cap;
   int n;
   const int* perm;     // caller ids in sorted order
-  const int* pos_in;   // caller id -> physical index in sin
+  const int* src;      // sorted position -> physical index in sin
   int* pos_out;        // caller id -> physical index in sout
   const float4* gvel;
   float dt;
@@ -118,6 +145,7 @@ struct G2PArgs {
   float* cand;      // candidate records, 21 planes (sigma 3, U 9, V 9) of n floats
   int* cand_slot;   // sorted slot of each candidate
   int* cand_count;
+  const Clock* clk;
 };
 
 // Candidate planes of the surrogate return map.
@@ -137,6 +165,7 @@ struct SurrogateArgs {
   int write_stress;
   unsigned long long* accepted;
   unsigned long long* candidates;  // cumulative candidate count (block 0 adds this substep's)
+  const Clock* clk;
 };
 
 void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const int* pos, const float* thr,
@@ -144,8 +173,10 @@ void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap
 void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
 void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n);
+// Also writes src[j] = pos[perm[j]]: the physical index of the particle at sorted position j.
 void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
-                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters);
+                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
+                   const int* perm, const int* pos, int* src, const Clock* clk);
 void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
 void launch_g2p(hipStream_t s, const G2PArgs& a);
@@ -158,6 +189,10 @@ void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, cons
                         unsigned long long* out, int grid);
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4);
+// Dt from the max node speed (init: the first Dt, no frame bookkeeping), then the frame clamp.
+void launch_clock_update(hipStream_t s, Clock* c, int init);
+// End of a substep: count it, close a completed frame, stop the batch after a frame or the run.
+void launch_clock_end(hipStream_t s, Clock* c);
 void surrogate_set_smem_limit();
 void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid);
 

@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void k_reset(uint32_t* __restrict__ zero_words
                                                int* __restrict__ cell_start, int* __restrict__ cell_end,
                                                const int* __restrict__ counters, const int* __restrict__ active_tiles,
                                                int* __restrict__ tile_slot, const int* __restrict__ active_blocks,
-                                               int* __restrict__ block_slot) {
+                                               int* __restrict__ block_slot, const Clock* clk) {
+  if (clock_skip(clk)) return;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = t0; i < n_zero; i += stride) zero_words[i] = 0u;
@@ -118,7 +119,8 @@ __device__ __forceinline__ uint64_t digit_peers(bool valid, uint32_t d) {
 }
 
 __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ hist,
-                                                   int* __restrict__ counters) {
+                                                   int* __restrict__ counters, const Clock* clk) {
+  if (clock_skip(clk)) return;
   __shared__ uint32_t h[3][256];
   for (int i = threadIdx.x; i < 768; i += 256) (&h[0][0])[i] = 0u;
   if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
@@ -149,10 +151,13 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ 
   }
 }
 
+constexpr int LOOKBACK = 8;  // predecessor status words read per look-back step
+
 __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
                                                    uint32_t* __restrict__ kout, int* __restrict__ vout, int n, int pass,
                                                    const uint32_t* __restrict__ hist, uint32_t* __restrict__ tickets,
-                                                   uint32_t* __restrict__ status) {
+                                                   uint32_t* __restrict__ status, const Clock* clk) {
+  if (clock_skip(clk)) return;
   __shared__ uint32_t s_k[SORT_TILE];
   __shared__ int s_v[SORT_TILE];
   __shared__ uint32_t s_wh[4][256];
@@ -203,13 +208,30 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
       __hip_atomic_store(my, FLAG_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       __hip_atomic_store(my, FLAG_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      for (int bb = b - 1; bb >= 0; --bb) {
-        uint32_t sv;
-        do {
-          sv = __hip_atomic_load(status + (size_t)bb * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } while ((sv & ~VALUE_MASK) == 0u);
-        excl += sv & VALUE_MASK;
-        if ((sv & ~VALUE_MASK) == FLAG_PREFIX) break;
+      // look back over the predecessors LOOKBACK at a time (independent loads), summing published
+      // aggregates nearest first until an inclusive prefix is found; an unpublished entry is
+      // polled again. Every block publishes its aggregate before it looks back, so the wait ends.
+      int bb = b - 1;
+      while (true) {
+        uint32_t sv[LOOKBACK];
+#pragma unroll
+        for (int j = 0; j < LOOKBACK; ++j)
+          sv[j] = bb - j >= 0 ? __hip_atomic_load(status + (size_t)(bb - j) * 256 + d, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : FLAG_PREFIX;
+        int used = 0;
+        bool found = false;
+#pragma unroll
+        for (int j = 0; j < LOOKBACK; ++j) {
+          if (found || used < j) continue;           // stop at the first prefix or unpublished entry
+          const uint32_t fl = sv[j] & ~VALUE_MASK;
+          if (fl == 0u) continue;
+          excl += sv[j] & VALUE_MASK;
+          used = j + 1;
+          if (fl == FLAG_PREFIX) found = true;
+        }
+        if (found) break;
+        bb -= used;
       }
       __hip_atomic_store(my, FLAG_PREFIX | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -266,9 +288,11 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 // start and flags the cell's tile (plain stores of one value), the last position writes the end.
 __global__ __launch_bounds__(256) void k_bounds(DevParams P, const uint32_t* __restrict__ ks, int n,
                                                 int* __restrict__ cell_start, int* __restrict__ cell_end,
-                                                int* __restrict__ tile_slot) {
+                                                int* __restrict__ tile_slot, const int* __restrict__ perm,
+                                                const int* __restrict__ pos, int* __restrict__ src, const Clock* clk) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= n || clock_skip(clk)) return;
+  src[j] = clamp_index(pos[clamp_index(perm[j], n)], n);
   const uint32_t k = ks[j];
   if (j == n - 1 || ks[j + 1] != k) cell_end[k] = j + 1;
   if (j != 0 && ks[j - 1] == k) return;
@@ -285,7 +309,8 @@ __global__ __launch_bounds__(256) void k_bounds(DevParams P, const uint32_t* __r
 template <int MODE>
 __global__ __launch_bounds__(256) void k_claim(DevParams P, int64_t count, int* __restrict__ slot_of,
                                                int* __restrict__ active, int* __restrict__ counter,
-                                               int* __restrict__ block_slot) {
+                                               int* __restrict__ block_slot, const Clock* clk) {
+  if (clock_skip(clk)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool flagged = t < count && slot_of[t] == -2;
   const uint64_t mask = __ballot(flagged);
@@ -312,16 +337,31 @@ __global__ __launch_bounds__(256) void k_claim(DevParams P, int64_t count, int* 
 }
 
 // ================================================================== P2G
-// One workgroup (4 waves) per active 4x4x4-cell tile. Wave w owns cell layer z = w; lane l owns
-// cell (l>>2 & 3, l>>4) of that layer and, for accumulation, the node slab zo = l & 3 of the cell's
-// 4x4x4 node window (nodes c-1 .. c+2, a superset of the quadratic stencil and exactly the cubic one).
-// The cell's particles are staged into LDS four at a time (lane (cell, r) stages rank 4k + r) with
-// their payload computed once per particle; each lane then accumulates its 16 window nodes of its
-// cell's particles in registers, in ascending rank. Lanes write to the tile's LDS node array in
-// fixed phases (every address is written by at most one lane per instruction; waves take turns),
-// and the array is stored as the tile's partial. Node values are therefore deterministic sums in a
-// fixed order, without float atomics on global memory.
-constexpr int PAY = 28;  // payload floats per staged particle (7 x float4)
+// One workgroup (4 waves) per active 4x4x4-cell tile; wave w owns the cell layer z = w and walks
+// its four cell rows (y) one after another, with no workgroup barrier until the tile is done.
+//
+// Row pass. The four cells of a row (x = 0..3) are contiguous in the sorted order. In chunk k,
+// lane (cx, r), r < 8, stages the particle of rank 8k + r of cell cx: it loads the particle once
+// and writes a 48-float payload (B-spline weights and derivatives of the window's 4 nodes per axis,
+// mass, affine momentum terms, stress impulse) into the wave's LDS slots. Then lane (cx, zo, oy) adds the chunk's
+// particles of cell cx, in ascending rank, to its 4 register sums: nodes (cx-1+ox, cy-1+oy,
+// w-1+zo), ox = 0..3, of the cell's 4x4x4 node window c-1 .. c+2 (a superset of the quadratic
+// stencil and exactly the cubic one). At the end of the row the sums are added into the wave's
+// own LDS node slab (7 x 7 x 4 nodes) in four phases ox = 0..3; within a phase every lane has a
+// different node, so the slab receives the row sums in a fixed order.
+//
+// When all waves are done, node (X, Y, Z) of the tile's 7x7x7 footprint is the sum of the slabs of
+// the waves w with 0 <= Z - w <= 3, in ascending w, and is stored as the tile's partial. k_grid sums
+// the covering tiles' partials in ascending tile order: every node value is a sum in a fixed order,
+// without float atomics on global memory, and reruns are bitwise identical.
+constexpr int PAY = 48;                 // payload floats per staged particle (12 x float4)
+constexpr int P2G_RANKS = 8;            // ranks per cell staged per chunk
+constexpr int PAY_CELL = P2G_RANKS * PAY + 4;  // one cell's staged payloads (+1 float4 against bank conflicts)
+constexpr int SLAB = FOOT * FOOT * 4;   // nodes of a wave's slab
+// payload layout: weights and derivatives (per cell width) of window nodes c-1 .. c+2 on each axis
+constexpr int PY_WX = 0, PY_DWX = 4, PY_WY = 8, PY_DWY = 12, PY_WZ = 16, PY_DWZ = 20;
+constexpr int PY_M = 24, PY_Q0 = 25, PY_Q = 28, PY_S = 37;  // momentum / stress passes
+constexpr int PY_SF = 24;                                    // force pass: 9 floats of the stress matrix
 
 template <int ORDER>
 __device__ __forceinline__ void axis_weights(float f, float* w, float* dw) {
@@ -330,215 +370,236 @@ __device__ __forceinline__ void axis_weights(float f, float* w, float* dw) {
 }
 
 template <int ORDER, int MODE>
+__device__ __forceinline__ void p2g_stage(const P2GArgs& a, float dt, int idx, int g0, int g1, int g2, float4* dst) {
+  const DevParams& P = a.P;
+  float pay[PAY];
+#pragma unroll
+  for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
+  const int pi = clamp_index(a.src[idx], a.n);
+  const float* st = a.state;
+  const int64_t cap = a.cap;
+  float xp[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) xp[d] = st[(PX + d) * cap + pi];
+  const int cc[3] = {g0, g1, g2};
+  float f[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) f[d] = (xp[d] - P.gmin[d]) * P.inv_h[d] - (float)cc[d];
+  axis_weights<ORDER>(f[0], &pay[PY_WX], &pay[PY_DWX]);
+  axis_weights<ORDER>(f[1], &pay[PY_WY], &pay[PY_DWY]);
+  axis_weights<ORDER>(f[2], &pay[PY_WZ], &pay[PY_DWZ]);
+  if constexpr (MODE != P2G_FORCE) {
+    // momentum w*(q0 + Q o): Q = m D^-1 B diag(h), q0 = m v - Q (1 + f)  (HybridSolver.cpp:134,169-170)
+    const float mp = st[PM * cap + pi];
+    float Q[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Q[3 * r + c] = mp * P.dinv * st[(PB + 3 * r + c) * cap + pi] * P.h[c];
+    pay[PY_M] = mp;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      pay[PY_Q0 + r] = mp * st[(PV + r) * cap + pi] - (Q[3 * r] * (1.0f + f[0]) + Q[3 * r + 1] * (1.0f + f[1]) + Q[3 * r + 2] * (1.0f + f[2]));
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pay[PY_Q + i] = Q[i];
+  }
+  if constexpr (MODE == P2G_MOM_STRESS) {
+    // stress at F^n: the Kirchhoff stress V P(F) F^T written by the previous G2P (or by
+    // k_stress_init), columns scaled by dt/h so that the impulse folds into the momentum
+    const int R[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};  // symmetric 3x3 from (xx yy zz xy xz yz)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pay[PY_S + 3 * r + c] = dt * st[(PK + R[3 * r + c]) * cap + pi] * P.inv_h[c];
+  }
+  if constexpr (MODE == P2G_FORCE) {
+    // stress matrix V P(F^) F^n^T at the predicted F^ = F + dt (sum_i v_i grad w_ip^T) F from the
+    // post-P2G grid velocity (HybridSolver.cpp:269-339), columns scaled by 1/h
+    float Fn[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+    const float* wx = &pay[PY_WX]; const float* dx = &pay[PY_DWX];
+    const float* wy = &pay[PY_WY]; const float* dy = &pay[PY_DWY];
+    const float* wz = &pay[PY_WZ]; const float* dz = &pay[PY_DWZ];
+    float G[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) G[i] = 0.0f;
+    for (int oz = 0; oz < 4; ++oz) {
+      int n2 = g2 - 1 + oz;
+      if (n2 < 0 || n2 >= P.res[2]) continue;
+      for (int oy = 0; oy < 4; ++oy) {
+        int n1 = g1 - 1 + oy;
+        if (n1 < 0 || n1 >= P.res[1]) continue;
+        for (int ox = 0; ox < 4; ++ox) {
+          int n0 = g0 - 1 + ox;
+          if (n0 < 0 || n0 >= P.res[0]) continue;
+          float gw[3] = {dx[ox] * wy[oy] * wz[oz] * P.inv_h[0], wx[ox] * dy[oy] * wz[oz] * P.inv_h[1],
+                         wx[ox] * wy[oy] * dz[oz] * P.inv_h[2]};
+          int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+          const float4 gv4 = a.gvel[2 * node];
+          const float vr[3] = {gv4.x, gv4.y, gv4.z};
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) G[3 * r + c] += vr[r] * gw[c];
+        }
+      }
+    }
+    float GF[9], Fh[9];
+    mat_mul(G, Fn, GF);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + dt * GF[i];
+    float Jp = st[PPL * cap + pi];
+    float vol = st[PVOL * cap + pi];
+    float S[9];
+    int fb = 0;
+    stress_matrix(P, Fh, Fn, Jp, vol, S, fb);
+    if (fb) atomicAdd(a.fallback_svd, (unsigned long long)fb);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pay[PY_SF + 3 * r + c] = S[3 * r + c] * P.inv_h[c];
+  }
+#pragma unroll
+  for (int i = 0; i < PAY / 4; ++i) dst[i] = make_float4(pay[4 * i], pay[4 * i + 1], pay[4 * i + 2], pay[4 * i + 3]);
+}
+
+template <int ORDER, int MODE>
 __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
-  __shared__ float4 s_pay[4][64][PAY / 4];
-  __shared__ float s_acc[4][FOOT3];
-  __shared__ int s_max;
+  if (clock_skip(a.clk)) return;
+  const float dt = clock_dt(a.clk, a.dt);
+  __shared__ float4 s_pay[4][4 * PAY_CELL / 4];
+  __shared__ float s_slab[4][4][SLAB];  // [wave][quantity][node]
   const DevParams& P = a.P;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int cxy = lane >> 2, sub = lane & 3;
-  const int cx = cxy & 3, cy = cxy >> 2;
   const int ntile = a.counters[CNT_TILES];
   constexpr int NQ = (MODE == P2G_FORCE) ? 3 : 4;
+  // staging role: lane (cx, sr), sr < 8; accumulation role: lane (cx, zo, oy)
+  const int cx = lane >> 4, sr = lane & 15;
+  const int zo = (lane >> 2) & 3, oy = lane & 3;
+  float* slab = &s_slab[w][0][0];
+  for (int i = lane; i < 4 * SLAB; i += 64) slab[i] = 0.0f;
   for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
     const int tile = a.active_tiles[slot];
     const int t0 = tile % P.tiles[0], t1 = (tile / P.tiles[0]) % P.tiles[1], t2 = tile / (P.tiles[0] * P.tiles[1]);
-    for (int i = tid; i < 4 * FOOT3; i += 256) (&s_acc[0][0])[i] = 0.0f;
-    if (tid == 0) s_max = 0;
-    __syncthreads();
-    const int g0 = 4 * t0 + cx, g1 = 4 * t1 + cy, g2 = 4 * t2 + w;
-    int start = 0, cnt = 0;
-    if (g0 < P.res[0] && g1 < P.res[1] && g2 < P.res[2]) {
-      int key = (g2 * P.res[1] + g1) * P.res[0] + g0;
-      start = a.cell_start[key];
-      cnt = a.cell_end[key] - start;
-    }
-    if (sub == 0 && cnt > 0) atomicMax(&s_max, cnt);
-    float acc[16][NQ];
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0f;
-    __syncthreads();
-    const int nch = (s_max + 3) >> 2;
-    for (int ch = 0; ch < nch; ++ch) {
-      // ---- stage: lane (cell, sub) prepares rank 4*ch + sub of its cell
-      {
-        float pay[PAY];
-#pragma unroll
-        for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
-        const int rank = 4 * ch + sub;
-        if (rank < cnt) {
-          const int pi = clamp_index(a.pos[clamp_index(a.perm[start + rank], a.n)], a.n);
-          const float* st = a.state;
-          const int64_t cap = a.cap;
-          float xp[3];
-#pragma unroll
-          for (int d = 0; d < 3; ++d) xp[d] = st[(PX + d) * cap + pi];
-          const int cc[3] = {g0, g1, g2};
-          float f[3];
-#pragma unroll
-          for (int d = 0; d < 3; ++d) f[d] = (xp[d] - P.gmin[d]) * P.inv_h[d] - (float)cc[d];
-          pay[0] = f[0]; pay[1] = f[1]; pay[2] = f[2];
-          const float mp = st[PM * cap + pi];
-          if constexpr (MODE != P2G_FORCE) {
-            // momentum w*(q0 + Q o): Q = m D^-1 B diag(h), q0 = m v - Q (1 + f)  (HybridSolver.cpp:134,169-170)
-            float Q[9];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) Q[3 * r + c] = mp * P.dinv * st[(PB + 3 * r + c) * cap + pi] * P.h[c];
-            pay[3] = mp;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-              pay[4 + r] = mp * st[(PV + r) * cap + pi] - (Q[3 * r] * (1.0f + f[0]) + Q[3 * r + 1] * (1.0f + f[1]) + Q[3 * r + 2] * (1.0f + f[2]));
-#pragma unroll
-            for (int i = 0; i < 9; ++i) pay[7 + i] = Q[i];
-          }
-          if constexpr (MODE == P2G_MOM_STRESS) {
-            // stress at F^n: the Kirchhoff stress V P(F) F^T written by the previous G2P (or by
-            // k_stress_init), columns scaled by dt/h so that the impulse folds into the momentum
-            const int R[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};  // symmetric 3x3 from (xx yy zz xy xz yz)
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) pay[16 + 3 * r + c] = a.dt * st[(PK + R[3 * r + c]) * cap + pi] * P.inv_h[c];
-          }
-          if constexpr (MODE == P2G_FORCE) {
-            // stress matrix V P(F^) F^n^T at the predicted F^ (HybridSolver.cpp:269-339), columns
-            // scaled by 1/h
-            float Fn[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
-            float Fh[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Fh[i] = Fn[i];
-            if constexpr (MODE == P2G_FORCE) {
-              // F^ = F + dt (sum_i v_i grad w_ip^T) F from the post-P2G grid velocity (:269-306)
-              float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
-              axis_weights<ORDER>(f[0], wx, dx);
-              axis_weights<ORDER>(f[1], wy, dy);
-              axis_weights<ORDER>(f[2], wz, dz);
-              float G[9];
-#pragma unroll
-              for (int i = 0; i < 9; ++i) G[i] = 0.0f;
-              for (int oz = 0; oz < 4; ++oz) {
-                int n2 = g2 - 1 + oz;
-                if (n2 < 0 || n2 >= P.res[2]) continue;
-                for (int oy = 0; oy < 4; ++oy) {
-                  int n1 = g1 - 1 + oy;
-                  if (n1 < 0 || n1 >= P.res[1]) continue;
-                  for (int ox = 0; ox < 4; ++ox) {
-                    int n0 = g0 - 1 + ox;
-                    if (n0 < 0 || n0 >= P.res[0]) continue;
-                    float gw[3] = {dx[ox] * wy[oy] * wz[oz] * P.inv_h[0], wx[ox] * dy[oy] * wz[oz] * P.inv_h[1],
-                                   wx[ox] * wy[oy] * dz[oz] * P.inv_h[2]};
-                    int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
-                    const float4 gv4 = a.gvel[2 * node];
-                    const float vr[3] = {gv4.x, gv4.y, gv4.z};
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                      for (int c = 0; c < 3; ++c) G[3 * r + c] += vr[r] * gw[c];
-                  }
-                }
-              }
-              float GF[9];
-              mat_mul(G, Fn, GF);
-#pragma unroll
-              for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + a.dt * GF[i];
-            }
-            float Jp = st[PPL * cap + pi];
-            float vol = st[PVOL * cap + pi];
-            float S[9];
-            int fb = 0;
-            stress_matrix(P, Fh, Fn, Jp, vol, S, fb);
-            if (fb) atomicAdd(a.fallback_svd, (unsigned long long)fb);
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) pay[3 + 3 * r + c] = S[3 * r + c] * P.inv_h[c];
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < PAY / 4; ++i) s_pay[w][lane][i] = make_float4(pay[4 * i], pay[4 * i + 1], pay[4 * i + 2], pay[4 * i + 3]);
+    const int g2 = 4 * t2 + w;
+    for (int cy = 0; cy < 4; ++cy) {
+      const int g1 = 4 * t1 + cy;
+      const int g0 = 4 * t0 + cx;  // this lane's cell of the row (both roles)
+      int start = 0, cnt = 0;
+      if (g0 < P.res[0] && g1 < P.res[1] && g2 < P.res[2]) {
+        const int key = (g2 * P.res[1] + g1) * P.res[0] + g0;
+        start = a.cell_start[key];
+        cnt = a.cell_end[key] - start;
       }
-      __syncthreads();
-      // ---- accumulate: lane (cell, zo) over the staged particles of its cell
-      const int zo = sub;
-      for (int r2 = 0; r2 < 4; ++r2) {
-        if (4 * ch + r2 >= cnt) break;
-        float pay[PAY];
+      int cmax = max(max(__builtin_amdgcn_readlane(cnt, 0), __builtin_amdgcn_readlane(cnt, 16)),
+                     max(__builtin_amdgcn_readlane(cnt, 32), __builtin_amdgcn_readlane(cnt, 48)));
+      if (cmax == 0) continue;
+      float acc[4][NQ];
 #pragma unroll
-        for (int i = 0; i < PAY / 4; ++i) {
-          float4 t = s_pay[w][cxy * 4 + r2][i];
-          pay[4 * i] = t.x; pay[4 * i + 1] = t.y; pay[4 * i + 2] = t.z; pay[4 * i + 3] = t.w;
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0f;
+      for (int k = 0; k * P2G_RANKS < cmax; ++k) {
+        // ---- stage ranks 8k .. 8k+7 of each cell of the row
+        if (sr < P2G_RANKS) {
+          const int rank = P2G_RANKS * k + sr;
+          if (rank < cnt)
+            p2g_stage<ORDER, MODE>(a, dt, start + rank, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
         }
-        float wx[4], dwx[4], wy[4], dwy[4];
-        axis_weights<ORDER>(pay[0], wx, dwx);
-        axis_weights<ORDER>(pay[1], wy, dwy);
-        float wz, dwz;
-        spline<ORDER>(pay[2] + 1.0f - (float)zo, wz, dwz);
-        const float* S = (MODE == P2G_FORCE) ? &pay[3] : &pay[16];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // ---- accumulate this lane's cell, ascending rank
+        const int nr = min(P2G_RANKS, cnt - P2G_RANKS * k);
+        for (int r = 0; r < nr; ++r) {
+          const float4* src = &s_pay[w][(cx * PAY_CELL + r * PAY) / 4];
+          float pay[PAY];
 #pragma unroll
-        for (int oy = 0; oy < 4; ++oy) {
-          const float wyz = wy[oy] * wz, dyz = dwy[oy] * wz, ydz = wy[oy] * dwz;
+          for (int i = 0; i < 2; ++i) {  // x weights and derivatives
+            const float4 t = src[i];
+            pay[4 * i] = t.x; pay[4 * i + 1] = t.y; pay[4 * i + 2] = t.z; pay[4 * i + 3] = t.w;
+          }
+#pragma unroll
+          for (int i = 6; i < PAY / 4; ++i) {  // mass, momentum and stress terms
+            const float4 t = src[i];
+            pay[4 * i] = t.x; pay[4 * i + 1] = t.y; pay[4 * i + 2] = t.z; pay[4 * i + 3] = t.w;
+          }
+          // this lane's y and z node weights
+          const float* pf = reinterpret_cast<const float*>(src);
+          const float wy = pf[PY_WY + oy], dwy = pf[PY_DWY + oy], wz = pf[PY_WZ + zo], dwz = pf[PY_DWZ + zo];
+          const float wyz = wy * wz, dyz = dwy * wz, ydz = wy * dwz;
+          const float* S = (MODE == P2G_FORCE) ? &pay[PY_SF] : &pay[PY_S];
           float e[3], dv[3], av[3], mw = 0.0f;
           if constexpr (MODE != P2G_FORCE) {
-            mw = pay[3] * wyz;
+            mw = pay[PY_M] * wyz;
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-              float qrow = pay[4 + r] + pay[7 + 3 * r + 1] * (float)oy + pay[7 + 3 * r + 2] * (float)zo;
-              e[r] = wyz * qrow;
-              dv[r] = wyz * pay[7 + 3 * r];
+            for (int rr = 0; rr < 3; ++rr) {
+              const float qrow = pay[PY_Q0 + rr] + pay[PY_Q + 3 * rr + 1] * (float)oy + pay[PY_Q + 3 * rr + 2] * (float)zo;
+              e[rr] = wyz * qrow;
+              dv[rr] = wyz * pay[PY_Q + 3 * rr];
             }
           } else {
 #pragma unroll
-            for (int r = 0; r < 3; ++r) { e[r] = 0.0f; dv[r] = 0.0f; }
+            for (int rr = 0; rr < 3; ++rr) { e[rr] = 0.0f; dv[rr] = 0.0f; }
           }
           if constexpr (MODE != P2G_MOM) {
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-              e[r] -= S[3 * r + 1] * dyz + S[3 * r + 2] * ydz;
-              av[r] = S[3 * r] * wyz;
+            for (int rr = 0; rr < 3; ++rr) {
+              e[rr] -= S[3 * rr + 1] * dyz + S[3 * rr + 2] * ydz;
+              av[rr] = S[3 * rr] * wyz;
             }
           } else {
 #pragma unroll
-            for (int r = 0; r < 3; ++r) av[r] = 0.0f;
+            for (int rr = 0; rr < 3; ++rr) av[rr] = 0.0f;
           }
 #pragma unroll
           for (int ox = 0; ox < 4; ++ox) {
-            const int ni = oy * 4 + ox;
-            const float wxo = wx[ox] * (float)ox;
+            const float wx = pay[PY_WX + ox], dwx = pay[PY_DWX + ox], wxo = wx * (float)ox;
             if constexpr (MODE != P2G_FORCE) {
-              acc[ni][0] += wx[ox] * mw;
+              acc[ox][0] += wx * mw;
 #pragma unroll
-              for (int r = 0; r < 3; ++r) acc[ni][1 + r] += wx[ox] * e[r] + wxo * dv[r] - dwx[ox] * av[r];
+              for (int rr = 0; rr < 3; ++rr) acc[ox][1 + rr] += wx * e[rr] + wxo * dv[rr] - dwx * av[rr];
             } else {
 #pragma unroll
-              for (int r = 0; r < 3; ++r) acc[ni][r] += wx[ox] * e[r] - dwx[ox] * av[r];
+              for (int rr = 0; rr < 3; ++rr) acc[ox][rr] += wx * e[rr] - dwx * av[rr];
             }
           }
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
-      __syncthreads();
-    }
-    // ---- write the lane sums into the tile's node array: waves in turn, 16 conflict-free phases each
-    for (int turn = 0; turn < 4; ++turn) {
-      if (w == turn) {
+      // ---- row sums into the wave's slab: phase ox, node (cx + ox, cy + oy, zo)
 #pragma unroll
-        for (int oy = 0; oy < 4; ++oy)
+      for (int ox = 0; ox < 4; ++ox) {
+        const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * zo;
 #pragma unroll
-          for (int ox = 0; ox < 4; ++ox) {
-            const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * (w + sub);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) atomicAdd(&s_acc[q][ln], acc[oy * 4 + ox][q]);
-          }
+        for (int q = 0; q < NQ; ++q) atomicAdd(&slab[q * SLAB + ln], acc[ox][q]);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
-      __syncthreads();
     }
+    __syncthreads();
+    // ---- tile partial: node (X, Y, Z) = sum over waves w, 0 <= Z - w <= 3, ascending; slabs cleared
     float* part = a.partials + (size_t)slot * PART_STRIDE + (MODE == P2G_FORCE ? 4 * FOOT3 : 0);
-    for (int i = tid; i < NQ * FOOT3; i += 256) part[i] = (&s_acc[0][0])[i];
+    for (int i = tid; i < FOOT3; i += 256) {
+      const int X = i % FOOT, Y = (i / FOOT) % FOOT, Z = i / (FOOT * FOOT);
+      float s[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) s[q] = 0.0f;
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) {
+        const int z = Z - ww;
+        if (z < 0 || z > 3) continue;
+        const int ln = X + FOOT * Y + FOOT * FOOT * z;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          s[q] += s_slab[ww][q][ln];
+          s_slab[ww][q][ln] = 0.0f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = s[q];
+    }
     __syncthreads();
   }
 }
@@ -549,8 +610,11 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
 // v = p/m (HybridSolver.cpp:233-240), v += dt (f/m + g) (:457, :725-737) and collision (:460-511).
 template <int MODE>
 __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
+  if (clock_skip(a.clk)) return;
+  const float dt = clock_dt(a.clk, a.dt);
   const DevParams& P = a.P;
   const int nblk = a.counters[CNT_BLOCKS];
+  float vmax = 0.0f;  // max |v| over this lane's nodes after the velocity update
   const int lx = threadIdx.x & 3, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x >> 4;
   for (int s = blockIdx.x; s < nblk; s += gridDim.x) {
     const int blk = a.active_blocks[s];
@@ -587,7 +651,7 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
       v[0] = g4.x; v[1] = g4.y; v[2] = g4.z;
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        if (m > 0.0f) v[d] += a.dt * (sum[d] / m + P.gravity[d]);
+        if (m > 0.0f) v[d] += dt * (sum[d] / m + P.gravity[d]);
     } else {
       const float m = sum[0];
       a.gm[node] = m;
@@ -595,9 +659,10 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
       for (int d = 0; d < 3; ++d) {
         a.gmom[d * a.ng + node] = sum[1 + d];
         v[d] = m > 0.0f ? sum[1 + d] / m : 0.0f;
-        if (MODE == GRID_FUSED && m > 0.0f) v[d] += a.dt * P.gravity[d];
+        if (MODE == GRID_FUSED && m > 0.0f) v[d] += dt * P.gravity[d];
       }
     }
+    vmax = fmaxf(vmax, sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
     if constexpr (MODE == GRID_NORMALIZE) {
       vbf[0] = v[0]; vbf[1] = v[1]; vbf[2] = v[2];
     } else {
@@ -606,6 +671,54 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
     }
     a.gvel[2 * node] = make_float4(v[0], v[1], v[2], vbf[0]);
     a.gvel[2 * node + 1] = make_float4(vbf[1], vbf[2], 0.0f, 0.0f);
+  }
+  if (a.vmax_out != nullptr) {
+    // max over the wave, one atomic per wave (non-negative floats order as their bit patterns)
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+    if (threadIdx.x == 0 && vmax > 0.0f) atomicMax(a.vmax_out, __float_as_uint(vmax));
+  }
+}
+
+// ================================================================== adaptive time step
+// Dt = CFL / max(300, max|v| / h_min) (HybridSolver.cpp:860,878; RegularGrid.cpp:188-200,
+// RegularGrid.h:60), evaluated in double as in the reference from the max node speed after the
+// velocity update; then the frame clamp to 1/60 s (:880-892): a substep that reaches the frame end
+// is shortened to it, t advances by 1/60 and the frame time restarts.
+__global__ void k_clock_update(Clock* c, int init) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || c->skip) return;
+  const double vmax = (double)__uint_as_float(c->vmax_bits);
+  c->vmax_bits = 0u;
+  const double rate = vmax / c->hmin;
+  double Dt = c->cfl / (rate > 300.0 ? rate : 300.0);
+  if (!init) {
+    if (c->inner + Dt >= 1.0 / 60.0) {
+      Dt = 1.0 / 60.0 - c->inner;
+      c->t += 1.0 / 60.0;
+      c->inner = 0.0;
+      c->frame_now = 1;
+    } else {
+      c->inner += Dt;
+    }
+  }
+  c->dt = Dt;
+  c->dt_f = (float)Dt;
+}
+
+// End of a substep: the reference closes a frame after the substep that completed it (output at
+// HybridSolver.cpp:990-1029) and leaves its loop when t > maxt (:867).
+__global__ void k_clock_end(Clock* c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || c->skip) return;
+  c->executed += 1;
+  c->substeps += 1;
+  if (c->frame_now) {
+    c->frame_now = 0;
+    c->frame += 1;
+    c->frame_done = 1;
+    c->skip = 1;
+  }
+  if (c->t > c->maxt) {
+    c->done = 1;
+    c->skip = 1;
   }
 }
 
@@ -617,12 +730,14 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
 // physically sorted by cell. It also records where pid now lives and pid's next cell key.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
+  if (clock_skip(a.clk)) return;
+  const float dt = clock_dt(a.clk, a.dt);
   const DevParams& P = a.P;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
   if (j < a.n) {
     const int pid = clamp_index(a.perm[j], a.n);
-    const int pi = clamp_index(a.pos_in[pid], a.n);
+    const int pi = clamp_index(a.src[j], a.n);
     const float* st = a.sin;
     const int64_t cap = a.cap;
     float xp[3], g[3];
@@ -697,7 +812,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
     mat_mul(Gv, Fn, GF);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + a.dt * GF[i];
+    for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + dt * GF[i];
     float plastic = st[PPL * cap + pi];
     const float phi_p = st[PPHI * cap + pi];
     const float vol = st[PVOL * cap + pi];
@@ -735,7 +850,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     float xn[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      xn[d] = xp[d] + a.dt * vb[d];
+      xn[d] = xp[d] + dt * vb[d];
       if (fabsf(xn[d] - xp[d]) > P.h[d]) cfl = 1;
     }
     float* so = a.sout;
@@ -783,6 +898,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
 // prediction reproduces the exact path bit for bit.
 __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
 #pragma clang fp contract(off)
+  if (clock_skip(a.clk)) return;
   extern __shared__ __align__(16) unsigned char smem[];
   MLPShared& sh = *reinterpret_cast<MLPShared*>(smem);
   uint16_t* acts = reinterpret_cast<uint16_t*>(smem + sizeof(MLPShared));
@@ -984,13 +1100,14 @@ void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t ca
 
 void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n) {
   const int nb = blocks_for(n, SORT_TILE);
-  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, b.hist, b.counters);
+  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, b.hist, b.counters,
+                     b.clk);
   hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, keys_in, (const int*)nullptr, b.k1, b.v1, n, 0, b.hist,
-                     b.tickets, b.status);
+                     b.tickets, b.status, b.clk);
   hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k1, (const int*)b.v1, b.k2, b.v2, n, 1,
-                     b.hist, b.tickets, b.status + (size_t)nb * 256);
+                     b.hist, b.tickets, b.status + (size_t)nb * 256, b.clk);
   hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k2, (const int*)b.v2, b.k1, b.v1, n, 2,
-                     b.hist, b.tickets, b.status + (size_t)2 * nb * 256);
+                     b.hist, b.tickets, b.status + (size_t)2 * nb * 256, b.clk);
 }
 
 void launch_reset(hipStream_t s, const ResetArgs& r) {
@@ -1000,17 +1117,19 @@ void launch_reset(hipStream_t s, const ResetArgs& r) {
   if (grid > 4096) grid = 4096;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, r.zero_words, r.n_zero, r.prev_keys, r.n, r.cell_start,
-                     r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot);
+                     r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot, r.clk);
 }
 
 void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
-                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters) {
+                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
+                   const int* perm, const int* pos, int* src, const Clock* clk) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, n, cell_start, cell_end, tile_slot);
+  hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, n, cell_start, cell_end, tile_slot,
+                     perm, pos, src, clk);
   hipLaunchKernelGGL(k_claim<0>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, P, ntiles, tile_slot, active_tiles,
-                     counters + CNT_TILES, block_slot);
+                     counters + CNT_TILES, block_slot, clk);
   hipLaunchKernelGGL(k_claim<1>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, P, ntiles, block_slot, active_blocks,
-                     counters + CNT_BLOCKS, (int*)nullptr);
+                     counters + CNT_BLOCKS, (int*)nullptr, clk);
 }
 
 void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid) {
@@ -1026,6 +1145,12 @@ void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid) {
     else hipLaunchKernelGGL((k_p2g<3, P2G_FORCE>), dim3(grid), dim3(256), 0, s, a);
   }
 }
+
+void launch_clock_update(hipStream_t s, Clock* c, int init) {
+  hipLaunchKernelGGL(k_clock_update, dim3(1), dim3(64), 0, s, c, init);
+}
+
+void launch_clock_end(hipStream_t s, Clock* c) { hipLaunchKernelGGL(k_clock_end, dim3(1), dim3(64), 0, s, c); }
 
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid) {
   if (mode == GRID_FUSED) hipLaunchKernelGGL(k_grid<GRID_FUSED>, dim3(grid), dim3(64), 0, s, a);

@@ -62,6 +62,7 @@ struct mpm_ctx {
   int* pos[2] = {nullptr, nullptr};  // caller id -> physical index in state[b]
   uint32_t *k1 = nullptr, *k2 = nullptr;
   int *v1 = nullptr, *v2 = nullptr;
+  int* src = nullptr;  // sorted position -> physical index of the particle in state[cur]
   uint32_t* sortws = nullptr;
   int64_t sortws_words = 0;
   int nb_sort = 0;
@@ -80,6 +81,11 @@ struct mpm_ctx {
   float graph_dt[2] = {0.0f, 0.0f};
 
   int p2g_grid = 1024;
+  // adaptive time step (mpm_solve): the device clock, and the pointer the kernels get (null for
+  // fixed-step runs)
+  Clock* clock = nullptr;
+  Clock* clk_active = nullptr;
+  hipGraphExec_t graph_adapt[2] = {nullptr, nullptr};
   int grid_grid = 2048;
   int n_cu = 256;
 
@@ -136,11 +142,16 @@ static void dfree(T*& p) {
 }
 
 static void drop_graphs(mpm_ctx* ctx) {
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
     if (ctx->graph[i]) {
       (void)hipGraphExecDestroy(ctx->graph[i]);
       ctx->graph[i] = nullptr;
     }
+    if (ctx->graph_adapt[i]) {
+      (void)hipGraphExecDestroy(ctx->graph_adapt[i]);
+      ctx->graph_adapt[i] = nullptr;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ parameters
@@ -209,6 +220,7 @@ static int stage_sort(mpm_ctx* ctx) {
   r.active_blocks = ctx->active_blocks;
   r.block_slot = ctx->block_slot;
   r.max_tiles = ctx->ntiles;
+  r.clk = ctx->clk_active;
   launch_reset(s, r);
   SortBufs b;
   b.k1 = ctx->k1;
@@ -219,9 +231,11 @@ static int stage_sort(mpm_ctx* ctx) {
   b.tickets = ctx->sortws + 768;
   b.status = ctx->sortws + 772;
   b.counters = ctx->counters;
+  b.clk = ctx->clk_active;
   launch_sort(s, b, ctx->keys, ctx->n);
   launch_bounds(s, ctx->dp, ctx->k1, ctx->n, ctx->ntiles, ctx->cell_start, ctx->cell_end, ctx->tile_slot,
-                ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters);
+                ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters, ctx->v1, ctx->pos[ctx->cur],
+                ctx->src, ctx->clk_active);
   ctx->sort_buf = ctx->cur;
   return MPM_OK;
 }
@@ -232,8 +246,7 @@ static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   a.state = ctx->state[ctx->cur];
   a.cap = ctx->cap;
   a.n = ctx->n;
-  a.perm = ctx->v1;
-  a.pos = ctx->pos[ctx->cur];
+  a.src = ctx->src;
   a.cell_start = ctx->cell_start;
   a.cell_end = ctx->cell_end;
   a.counters = ctx->counters;
@@ -242,6 +255,7 @@ static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
   a.fallback_svd = ctx->fallbacks;
+  a.clk = ctx->clk_active;
   return a;
 }
 
@@ -257,6 +271,8 @@ static GridArgs grid_args(mpm_ctx* ctx, float dt) {
   g.gvel = reinterpret_cast<float4*>(ctx->grid + GRID_VEL * ctx->ng);
   g.ng = ctx->ng;
   g.dt = dt;
+  g.clk = ctx->clk_active;
+  g.vmax_out = nullptr;
   return g;
 }
 
@@ -278,7 +294,11 @@ static int stage_p2g(mpm_ctx* ctx, float dt) {
 }
 
 static int stage_grid(mpm_ctx* ctx, float dt) {
-  launch_grid(ctx->stream, grid_args(ctx, dt), ctx->dp.predicted ? GRID_FORCE_UPDATE : GRID_FUSED, ctx->grid_grid);
+  GridArgs g = grid_args(ctx, dt);
+  if (ctx->clk_active) g.vmax_out = &ctx->clk_active->vmax_bits;
+  launch_grid(ctx->stream, g, ctx->dp.predicted ? GRID_FORCE_UPDATE : GRID_FUSED, ctx->grid_grid);
+  // adaptive step: the new Dt from the updated velocities, used by G2P (HybridSolver.cpp:878)
+  if (ctx->clk_active) launch_clock_update(ctx->stream, ctx->clk_active, 0);
   return MPM_OK;
 }
 
@@ -290,7 +310,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.cap = ctx->cap;
   a.n = ctx->n;
   a.perm = ctx->v1;
-  a.pos_in = ctx->pos[ctx->cur];
+  a.src = ctx->src;
   a.pos_out = ctx->pos[1 - ctx->cur];
   a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
@@ -308,6 +328,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.cand = ctx->cand;
   a.cand_slot = ctx->cand_slot;
   a.cand_count = ctx->counters + CNT_CAND;
+  a.clk = ctx->clk_active;
   launch_g2p(ctx->stream, a);
   if (surrogate) {
     SurrogateArgs sa;
@@ -324,8 +345,10 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     sa.write_stress = a.write_stress;
     sa.accepted = ctx->sur_counts + 1;
     sa.candidates = ctx->sur_counts;
+    sa.clk = ctx->clk_active;
     launch_surrogate_apply(ctx->stream, sa, ctx->n_cu);
   }
+  if (ctx->clk_active) launch_clock_end(ctx->stream, ctx->clk_active);
   ctx->cur = 1 - ctx->cur;
   ctx->keys_valid = true;
   ctx->stress_valid = !ctx->dp.predicted;
@@ -442,8 +465,9 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
   dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
-  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->sortws);
+  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
+  dfree(ctx->clock);
   dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
   dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -552,6 +576,7 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
     if (!rc) rc = dalloc(ctx, ctx->k2, n);
     if (!rc) rc = dalloc(ctx, ctx->v1, n);
     if (!rc) rc = dalloc(ctx, ctx->v2, n);
+    if (!rc) rc = dalloc(ctx, ctx->src, n);
     ctx->nb_sort = (n + SORT_TILE - 1) / SORT_TILE;
     ctx->sortws_words = 772 + (int64_t)3 * ctx->nb_sort * 256;
     if (!rc) rc = dalloc(ctx, ctx->sortws, ctx->sortws_words);
@@ -563,6 +588,7 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
     CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->v1, 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->v2, 0, sizeof(int) * n, ctx->stream));
+    CK(hipMemsetAsync(ctx->src, 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->pos[1], 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->sortws, 0, sizeof(uint32_t) * ctx->sortws_words, ctx->stream));
     ctx->n = n;
@@ -705,6 +731,106 @@ int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps) {
     if (rc) return rc;
   }
   CK(hipGetLastError());
+  return MPM_OK;
+}
+
+// ------------------------------------------------------------------ adaptive time step
+// HybridSolver::solve (HybridSolver.cpp:827-1034): Dt = CFL / max(300, max|v_i| / h_min) from the
+// grid velocities of the first P2G (:854-860), then per substep from the velocities after the
+// velocity update (:878), clamped to the end of the 1/60 s frame (:880-892); the loop ends when
+// t > maxt after a frame (:867). The clock lives on the device; substeps are replayed two at a time
+// from a captured graph, and a substep that completes a frame makes the rest of the batch do nothing,
+// so the host sees every frame boundary exactly and calls `cb` there (the reference writes its OBJ
+// frames at that point, :990-1029).
+int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void* user) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!(cfl > 0.0) || !(maxt >= 0.0)) return fail(ctx, MPM_ERR_INVALID, "mpm_solve: cfl must be > 0 and maxt >= 0");
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  if (!ctx->clock) {
+    rc = dalloc(ctx, ctx->clock, 1);
+    if (rc) return rc;
+  }
+  Clock h{};
+  h.cfl = cfl;
+  h.maxt = maxt;
+  h.hmin = std::min(ctx->h[0], std::min(ctx->h[1], ctx->h[2]));
+  CK(hipMemcpyAsync(ctx->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+  ctx->clk_active = ctx->clock;
+  struct Off {  // fixed-step runs never see the clock
+    mpm_ctx* c;
+    ~Off() { c->clk_active = nullptr; }
+  } off{ctx};
+  // first Dt from the velocities p/m of a P2G without forces
+  rc = stage_sort(ctx);
+  if (rc) return rc;
+  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), P2G_MOM, ctx->p2g_grid);
+  GridArgs g0 = grid_args(ctx, 0.0f);
+  g0.vmax_out = &ctx->clock->vmax_bits;
+  launch_grid(ctx->stream, g0, GRID_NORMALIZE, ctx->grid_grid);
+  launch_clock_update(ctx->stream, ctx->clock, 1);
+  CK(hipGetLastError());
+  while (true) {
+    int executed = 0;
+    const bool eager = !ctx->keys_valid || (!ctx->dp.predicted && !ctx->stress_valid);
+    if (eager) {
+      rc = one_substep(ctx, 0.0f);
+      if (rc) return rc;
+    } else {
+      const int c0 = ctx->cur;
+      if (!ctx->graph_adapt[c0]) {
+        hipGraph_t gr = nullptr;
+        const int64_t steps0 = ctx->substeps;
+        CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int r1 = one_substep(ctx, 0.0f);
+        int r2 = r1 ? r1 : one_substep(ctx, 0.0f);
+        hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
+        ctx->substeps = steps0;
+        ctx->cur = c0;  // capture did not execute anything
+        if (r2) return r2;
+        if (ec != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&ctx->graph_adapt[c0], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      }
+      CK(hipGraphLaunch(ctx->graph_adapt[c0], ctx->stream));
+      executed = -1;  // read back below
+    }
+    CK(hipMemcpyAsync(&h, ctx->clock, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (executed < 0) {
+      // the replayed pair ran one or two substeps; the state is in the buffer the last one wrote
+      const int c0 = ctx->cur;
+      const int e = h.executed;
+      ctx->substeps += e;
+      ctx->cur = (e == 2) ? c0 : 1 - c0;
+      ctx->sort_buf = (e == 2) ? 1 - c0 : c0;
+    }
+    const bool frame = h.frame_done != 0;
+    const bool done = h.done != 0;
+    h.frame_done = 0;
+    h.skip = 0;
+    h.executed = 0;
+    CK(hipMemcpyAsync(ctx->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+    if (frame && cb) cb(ctx, h.frame - 1, h.t, user);
+    if (done) break;
+  }
+  CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+int mpm_get_clock(mpm_ctx* ctx, double* t, double* dt, int32_t* frames, int64_t* substeps) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (!ctx->clock) return fail(ctx, MPM_ERR_STATE, "mpm_solve has not been called");
+  CK(hipSetDevice(ctx->device));
+  Clock h{};
+  CK(hipMemcpyAsync(&h, ctx->clock, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+  CK(hipStreamSynchronize(ctx->stream));
+  if (t) *t = h.t;
+  if (dt) *dt = h.dt;
+  if (frames) *frames = h.frame;
+  if (substeps) *substeps = h.substeps;
   return MPM_OK;
 }
 

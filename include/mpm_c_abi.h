@@ -111,7 +111,7 @@ typedef struct mpm_params {
   float friction;                /* grid collision friction coefficient; reference 0.2 (:465) */
   float apic_damping;            /* C <- skew + (1-d) sym (:808-824); reference d = 0 for particles */
   float fp64_svd_ratio;          /* redo the SVD in FP64 when sigma_min < ratio * sigma_max */
-  float fp64_apex_radius;        /* run the DP projection in FP64 when |eps_trial| < radius */
+  float fp64_apex_radius;        /* run the DP cone projection in FP64 when |eps_trial| < radius */
   int32_t reserved[4];
 } mpm_params;
 
@@ -182,6 +182,17 @@ int mpm_estimate_volumes(mpm_ctx* ctx);
 int mpm_substep(mpm_ctx* ctx, float dt);
 /* nsteps substeps with fixed dt, replayed from one captured HIP graph. */
 int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps);
+/* Called by mpm_solve on the calling thread after the substep that completes frame `frame`
+ * (0-based) at simulated time t; the particle state may be read from inside the callback. */
+typedef void (*mpm_frame_callback)(mpm_ctx* ctx, int32_t frame, double t, void* user);
+/* The reference's adaptive loop, HybridSolver::solve(CFL, maxt, alpha) (HybridSolver.cpp:827-1034;
+ * alpha is unused there as here): Dt = CFL / max(300, max|v_i| / h_min) from the grid velocities of
+ * a first P2G, then from the velocities after each velocity update; each substep is clamped to the
+ * end of its 1/60 s frame, t advances by 1/60 per frame, and the loop ends after the frame that
+ * takes t past maxt. Dt is evaluated on the device; cb (may be null) runs at every frame end. */
+int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void* user);
+/* Clock of the last mpm_solve: simulated time, last Dt, frames completed, substeps run. */
+int mpm_get_clock(mpm_ctx* ctx, double* t, double* dt, int32_t* frames, int64_t* substeps);
 /* Run a single stage (MPM_STAGE_*) of the current substep; stages must be issued in order
  * SORT, P2G, GRID, G2P. Used by per-stage parity tests. */
 int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt);

@@ -32,6 +32,8 @@ def load():
         "orc_stage": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
         "orc_substep": (ctypes.c_int, [vp, ctypes.c_double]),
         "orc_run": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
+        "orc_solve": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d,
+                                     ctypes.POINTER(ctypes.c_int), _d]),
         "orc_estimate_volumes": (ctypes.c_int, [vp]),
         "orc_get_grid": (ctypes.c_int, [vp, _d, _d, _d, _d, _d]),
         "orc_cell_keys": (None, [ctypes.c_int, _d, _d, _d, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint32)]),
@@ -105,6 +107,15 @@ class Oracle:
 
     def run(self, dt, n):
         self.lib.orc_run(self.h, float(dt), int(n))
+
+    def solve(self, cfl, maxt, max_substeps=1000000):
+        """The reference's adaptive loop; returns (dts, frame_end flags, t)."""
+        dts = np.zeros(max_substeps)
+        fe = np.zeros(max_substeps, np.int32)
+        t = ctypes.c_double()
+        k = self.lib.orc_solve(self.h, float(cfl), float(maxt), int(max_substeps), _dp(dts),
+                               fe.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(t))
+        return dts[:k], fe[:k], t.value
 
     def estimate_volumes(self):
         self.lib.orc_estimate_volumes(self.h)

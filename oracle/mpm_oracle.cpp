@@ -431,6 +431,10 @@ template <class T> struct Sim {
 
   // gravity (:457), updateGridVelocities_ (:725-737), gridCollisionHandling_ (:460-511)
   void grid_update(T dt) {
+    velocity_update(dt);
+    collide();
+  }
+  void velocity_update(T dt) {
     for (int c = 0; c < ng; ++c) {
       if (!(gm[c] > T(0))) continue;
       for (int a = 0; a < 3; ++a) {
@@ -438,6 +442,18 @@ template <class T> struct Sim {
         gv[(size_t)a * ng + c] += dt * f / gm[c];
       }
     }
+  }
+  // RegularGrid::max_velocity (RegularGrid.cpp:188-200): largest node speed (zero where m = 0)
+  double max_speed() const {
+    double vm = 0.0;
+    for (int c = 0; c < ng; ++c) {
+      double s2 = 0.0;
+      for (int a = 0; a < 3; ++a) s2 += double(gv[(size_t)a * ng + c]) * double(gv[(size_t)a * ng + c]);
+      vm = std::max(vm, std::sqrt(s2));
+    }
+    return vm;
+  }
+  void collide() {
     gvbf = gv;
     const T mu = T(P.friction);
     for (int c = 0; c < ng; ++c) {
@@ -615,6 +631,42 @@ template <class T> void set_particles(orc::Sim<T>& s, int n, const double* x, co
     for (int a = 0; a < 3; ++a) s.FP[(size_t)(4 * a) * n + p] = c;
   }
 }
+// HybridSolver::solve (HybridSolver.cpp:854-990) with the adaptive Dt: first P2G, Dt = CFL /
+// max(300, vmax/h_min) (:860); per iteration forces and velocity update with the current Dt, new Dt
+// from the updated velocities (:878), frame clamp to 1/60 s (:880-892), collision, G2P with the new
+// Dt, then the next P2G. dts[i] is the Dt of substep i, frame_end[i] is 1 where substep i completed
+// a frame. Returns the number of substeps.
+template <class T>
+int solve(orc::Sim<T>& s, double cfl, double maxt, int max_sub, double* dts, int* frame_end, double* t_out) {
+  const double hmin = std::min(s.h[0], std::min(s.h[1], s.h[2]));
+  s.p2g();
+  double Dt = cfl / std::max(3e2, s.max_speed() / hmin);
+  double t = 0.0, inner = 0.0;
+  int k = 0;
+  while (t <= maxt && k < max_sub) {
+    s.forces(T(Dt));
+    s.velocity_update(T(Dt));
+    Dt = cfl / std::max(3e2, s.max_speed() / hmin);
+    int fe = 0;
+    if (inner + Dt >= 1.0 / 60.0) {
+      Dt = 1.0 / 60.0 - inner;
+      t += 1.0 / 60.0;
+      inner = 0.0;
+      fe = 1;
+    } else {
+      inner += Dt;
+    }
+    s.collide();
+    s.g2p(T(Dt));
+    if (dts) dts[k] = Dt;
+    if (frame_end) frame_end[k] = fe;
+    ++k;
+    s.p2g();
+  }
+  if (t_out) *t_out = t;
+  return k;
+}
+
 template <class T> int stage(orc::Sim<T>& s, int st, double dt) {
   switch (st) {
     case MPM_STAGE_P2G: s.p2g(); s.forces(T(dt)); return 0;
@@ -676,6 +728,11 @@ int orc_substep(void* hp, double dt) {
   return 0;
 }
 
+int orc_solve(void* hp, double cfl, double maxt, int max_substeps, double* dts, int* frame_end, double* t_out) {
+  Handle* h = static_cast<Handle*>(hp);
+  return h->prec == 64 ? solve(h->d, cfl, maxt, max_substeps, dts, frame_end, t_out)
+                       : solve(h->f, cfl, maxt, max_substeps, dts, frame_end, t_out);
+}
 int orc_run(void* hp, double dt, int nsteps) {
   for (int i = 0; i < nsteps; ++i) orc_substep(hp, dt);
   return 0;

@@ -1,0 +1,90 @@
+"""GPU tests of the adaptive time step (mpm_solve) against the oracle's restatement of
+HybridSolver::solve (HybridSolver.cpp:854-990).
+
+  free fall at 9 m/s (speed limit active)   same substeps per frame, same final t (bitwise: both add
+                                            1/60 in double), positions within 1e-5
+  sand impact, 3 frames                     same frame count and final t; substeps per frame within 1
+                                            (Dt comes from FP32 node speeds); positions within 2e-3
+  frame callback                            called once per frame, in order, with readable state
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from anisotropicelastoplasticity_amd import Solver, _lib, scenes
+from oracle.bind import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _free_fall(speed):
+    sc = scenes.small_scene(_lib.SAND, 2, predicted=0, friction_model=_lib.FRICTION_COULOMB, seed=3,
+                            res=(64, 64, 64), cells=(4, 4, 4), lo=(0.2, 0.4, 0.4), hi=(0.3, 0.5, 0.5),
+                            v0=(speed, 0.0, 0.0), prims=[])
+    sc.params.youngs_modulus = 1e-6
+    return sc
+
+
+def _gpu_solve(sc, cfl, maxt):
+    frames = []
+    with Solver() as s:
+        sc.load_into(s)
+
+        def on_frame(frame, t):
+            c = s.clock()
+            x = s.get_particles()["x"]
+            frames.append((frame, t, c["substeps"], x.copy()))
+
+        s.solve(cfl, maxt, on_frame)
+        out = s.get_particles()
+        out["clock"] = s.clock()
+        out["stats"] = s.stats()
+    return out, frames
+
+
+def _oracle_solve(sc, cfl, maxt):
+    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
+    o.load_scene(sc)
+    dts, fe, t = o.solve(cfl, maxt)
+    return o.get_particles(), dts, fe, t
+
+
+def test_solve_free_fall_matches_oracle():
+    sc = _free_fall(9.0)
+    cfl, maxt = 0.3, 1.5 / 60.0
+    say(f"[solve free fall] n={sc.n}")
+    g, frames = _gpu_solve(sc, cfl, maxt)
+    q, dts, fe, t = _oracle_solve(sc, cfl, maxt)
+    ends = np.flatnonzero(fe) + 1  # substeps completed at each frame end
+    say(f"  gpu frames {[(f, s) for f, _, s, _ in frames]}, oracle ends {list(ends)}, t {g['clock']['t']} vs {t}")
+    assert [f for f, _, _, _ in frames] == list(range(len(ends)))
+    assert [s for _, _, s, _ in frames] == list(ends)
+    assert g["clock"]["t"] == t and g["clock"]["frames"] == len(ends)
+    assert g["clock"]["substeps"] == len(dts)
+    assert np.max(np.abs(g["x"] - q["x"])) <= 1e-5
+    assert g["stats"]["status"] == 0
+
+
+def test_solve_sand_impact_matches_oracle():
+    sc = scenes.small_scene(_lib.SAND, 2, predicted=0, friction_model=_lib.FRICTION_COULOMB, seed=5,
+                            res=(24, 24, 24), cells=(6, 5, 6), lo=(0.3, 0.2, 0.35), hi=(0.55, 0.42, 0.6),
+                            v0=(0.3, -1.5, 0.2))
+    cfl, maxt = 0.3, 2.5 / 60.0
+    say(f"[solve sand impact] n={sc.n}")
+    g, frames = _gpu_solve(sc, cfl, maxt)
+    q, dts, fe, t = _oracle_solve(sc, cfl, maxt)
+    ends = np.flatnonzero(fe) + 1
+    gsub = [s for _, _, s, _ in frames]
+    say(f"  gpu substeps at frame ends {gsub}, oracle {list(ends)}; max|dx| {np.max(np.abs(g['x'] - q['x'])):.3e}")
+    assert len(frames) == len(ends) == 3
+    assert g["clock"]["t"] == t
+    assert all(abs(a - b) <= 1 for a, b in zip(gsub, ends))
+    assert np.max(np.abs(g["x"] - q["x"])) <= 2e-3
+    # the state seen in the last frame callback is the final state
+    assert np.array_equal(frames[-1][3], g["x"])

@@ -163,3 +163,49 @@ def test_snow_plastic_determinant_tracks_clamp():
         want = np.prod(s) / np.prod(np.clip(s, lo, hi))
         assert abs(q["plastic"][i] - want) < 1e-12
         assert np.allclose(np.linalg.svd(q["F"][:, i].reshape(3, 3), compute_uv=False), np.clip(s, lo, hi))
+
+
+@pytest.mark.parametrize("speed", [1.0, 9.0])
+def test_adaptive_step_schedule_free_fall(speed):
+    # The reference's solve loop (HybridSolver.cpp:854-990) on a stress-free block in free fall:
+    # every node speed equals the particle speed, so the Dt sequence and the positions follow a
+    # closed recurrence. Forces and the velocity update use the previous Dt, advection the new one.
+    sc = scenes.small_scene(_lib.SAND, 2, predicted=0, friction_model=_lib.FRICTION_COULOMB, seed=3,
+                            res=(64, 64, 64), cells=(4, 4, 4), lo=(0.2, 0.4, 0.4), hi=(0.3, 0.5, 0.5),
+                            v0=(speed, 0.0, 0.0), prims=[])
+    sc.params.youngs_modulus = 1e-6
+    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res)
+    o.load_scene(sc)
+    cfl, maxt = 0.3, 1.5 / 60.0
+    x0 = sc.particles["x"].astype(np.float64)
+    dts, fe, t = o.solve(cfl, maxt)
+    h = 1.0 / sc.res[0]
+    g = np.array([0.0, float(np.float32(-9.8)), 0.0])
+    v = np.array([speed, 0.0, 0.0], dtype=np.float32).astype(np.float64)
+    want_dt = []
+    disp = np.zeros(3)
+    dt_prev = cfl / max(300.0, np.linalg.norm(v) / h)  # first Dt from the first P2G's velocities
+    inner, tt = 0.0, 0.0
+    while tt <= maxt:
+        v = v + dt_prev * g                              # velocity update with the previous Dt
+        dt = cfl / max(300.0, np.linalg.norm(v) / h)     # new Dt from the updated velocities
+        if inner + dt >= 1.0 / 60.0:
+            dt, tt, inner = 1.0 / 60.0 - inner, tt + 1.0 / 60.0, 0.0
+        else:
+            inner += dt
+        disp += dt * v                                   # advection with the new Dt
+        want_dt.append(dt)
+        dt_prev = dt
+    assert len(dts) == len(want_dt)
+    assert np.allclose(dts, want_dt, rtol=1e-9, atol=0)  # node speeds carry rounding of the P2G sums
+    assert fe.sum() == 2 and t == tt
+    # each frame spans exactly 1/60 s
+    ends = np.flatnonzero(fe)
+    assert abs(dts[:ends[0] + 1].sum() - 1.0 / 60.0) < 1e-15
+    assert abs(dts[ends[0] + 1:ends[1] + 1].sum() - 1.0 / 60.0) < 1e-15
+    if speed > 300 * h:
+        assert np.max(dts[:-1]) < cfl / 300.0  # the speed limit is active
+    else:
+        assert np.all(dts[fe == 0] == cfl / 300.0)
+    q = o.get_particles()
+    assert np.allclose(q["x"] - x0, disp[:, None], atol=1e-9)
