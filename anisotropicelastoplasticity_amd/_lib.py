@@ -45,6 +45,21 @@ class LevelSetPrim(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("a", ctypes.c_float * 4), ("b", ctypes.c_float * 4)]
 
 
+MAX_LEVELSETS = 16
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("min_bound", ctypes.c_double * 3),
+        ("max_bound", ctypes.c_double * 3),
+        ("res", ctypes.c_int32 * 3),
+        ("n_particles", ctypes.c_int32),
+        ("params", Params),
+        ("n_levelsets", ctypes.c_int32),
+        ("levelsets", LevelSetPrim * MAX_LEVELSETS),
+    ]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("substeps", ctypes.c_int64),
@@ -77,6 +92,7 @@ SIGNATURES = [
     ("mpm_set_grid", ctypes.c_int, [_ctx, _d, _d, _i32]),
     ("mpm_set_params", ctypes.c_int, [_ctx, _P(Params)]),
     ("mpm_set_levelset", ctypes.c_int, [_ctx, _P(LevelSetPrim), ctypes.c_int32]),
+    ("mpm_set_scenes", ctypes.c_int, [_ctx, ctypes.c_int32, _P(SceneDesc)]),
     ("mpm_set_particles", ctypes.c_int, [_ctx, ctypes.c_int32, _f, _f, _f, _f, _f, _f, _f, _f]),
     ("mpm_get_particles", ctypes.c_int, [_ctx, _f, _f, _f, _f, _f, _f]),
     ("mpm_estimate_volumes", ctypes.c_int, [_ctx]),

@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import LevelSetPrim, Params, Stats
+from ._lib import LevelSetPrim, Params, SceneDesc, Stats
 
 
 class MPMError(RuntimeError):
@@ -96,6 +96,49 @@ class Solver:
         self._check(rc, "mpm_set_particles")
         self.n = n
 
+    def set_scenes(self, scenes):
+        """Several independent scenes in this context (mpm_set_scenes); `scenes` is a list of
+        scenes.Scene. Their particles are then set concatenated in the same order (load_batch)."""
+        descs = (SceneDesc * len(scenes))()
+        for k, sc in enumerate(scenes):
+            d = descs[k]
+            for a in range(3):
+                d.min_bound[a] = float(sc.gmin[a])
+                d.max_bound[a] = float(sc.gmax[a])
+                d.res[a] = int(sc.res[a])
+            d.n_particles = int(sc.n)
+            d.params = sc.params
+            if len(sc.prims) > _lib.MAX_LEVELSETS:
+                raise MPMError("at most 16 level-set primitives per scene")
+            d.n_levelsets = len(sc.prims)
+            for i, pr in enumerate(sc.prims):
+                d.levelsets[i] = pr
+        self._check(self.lib.mpm_set_scenes(self.h, len(scenes), descs), "mpm_set_scenes")
+        self.res = None
+        self.scene_counts = [int(sc.n) for sc in scenes]
+
+    def load_batch(self, scenes):
+        """set_scenes + set_particles with every scene's particles concatenated in order."""
+        self.set_scenes(scenes)
+        keys = ["x", "v", "B", "F", "plastic", "mass", "volume", "phi"]
+        cat = {}
+        for k in keys:
+            parts = [sc.particles.get(k) for sc in scenes]
+            if all(p is None for p in parts):
+                cat[k] = None
+                continue
+            filled = []
+            for sc, p in zip(scenes, parts):
+                if p is None:  # a scene without the field: its default
+                    n = sc.n
+                    p = {"v": np.zeros((3, n)), "B": np.zeros((9, n)), "phi": np.zeros(n),
+                         "F": np.tile(np.eye(3).reshape(9, 1), (1, n)),
+                         "plastic": np.full(n, 1.0 if sc.params.material == _lib.SNOW else 0.0)}[k]
+                filled.append(np.asarray(p, np.float32))
+            cat[k] = np.concatenate(filled, axis=-1)
+        self.set_particles(cat["x"], cat["mass"], v=cat["v"], B=cat["B"], F=cat["F"], plastic=cat["plastic"],
+                           volume=cat["volume"], phi=cat["phi"])
+
     # ---------------------------------------------------------------- stepping
     def estimate_volumes(self):
         self._check(self.lib.mpm_estimate_volumes(self.h), "mpm_estimate_volumes")
@@ -149,17 +192,17 @@ class Solver:
         self._check(rc, "mpm_get_particles")
         return out
 
-    def get_sorted(self):
+    def get_sorted(self, bins=True):
+        """Sorted cell keys, caller ids in sorted order and (single scene) the cell bin offsets."""
         n = self.n
-        ncell = int(np.prod(self.res))
         keys = np.empty(n, np.uint32)
         ids = np.empty(n, np.int32)
-        bins = np.empty(ncell + 1, np.int32)
+        b = np.empty(int(np.prod(self.res)) + 1, np.int32) if bins else None
         rc = self.lib.mpm_get_sorted(self.h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                      ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                     bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+                                     None if b is None else b.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         self._check(rc, "mpm_get_sorted")
-        return keys, ids, bins
+        return keys, ids, b
 
     def get_grid(self):
         ng = int(np.prod(self.res))

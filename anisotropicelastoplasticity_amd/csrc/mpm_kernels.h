@@ -39,6 +39,45 @@ enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_CAND = 2, CNT_NUM = 4 };
 enum P2GMode : int { P2G_MOM_STRESS = 0, P2G_MOM = 1, P2G_FORCE = 2 };
 enum GridMode : int { GRID_FUSED = 0, GRID_NORMALIZE = 1, GRID_FORCE_UPDATE = 2 };
 
+// Scenes of one context. A context simulates S >= 1 independent scenes; their particles, cells,
+// tiles and nodes are concatenated scene-major, so that one radix sort, one P2G launch, one grid
+// launch and one G2P launch cover all of them. Each table has S + 1 entries (prefix offsets).
+// Particle-indexed launches run over 256-particle blocks that never straddle two scenes, so the
+// scene of a workgroup is uniform.
+struct SceneTable {
+  int nscenes;
+  const DevParams* params;   // [S] per-scene parameters, grid geometry and level sets
+  const int* part_off;       // [S+1] particles (caller order and sorted order are both scene-major)
+  const int* pblock_off;     // [S+1] 256-particle blocks
+  const int* tile_off;       // [S+1] 4x4x4-cell tiles (= 4x4x4-node blocks)
+  const int64_t* node_off;   // [S+1] nodes (= cells): global cell key = node_off[s] + local key
+  const int* thr_off;        // [S] per-axis cell thresholds of the scene
+};
+
+// Largest s in [0, S) with off[s] <= x (off ascending).
+template <class T>
+__device__ __forceinline__ int scene_of(const T* off, int S, T x) {
+  int lo = 0, hi = S - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Scene and particle index of thread threadIdx.x of particle block blockIdx.x (valid: j < end).
+struct PBlock {
+  int s, j, end;
+};
+__device__ __forceinline__ PBlock particle_block(const SceneTable& T) {
+  PBlock b;
+  b.s = __builtin_amdgcn_readfirstlane(scene_of(T.pblock_off, T.nscenes, (int)blockIdx.x));
+  b.j = T.part_off[b.s] + ((int)blockIdx.x - T.pblock_off[b.s]) * 256 + (int)threadIdx.x;
+  b.end = T.part_off[b.s + 1];
+  return b;
+}
+
 // Device clock of the adaptive time step (HybridSolver::solve, HybridSolver.cpp:860,878-892). The
 // substep kernels take a pointer to it (null for fixed-step runs): they read Dt from it and do
 // nothing once `skip` is set, which the last kernel of a substep does when the substep completed a
@@ -89,7 +128,7 @@ struct ResetArgs {
 };
 
 struct P2GArgs {
-  DevParams P;
+  SceneTable T;
   const float* state;
   int64_t cap;
   int n;
@@ -106,7 +145,7 @@ struct P2GArgs {
 };
 
 struct GridArgs {
-  DevParams P;
+  SceneTable T;
   const int* counters;
   const int* active_blocks;
   const int* tile_slot;
@@ -121,7 +160,7 @@ struct GridArgs {
 };
 
 struct G2PArgs {
-  DevParams P;
+  SceneTable T;
   const float* sin;
   float* sout;
   int64_t cap;
@@ -152,7 +191,7 @@ struct G2PArgs {
 constexpr int CAND_SIGMA = 0, CAND_U = 3, CAND_V = 12, CAND_PLANES = 21;
 
 struct SurrogateArgs {
-  DevParams P;
+  SceneTable T;
   const uint16_t* wbf;   // BF16 forward weights
   const float* master;   // FP32 master weights (biases)
   float* st;             // particle state written by k_g2p (sorted slots)
@@ -168,24 +207,29 @@ struct SurrogateArgs {
   const Clock* clk;
 };
 
-void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const int* pos, const float* thr,
-                 uint32_t* keys, int* status);
-void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, unsigned long long* fallback);
+// Particle-indexed launchers take the number of 256-particle blocks (SceneTable::pblock_off[S]).
+void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* st, int64_t cap, int n, const int* pos,
+                 const float* thr, uint32_t* keys, int* status);
+void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* st, int64_t cap, int n,
+                        unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
-void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n);
+// `passes` 8-bit digit passes (1..4); the result is in k1/v1 for an odd count, k2/v2 for an even one.
+void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n, int passes);
 // Also writes src[j] = pos[perm[j]]: the physical index of the particle at sorted position j.
-void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
+void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
                    int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
                    const int* perm, const int* pos, int* src, const Clock* clk);
-void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid);
+// order: B-spline order of every scene (2 or 3)
+void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
-void launch_g2p(hipStream_t s, const G2PArgs& a);
-void launch_volumes(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, const float* gm);
+void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks);
+void launch_volumes(hipStream_t s, const SceneTable& T, int order, int nblocks, float* st, int64_t cap, int n,
+                    const float* gm);
 void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out);
-void launch_grid_export(hipStream_t s, const DevParams& P, const float* grid, int64_t ng, int which, const int* block_slot,
+void launch_grid_export(hipStream_t s, const SceneTable& T, const float* grid, int64_t ng, int which, const int* block_slot,
                         float* out);
 void launch_iota(hipStream_t s, int* p, int n);
-void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
+void launch_count_nodes(hipStream_t s, const SceneTable& T, const float* gm, const int* counters, const int* active_blocks,
                         unsigned long long* out, int grid);
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v);
 void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4);

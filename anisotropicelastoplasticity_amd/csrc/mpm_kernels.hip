@@ -53,23 +53,28 @@ __device__ __forceinline__ uint32_t cell_key(const DevParams& P, const float* th
 // ================================================================== keys of the caller order
 // keys[id] = cell key of caller particle id (physical index pos[id]); the radix sort consumes the
 // keys in caller order, so that equal keys keep caller order as in the reference.
-__global__ __launch_bounds__(256) void k_keys(DevParams P, const float* __restrict__ st, int64_t cap, int n,
+__global__ __launch_bounds__(256) void k_keys(SceneTable T, const float* __restrict__ st, int64_t cap, int n,
                                               const int* __restrict__ pos, const float* __restrict__ thr,
                                               uint32_t* __restrict__ keys, int* __restrict__ status) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const PBlock b = particle_block(T);
+  const DevParams& P = T.params[b.s];
+  const int i = b.j;
   int oob = 0;
-  if (i < n) {
+  if (i < b.end) {
     int pi = clamp_index(pos[i], n);
-    keys[i] = cell_key(P, thr, st[PX * cap + pi], st[(PX + 1) * cap + pi], st[(PX + 2) * cap + pi], oob);
+    keys[i] = (uint32_t)T.node_off[b.s] +
+              cell_key(P, thr + T.thr_off[b.s], st[PX * cap + pi], st[(PX + 1) * cap + pi], st[(PX + 2) * cap + pi], oob);
   }
   if (__ballot(oob) && lane_id() == 0) atomicOr(status, MPM_STATUS_OUT_OF_DOMAIN);
 }
 
 // Volume-weighted Kirchhoff stress of the current F (after the caller sets particles or parameters).
-__global__ __launch_bounds__(256) void k_stress_init(DevParams P, float* __restrict__ st, int64_t cap, int n,
+__global__ __launch_bounds__(256) void k_stress_init(SceneTable T, float* __restrict__ st, int64_t cap, int n,
                                                      unsigned long long* __restrict__ fallback) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const PBlock b = particle_block(T);
+  const DevParams& P = T.params[b.s];
+  const int i = b.j;
+  if (i >= b.end) return;
   float F[9], K[6];
 #pragma unroll
   for (int c = 0; c < 9; ++c) F[c] = st[(PF + c) * cap + i];
@@ -118,11 +123,12 @@ __device__ __forceinline__ uint64_t digit_peers(bool valid, uint32_t d) {
   return peers;
 }
 
-__global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ hist,
-                                                   int* __restrict__ counters, const Clock* clk) {
+__global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, int n, int passes,
+                                                   uint32_t* __restrict__ hist, int* __restrict__ counters,
+                                                   const Clock* clk) {
   if (clock_skip(clk)) return;
-  __shared__ uint32_t h[3][256];
-  for (int i = threadIdx.x; i < 768; i += 256) (&h[0][0])[i] = 0u;
+  __shared__ uint32_t h[4][256];
+  for (int i = threadIdx.x; i < 1024; i += 256) (&h[0][0])[i] = 0u;
   if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
   __syncthreads();
   const int base = blockIdx.x * HIST_TILE;
@@ -133,8 +139,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ 
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0ull) continue;
     const int first = __ffsll((long long)vmask) - 1;
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
+    for (int p = 0; p < passes; ++p) {
       const uint32_t d = (key >> (8 * p)) & 255u;
       const uint32_t d0 = __shfl(d, first);
       if (__ballot(valid && d == d0) == vmask) {  // one digit across the wave: one add
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ 
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 768; i += 256) {
+  for (int i = threadIdx.x; i < 256 * passes; i += 256) {
     uint32_t c = (&h[0][0])[i];
     if (c) atomicAdd(&hist[i], c);
   }
@@ -286,28 +291,32 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 // ================================================================== bins and tiles
 // Thread per sorted position: the first position of each run of equal keys writes the cell's bin
 // start and flags the cell's tile (plain stores of one value), the last position writes the end.
-__global__ __launch_bounds__(256) void k_bounds(DevParams P, const uint32_t* __restrict__ ks, int n,
+__global__ __launch_bounds__(256) void k_bounds(SceneTable T, const uint32_t* __restrict__ ks, int n,
                                                 int* __restrict__ cell_start, int* __restrict__ cell_end,
                                                 int* __restrict__ tile_slot, const int* __restrict__ perm,
                                                 const int* __restrict__ pos, int* __restrict__ src, const Clock* clk) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || clock_skip(clk)) return;
+  if (clock_skip(clk)) return;
+  const PBlock b = particle_block(T);  // sorted positions are scene-major as well
+  const int j = b.j;
+  if (j >= b.end) return;
+  const DevParams& P = T.params[b.s];
   src[j] = clamp_index(pos[clamp_index(perm[j], n)], n);
   const uint32_t k = ks[j];
   if (j == n - 1 || ks[j + 1] != k) cell_end[k] = j + 1;
   if (j != 0 && ks[j - 1] == k) return;
   cell_start[k] = j;
+  const uint32_t lk = k - (uint32_t)T.node_off[b.s];
   const int r01 = P.res[0] * P.res[1];
-  const int c2 = (int)(k / (uint32_t)r01), rem = (int)(k % (uint32_t)r01);
+  const int c2 = (int)(lk / (uint32_t)r01), rem = (int)(lk % (uint32_t)r01);
   const int c1 = rem / P.res[0], c0 = rem % P.res[0];
-  tile_slot[((c2 >> 2) * P.tiles[1] + (c1 >> 2)) * P.tiles[0] + (c0 >> 2)] = -2;
+  tile_slot[T.tile_off[b.s] + ((c2 >> 2) * P.tiles[1] + (c1 >> 2)) * P.tiles[0] + (c0 >> 2)] = -2;
 }
 
 // Thread per tile (MODE 0) or node block (MODE 1): a flagged entry (-2) receives the next slot of
 // the active list (one atomic per wave). A claimed tile flags the 27 node blocks around it. Slot
 // numbers depend on the order in which waves run; no result depends on them.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_claim(DevParams P, int64_t count, int* __restrict__ slot_of,
+__global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int* __restrict__ slot_of,
                                                int* __restrict__ active, int* __restrict__ counter,
                                                int* __restrict__ block_slot, const Clock* clk) {
   if (clock_skip(clk)) return;
@@ -325,13 +334,16 @@ __global__ __launch_bounds__(256) void k_claim(DevParams P, int64_t count, int* 
   slot_of[t] = slot;
   active[slot] = (int)t;
   if (MODE == 0) {
-    const int t0 = (int)(t % P.tiles[0]), t1 = (int)((t / P.tiles[0]) % P.tiles[1]), t2 = (int)(t / ((int64_t)P.tiles[0] * P.tiles[1]));
+    const int sc = scene_of(T.tile_off, T.nscenes, (int)t);
+    const int* tl = T.params[sc].tiles;
+    const int lt = (int)t - T.tile_off[sc];
+    const int t0 = lt % tl[0], t1 = (lt / tl[0]) % tl[1], t2 = lt / (tl[0] * tl[1]);
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
           const int b0 = t0 + dx, b1 = t1 + dy, b2 = t2 + dz;
-          if (b0 < 0 || b1 < 0 || b2 < 0 || b0 >= P.tiles[0] || b1 >= P.tiles[1] || b2 >= P.tiles[2]) continue;
-          block_slot[(b2 * P.tiles[1] + b1) * P.tiles[0] + b0] = -2;
+          if (b0 < 0 || b1 < 0 || b2 < 0 || b0 >= tl[0] || b1 >= tl[1] || b2 >= tl[2]) continue;
+          block_slot[T.tile_off[sc] + (b2 * tl[1] + b1) * tl[0] + b0] = -2;
         }
   }
 }
@@ -370,8 +382,8 @@ __device__ __forceinline__ void axis_weights(float f, float* w, float* dw) {
 }
 
 template <int ORDER, int MODE>
-__device__ __forceinline__ void p2g_stage(const P2GArgs& a, float dt, int idx, int g0, int g1, int g2, float4* dst) {
-  const DevParams& P = a.P;
+__device__ __forceinline__ void p2g_stage(const P2GArgs& a, const DevParams& P, int64_t noff, float dt, int idx, int g0,
+                                          int g1, int g2, float4* dst) {
   float pay[PAY];
 #pragma unroll
   for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
@@ -435,7 +447,7 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, float dt, int idx, i
           if (n0 < 0 || n0 >= P.res[0]) continue;
           float gw[3] = {dx[ox] * wy[oy] * wz[oz] * P.inv_h[0], wx[ox] * dy[oy] * wz[oz] * P.inv_h[1],
                          wx[ox] * wy[oy] * dz[oz] * P.inv_h[2]};
-          int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+          int64_t node = noff + ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
           const float4 gv4 = a.gvel[2 * node];
           const float vr[3] = {gv4.x, gv4.y, gv4.z};
 #pragma unroll
@@ -470,7 +482,6 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
   const float dt = clock_dt(a.clk, a.dt);
   __shared__ float4 s_pay[4][4 * PAY_CELL / 4];
   __shared__ float s_slab[4][4][SLAB];  // [wave][quantity][node]
-  const DevParams& P = a.P;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int ntile = a.counters[CNT_TILES];
   constexpr int NQ = (MODE == P2G_FORCE) ? 3 : 4;
@@ -480,7 +491,11 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
   float* slab = &s_slab[w][0][0];
   for (int i = lane; i < 4 * SLAB; i += 64) slab[i] = 0.0f;
   for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
-    const int tile = a.active_tiles[slot];
+    const int gtile = a.active_tiles[slot];
+    const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
+    const DevParams& P = a.T.params[sc];
+    const int64_t noff = a.T.node_off[sc];
+    const int tile = gtile - a.T.tile_off[sc];
     const int t0 = tile % P.tiles[0], t1 = (tile / P.tiles[0]) % P.tiles[1], t2 = tile / (P.tiles[0] * P.tiles[1]);
     const int g2 = 4 * t2 + w;
     for (int cy = 0; cy < 4; ++cy) {
@@ -488,7 +503,7 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
       const int g0 = 4 * t0 + cx;  // this lane's cell of the row (both roles)
       int start = 0, cnt = 0;
       if (g0 < P.res[0] && g1 < P.res[1] && g2 < P.res[2]) {
-        const int key = (g2 * P.res[1] + g1) * P.res[0] + g0;
+        const int64_t key = noff + (g2 * P.res[1] + g1) * P.res[0] + g0;
         start = a.cell_start[key];
         cnt = a.cell_end[key] - start;
       }
@@ -505,7 +520,7 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
         if (sr < P2G_RANKS) {
           const int rank = P2G_RANKS * k + sr;
           if (rank < cnt)
-            p2g_stage<ORDER, MODE>(a, dt, start + rank, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
+            p2g_stage<ORDER, MODE>(a, P, noff, dt, start + rank, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -612,16 +627,19 @@ template <int MODE>
 __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
   if (clock_skip(a.clk)) return;
   const float dt = clock_dt(a.clk, a.dt);
-  const DevParams& P = a.P;
   const int nblk = a.counters[CNT_BLOCKS];
   float vmax = 0.0f;  // max |v| over this lane's nodes after the velocity update
   const int lx = threadIdx.x & 3, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x >> 4;
   for (int s = blockIdx.x; s < nblk; s += gridDim.x) {
-    const int blk = a.active_blocks[s];
+    const int gblk = a.active_blocks[s];
+    const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gblk));
+    const DevParams& P = a.T.params[sc];
+    const int toff = a.T.tile_off[sc];
+    const int blk = gblk - toff;
     const int b0 = blk % P.tiles[0], b1 = (blk / P.tiles[0]) % P.tiles[1], b2 = blk / (P.tiles[0] * P.tiles[1]);
     const int n0 = 4 * b0 + lx, n1 = 4 * b1 + ly, n2 = 4 * b2 + lz;
     if (n0 >= P.res[0] || n1 >= P.res[1] || n2 >= P.res[2]) continue;
-    const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+    const int64_t node = a.T.node_off[sc] + ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
     const int nn[3] = {n0, n1, n2};
     int tlo[3], thi[3];
 #pragma unroll
@@ -638,7 +656,7 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
     for (int t2 = tlo[2]; t2 <= thi[2]; ++t2)
       for (int t1 = tlo[1]; t1 <= thi[1]; ++t1)
         for (int t0 = tlo[0]; t0 <= thi[0]; ++t0) {
-          const int sl = a.tile_slot[(t2 * P.tiles[1] + t1) * P.tiles[0] + t0];
+          const int sl = a.tile_slot[toff + (t2 * P.tiles[1] + t1) * P.tiles[0] + t0];
           if (sl < 0) continue;
           const int ln = (n0 - (4 * t0 - 1)) + FOOT * (n1 - (4 * t1 - 1)) + FOOT * FOOT * (n2 - (4 * t2 - 1));
           const float* part = a.partials + (size_t)sl * PART_STRIDE + qoff * FOOT3 + ln;
@@ -732,10 +750,13 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   if (clock_skip(a.clk)) return;
   const float dt = clock_dt(a.clk, a.dt);
-  const DevParams& P = a.P;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const PBlock pb = particle_block(a.T);
+  const DevParams& P = a.T.params[pb.s];
+  const int64_t noff = a.T.node_off[pb.s];
+  const float4* gvel = a.gvel + 2 * noff;
+  const int j = pb.j;
   int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
-  if (j < a.n) {
+  if (j < pb.end) {
     const int pid = clamp_index(a.perm[j], a.n);
     const int pi = clamp_index(a.src[j], a.n);
     const float* st = a.sin;
@@ -774,8 +795,8 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
           const int n0 = base[0] + ox;
           if (n0 < 0 || n0 >= P.res[0]) continue;
           const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
-          const float4 q0 = a.gvel[2 * node];
-          const float4 q1 = a.gvel[2 * node + 1];
+          const float4 q0 = gvel[2 * node];
+          const float4 q1 = gvel[2 * node + 1];
           const float vi[3] = {q0.x, q0.y, q0.z};
           const float bi[3] = {q0.w, q1.x, q1.y};
           const float wt = w[0][ox] * wyz;
@@ -874,7 +895,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     so[PPHI * cap + j] = phi_p;
     reinterpret_cast<int*>(so + (size_t)PID * cap)[j] = pid;
     a.pos_out[pid] = j;
-    a.keys[pid] = cell_key(P, a.thr, xn[0], xn[1], xn[2], oob);
+    a.keys[pid] = (uint32_t)noff + cell_key(P, a.thr + a.T.thr_off[pb.s], xn[0], xn[1], xn[2], oob);
     bad = !(isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]) && isfinite(vp[0]) && isfinite(vp[1]) &&
             isfinite(vp[2]) && isfinite(Fh[0]) && isfinite(Fh[4]) && isfinite(Fh[8]) && isfinite(plastic));
   }
@@ -902,7 +923,6 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   MLPShared& sh = *reinterpret_cast<MLPShared*>(smem);
   uint16_t* acts = reinterpret_cast<uint16_t*>(smem + sizeof(MLPShared));
-  const DevParams& P = a.P;
   // the counter is cleared by the sort stage; a G2P stage repeated without it cannot list more than n
   const int count = min(*a.cand_count, a.n);
   if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
@@ -912,11 +932,14 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint16_t* act = acts + w * 16 * LDA;
   const int64_t cap = a.cap, n = a.n;
-  const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
   for (int blk = blockIdx.x * 4 + w; blk * 16 < count; blk += gridDim.x * 4) {
     const int row = blk * 16 + (lane & 15);
     const bool valid = lane < 16 && row < count;
     int j = 0;
+    // the candidate's scene: its sorted slot lies in the scene's particle range
+    const int sc = valid ? scene_of(a.T.part_off, a.T.nscenes, clamp_index(a.cand_slot[row], (int)n)) : 0;
+    const DevParams& P = a.T.params[sc];
+    const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
     float S[3] = {1.0f, 1.0f, 1.0f}, eps[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f, phi_p = 0.0f, phi = 0.0f;
     if (valid) {
       j = clamp_index(a.cand_slot[row], (int)n);
@@ -999,10 +1022,13 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
 // ================================================================== auxiliary kernels
 // rho_p = sum_i w_ip m_i / h^3, V_p = m_p / rho_p (HybridSolver.cpp:242-249), at the current positions.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_volumes(DevParams P, float* __restrict__ st, int64_t cap, int n,
-                                                 const float* __restrict__ gm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(256) void k_volumes(SceneTable T, float* __restrict__ st, int64_t cap, int n,
+                                                 const float* __restrict__ gm_all) {
+  const PBlock pb = particle_block(T);
+  const int i = pb.j;
+  if (i >= pb.end) return;
+  const DevParams& P = T.params[pb.s];
+  const float* gm = gm_all + T.node_off[pb.s];
   float g[3];
   int base[3];
   float w[3][4], dw[3][4];
@@ -1037,12 +1063,15 @@ __global__ __launch_bounds__(256) void k_unpermute(const float* __restrict__ st,
 
 // Dense planar copy of one grid quantity (0 = m, 1 = momentum, 2 = v, 3 = v_BF), with nodes
 // outside the active blocks set to zero.
-__global__ __launch_bounds__(256) void k_grid_export(DevParams P, const float* __restrict__ grid, int64_t ng, int which,
+__global__ __launch_bounds__(256) void k_grid_export(SceneTable T, const float* __restrict__ grid, int64_t ng, int which,
                                                      const int* __restrict__ block_slot, float* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ng) return;
-  const int n0 = (int)(i % P.res[0]), n1 = (int)((i / P.res[0]) % P.res[1]), n2 = (int)(i / ((int64_t)P.res[0] * P.res[1]));
-  const int b = ((n2 >> 2) * P.tiles[1] + (n1 >> 2)) * P.tiles[0] + (n0 >> 2);
+  const int sc = scene_of(T.node_off, T.nscenes, i);
+  const DevParams& P = T.params[sc];
+  const int64_t li = i - T.node_off[sc];
+  const int n0 = (int)(li % P.res[0]), n1 = (int)((li / P.res[0]) % P.res[1]), n2 = (int)(li / ((int64_t)P.res[0] * P.res[1]));
+  const int b = T.tile_off[sc] + ((n2 >> 2) * P.tiles[1] + (n1 >> 2)) * P.tiles[0] + (n0 >> 2);
   const bool on = block_slot[b] >= 0;
   if (which == 0) {
     out[i] = on ? grid[GRID_M * ng + i] : 0.0f;
@@ -1060,17 +1089,20 @@ __global__ __launch_bounds__(256) void k_iota(int* __restrict__ p, int n) {
 }
 
 // Number of nodes with m > 0 in the active blocks.
-__global__ __launch_bounds__(64) void k_count_nodes(DevParams P, const float* __restrict__ gm, const int* __restrict__ counters,
+__global__ __launch_bounds__(64) void k_count_nodes(SceneTable T, const float* __restrict__ gm, const int* __restrict__ counters,
                                                     const int* __restrict__ active_blocks, unsigned long long* __restrict__ out) {
   const int nblk = counters[CNT_BLOCKS];
   const int lx = threadIdx.x & 3, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x >> 4;
   unsigned long long c = 0;
   for (int s = blockIdx.x; s < nblk; s += gridDim.x) {
-    const int blk = active_blocks[s];
+    const int gblk = active_blocks[s];
+    const int sc = scene_of(T.tile_off, T.nscenes, gblk);
+    const DevParams& P = T.params[sc];
+    const int blk = gblk - T.tile_off[sc];
     const int b0 = blk % P.tiles[0], b1 = (blk / P.tiles[0]) % P.tiles[1], b2 = blk / (P.tiles[0] * P.tiles[1]);
     const int n0 = 4 * b0 + lx, n1 = 4 * b1 + ly, n2 = 4 * b2 + lz;
     if (n0 >= P.res[0] || n1 >= P.res[1] || n2 >= P.res[2]) continue;
-    if (gm[((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0] > 0.0f) ++c;
+    if (gm[T.node_off[sc] + ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0] > 0.0f) ++c;
   }
   if (c) atomicAdd(out, c);
 }
@@ -1089,25 +1121,30 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, int64_t n
 // ================================================================== launchers
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-void launch_keys(hipStream_t s, const DevParams& P, const float* st, int64_t cap, int n, const int* pos, const float* thr,
-                 uint32_t* keys, int* status) {
-  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, pos, thr, keys, status);
+void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* st, int64_t cap, int n, const int* pos,
+                 const float* thr, uint32_t* keys, int* status) {
+  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, pos, thr, keys, status);
 }
 
-void launch_stress_init(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, unsigned long long* fallback) {
-  if (n > 0) hipLaunchKernelGGL(k_stress_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, fallback);
+void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* st, int64_t cap, int n,
+                        unsigned long long* fallback) {
+  if (n > 0) hipLaunchKernelGGL(k_stress_init, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, fallback);
 }
 
-void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n) {
+void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n, int passes) {
   const int nb = blocks_for(n, SORT_TILE);
-  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, b.hist, b.counters,
-                     b.clk);
-  hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, keys_in, (const int*)nullptr, b.k1, b.v1, n, 0, b.hist,
-                     b.tickets, b.status, b.clk);
-  hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k1, (const int*)b.v1, b.k2, b.v2, n, 1,
-                     b.hist, b.tickets, b.status + (size_t)nb * 256, b.clk);
-  hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, (const uint32_t*)b.k2, (const int*)b.v2, b.k1, b.v1, n, 2,
-                     b.hist, b.tickets, b.status + (size_t)2 * nb * 256, b.clk);
+  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, passes, b.hist,
+                     b.counters, b.clk);
+  const uint32_t* kin = keys_in;
+  const int* vin = nullptr;
+  for (int p = 0; p < passes; ++p) {
+    uint32_t* ko = (p & 1) ? b.k2 : b.k1;
+    int* vo = (p & 1) ? b.v2 : b.v1;
+    hipLaunchKernelGGL(k_sort_pass, dim3(nb), dim3(256), 0, s, kin, vin, ko, vo, n, p, b.hist, b.tickets,
+                       b.status + (size_t)p * nb * 256, b.clk);
+    kin = ko;
+    vin = vo;
+  }
 }
 
 void launch_reset(hipStream_t s, const ResetArgs& r) {
@@ -1120,20 +1157,20 @@ void launch_reset(hipStream_t s, const ResetArgs& r) {
                      r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot, r.clk);
 }
 
-void launch_bounds(hipStream_t s, const DevParams& P, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
+void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
                    int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
                    const int* perm, const int* pos, int* src, const Clock* clk) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, ks, n, cell_start, cell_end, tile_slot,
-                     perm, pos, src, clk);
-  hipLaunchKernelGGL(k_claim<0>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, P, ntiles, tile_slot, active_tiles,
+  hipLaunchKernelGGL(k_bounds, dim3(nblocks), dim3(256), 0, s, T, ks, n, cell_start, cell_end, tile_slot, perm, pos,
+                     src, clk);
+  hipLaunchKernelGGL(k_claim<0>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, tile_slot, active_tiles,
                      counters + CNT_TILES, block_slot, clk);
-  hipLaunchKernelGGL(k_claim<1>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, P, ntiles, block_slot, active_blocks,
+  hipLaunchKernelGGL(k_claim<1>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, block_slot, active_blocks,
                      counters + CNT_BLOCKS, (int*)nullptr, clk);
 }
 
-void launch_p2g(hipStream_t s, const P2GArgs& a, int mode, int grid) {
-  const int o = a.P.order;
+void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid) {
+  const int o = order;
   if (mode == P2G_MOM_STRESS) {
     if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM_STRESS>), dim3(grid), dim3(256), 0, s, a);
     else hipLaunchKernelGGL((k_p2g<3, P2G_MOM_STRESS>), dim3(grid), dim3(256), 0, s, a);
@@ -1158,34 +1195,35 @@ void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid) {
   else hipLaunchKernelGGL(k_grid<GRID_FORCE_UPDATE>, dim3(grid), dim3(64), 0, s, a);
 }
 
-void launch_g2p(hipStream_t s, const G2PArgs& a) {
+void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks) {
   if (a.n <= 0) return;
-  if (a.P.order == 2) hipLaunchKernelGGL(k_g2p<2>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_g2p<3>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+  if (order == 2) hipLaunchKernelGGL(k_g2p<2>, dim3(nblocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_g2p<3>, dim3(nblocks), dim3(256), 0, s, a);
 }
 
-void launch_volumes(hipStream_t s, const DevParams& P, float* st, int64_t cap, int n, const float* gm) {
+void launch_volumes(hipStream_t s, const SceneTable& T, int order, int nblocks, float* st, int64_t cap, int n,
+                    const float* gm) {
   if (n <= 0) return;
-  if (P.order == 2) hipLaunchKernelGGL(k_volumes<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, gm);
-  else hipLaunchKernelGGL(k_volumes<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, cap, n, gm);
+  if (order == 2) hipLaunchKernelGGL(k_volumes<2>, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, gm);
+  else hipLaunchKernelGGL(k_volumes<3>, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, gm);
 }
 
 void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out) {
   if (n > 0) hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, cap, n, p0, np, out);
 }
 
-void launch_grid_export(hipStream_t s, const DevParams& P, const float* grid, int64_t ng, int which, const int* block_slot,
+void launch_grid_export(hipStream_t s, const SceneTable& T, const float* grid, int64_t ng, int which, const int* block_slot,
                         float* out) {
-  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(ng, 256)), dim3(256), 0, s, P, grid, ng, which, block_slot, out);
+  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(ng, 256)), dim3(256), 0, s, T, grid, ng, which, block_slot, out);
 }
 
 void launch_iota(hipStream_t s, int* p, int n) {
   if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
 }
 
-void launch_count_nodes(hipStream_t s, const DevParams& P, const float* gm, const int* counters, const int* active_blocks,
+void launch_count_nodes(hipStream_t s, const SceneTable& T, const float* gm, const int* counters, const int* active_blocks,
                         unsigned long long* out, int grid) {
-  hipLaunchKernelGGL(k_count_nodes, dim3(grid), dim3(64), 0, s, P, gm, counters, active_blocks, out);
+  hipLaunchKernelGGL(k_count_nodes, dim3(grid), dim3(64), 0, s, T, gm, counters, active_blocks, out);
 }
 
 void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4) {

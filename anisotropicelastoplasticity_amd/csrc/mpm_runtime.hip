@@ -16,6 +16,9 @@
 
 using namespace mpm;
 
+static_assert(sizeof(mpm_params) == 100 && sizeof(mpm_levelset) == 36, "C ABI struct layout");
+static_assert(sizeof(mpm_scene_desc) == 48 + 16 + sizeof(mpm_params) + 4 + 16 * 36, "C ABI struct layout");
+
 namespace {
 thread_local std::string g_create_error;
 
@@ -25,21 +28,36 @@ struct DevBuf {
 };
 }  // namespace
 
+// Host description of one scene (grid geometry, parameters, level sets, particle count).
+struct SceneHost {
+  double gmin[3] = {0, 0, 0}, gmax[3] = {1, 1, 1}, h[3] = {1, 1, 1};
+  int res[3] = {0, 0, 0};
+  int tiles[3] = {0, 0, 0};
+  int64_t ng = 0, ntiles = 0;
+  mpm_params params{};
+  std::vector<mpm_levelset> ls;
+  int n = 0;
+};
+
 struct mpm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
 
-  // grid
+  // scenes: one for the single-scene calls (mpm_set_grid / _params / _levelset), S for mpm_set_scenes
   bool have_grid = false;
-  double gmin[3] = {0, 0, 0}, gmax[3] = {1, 1, 1}, h[3] = {1, 1, 1};
-  int res[3] = {0, 0, 0};
-  int64_t ng = 0;
-  int tiles[3] = {0, 0, 0};
-  int64_t ntiles = 0;
-  mpm_params params{};
-  std::vector<mpm_levelset> ls;
-  DevParams dp{};
+  bool batch = false;
+  std::vector<SceneHost> scenes;
+  int64_t ng = 0;      // nodes (= cells) of all scenes
+  int64_t ntiles = 0;  // tiles of all scenes
+  int order = 3, predicted = 0, any_sand = 0;
+  // device scene table (SceneTable points into it)
+  unsigned char* table_buf = nullptr;
+  SceneTable T{};
+  int nblocks_p = 0;    // 256-particle blocks over all scenes
+  int sort_passes = 3;  // 8-bit digit passes covering the largest global cell key
+  uint32_t* ks_sorted = nullptr;  // sorted keys / ids of the last sort (k1/v1 or k2/v2)
+  int* vs_sorted = nullptr;
 
   float* thr = nullptr;      // per-axis cell thresholds
   float* grid = nullptr;     // m, mom(3), v(3), vbf(3)
@@ -154,10 +172,18 @@ static void drop_graphs(mpm_ctx* ctx) {
   }
 }
 
+static SceneHost& scene0(mpm_ctx* ctx) {
+  if (ctx->scenes.empty()) {
+    ctx->scenes.resize(1);
+    mpm_params_default(&ctx->scenes[0].params);
+  }
+  return ctx->scenes[0];
+}
+
 // ------------------------------------------------------------------ parameters
-static void build_devparams(mpm_ctx* ctx) {
-  DevParams& d = ctx->dp;
-  const mpm_params& p = ctx->params;
+static DevParams devparams_of(const SceneHost& sc) {
+  DevParams d{};
+  const mpm_params& p = sc.params;
   d.order = p.kernel_order == MPM_KERNEL_QUADRATIC ? 2 : 3;
   d.material = p.material;
   d.predicted = p.stress_at_predicted_F;
@@ -178,19 +204,74 @@ static void build_devparams(mpm_ctx* ctx) {
   d.damping = p.apic_damping;
   d.svd_ratio = p.fp64_svd_ratio;
   d.apex_radius = p.fp64_apex_radius;
-  double hmin = std::min(ctx->h[0], std::min(ctx->h[1], ctx->h[2]));
+  double hmin = std::min(sc.h[0], std::min(sc.h[1], sc.h[2]));
   d.dinv = float((d.order == 3 ? 3.0 : 4.0) / hmin / hmin);
   for (int a = 0; a < 3; ++a) {
-    d.gmin[a] = float(ctx->gmin[a]);
-    d.h[a] = float(ctx->h[a]);
-    d.inv_h[a] = float(1.0 / ctx->h[a]);
-    d.res[a] = ctx->res[a];
-    d.tiles[a] = ctx->tiles[a];
+    d.gmin[a] = float(sc.gmin[a]);
+    d.h[a] = float(sc.h[a]);
+    d.inv_h[a] = float(1.0 / sc.h[a]);
+    d.res[a] = sc.res[a];
+    d.tiles[a] = sc.tiles[a];
   }
-  d.nls = (int)ctx->ls.size();
-  for (int k = 0; k < d.nls; ++k) d.ls[k] = ctx->ls[k];
+  d.nls = (int)sc.ls.size();
+  for (int k = 0; k < d.nls; ++k) d.ls[k] = sc.ls[k];
+  return d;
+}
+
+// Scene table on the device: per-scene DevParams and the offset tables of SceneTable.
+static int upload_table(mpm_ctx* ctx) {
+  const int S = (int)ctx->scenes.size();
+  if (S == 0) return MPM_OK;
+  for (int k = 1; k < S; ++k) {
+    if (ctx->scenes[k].params.kernel_order != ctx->scenes[0].params.kernel_order ||
+        ctx->scenes[k].params.stress_at_predicted_F != ctx->scenes[0].params.stress_at_predicted_F)
+      return fail(ctx, MPM_ERR_UNSUPPORTED, "the scenes of one context share kernel_order and stress_at_predicted_F");
+  }
+  std::vector<DevParams> dp(S);
+  std::vector<int> part_off(S + 1, 0), pblock_off(S + 1, 0), tile_off(S + 1, 0), thr_off(S, 0);
+  std::vector<int64_t> node_off(S + 1, 0);
+  int thr = 0;
+  ctx->any_sand = 0;
+  for (int k = 0; k < S; ++k) {
+    const SceneHost& sc = ctx->scenes[k];
+    dp[k] = devparams_of(sc);
+    ctx->any_sand |= sc.params.material == MPM_SAND;
+    part_off[k + 1] = part_off[k] + sc.n;
+    pblock_off[k + 1] = pblock_off[k] + (sc.n + 255) / 256;
+    tile_off[k + 1] = tile_off[k] + (int)sc.ntiles;
+    node_off[k + 1] = node_off[k] + sc.ng;
+    thr_off[k] = thr;
+    thr += sc.res[0] + sc.res[1] + sc.res[2] + 3;
+  }
+  ctx->order = dp[0].order;
+  ctx->predicted = dp[0].predicted;
+  ctx->nblocks_p = pblock_off[S];
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_dp = 0, o_part = al(o_dp + sizeof(DevParams) * S), o_pb = al(o_part + sizeof(int) * (S + 1));
+  const size_t o_tile = al(o_pb + sizeof(int) * (S + 1)), o_node = al(o_tile + sizeof(int) * (S + 1));
+  const size_t o_thr = al(o_node + sizeof(int64_t) * (S + 1)), total = al(o_thr + sizeof(int) * S);
+  std::vector<unsigned char> h(total, 0);
+  std::memcpy(h.data() + o_dp, dp.data(), sizeof(DevParams) * S);
+  std::memcpy(h.data() + o_part, part_off.data(), sizeof(int) * (S + 1));
+  std::memcpy(h.data() + o_pb, pblock_off.data(), sizeof(int) * (S + 1));
+  std::memcpy(h.data() + o_tile, tile_off.data(), sizeof(int) * (S + 1));
+  std::memcpy(h.data() + o_node, node_off.data(), sizeof(int64_t) * (S + 1));
+  std::memcpy(h.data() + o_thr, thr_off.data(), sizeof(int) * S);
+  CK(hipStreamSynchronize(ctx->stream));
+  int rc = dalloc(ctx, ctx->table_buf, total);
+  if (rc) return rc;
+  CK(hipMemcpy(ctx->table_buf, h.data(), total, hipMemcpyHostToDevice));
+  SceneTable& T = ctx->T;
+  T.nscenes = S;
+  T.params = reinterpret_cast<const DevParams*>(ctx->table_buf + o_dp);
+  T.part_off = reinterpret_cast<const int*>(ctx->table_buf + o_part);
+  T.pblock_off = reinterpret_cast<const int*>(ctx->table_buf + o_pb);
+  T.tile_off = reinterpret_cast<const int*>(ctx->table_buf + o_tile);
+  T.node_off = reinterpret_cast<const int64_t*>(ctx->table_buf + o_node);
+  T.thr_off = reinterpret_cast<const int*>(ctx->table_buf + o_thr);
   ctx->stress_valid = false;
   drop_graphs(ctx);
+  return MPM_OK;
 }
 
 // smallest float f with (double(f) - gmin)/h >= c, evaluated exactly as HybridSolver.cpp:33-36
@@ -202,15 +283,19 @@ static float threshold(double gmin, double h, int c) {
   return f;
 }
 
+// Radix-sort workspace words: 4 x 256 histogram words, 4 pass tickets, then the look-back words.
+static constexpr int64_t SORTWS_TICKETS = 1024, SORTWS_STATUS = 1028;
+
 // ------------------------------------------------------------------ substep stages
 static int stage_sort(mpm_ctx* ctx) {
   hipStream_t s = ctx->stream;
   float* st = ctx->state[ctx->cur];
-  if (!ctx->keys_valid) launch_keys(s, ctx->dp, st, ctx->cap, ctx->n, ctx->pos[ctx->cur], ctx->thr, ctx->keys, ctx->status);
+  if (!ctx->keys_valid)
+    launch_keys(s, ctx->T, ctx->nblocks_p, st, ctx->cap, ctx->n, ctx->pos[ctx->cur], ctx->thr, ctx->keys, ctx->status);
   ResetArgs r;
   r.zero_words = ctx->sortws;
   r.n_zero = ctx->sortws_words;
-  r.prev_keys = ctx->k1;
+  r.prev_keys = ctx->ks_sorted;
   r.n = ctx->n;
   r.cell_start = ctx->cell_start;
   r.cell_end = ctx->cell_end;
@@ -228,21 +313,21 @@ static int stage_sort(mpm_ctx* ctx) {
   b.v1 = ctx->v1;
   b.v2 = ctx->v2;
   b.hist = ctx->sortws;
-  b.tickets = ctx->sortws + 768;
-  b.status = ctx->sortws + 772;
+  b.tickets = ctx->sortws + SORTWS_TICKETS;
+  b.status = ctx->sortws + SORTWS_STATUS;
   b.counters = ctx->counters;
   b.clk = ctx->clk_active;
-  launch_sort(s, b, ctx->keys, ctx->n);
-  launch_bounds(s, ctx->dp, ctx->k1, ctx->n, ctx->ntiles, ctx->cell_start, ctx->cell_end, ctx->tile_slot,
-                ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters, ctx->v1, ctx->pos[ctx->cur],
-                ctx->src, ctx->clk_active);
+  launch_sort(s, b, ctx->keys, ctx->n, ctx->sort_passes);
+  launch_bounds(s, ctx->T, ctx->nblocks_p, ctx->ks_sorted, ctx->n, ctx->ntiles, ctx->cell_start, ctx->cell_end,
+                ctx->tile_slot, ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters, ctx->vs_sorted,
+                ctx->pos[ctx->cur], ctx->src, ctx->clk_active);
   ctx->sort_buf = ctx->cur;
   return MPM_OK;
 }
 
 static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   P2GArgs a;
-  a.P = ctx->dp;
+  a.T = ctx->T;
   a.state = ctx->state[ctx->cur];
   a.cap = ctx->cap;
   a.n = ctx->n;
@@ -261,7 +346,7 @@ static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
 
 static GridArgs grid_args(mpm_ctx* ctx, float dt) {
   GridArgs g;
-  g.P = ctx->dp;
+  g.T = ctx->T;
   g.counters = ctx->counters;
   g.active_blocks = ctx->active_blocks;
   g.tile_slot = ctx->tile_slot;
@@ -279,16 +364,16 @@ static GridArgs grid_args(mpm_ctx* ctx, float dt) {
 static int stage_p2g(mpm_ctx* ctx, float dt) {
   hipStream_t s = ctx->stream;
   P2GArgs a = p2g_args(ctx, dt);
-  if (!ctx->dp.predicted) {
+  if (!ctx->predicted) {
     if (!ctx->stress_valid) {
-      launch_stress_init(s, ctx->dp, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
+      launch_stress_init(s, ctx->T, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
       ctx->stress_valid = true;
     }
-    launch_p2g(s, a, P2G_MOM_STRESS, ctx->p2g_grid);
+    launch_p2g(s, a, ctx->order, P2G_MOM_STRESS, ctx->p2g_grid);
   } else {
-    launch_p2g(s, a, P2G_MOM, ctx->p2g_grid);
+    launch_p2g(s, a, ctx->order, P2G_MOM, ctx->p2g_grid);
     launch_grid(s, grid_args(ctx, dt), GRID_NORMALIZE, ctx->grid_grid);
-    launch_p2g(s, a, P2G_FORCE, ctx->p2g_grid);
+    launch_p2g(s, a, ctx->order, P2G_FORCE, ctx->p2g_grid);
   }
   return MPM_OK;
 }
@@ -296,7 +381,7 @@ static int stage_p2g(mpm_ctx* ctx, float dt) {
 static int stage_grid(mpm_ctx* ctx, float dt) {
   GridArgs g = grid_args(ctx, dt);
   if (ctx->clk_active) g.vmax_out = &ctx->clk_active->vmax_bits;
-  launch_grid(ctx->stream, g, ctx->dp.predicted ? GRID_FORCE_UPDATE : GRID_FUSED, ctx->grid_grid);
+  launch_grid(ctx->stream, g, ctx->predicted ? GRID_FORCE_UPDATE : GRID_FUSED, ctx->grid_grid);
   // adaptive step: the new Dt from the updated velocities, used by G2P (HybridSolver.cpp:878)
   if (ctx->clk_active) launch_clock_update(ctx->stream, ctx->clk_active, 0);
   return MPM_OK;
@@ -304,23 +389,23 @@ static int stage_grid(mpm_ctx* ctx, float dt) {
 
 static int stage_g2p(mpm_ctx* ctx, float dt) {
   G2PArgs a;
-  a.P = ctx->dp;
+  a.T = ctx->T;
   a.sin = ctx->state[ctx->cur];
   a.sout = ctx->state[1 - ctx->cur];
   a.cap = ctx->cap;
   a.n = ctx->n;
-  a.perm = ctx->v1;
+  a.perm = ctx->vs_sorted;
   a.src = ctx->src;
   a.pos_out = ctx->pos[1 - ctx->cur];
   a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
-  a.write_stress = ctx->dp.predicted ? 0 : 1;
+  a.write_stress = ctx->predicted ? 0 : 1;
   a.thr = ctx->thr;
   a.keys = ctx->keys;
   a.status = ctx->status;
   a.fallback_svd = ctx->fallbacks;
   a.fallback_apex = ctx->fallbacks + 1;
-  const bool surrogate = ctx->sur_enabled && ctx->mlp_ready && ctx->cand && ctx->dp.material == MPM_SAND;
+  const bool surrogate = ctx->sur_enabled && ctx->mlp_ready && ctx->cand && ctx->any_sand;
   a.sur_mode = surrogate ? 2 : (ctx->pair_cap > 0 ? 1 : 0);
   a.pair_log = ctx->pair_log;
   a.pair_cap = ctx->pair_cap;
@@ -329,10 +414,10 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.cand_slot = ctx->cand_slot;
   a.cand_count = ctx->counters + CNT_CAND;
   a.clk = ctx->clk_active;
-  launch_g2p(ctx->stream, a);
+  launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p);
   if (surrogate) {
     SurrogateArgs sa;
-    sa.P = ctx->dp;
+    sa.T = ctx->T;
     sa.wbf = ctx->mlp_wbf;
     sa.master = ctx->mlp_master;
     sa.st = ctx->state[1 - ctx->cur];
@@ -351,7 +436,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   if (ctx->clk_active) launch_clock_end(ctx->stream, ctx->clk_active);
   ctx->cur = 1 - ctx->cur;
   ctx->keys_valid = true;
-  ctx->stress_valid = !ctx->dp.predicted;
+  ctx->stress_valid = !ctx->predicted;
   ctx->substeps += 1;
   return MPM_OK;
 }
@@ -370,6 +455,9 @@ static int ready(mpm_ctx* ctx) {
   if (!ctx) return MPM_ERR_INVALID;
   if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid has not been called");
   if (ctx->n <= 0) return fail(ctx, MPM_ERR_STATE, "mpm_set_particles has not been called");
+  int64_t total = 0;
+  for (const SceneHost& sc : ctx->scenes) total += sc.n;
+  if (total != ctx->n) return fail(ctx, MPM_ERR_STATE, "mpm_set_particles must follow mpm_set_scenes");
   return MPM_OK;
 }
 
@@ -433,7 +521,7 @@ int mpm_create(mpm_ctx** out, int32_t device) {
     delete ctx;
     return MPM_ERR_HIP;
   }
-  mpm_params_default(&ctx->params);
+  scene0(ctx);
   int rc = dalloc(ctx, ctx->counters, CNT_NUM);
   if (!rc) rc = dalloc(ctx, ctx->status, 1);
   if (!rc) rc = dalloc(ctx, ctx->fallbacks, 4);
@@ -461,6 +549,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   if (!ctx) return MPM_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graphs(ctx);
+  dfree(ctx->table_buf);
   dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_end);
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
@@ -487,82 +576,174 @@ static int alloc_partials(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
-int mpm_set_grid(mpm_ctx* ctx, const double min_bound[3], const double max_bound[3], const int32_t res[3]) {
-  if (!ctx || !min_bound || !max_bound || !res) return MPM_ERR_INVALID;
+// Radix-sort workspace: histograms, tickets and look-back words of every pass.
+static int alloc_sortws(mpm_ctx* ctx) {
+  if (ctx->n <= 0) return MPM_OK;
+  ctx->nb_sort = (ctx->n + SORT_TILE - 1) / SORT_TILE;
+  const int64_t words = SORTWS_STATUS + (int64_t)ctx->sort_passes * ctx->nb_sort * 256;
+  if (words != ctx->sortws_words || !ctx->sortws) {
+    int rc = dalloc(ctx, ctx->sortws, words);
+    if (rc) return rc;
+    ctx->sortws_words = words;
+  }
+  CK(hipMemsetAsync(ctx->sortws, 0, sizeof(uint32_t) * ctx->sortws_words, ctx->stream));
+  return MPM_OK;
+}
+
+// The last sort pass writes k1/v1 for an odd pass count, k2/v2 for an even one.
+static void pick_sorted(mpm_ctx* ctx) {
+  ctx->ks_sorted = (ctx->sort_passes & 1) ? ctx->k1 : ctx->k2;
+  ctx->vs_sorted = (ctx->sort_passes & 1) ? ctx->v1 : ctx->v2;
+}
+
+static int set_geometry(mpm_ctx* ctx, SceneHost& sc, const double min_bound[3], const double max_bound[3],
+                        const int32_t res[3]) {
   for (int a = 0; a < 3; ++a) {
     if (!(max_bound[a] > min_bound[a])) return fail(ctx, MPM_ERR_INVALID, "maxBound must be bigger than minBound");
     if (res[a] < 1) return fail(ctx, MPM_ERR_INVALID, "resolution must be positive");
   }
-  int64_t ng = (int64_t)res[0] * res[1] * res[2];
-  if (ng > (1ll << 24)) return fail(ctx, MPM_ERR_UNSUPPORTED, "grids above 2^24 cells need more radix passes");
-  CK(hipSetDevice(ctx->device));
-  CK(hipStreamSynchronize(ctx->stream));
   for (int a = 0; a < 3; ++a) {
-    ctx->gmin[a] = min_bound[a];
-    ctx->gmax[a] = max_bound[a];
-    ctx->res[a] = res[a];
-    ctx->h[a] = (max_bound[a] - min_bound[a]) / res[a];  // RegularGrid.cpp:144-146
-    ctx->tiles[a] = (res[a] + TILE - 1) / TILE;
+    sc.gmin[a] = min_bound[a];
+    sc.gmax[a] = max_bound[a];
+    sc.res[a] = res[a];
+    sc.h[a] = (max_bound[a] - min_bound[a]) / res[a];  // RegularGrid.cpp:144-146
+    sc.tiles[a] = (res[a] + TILE - 1) / TILE;
   }
-  ctx->ng = ng;
-  ctx->ntiles = (int64_t)ctx->tiles[0] * ctx->tiles[1] * ctx->tiles[2];
-  int rc = dalloc(ctx, ctx->grid, (size_t)GRID_FLOATS * ng);
-  if (!rc) rc = dalloc(ctx, ctx->cell_start, ng);
-  if (!rc) rc = dalloc(ctx, ctx->cell_end, ng);
-  if (!rc) rc = dalloc(ctx, ctx->tile_slot, ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, ctx->active_tiles, ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, ctx->block_slot, ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, ctx->active_blocks, ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, ctx->thr, (size_t)(res[0] + res[1] + res[2] + 3));
-  if (rc) return rc;
-  CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * GRID_FLOATS * ng, ctx->stream));
-  CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * ng, ctx->stream));
-  CK(hipMemsetAsync(ctx->cell_end, 0, sizeof(int) * ng, ctx->stream));
-  launch_fill_int(ctx->stream, ctx->tile_slot, ctx->ntiles, -1);
-  launch_fill_int(ctx->stream, ctx->block_slot, ctx->ntiles, -1);
-  CK(hipMemsetAsync(ctx->counters, 0, CNT_NUM * sizeof(int), ctx->stream));
-  std::vector<float> thr;
-  for (int a = 0; a < 3; ++a)
-    for (int c = 0; c <= res[a]; ++c) thr.push_back(threshold(ctx->gmin[a], ctx->h[a], c));
-  CK(hipMemcpyAsync(ctx->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  CK(hipStreamSynchronize(ctx->stream));
-  ctx->have_grid = true;
-  ctx->keys_valid = false;
-  ctx->partial_slots = 0;
-  if (ctx->k1) CK(hipMemset(ctx->k1, 0, sizeof(uint32_t) * ctx->cap));
-  rc = alloc_partials(ctx);
-  if (rc) return rc;
-  build_devparams(ctx);
+  sc.ng = (int64_t)res[0] * res[1] * res[2];
+  sc.ntiles = (int64_t)sc.tiles[0] * sc.tiles[1] * sc.tiles[2];
   return MPM_OK;
 }
 
-int mpm_set_params(mpm_ctx* ctx, const mpm_params* p) {
-  if (!ctx || !p) return MPM_ERR_INVALID;
+static int check_params(mpm_ctx* ctx, const mpm_params* p) {
   if (p->kernel_order != MPM_KERNEL_QUADRATIC && p->kernel_order != MPM_KERNEL_CUBIC)
     return fail(ctx, MPM_ERR_INVALID, "kernel_order must be 2 or 3");
   if (p->material != MPM_SNOW && p->material != MPM_SAND) return fail(ctx, MPM_ERR_INVALID, "unknown material");
   if (p->friction_model < 0 || p->friction_model > 2) return fail(ctx, MPM_ERR_INVALID, "unknown friction model");
   if (!(p->poisson_ratio < 0.5f) || !(p->youngs_modulus > 0.0f)) return fail(ctx, MPM_ERR_INVALID, "bad elastic moduli");
-  ctx->params = *p;
-  build_devparams(ctx);
   return MPM_OK;
 }
 
-int mpm_set_levelset(mpm_ctx* ctx, const mpm_levelset* prims, int32_t n) {
-  if (!ctx || n < 0 || n > 16 || (n > 0 && !prims)) return ctx ? fail(ctx, MPM_ERR_INVALID, "0 <= n <= 16 primitives") : MPM_ERR_INVALID;
+static int check_levelsets(mpm_ctx* ctx, const mpm_levelset* prims, int32_t n) {
+  if (n < 0 || n > MPM_MAX_LEVELSETS || (n > 0 && !prims)) return fail(ctx, MPM_ERR_INVALID, "0 <= n <= 16 primitives");
   for (int i = 0; i < n; ++i)
     if (prims[i].type < MPM_LS_PLANE || prims[i].type > MPM_LS_SPHERE) return fail(ctx, MPM_ERR_INVALID, "unknown level-set type");
-  ctx->ls.assign(prims, prims + n);
-  build_devparams(ctx);
   return MPM_OK;
+}
+
+// Grid buffers of all scenes, after their geometry changed: nodes, bins, tiles and cell thresholds
+// concatenated scene-major.
+static int layout(mpm_ctx* ctx) {
+  int64_t ng = 0, nt = 0, nthr = 0;
+  for (const SceneHost& sc : ctx->scenes) {
+    ng += sc.ng;
+    nt += sc.ntiles;
+    nthr += sc.res[0] + sc.res[1] + sc.res[2] + 3;
+  }
+  if (ng > (int64_t)0xFFFFFFFFll) return fail(ctx, MPM_ERR_UNSUPPORTED, "more than 2^32 cells over all scenes");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->ng = ng;
+  ctx->ntiles = nt;
+  int bits = 1;
+  while ((int64_t(1) << bits) < ng) ++bits;  // keys lie in [0, ng)
+  ctx->sort_passes = std::max(1, (bits + 7) / 8);
+  int rc = dalloc(ctx, ctx->grid, (size_t)GRID_FLOATS * ng);
+  if (!rc) rc = dalloc(ctx, ctx->cell_start, ng);
+  if (!rc) rc = dalloc(ctx, ctx->cell_end, ng);
+  if (!rc) rc = dalloc(ctx, ctx->tile_slot, nt);
+  if (!rc) rc = dalloc(ctx, ctx->active_tiles, nt);
+  if (!rc) rc = dalloc(ctx, ctx->block_slot, nt);
+  if (!rc) rc = dalloc(ctx, ctx->active_blocks, nt);
+  if (!rc) rc = dalloc(ctx, ctx->thr, (size_t)nthr);
+  if (rc) return rc;
+  CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * GRID_FLOATS * ng, ctx->stream));
+  CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * ng, ctx->stream));
+  CK(hipMemsetAsync(ctx->cell_end, 0, sizeof(int) * ng, ctx->stream));
+  launch_fill_int(ctx->stream, ctx->tile_slot, nt, -1);
+  launch_fill_int(ctx->stream, ctx->block_slot, nt, -1);
+  CK(hipMemsetAsync(ctx->counters, 0, CNT_NUM * sizeof(int), ctx->stream));
+  std::vector<float> thr;
+  thr.reserve(nthr);
+  for (const SceneHost& sc : ctx->scenes)
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c <= sc.res[a]; ++c) thr.push_back(threshold(sc.gmin[a], sc.h[a], c));
+  CK(hipMemcpyAsync(ctx->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  ctx->have_grid = true;
+  ctx->keys_valid = false;
+  ctx->partial_slots = 0;
+  pick_sorted(ctx);
+  if (ctx->k1) CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * ctx->cap, ctx->stream));
+  if (ctx->k2) CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * ctx->cap, ctx->stream));
+  rc = alloc_sortws(ctx);
+  if (!rc) rc = alloc_partials(ctx);
+  CK(hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+
+int mpm_set_grid(mpm_ctx* ctx, const double min_bound[3], const double max_bound[3], const int32_t res[3]) {
+  if (!ctx || !min_bound || !max_bound || !res) return MPM_ERR_INVALID;
+  if (ctx->batch) return fail(ctx, MPM_ERR_STATE, "the context holds several scenes (mpm_set_scenes)");
+  SceneHost& sc = scene0(ctx);
+  int rc = set_geometry(ctx, sc, min_bound, max_bound, res);
+  if (!rc) rc = layout(ctx);
+  if (!rc) rc = upload_table(ctx);
+  return rc;
+}
+
+int mpm_set_params(mpm_ctx* ctx, const mpm_params* p) {
+  if (!ctx || !p) return MPM_ERR_INVALID;
+  if (ctx->batch) return fail(ctx, MPM_ERR_STATE, "the context holds several scenes (mpm_set_scenes)");
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  scene0(ctx).params = *p;
+  return ctx->have_grid ? upload_table(ctx) : MPM_OK;
+}
+
+int mpm_set_levelset(mpm_ctx* ctx, const mpm_levelset* prims, int32_t n) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (ctx->batch) return fail(ctx, MPM_ERR_STATE, "the context holds several scenes (mpm_set_scenes)");
+  int rc = check_levelsets(ctx, prims, n);
+  if (rc) return rc;
+  scene0(ctx).ls.assign(prims, prims + n);
+  return ctx->have_grid ? upload_table(ctx) : MPM_OK;
+}
+
+int mpm_set_scenes(mpm_ctx* ctx, int32_t nscenes, const mpm_scene_desc* descs) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (nscenes < 1 || !descs) return fail(ctx, MPM_ERR_INVALID, "nscenes >= 1 scene descriptions are required");
+  std::vector<SceneHost> sc(nscenes);
+  for (int k = 0; k < nscenes; ++k) {
+    const mpm_scene_desc& d = descs[k];
+    int rc = set_geometry(ctx, sc[k], d.min_bound, d.max_bound, d.res);
+    if (!rc) rc = check_params(ctx, &d.params);
+    if (!rc) rc = check_levelsets(ctx, d.levelsets, d.n_levelsets);
+    if (!rc && d.n_particles < 1) rc = fail(ctx, MPM_ERR_INVALID, "every scene needs particles");
+    if (rc) return rc;
+    sc[k].params = d.params;
+    sc[k].ls.assign(d.levelsets, d.levelsets + d.n_levelsets);
+    sc[k].n = d.n_particles;
+  }
+  ctx->scenes = std::move(sc);
+  ctx->batch = true;
+  int rc = layout(ctx);
+  if (!rc) rc = upload_table(ctx);
+  return rc;
 }
 
 int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, const float* B, const float* F,
                       const float* plastic, const float* mass, const float* volume, const float* phi) {
   if (!ctx) return MPM_ERR_INVALID;
-  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid must precede mpm_set_particles");
+  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid or mpm_set_scenes must precede mpm_set_particles");
   if (n <= 0 || !x || !mass) return fail(ctx, MPM_ERR_INVALID, "n > 0, x and mass are required");
-  if (ctx->params.per_particle_phi && !phi) return fail(ctx, MPM_ERR_INVALID, "per_particle_phi needs phi");
+  if (ctx->batch) {
+    int64_t total = 0;
+    for (const SceneHost& sc : ctx->scenes) total += sc.n;
+    if (total != n) return fail(ctx, MPM_ERR_INVALID, "n must equal the scenes' particle counts summed");
+  } else {
+    ctx->scenes[0].n = n;
+  }
+  for (const SceneHost& sc : ctx->scenes)
+    if (sc.params.per_particle_phi && !phi) return fail(ctx, MPM_ERR_INVALID, "per_particle_phi needs phi");
   CK(hipSetDevice(ctx->device));
   CK(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);
@@ -577,45 +758,49 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
     if (!rc) rc = dalloc(ctx, ctx->v1, n);
     if (!rc) rc = dalloc(ctx, ctx->v2, n);
     if (!rc) rc = dalloc(ctx, ctx->src, n);
-    ctx->nb_sort = (n + SORT_TILE - 1) / SORT_TILE;
-    ctx->sortws_words = 772 + (int64_t)3 * ctx->nb_sort * 256;
-    if (!rc) rc = dalloc(ctx, ctx->sortws, ctx->sortws_words);
     if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
     if (rc) return rc;
     CK(hipMemsetAsync(ctx->state[1], 0, sizeof(float) * NPLANES * (size_t)n, ctx->stream));
     CK(hipMemsetAsync(ctx->keys, 0, sizeof(uint32_t) * n, ctx->stream));
-    CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->v1, 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->v2, 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->src, 0, sizeof(int) * n, ctx->stream));
     CK(hipMemsetAsync(ctx->pos[1], 0, sizeof(int) * n, ctx->stream));
-    CK(hipMemsetAsync(ctx->sortws, 0, sizeof(uint32_t) * ctx->sortws_words, ctx->stream));
     ctx->n = n;
     ctx->cap = n;
-    rc = alloc_partials(ctx);
+    rc = alloc_sortws(ctx);
+    if (!rc) rc = alloc_partials(ctx);
     if (rc) return rc;
   }
+  pick_sorted(ctx);
+  int rc = upload_table(ctx);
+  if (rc) return rc;
   const int64_t cap = ctx->cap;
   std::vector<float> h((size_t)NPLANES * cap, 0.0f);
-  for (int i = 0; i < n; ++i) {
-    for (int a = 0; a < 3; ++a) {
-      h[(PX + a) * cap + i] = x[(size_t)a * n + i];
-      h[(PV + a) * cap + i] = v ? v[(size_t)a * n + i] : 0.0f;
+  int i = 0;
+  for (const SceneHost& sc : ctx->scenes) {
+    const float jp0 = sc.params.material == MPM_SNOW ? 1.0f : 0.0f;
+    for (int e = i + sc.n; i < e; ++i) {
+      for (int a = 0; a < 3; ++a) {
+        h[(PX + a) * cap + i] = x[(size_t)a * n + i];
+        h[(PV + a) * cap + i] = v ? v[(size_t)a * n + i] : 0.0f;
+      }
+      for (int c = 0; c < 9; ++c) {
+        h[(PB + c) * cap + i] = B ? B[(size_t)c * n + i] : 0.0f;
+        h[(PF + c) * cap + i] = F ? F[(size_t)c * n + i] : ((c % 4 == 0) ? 1.0f : 0.0f);
+      }
+      h[PPL * cap + i] = plastic ? plastic[i] : jp0;
+      h[PM * cap + i] = mass[i];
+      h[PVOL * cap + i] = volume ? volume[i] : 0.0f;
+      h[PPHI * cap + i] = phi ? phi[i] : 0.0f;
+      int id = i;
+      std::memcpy(&h[PID * cap + i], &id, sizeof(int));
     }
-    for (int c = 0; c < 9; ++c) {
-      h[(PB + c) * cap + i] = B ? B[(size_t)c * n + i] : 0.0f;
-      h[(PF + c) * cap + i] = F ? F[(size_t)c * n + i] : ((c % 4 == 0) ? 1.0f : 0.0f);
-    }
-    h[PPL * cap + i] = plastic ? plastic[i] : (ctx->params.material == MPM_SNOW ? 1.0f : 0.0f);
-    h[PM * cap + i] = mass[i];
-    h[PVOL * cap + i] = volume ? volume[i] : 0.0f;
-    h[PPHI * cap + i] = phi ? phi[i] : 0.0f;
-    int id = i;
-    std::memcpy(&h[PID * cap + i], &id, sizeof(int));
   }
   CK(hipMemcpyAsync(ctx->state[0], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * n, ctx->stream));
+  CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
   launch_iota(ctx->stream, ctx->pos[0], n);
   CK(hipStreamSynchronize(ctx->stream));
   ctx->stress_valid = false;
@@ -658,9 +843,9 @@ int mpm_estimate_volumes(mpm_ctx* ctx) {
   rc = stage_sort(ctx);
   if (rc) return rc;
   P2GArgs a = p2g_args(ctx, 0.0f);
-  launch_p2g(ctx->stream, a, P2G_MOM, ctx->p2g_grid);
+  launch_p2g(ctx->stream, a, ctx->order, P2G_MOM, ctx->p2g_grid);
   launch_grid(ctx->stream, grid_args(ctx, 0.0f), GRID_NORMALIZE, ctx->grid_grid);
-  launch_volumes(ctx->stream, ctx->dp, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->grid);
+  launch_volumes(ctx->stream, ctx->T, ctx->order, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->grid);
   ctx->stress_valid = false;
   CK(hipGetLastError());
   CK(hipStreamSynchronize(ctx->stream));
@@ -693,7 +878,7 @@ int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps) {
   if (nsteps < 0) return fail(ctx, MPM_ERR_INVALID, "nsteps < 0");
   if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
   CK(hipSetDevice(ctx->device));
-  if (nsteps > 0 && (!ctx->keys_valid || (!ctx->dp.predicted && !ctx->stress_valid))) {
+  if (nsteps > 0 && (!ctx->keys_valid || (!ctx->predicted && !ctx->stress_valid))) {
     rc = one_substep(ctx, dt);
     if (rc) return rc;
     --nsteps;
@@ -746,6 +931,8 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
   int rc = ready(ctx);
   if (rc) return rc;
   if (!(cfl > 0.0) || !(maxt >= 0.0)) return fail(ctx, MPM_ERR_INVALID, "mpm_solve: cfl must be > 0 and maxt >= 0");
+  if (ctx->scenes.size() != 1)
+    return fail(ctx, MPM_ERR_UNSUPPORTED, "mpm_solve: the adaptive step couples scenes; use one scene per context");
   if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
   CK(hipSetDevice(ctx->device));
   if (!ctx->clock) {
@@ -755,7 +942,8 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
   Clock h{};
   h.cfl = cfl;
   h.maxt = maxt;
-  h.hmin = std::min(ctx->h[0], std::min(ctx->h[1], ctx->h[2]));
+  const SceneHost& sc0 = ctx->scenes[0];
+  h.hmin = std::min(sc0.h[0], std::min(sc0.h[1], sc0.h[2]));
   CK(hipMemcpyAsync(ctx->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
   ctx->clk_active = ctx->clock;
   struct Off {  // fixed-step runs never see the clock
@@ -765,7 +953,7 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
   // first Dt from the velocities p/m of a P2G without forces
   rc = stage_sort(ctx);
   if (rc) return rc;
-  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), P2G_MOM, ctx->p2g_grid);
+  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), ctx->order, P2G_MOM, ctx->p2g_grid);
   GridArgs g0 = grid_args(ctx, 0.0f);
   g0.vmax_out = &ctx->clock->vmax_bits;
   launch_grid(ctx->stream, g0, GRID_NORMALIZE, ctx->grid_grid);
@@ -773,7 +961,7 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
   CK(hipGetLastError());
   while (true) {
     int executed = 0;
-    const bool eager = !ctx->keys_valid || (!ctx->dp.predicted && !ctx->stress_valid);
+    const bool eager = !ctx->keys_valid || (!ctx->predicted && !ctx->stress_valid);
     if (eager) {
       rc = one_substep(ctx, 0.0f);
       if (rc) return rc;
@@ -840,8 +1028,8 @@ int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins) {
   CK(hipSetDevice(ctx->device));
   const int n = ctx->n;
   CK(hipStreamSynchronize(ctx->stream));
-  if (keys) CK(hipMemcpy(keys, ctx->k1, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-  if (ids) CK(hipMemcpy(ids, ctx->v1, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (keys) CK(hipMemcpy(keys, ctx->ks_sorted, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  if (ids) CK(hipMemcpy(ids, ctx->vs_sorted, sizeof(int) * n, hipMemcpyDeviceToHost));
   if (bins) {
     const int64_t ng = ctx->ng;
     std::vector<int> cs(ng), ce(ng);
@@ -866,7 +1054,7 @@ int mpm_get_grid(mpm_ctx* ctx, float* m, float* mom, float* v, float* vbf) {
   hipError_t e = hipSuccess;
   for (const Req& r : reqs) {
     if (!r.dst) continue;
-    launch_grid_export(ctx->stream, ctx->dp, ctx->grid, ng, r.which, ctx->block_slot, tmp);
+    launch_grid_export(ctx->stream, ctx->T, ctx->grid, ng, r.which, ctx->block_slot, tmp);
     e = hipMemcpyAsync(r.dst, tmp, sizeof(float) * r.np * ng, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) break;
   }
@@ -888,7 +1076,7 @@ int mpm_get_stats(mpm_ctx* ctx, mpm_stats* s) {
   unsigned long long sc[4] = {0, 0, 0, 0};
   if (ctx->have_grid && ctx->n > 0) {
     CK(hipMemsetAsync(ctx->fallbacks + 2, 0, sizeof(unsigned long long), ctx->stream));
-    launch_count_nodes(ctx->stream, ctx->dp, ctx->grid, ctx->counters, ctx->active_blocks, ctx->fallbacks + 2, ctx->grid_grid);
+    launch_count_nodes(ctx->stream, ctx->T, ctx->grid, ctx->counters, ctx->active_blocks, ctx->fallbacks + 2, ctx->grid_grid);
   }
   CK(hipMemcpyAsync(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   CK(hipMemcpyAsync(&status, ctx->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -144,3 +144,43 @@ def small_scene(material, order, predicted, friction_model, seed=7, res=(24, 24,
         else:
             prims = [levelset(_lib.LS_PLANE, (0.0, 1.0, 0.0, 0.18))]
     return Scene("small", p, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), res, prims, parts, dt, 1)
+
+
+def batch_scene(k, rng, n_range=(20000, 500000), res_choices=(64, 80, 96, 112, 128), dt=5e-5, substeps=500):
+    """Scene k of a batched run: snow or sand (50/50), a grid of 64^3..128^3 over the unit cube, a
+    particle count log-uniform in n_range, E log-uniform in [1e5, 1e6], nu uniform in [0.2, 0.35],
+    sand friction angle uniform in [25, 40] degrees. The particles form a cube of c^3 lattice cells
+    (8 jittered particles per cell, one lattice cell per grid cell) falling at 1 m/s inside box walls
+    [0.05, 0.95]^3 with Coulomb friction 0.4."""
+    sand = bool(rng.random() < 0.5)
+    r = int(rng.choice(res_choices))
+    n_want = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+    E = float(np.exp(rng.uniform(np.log(1e5), np.log(1e6))))
+    nu = float(rng.uniform(0.2, 0.35))
+    phi = float(rng.uniform(25.0, 40.0))
+    c = max(2, int(round((n_want / 8.0) ** (1.0 / 3.0))))
+    edge = c / r
+    lo = (0.5 - edge / 2, 0.1, 0.5 - edge / 2)
+    hi = (lo[0] + edge, lo[1] + edge, lo[2] + edge)
+    x = generate_block(1000 + k, lo, hi, (c, c, c), 2)
+    if sand:
+        params = sand_params(E=E, nu=nu, phi_deg=phi)
+        rho = 2200.0
+    else:
+        params = snow_params()
+        params.youngs_modulus = E
+        params.poisson_ratio = nu
+        params.friction_model = _lib.FRICTION_COULOMB
+        rho = 400.0
+    params.friction = 0.4
+    parts = _particles(x, rho, edge ** 3, (0.0, -1.0, 0.0), _lib.SAND if sand else _lib.SNOW)
+    prims = [levelset(_lib.LS_BOX, (0.05, 0.05, 0.05, 0.0), (0.95, 0.95, 0.95, 0.0))]
+    return Scene(f"batch{k}", params, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (r, r, r), prims, parts, dt, substeps,
+                 dict(sand=sand, E=E, nu=nu, phi=phi if sand else None))
+
+
+def config5_batch(seed=5, nscenes=64, **kw):
+    """Batched many-scene run: `nscenes` independent scenes drawn by batch_scene from one seed
+    (64 scenes: about 9.5 million particles), dt = 5e-5, 500 substeps."""
+    rng = np.random.default_rng(seed)
+    return [batch_scene(k, rng, **kw) for k in range(nscenes)]

@@ -4,6 +4,10 @@ Workload: Drucker-Prager sand column collapse, 1,048,576 particles on a 128^3 gr
 quadratic B-spline APIC, stress at F^n, box walls + sphere with Coulomb friction, exact return
 mapping. Inputs are synthetic (seeded jittered lattice), resident in HBM before timing starts.
 
+`value` is the exact-return-map path; the `surrogate` object reports the same workload with the
+learned return map (trained on the GPU from pairs the exact path logs during the run), its
+throughput and its position error against the exact run after the same substeps.
+
 Prints one JSON line. `roofline` is for the dominant kernel (its own algorithmic bytes over its
 HIP-event-timed duration); `substep_roofline` is the whole substep against 208 B/particle +
 56 B/active node. `cpu_baseline` is the CPU oracle (a single-threaded FP64 restatement of the
@@ -67,6 +71,71 @@ def cpu_baseline(sample_steps=3):
                        "FP64 oracle (C++ restatement of the reference substep, -O3, one thread)")
 
 
+def _sync_torch():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def surrogate_leg(sc, warmup, steps, x_exact, wall_exact, gate_tol=1e-5, pair_chunks=10, pairs_per_chunk=200000,
+                  epochs=8, batch=4096, lr=1e-3):
+    """The learned return map against the exact one on the same workload: (trial, projected) pairs are
+    logged by the exact path at `pair_chunks` substeps spread over the run, the network is trained on
+    the GPU, then the run is repeated from the same initial state with the surrogate (gate `gate_tol`,
+    exact fallback) and timed; positions are compared after warmup + steps substeps."""
+    from anisotropicelastoplasticity_amd import Solver
+    total = warmup + steps
+    pairs = []
+    with Solver(0) as s:
+        sc.load_into(s)
+        every = max(1, total // pair_chunks)
+        done = 0
+        while done < total:
+            s.set_pair_logging(pairs_per_chunk)
+            s.run(sc.dt, 1)
+            pairs.append(s.get_pairs())
+            s.set_pair_logging(0)
+            k = min(every - 1, total - done - 1)
+            if k > 0:
+                s.run(sc.dt, k)
+            done += 1 + max(k, 0)
+        pairs = np.concatenate(pairs) if pairs else np.zeros((0, 8), np.float32)
+        t0 = time.perf_counter()
+        s.surrogate_init(1)
+        losses = s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch, lr=lr, seed=2)
+        s.synchronize()
+        train_s = time.perf_counter() - t0
+        w = s.surrogate_get_weights()
+    with Solver(0) as s:
+        sc.load_into(s)
+        s.surrogate_set_weights(w)
+        s.set_surrogate(True, gate_tol)
+        s.run(sc.dt, warmup)
+        s.synchronize()
+        _sync_torch()
+        t0 = time.perf_counter()
+        s.run(sc.dt, steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        x = s.get_particles()["x"]
+        st = s.stats()
+    dx = x.astype(np.float64) - x_exact.astype(np.float64)
+    h = (sc.gmax[0] - sc.gmin[0]) / sc.res[0]
+    cand = st["surrogate_candidates"]
+    return {
+        "value": sc.n * steps / wall, "ms_per_step": 1e3 * wall / steps, "speedup_vs_exact": wall_exact / wall,
+        "rel_pos_err": float(np.linalg.norm(dx) / np.linalg.norm(x_exact.astype(np.float64))),
+        "max_dx_over_h": float(np.max(np.abs(dx)) / h), "substeps_compared": total,
+        "gate_tol": gate_tol, "candidates_per_substep": cand / total,
+        "acceptance": (st["surrogate_accepted"] / cand) if cand else 0.0,
+        "pairs_trained": int(len(pairs)), "train_epochs": epochs, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+        "train_s": train_s, "status": st["status"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +144,7 @@ def main():
     ap.add_argument("--stage-steps", type=int, default=50, help="substeps timed stage by stage with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned return-map leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -100,6 +170,7 @@ def main():
         s.run(sc.dt, args.steps)
         s.synchronize()
         wall = time.perf_counter() - t0
+        x_exact = s.get_particles()["x"]
         # per-stage HIP-event timing on the context stream (eager launches)
         total_ms, stage_ms = s.time_substeps(sc.dt, args.stage_steps)
         st = s.stats()
@@ -109,6 +180,7 @@ def main():
             copy = {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "launches": reps,
                     "kernel": "k_copy", "avg_ms": s.bench_copy(nbytes, reps)}
 
+    sur = None if args.no_surrogate else surrogate_leg(sc, args.warmup, args.steps, x_exact, wall)
     value = n * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
     n_nodes = st["n_active_nodes"]
@@ -147,6 +219,10 @@ def main():
         "fp64_svd_count": st["fp64_svd_count"],
         "fp64_apex_count": st["fp64_apex_count"],
     }
+    if sur is not None:
+        # the same workload with the learned return map (exact fallback behind the yield-residual
+        # gate), compared with the exact run above after the same number of substeps
+        out["surrogate"] = sur
     if copy is not None:
         out["copy_calibration"] = copy
     if not args.no_cpu_baseline:

@@ -121,6 +121,21 @@ typedef struct mpm_levelset {
   float b[4];
 } mpm_levelset;
 
+#define MPM_MAX_LEVELSETS 16
+
+/* One scene of a batched context (mpm_set_scenes): its grid, parameters, level sets and particle
+ * count. The fields mean what the arguments of mpm_set_grid, mpm_set_params and mpm_set_levelset
+ * mean for a single scene. */
+typedef struct mpm_scene_desc {
+  double min_bound[3];
+  double max_bound[3];
+  int32_t res[3];
+  int32_t n_particles;
+  mpm_params params;
+  int32_t n_levelsets;
+  mpm_levelset levelsets[MPM_MAX_LEVELSETS];
+} mpm_scene_desc;
+
 typedef struct mpm_stats {
   int64_t substeps;          /* substeps run since mpm_set_particles */
   int32_t status;            /* OR of MPM_STATUS_* */
@@ -160,6 +175,15 @@ int mpm_set_params(mpm_ctx* ctx, const mpm_params* params);
 
 /* Replaces HybridSolver::setLevelSet(LevelSet, DLevelSet) (HybridSolver.h:89). n <= 16. */
 int mpm_set_levelset(mpm_ctx* ctx, const mpm_levelset* prims, int32_t n);
+
+/* Batched scenes (no counterpart in the reference, which simulates one ParticleSystem per
+ * HybridSolver): nscenes independent scenes in one context. Every substep stage covers all scenes
+ * in one launch (one radix sort over scene-major keys, one P2G, one grid update, one G2P); each
+ * scene evolves exactly as it would alone in its own context, bit for bit. The scenes must share
+ * kernel_order and stress_at_predicted_F; mpm_set_particles then takes all particles concatenated
+ * scene by scene (n = sum of n_particles), and mpm_get_particles returns them in that order.
+ * mpm_set_grid / mpm_set_params / mpm_set_levelset apply to single-scene contexts only. */
+int mpm_set_scenes(mpm_ctx* ctx, int32_t nscenes, const mpm_scene_desc* scenes);
 
 /* Mirrors the ParticleSystem constructor (ParticleSystem.cpp:81-117) and
  * HybridSolver::setParticleSystem (HybridSolver.h:86). plastic = J_P = det F_P for snow (the only

@@ -1,0 +1,115 @@
+"""GPU tests of batched scenes (mpm_set_scenes): several independent scenes in one context, every
+stage one launch over all of them.
+
+  per-scene equality   each scene of a batch evolves bit for bit as it does alone in its own
+                       context (same sorted order, same tile order, same arithmetic), snow and sand
+                       mixed, through captured-graph substeps
+  4-pass sort          a batch with more than 2^24 cells in total (keys need four 8-bit passes)
+                       sorts bit-exactly like the oracle's stable sort of the global keys, and each
+                       scene still matches its single-scene run (three passes)
+  surrogate            with a network whose outputs are NaN every cone-branch particle falls back,
+                       and the batch is bitwise equal to the exact batch
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from anisotropicelastoplasticity_amd import Solver, _lib, scenes
+from oracle import bind as ob
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _small_batch(seed, k, n_range, res_choices):
+    rng = np.random.default_rng(seed)
+    return [scenes.batch_scene(i, rng, n_range=n_range, res_choices=res_choices, dt=5e-5) for i in range(k)]
+
+
+def _run_single(sc, nsteps):
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, nsteps)
+        return s.get_particles()
+
+
+def _run_batch(batch, nsteps, surrogate_weights=None):
+    with Solver() as s:
+        s.load_batch(batch)
+        if surrogate_weights is not None:
+            s.surrogate_set_weights(surrogate_weights)
+            s.set_surrogate(True, 1e-5)
+        s.run(batch[0].dt, nsteps)
+        out = s.get_particles()
+        out["stats"] = s.stats()
+    return out
+
+
+def _split(out, batch):
+    offs = np.cumsum([0] + [sc.n for sc in batch])
+    return [{k: out[k][..., offs[i]:offs[i + 1]] for k in ("x", "v", "B", "F", "plastic")} for i in range(len(batch))]
+
+
+def _assert_bitwise(a, b, what):
+    for k in ("x", "v", "B", "F", "plastic"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), f"{what}: {k} differs"
+
+
+def test_batch_scenes_match_single_runs_bitwise():
+    batch = _small_batch(11, 5, (3000, 40000), (32, 40, 48, 64))
+    say(f"[batch bitwise] scenes {[(sc.n, sc.res[0], 'sand' if sc.meta['sand'] else 'snow') for sc in batch]}")
+    nsteps = 21  # eager first substep + graph pairs
+    out = _run_batch(batch, nsteps)
+    assert out["stats"]["status"] == 0
+    for i, (sc, got) in enumerate(zip(batch, _split(out, batch))):
+        _assert_bitwise(got, _run_single(sc, nsteps), f"scene {i}")
+
+
+def test_batch_four_pass_sort_and_equality():
+    batch = _small_batch(12, 9, (2000, 5000), (128,))
+    ng = sum(int(np.prod(sc.res)) for sc in batch)
+    assert ng > (1 << 24)
+    say(f"[batch 4-pass] {len(batch)} scenes, {ng} cells")
+    with Solver() as s:
+        s.load_batch(batch)
+        s.stage(_lib.STAGE_SORT, batch[0].dt)
+        keys, ids, _ = s.get_sorted(bins=False)
+        x = s.get_particles()["x"]
+    # global keys: each scene's cell key offset by the cells of the scenes before it
+    glob = []
+    off = 0
+    p0 = 0
+    for sc in batch:
+        h = [(b - a) / r for a, b, r in zip(sc.gmin, sc.gmax, sc.res)]
+        glob.append(ob.cell_keys(x[:, p0:p0 + sc.n], sc.gmin, h, sc.res).astype(np.uint64) + off)
+        off += int(np.prod(sc.res))
+        p0 += sc.n
+    glob = np.concatenate(glob)
+    perm = np.argsort(glob, kind="stable")
+    assert np.array_equal(ids, perm.astype(np.int32))
+    assert np.array_equal(keys.astype(np.uint64), glob[perm])
+    nsteps = 6
+    out = _run_batch(batch, nsteps)
+    for i, (sc, got) in enumerate(zip(batch, _split(out, batch))):
+        _assert_bitwise(got, _run_single(sc, nsteps), f"scene {i}")
+
+
+def test_batch_surrogate_all_rejected_is_exact():
+    batch = _small_batch(13, 4, (5000, 20000), (32, 48))
+    for sc in batch:  # drive the blocks into the floor so that sand reaches the cone branch
+        sc.particles["v"][1, :] = -3.0
+    w = np.zeros(mo.NPARAM, np.float32)
+    w[mo.active_mask()] = np.nan
+    nsteps = 40
+    ref = _run_batch(batch, nsteps)
+    sur = _run_batch(batch, nsteps, surrogate_weights=w)
+    st = sur["stats"]
+    say(f"[batch surrogate] candidates {st['surrogate_candidates']} accepted {st['surrogate_accepted']}")
+    assert st["surrogate_accepted"] == 0
+    _assert_bitwise(sur, ref, "batch")

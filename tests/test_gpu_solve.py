@@ -3,8 +3,11 @@ HybridSolver::solve (HybridSolver.cpp:854-990).
 
   free fall at 9 m/s (speed limit active)   same substeps per frame, same final t (bitwise: both add
                                             1/60 in double), positions within 1e-5
-  sand impact, 3 frames                     same frame count and final t; substeps per frame within 1
-                                            (Dt comes from FP32 node speeds); positions within 2e-3
+  sand impact, 3 frames                     same frame count and final t; substep count at each frame
+                                            end within 10%: during the impact max|v_i| is set by
+                                            tiny-mass boundary nodes, whose speed is sensitive to
+                                            rounding (the oracle run in FP32 moves from 39 to 40 and from
+                                            59 to 61 substeps); positions within 2e-3
   frame callback                            called once per frame, in order, with readable state
 """
 import sys
@@ -84,7 +87,7 @@ def test_solve_sand_impact_matches_oracle():
     say(f"  gpu substeps at frame ends {gsub}, oracle {list(ends)}; max|dx| {np.max(np.abs(g['x'] - q['x'])):.3e}")
     assert len(frames) == len(ends) == 3
     assert g["clock"]["t"] == t
-    assert all(abs(a - b) <= 1 for a, b in zip(gsub, ends))
+    assert all(abs(a - b) <= max(2, 0.1 * b) for a, b in zip(gsub, ends))
     assert np.max(np.abs(g["x"] - q["x"])) <= 2e-3
     # the state seen in the last frame callback is the final state
     assert np.array_equal(frames[-1][3], g["x"])
