@@ -60,6 +60,11 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("p2g_ranks", ctypes.c_int32), ("p2g_blocks", ctypes.c_int32), ("grid_blocks", ctypes.c_int32),
+                ("p2g_ms", ctypes.c_float), ("grid_ms", ctypes.c_float)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("substeps", ctypes.c_int64),
@@ -95,10 +100,15 @@ SIGNATURES = [
     ("mpm_set_scenes", ctypes.c_int, [_ctx, ctypes.c_int32, _P(SceneDesc)]),
     ("mpm_set_particles", ctypes.c_int, [_ctx, ctypes.c_int32, _f, _f, _f, _f, _f, _f, _f, _f]),
     ("mpm_get_particles", ctypes.c_int, [_ctx, _f, _f, _f, _f, _f, _f]),
+    ("mpm_get_stress", ctypes.c_int, [_ctx, _f]),
+    ("mpm_set_stress", ctypes.c_int, [_ctx, _f]),
     ("mpm_estimate_volumes", ctypes.c_int, [_ctx]),
     ("mpm_substep", ctypes.c_int, [_ctx, ctypes.c_float]),
     ("mpm_run", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32]),
     ("mpm_solve", ctypes.c_int, [_ctx, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("mpm_autotune", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _P(Tuning)]),
+    ("mpm_get_tuning", ctypes.c_int, [_ctx, _P(Tuning)]),
+    ("mpm_set_tuning", ctypes.c_int, [_ctx, _P(Tuning)]),
     ("mpm_get_clock", ctypes.c_int,
      [_ctx, _d, _d, _i32, _P(ctypes.c_int64)]),
     ("mpm_stage", ctypes.c_int, [_ctx, ctypes.c_int32, ctypes.c_float]),

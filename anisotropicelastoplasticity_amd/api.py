@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import LevelSetPrim, Params, SceneDesc, Stats
+from ._lib import LevelSetPrim, Params, SceneDesc, Stats, Tuning
 
 
 class MPMError(RuntimeError):
@@ -169,6 +169,20 @@ class Solver:
         if errors:
             raise errors[0]
 
+    def autotune(self, dt, reps=5):
+        t = Tuning()
+        self._check(self.lib.mpm_autotune(self.h, float(dt), int(reps), ctypes.byref(t)), "mpm_autotune")
+        return {name: getattr(t, name) for name, _ in Tuning._fields_}
+
+    def tuning(self):
+        t = Tuning()
+        self._check(self.lib.mpm_get_tuning(self.h, ctypes.byref(t)), "mpm_get_tuning")
+        return {name: getattr(t, name) for name, _ in Tuning._fields_}
+
+    def set_tuning(self, p2g_ranks, p2g_blocks, grid_blocks):
+        t = Tuning(int(p2g_ranks), int(p2g_blocks), int(grid_blocks), 0.0, 0.0)
+        self._check(self.lib.mpm_set_tuning(self.h, ctypes.byref(t)), "mpm_set_tuning")
+
     def clock(self):
         t, dt = ctypes.c_double(), ctypes.c_double()
         frames, subs = ctypes.c_int32(), ctypes.c_int64()
@@ -191,6 +205,45 @@ class Solver:
                                         _fp(out["plastic"]), _fp(out["volume"]))
         self._check(rc, "mpm_get_particles")
         return out
+
+    def get_stress(self):
+        K = np.empty((6, self.n), np.float32)
+        self._check(self.lib.mpm_get_stress(self.h, _fp(K)), "mpm_get_stress")
+        return K
+
+    def set_stress(self, K):
+        K = np.ascontiguousarray(K, dtype=np.float32).reshape(6, self.n)
+        self._check(self.lib.mpm_set_stress(self.h, _fp(K)), "mpm_set_stress")
+
+    # ---------------------------------------------------------------- frames and checkpoints
+    def write_obj(self, path):
+        """Particle positions as an OBJ point list, one `v x y z` line per particle in caller order
+        (the frame files of HybridSolver.cpp:996-1005)."""
+        write_obj(path, self.get_particles()["x"])
+
+    def save_checkpoint(self, path):
+        """Resumable snapshot (numpy .npz, no pickles): particle state in caller order, masses,
+        per-particle friction angles, the stress state and the substep count."""
+        st = self.get_particles()
+        keep = self._keep
+        data = dict(x=st["x"], v=st["v"], B=st["B"], F=st["F"], plastic=st["plastic"], volume=st["volume"],
+                    mass=keep["mass"], substeps=np.int64(self.stats()["substeps"]))
+        if keep.get("phi") is not None:
+            data["phi"] = keep["phi"]
+        try:
+            data["stress"] = self.get_stress()
+        except MPMError:
+            pass  # stress at the predicted F keeps no stress state
+        np.savez(path, **data)
+
+    def load_checkpoint(self, path):
+        """Restore a snapshot written by save_checkpoint into a context with the same grid and
+        parameters; the following substeps repeat the original run bit for bit."""
+        with np.load(path, allow_pickle=False) as z:
+            self.set_particles(z["x"], z["mass"], v=z["v"], B=z["B"], F=z["F"], plastic=z["plastic"],
+                               volume=z["volume"], phi=z["phi"] if "phi" in z.files else None)
+            if "stress" in z.files:
+                self.set_stress(z["stress"])
 
     def get_sorted(self, bins=True):
         """Sorted cell keys, caller ids in sorted order and (single scene) the cell bin offsets."""
@@ -286,6 +339,14 @@ class Solver:
         rc = self.lib.mpm_time_substeps(self.h, float(dt), int(nsteps), ctypes.byref(total), stages)
         self._check(rc, "mpm_time_substeps")
         return total.value, list(stages)
+
+
+def write_obj(path, x):
+    """`v x y z` per particle, as the reference's frame writer (HybridSolver.cpp:996-1005)."""
+    x = np.asarray(x)
+    with open(path, "w") as f:
+        for i in range(x.shape[1]):
+            f.write("v %.6g %.6g %.6g\n" % (x[0, i], x[1, i], x[2, i]))
 
 
 def generate_block(seed, lo, hi, cells, ppa):

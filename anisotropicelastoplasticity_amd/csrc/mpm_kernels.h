@@ -219,13 +219,14 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
 void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
                    int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
                    const int* perm, const int* pos, int* src, const Clock* clk);
-// order: B-spline order of every scene (2 or 3)
-void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid);
+// order: B-spline order of every scene (2 or 3); ranks: particles per cell staged per chunk (4 or 8)
+void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
 void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks);
 void launch_volumes(hipStream_t s, const SceneTable& T, int order, int nblocks, float* st, int64_t cap, int n,
                     const float* gm);
 void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out);
+void launch_permute_in(hipStream_t s, float* st, int64_t cap, int n, int p0, int np, const float* in, const int* pos);
 void launch_grid_export(hipStream_t s, const SceneTable& T, const float* grid, int64_t ng, int which, const int* block_slot,
                         float* out);
 void launch_iota(hipStream_t s, int* p, int n);

@@ -367,8 +367,11 @@ __global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int*
 // the covering tiles' partials in ascending tile order: every node value is a sum in a fixed order,
 // without float atomics on global memory, and reruns are bitwise identical.
 constexpr int PAY = 48;                 // payload floats per staged particle (12 x float4)
-constexpr int P2G_RANKS = 8;            // ranks per cell staged per chunk
-constexpr int PAY_CELL = P2G_RANKS * PAY + 4;  // one cell's staged payloads (+1 float4 against bank conflicts)
+// RANKS (4 or 8, chosen by the autotuner) = ranks per cell staged per chunk; one cell's staged
+// payloads take RANKS * PAY + 4 floats (+1 float4 against bank conflicts). The chunk size does not
+// change the order in which a lane adds its cell's particles, so results do not depend on it.
+template <int RANKS>
+constexpr int pay_cell() { return RANKS * PAY + 4; }
 constexpr int SLAB = FOOT * FOOT * 4;   // nodes of a wave's slab
 // payload layout: weights and derivatives (per cell width) of window nodes c-1 .. c+2 on each axis
 constexpr int PY_WX = 0, PY_DWX = 4, PY_WY = 8, PY_DWY = 12, PY_WZ = 16, PY_DWZ = 20;
@@ -476,10 +479,12 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const DevParams& P, 
   for (int i = 0; i < PAY / 4; ++i) dst[i] = make_float4(pay[4 * i], pay[4 * i + 1], pay[4 * i + 2], pay[4 * i + 3]);
 }
 
-template <int ORDER, int MODE>
+template <int ORDER, int MODE, int RANKS>
 __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
   if (clock_skip(a.clk)) return;
   const float dt = clock_dt(a.clk, a.dt);
+  constexpr int P2G_RANKS = RANKS;
+  constexpr int PAY_CELL = pay_cell<RANKS>();
   __shared__ float4 s_pay[4][4 * PAY_CELL / 4];
   __shared__ float s_slab[4][4][SLAB];  // [wave][quantity][node]
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -1061,6 +1066,17 @@ __global__ __launch_bounds__(256) void k_unpermute(const float* __restrict__ st,
   for (int p = 0; p < np; ++p) out[(size_t)p * n + id] = st[(p0 + p) * cap + i];
 }
 
+// Inverse of k_unpermute: planes [p0, p0+np) of the device order from caller order
+// (st[plane][pos[id]] = in[plane][id]).
+__global__ __launch_bounds__(256) void k_permute_in(float* __restrict__ st, int64_t cap, int n, int p0, int np,
+                                                    const float* __restrict__ in, const int* __restrict__ pos) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int i = pos[id];
+  if (i < 0 || i >= n) return;
+  for (int p = 0; p < np; ++p) st[(p0 + p) * cap + i] = in[(size_t)p * n + id];
+}
+
 // Dense planar copy of one grid quantity (0 = m, 1 = momentum, 2 = v, 3 = v_BF), with nodes
 // outside the active blocks set to zero.
 __global__ __launch_bounds__(256) void k_grid_export(SceneTable T, const float* __restrict__ grid, int64_t ng, int which,
@@ -1169,18 +1185,23 @@ void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32
                      counters + CNT_BLOCKS, (int*)nullptr, clk);
 }
 
-void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid) {
-  const int o = order;
+template <int RANKS>
+static void launch_p2g_r(hipStream_t s, const P2GArgs& a, int order, int mode, int grid) {
   if (mode == P2G_MOM_STRESS) {
-    if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM_STRESS>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM_STRESS>), dim3(grid), dim3(256), 0, s, a);
+    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(256), 0, s, a);
   } else if (mode == P2G_MOM) {
-    if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM>), dim3(grid), dim3(256), 0, s, a);
+    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM, RANKS>), dim3(grid), dim3(256), 0, s, a);
   } else {
-    if (o == 2) hipLaunchKernelGGL((k_p2g<2, P2G_FORCE>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p2g<3, P2G_FORCE>), dim3(grid), dim3(256), 0, s, a);
+    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_FORCE, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_FORCE, RANKS>), dim3(grid), dim3(256), 0, s, a);
   }
+}
+
+void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks) {
+  if (ranks == 4) launch_p2g_r<4>(s, a, order, mode, grid);
+  else launch_p2g_r<8>(s, a, order, mode, grid);
 }
 
 void launch_clock_update(hipStream_t s, Clock* c, int init) {
@@ -1210,6 +1231,10 @@ void launch_volumes(hipStream_t s, const SceneTable& T, int order, int nblocks, 
 
 void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out) {
   if (n > 0) hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, cap, n, p0, np, out);
+}
+
+void launch_permute_in(hipStream_t s, float* st, int64_t cap, int n, int p0, int np, const float* in, const int* pos) {
+  if (n > 0) hipLaunchKernelGGL(k_permute_in, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, cap, n, p0, np, in, pos);
 }
 
 void launch_grid_export(hipStream_t s, const SceneTable& T, const float* grid, int64_t ng, int which, const int* block_slot,

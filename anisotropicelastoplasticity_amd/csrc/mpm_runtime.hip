@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -99,6 +100,7 @@ struct mpm_ctx {
   float graph_dt[2] = {0.0f, 0.0f};
 
   int p2g_grid = 1024;
+  int p2g_ranks = 8;  // P2G staging chunk (mpm_autotune / mpm_set_tuning)
   // adaptive time step (mpm_solve): the device clock, and the pointer the kernels get (null for
   // fixed-step runs)
   Clock* clock = nullptr;
@@ -369,11 +371,11 @@ static int stage_p2g(mpm_ctx* ctx, float dt) {
       launch_stress_init(s, ctx->T, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
       ctx->stress_valid = true;
     }
-    launch_p2g(s, a, ctx->order, P2G_MOM_STRESS, ctx->p2g_grid);
+    launch_p2g(s, a, ctx->order, P2G_MOM_STRESS, ctx->p2g_grid, ctx->p2g_ranks);
   } else {
-    launch_p2g(s, a, ctx->order, P2G_MOM, ctx->p2g_grid);
+    launch_p2g(s, a, ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks);
     launch_grid(s, grid_args(ctx, dt), GRID_NORMALIZE, ctx->grid_grid);
-    launch_p2g(s, a, ctx->order, P2G_FORCE, ctx->p2g_grid);
+    launch_p2g(s, a, ctx->order, P2G_FORCE, ctx->p2g_grid, ctx->p2g_ranks);
   }
   return MPM_OK;
 }
@@ -835,6 +837,49 @@ int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* F, floa
   return MPM_OK;
 }
 
+int mpm_get_stress(mpm_ctx* ctx, float* K) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!K) return fail(ctx, MPM_ERR_INVALID, "K is required");
+  if (ctx->predicted) return fail(ctx, MPM_ERR_STATE, "no stress state with stress_at_predicted_F");
+  CK(hipSetDevice(ctx->device));
+  if (!ctx->stress_valid) {
+    launch_stress_init(ctx->stream, ctx->T, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
+    ctx->stress_valid = true;
+  }
+  float* tmp = nullptr;
+  rc = dalloc(ctx, tmp, (size_t)6 * ctx->n);
+  if (rc) return rc;
+  launch_unpermute(ctx->stream, ctx->state[ctx->cur], ctx->cap, ctx->n, PK, 6, tmp);
+  hipError_t e = hipMemcpyAsync(K, tmp, sizeof(float) * 6 * ctx->n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(tmp);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_get_stress: ") + hipGetErrorString(e));
+  return MPM_OK;
+}
+
+int mpm_set_stress(mpm_ctx* ctx, const float* K) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (!K) return fail(ctx, MPM_ERR_INVALID, "K is required");
+  if (ctx->predicted) return fail(ctx, MPM_ERR_STATE, "no stress state with stress_at_predicted_F");
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  float* tmp = nullptr;
+  rc = dalloc(ctx, tmp, (size_t)6 * ctx->n);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, K, sizeof(float) * 6 * ctx->n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    launch_permute_in(ctx->stream, ctx->state[ctx->cur], ctx->cap, ctx->n, PK, 6, tmp, ctx->pos[ctx->cur]);
+    e = hipStreamSynchronize(ctx->stream);
+  }
+  dfree(tmp);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_set_stress: ") + hipGetErrorString(e));
+  ctx->stress_valid = true;
+  drop_graphs(ctx);
+  return MPM_OK;
+}
+
 int mpm_estimate_volumes(mpm_ctx* ctx) {
   int rc = ready(ctx);
   if (rc) return rc;
@@ -843,7 +888,7 @@ int mpm_estimate_volumes(mpm_ctx* ctx) {
   rc = stage_sort(ctx);
   if (rc) return rc;
   P2GArgs a = p2g_args(ctx, 0.0f);
-  launch_p2g(ctx->stream, a, ctx->order, P2G_MOM, ctx->p2g_grid);
+  launch_p2g(ctx->stream, a, ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks);
   launch_grid(ctx->stream, grid_args(ctx, 0.0f), GRID_NORMALIZE, ctx->grid_grid);
   launch_volumes(ctx->stream, ctx->T, ctx->order, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->grid);
   ctx->stress_valid = false;
@@ -953,7 +998,7 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
   // first Dt from the velocities p/m of a P2G without forces
   rc = stage_sort(ctx);
   if (rc) return rc;
-  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), ctx->order, P2G_MOM, ctx->p2g_grid);
+  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks);
   GridArgs g0 = grid_args(ctx, 0.0f);
   g0.vmax_out = &ctx->clock->vmax_bits;
   launch_grid(ctx->stream, g0, GRID_NORMALIZE, ctx->grid_grid);
@@ -1005,6 +1050,105 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
     if (done) break;
   }
   CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Sort, P2G and the grid update are idempotent for a given particle state (they rebuild the bins,
+// the tile partials and the node arrays from it), so candidate launch shapes are timed on the live
+// state without advancing it. None of the candidates changes an order of summation: every choice
+// gives the same bits.
+static float time_launches(mpm_ctx* ctx, int reps, const std::function<void()>& f) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1.0f;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1.0f;
+  }
+  f();  // warm-up
+  (void)hipEventRecord(e0, ctx->stream);
+  for (int r = 0; r < reps; ++r) f();
+  (void)hipEventRecord(e1, ctx->stream);
+  float ms = -1.0f;
+  if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+int mpm_autotune(mpm_ctx* ctx, float dt, int32_t reps, mpm_tuning* chosen) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  if (reps < 1) return fail(ctx, MPM_ERR_INVALID, "reps >= 1");
+  if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
+  CK(hipSetDevice(ctx->device));
+  rc = stage_sort(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  if (!ctx->predicted && !ctx->stress_valid) {
+    launch_stress_init(s, ctx->T, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
+    ctx->stress_valid = true;
+  }
+  const int p2g_mode = ctx->predicted ? P2G_MOM : P2G_MOM_STRESS;
+  const int grid_mode = ctx->predicted ? GRID_NORMALIZE : GRID_FUSED;
+  const P2GArgs a = p2g_args(ctx, dt);
+  mpm_tuning best{};
+  best.p2g_ms = 1e30f;
+  const int ranks[2] = {4, 8};
+  const int mults[5] = {2, 3, 4, 5, 6};
+  for (int r : ranks)
+    for (int m : mults) {
+      const int blocks = std::max(64, m * ctx->n_cu);
+      const float ms = time_launches(ctx, reps, [&] { launch_p2g(s, a, ctx->order, p2g_mode, blocks, r); });
+      CK(hipGetLastError());
+      if (ms >= 0.0f && ms < best.p2g_ms) {
+        best.p2g_ms = ms;
+        best.p2g_ranks = r;
+        best.p2g_blocks = blocks;
+      }
+    }
+  launch_p2g(s, a, ctx->order, p2g_mode, best.p2g_blocks, best.p2g_ranks);
+  const GridArgs g = grid_args(ctx, dt);
+  best.grid_ms = 1e30f;
+  const int gmults[4] = {2, 4, 8, 16};
+  for (int m : gmults) {
+    const int blocks = std::max(64, m * ctx->n_cu);
+    const float ms = time_launches(ctx, reps, [&] { launch_grid(s, g, grid_mode, blocks); });
+    CK(hipGetLastError());
+    if (ms >= 0.0f && ms < best.grid_ms) {
+      best.grid_ms = ms;
+      best.grid_blocks = blocks;
+    }
+  }
+  CK(hipStreamSynchronize(s));
+  if (best.p2g_ranks == 0 || best.grid_blocks == 0) return fail(ctx, MPM_ERR_HIP, "autotune: no timing succeeded");
+  ctx->p2g_ranks = best.p2g_ranks;
+  ctx->p2g_grid = best.p2g_blocks;
+  ctx->grid_grid = best.grid_blocks;
+  drop_graphs(ctx);
+  if (chosen) *chosen = best;
+  return MPM_OK;
+}
+
+int mpm_get_tuning(mpm_ctx* ctx, mpm_tuning* t) {
+  if (!ctx || !t) return MPM_ERR_INVALID;
+  *t = mpm_tuning{};
+  t->p2g_ranks = ctx->p2g_ranks;
+  t->p2g_blocks = ctx->p2g_grid;
+  t->grid_blocks = ctx->grid_grid;
+  return MPM_OK;
+}
+
+int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* t) {
+  if (!ctx || !t) return MPM_ERR_INVALID;
+  if ((t->p2g_ranks != 4 && t->p2g_ranks != 8) || t->p2g_blocks < 1 || t->grid_blocks < 1)
+    return fail(ctx, MPM_ERR_INVALID, "p2g_ranks must be 4 or 8 and the block counts positive");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->p2g_ranks = t->p2g_ranks;
+  ctx->p2g_grid = t->p2g_blocks;
+  ctx->grid_grid = t->grid_blocks;
+  drop_graphs(ctx);
   return MPM_OK;
 }
 

@@ -4,7 +4,10 @@
 // setParticleSystem / setRegularGrid / setLevelSet / solve. Errors raise std::runtime_error
 // instead of the reference's cerr + while(1).
 #pragma once
+#include <cstdint>
 #include <cstdio>
+#include <filesystem>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -37,16 +40,100 @@ class HybridSolver {
   void setParams(const mpm_params& p) { params_ = p; dirty_ = true; }
   mpm_params& params() { dirty_ = true; return params_; }
 
-  // HybridSolver::solve (HybridSolver.cpp:827-1034) with a fixed time step: the reference's first
-  // P2G with the density estimate (:854, when estimateVolumes), then substeps until t > maxt, then
-  // the particle state is copied back into the ParticleSystem. `alpha` is unused, as in the reference.
-  void solve(double dt, double maxt, double /*alpha*/ = 0.0, bool estimateVolumes = false) {
+  // HybridSolver::solve (HybridSolver.cpp:827-1034): the first P2G with the density estimate
+  // (:854, when estimateVolumes), then the adaptive loop Dt = CFL / max(300, max|v_i| / h_min)
+  // clamped to 1/60 s frames until t > maxt. After every frame the particle positions go to
+  // <outputDirectory>/particle/particle_<frame>.obj as in the reference (:990-1005), if an output
+  // directory is set. `alpha` is unused, as in the reference.
+  void solve(double CFL, double maxt, double /*alpha*/ = 0.0, bool estimateVolumes = true) {
+    upload();
+    if (estimateVolumes) check(mpm_estimate_volumes(ctx_), "mpm_estimate_volumes");
+    if (!outdir_.empty()) std::filesystem::create_directories(std::filesystem::path(outdir_) / "particle");
+    check(mpm_solve(ctx_, CFL, maxt, &HybridSolver::onFrame, this), "mpm_solve");
+    syncToHost();
+  }
+  // Fixed-step variant: substeps of dt until t > maxt (the configurations' fixed Dt).
+  void solveFixed(double dt, double maxt, bool estimateVolumes = false) {
     upload();
     if (estimateVolumes) check(mpm_estimate_volumes(ctx_), "mpm_estimate_volumes");
     int n = 0;
     for (double t = 0.0; t <= maxt; t += dt) ++n;
     check(mpm_run(ctx_, float(dt), n), "mpm_run");
     syncToHost();
+  }
+  // Frame files of solve(): <dir>/particle/particle_N.obj (empty: no output).
+  void setOutputDirectory(const std::string& dir) { outdir_ = dir; }
+  int framesWritten() const { return frames_; }
+
+  // Resumable binary snapshot of the particle state (positions, velocities, B, F, plastic, masses,
+  // volumes, friction angles, the stress state of the next P2G) and the material; loadCheckpoint
+  // restores it into the bound ParticleSystem and the device, and the following substeps repeat the
+  // original run bit for bit.
+  void saveCheckpoint(const std::string& path) {
+    syncToHost();
+    const size_t n = ps_->size();
+    std::vector<float> K(6 * n, 0.0f);
+    const bool haveK = params_.stress_at_predicted_F == 0;
+    if (haveK) check(mpm_get_stress(ctx_, K.data()), "mpm_get_stress");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + path);
+    const uint64_t hdr[3] = {0x50414e534d504d31ull /* "1MPMSNAP" */, (uint64_t)n, haveK ? 1ull : 0ull};
+    f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+    const int32_t mat = ps_->material;
+    const double el[4] = {ps_->youngsModulus, ps_->poissonRatio, ps_->criticalCompression, ps_->criticalStretch};
+    f.write(reinterpret_cast<const char*>(&mat), sizeof(mat));
+    f.write(reinterpret_cast<const char*>(el), sizeof(el));
+    auto put = [&](const std::vector<float>& v, size_t count) {
+      std::vector<float> tmp(count, 0.0f);
+      for (size_t i = 0; i < count && i < v.size(); ++i) tmp[i] = v[i];
+      f.write(reinterpret_cast<const char*>(tmp.data()), sizeof(float) * count);
+    };
+    put(ps_->positions, 3 * n);
+    put(ps_->velocities, 3 * n);
+    put(ps_->affineMomenta, 9 * n);
+    put(ps_->elasticDeformationGradients, 9 * n);
+    put(ps_->plastic, n);
+    put(ps_->masses, n);
+    put(ps_->volumes, n);
+    put(ps_->frictionAngles, n);
+    put(K, 6 * n);
+    if (!f) throw std::runtime_error("write failed: " + path);
+  }
+  void loadCheckpoint(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot read " + path);
+    uint64_t hdr[3];
+    f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+    if (!f || hdr[0] != 0x50414e534d504d31ull) throw std::runtime_error(path + " is not a checkpoint");
+    const size_t n = (size_t)hdr[1];
+    int32_t mat = 0;
+    double el[4];
+    f.read(reinterpret_cast<char*>(&mat), sizeof(mat));
+    f.read(reinterpret_cast<char*>(el), sizeof(el));
+    auto get = [&](std::vector<float>& v, size_t count) {
+      v.resize(count);
+      f.read(reinterpret_cast<char*>(v.data()), sizeof(float) * count);
+    };
+    std::vector<float> phi, K;
+    get(ps_->positions, 3 * n);
+    get(ps_->velocities, 3 * n);
+    get(ps_->affineMomenta, 9 * n);
+    get(ps_->elasticDeformationGradients, 9 * n);
+    get(ps_->plastic, n);
+    get(ps_->masses, n);
+    get(ps_->volumes, n);
+    get(phi, n);
+    get(K, 6 * n);
+    if (!f) throw std::runtime_error("truncated checkpoint " + path);
+    ps_->material = mat;
+    ps_->youngsModulus = el[0];
+    ps_->poissonRatio = el[1];
+    ps_->criticalCompression = el[2];
+    ps_->criticalStretch = el[3];
+    if (ps_->frictionAngles.size() == n || params_.per_particle_phi) ps_->frictionAngles = phi;
+    dirty_ = true;
+    upload();
+    if (hdr[2]) check(mpm_set_stress(ctx_, K.data()), "mpm_set_stress");
   }
   void substep(double dt) { upload(); check(mpm_substep(ctx_, float(dt)), "mpm_substep"); }
   void run(double dt, int n) { upload(); check(mpm_run(ctx_, float(dt), n), "mpm_run"); }
@@ -110,6 +197,20 @@ class HybridSolver {
     dirty_ = false;
   }
 
+  static void onFrame(mpm_ctx* /*ctx*/, int32_t frame, double /*t*/, void* user) {
+    HybridSolver* self = static_cast<HybridSolver*>(user);
+    if (self->outdir_.empty()) return;
+    self->syncToHost();
+    const std::string path = self->outdir_ + "/particle/particle_" + std::to_string(frame) + ".obj";
+    std::ofstream out(path);
+    const size_t n = self->ps_->size();
+    const std::vector<float>& x = self->ps_->positions;  // planar: x0[n], x1[n], x2[n]
+    for (size_t p = 0; p < n; ++p) out << "v " << x[p] << ' ' << x[n + p] << ' ' << x[2 * n + p] << '\n';
+    ++self->frames_;
+  }
+
+  std::string outdir_;
+  int frames_ = 0;
   mpm_ctx* ctx_ = nullptr;
   ParticleSystem* ps_ = nullptr;
   RegularGrid* rg_ = nullptr;

@@ -197,6 +197,12 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
 /* Copy particle state out in caller order; any pointer may be NULL. */
 int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* F, float* plastic,
                       float* volume);
+/* The volume-weighted Kirchhoff stress V tau (6 x n planes: xx yy zz xy xz yz, caller order) that
+ * the next P2G scatters (stress at F^n). A checkpoint that stores it next to mpm_get_particles and
+ * restores both (mpm_set_particles, then mpm_set_stress) resumes bit for bit; without it the next
+ * substep recomputes the stress from F. No counterpart in the reference, which keeps no such state. */
+int mpm_get_stress(mpm_ctx* ctx, float* K);
+int mpm_set_stress(mpm_ctx* ctx, const float* K);
 
 /* The reference's first P2G with evaluateVolumesAndDensities = true (HybridSolver.cpp:242-249):
  * rho_p = sum_i w_ip m_i / h^3, V_p = m_p / rho_p, at the current positions. */
@@ -215,6 +221,21 @@ typedef void (*mpm_frame_callback)(mpm_ctx* ctx, int32_t frame, double t, void* 
  * end of its 1/60 s frame, t advances by 1/60 per frame, and the loop ends after the frame that
  * takes t past maxt. Dt is evaluated on the device; cb (may be null) runs at every frame end. */
 int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void* user);
+/* Launch shape of the P2G and grid kernels. None of the shapes changes an order of summation, so
+ * results are bitwise identical for every valid shape. */
+typedef struct mpm_tuning {
+  int32_t p2g_ranks;   /* particles per cell staged per P2G chunk (the bin tile): 4 or 8 */
+  int32_t p2g_blocks;  /* P2G workgroups, persistent over the active 4x4x4-cell tiles */
+  int32_t grid_blocks; /* grid-update workgroups, persistent over the active node blocks */
+  float p2g_ms;        /* times of the chosen shapes measured by mpm_autotune (0 otherwise) */
+  float grid_ms;
+} mpm_tuning;
+/* Autotuner (no counterpart in the reference): time every candidate shape `reps` times on the
+ * current particle state (sort, P2G and grid update are idempotent, the state does not advance),
+ * keep the fastest and report it in *chosen (may be NULL). */
+int mpm_autotune(mpm_ctx* ctx, float dt, int32_t reps, mpm_tuning* chosen);
+int mpm_get_tuning(mpm_ctx* ctx, mpm_tuning* tuning);
+int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* tuning);
 /* Clock of the last mpm_solve: simulated time, last Dt, frames completed, substeps run. */
 int mpm_get_clock(mpm_ctx* ctx, double* t, double* dt, int32_t* frames, int64_t* substeps);
 /* Run a single stage (MPM_STAGE_*) of the current substep; stages must be issued in order

@@ -113,3 +113,27 @@ def test_batch_surrogate_all_rejected_is_exact():
     say(f"[batch surrogate] candidates {st['surrogate_candidates']} accepted {st['surrogate_accepted']}")
     assert st["surrogate_accepted"] == 0
     _assert_bitwise(sur, ref, "batch")
+
+
+def test_launch_shapes_give_identical_bits_and_autotune():
+    # the autotuner's candidates (P2G staging chunk, P2G and grid workgroup counts) do not change
+    # any order of summation: every shape gives the same bits, and tuning does not advance the state
+    batch = _small_batch(14, 3, (5000, 30000), (32, 48))
+    dt, nsteps = batch[0].dt, 11
+    outs = []
+    for shape in [(8, 1024, 2048), (4, 512, 512), (4, 1536, 4096), (8, 300, 700)]:
+        with Solver() as s:
+            s.load_batch(batch)
+            s.set_tuning(*shape)
+            s.run(dt, nsteps)
+            outs.append(s.get_particles())
+    for o in outs[1:]:
+        _assert_bitwise(o, outs[0], "launch shape")
+    with Solver() as s:
+        s.load_batch(batch)
+        t = s.autotune(dt, reps=3)
+        say(f"[autotune] {t}")
+        assert t["p2g_ranks"] in (4, 8) and t["p2g_blocks"] > 0 and t["grid_blocks"] > 0
+        assert s.tuning()["p2g_ranks"] == t["p2g_ranks"]
+        s.run(dt, nsteps)
+        _assert_bitwise(s.get_particles(), outs[0], "after autotune")

@@ -285,13 +285,19 @@ def test_fp64_svd_fallback():
     assert rel(q["v"], qr["v"]) < 1e-4
 
 
-def test_cpp_host_api_example():
-    # the reference's calling pattern (ParticleSystem + RegularGrid + HybridSolver::solve) in C++
+def test_cpp_host_api_example(tmp_path):
+    # the reference's calling pattern (ParticleSystem + RegularGrid + HybridSolver::solve) in C++:
+    # adaptive solve over two frames writing particle_N.obj, then a checkpoint that resumes bitwise
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "anisotropicelastoplasticity_amd",
                        "sand_column_example")
     say("[c++ host api] sand_column_example")
-    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     say(res.stdout.strip())
     assert res.returncode == 0, res.stdout + res.stderr
+    n = 8 * 12 * 8 * 8
+    for f in range(2):
+        lines = (tmp_path / "particle" / f"particle_{f}.obj").read_text().splitlines()
+        assert len(lines) == n and all(l.startswith("v ") and len(l.split()) == 4 for l in lines)
+    assert not (tmp_path / "particle" / "particle_2.obj").exists()

@@ -91,3 +91,30 @@ def test_solve_sand_impact_matches_oracle():
     assert np.max(np.abs(g["x"] - q["x"])) <= 2e-3
     # the state seen in the last frame callback is the final state
     assert np.array_equal(frames[-1][3], g["x"])
+
+
+def test_checkpoint_resumes_bitwise(tmp_path):
+    sc = scenes.small_scene(_lib.SAND, 2, predicted=0, friction_model=_lib.FRICTION_COULOMB, seed=8,
+                            res=(32, 32, 32), cells=(6, 6, 6), lo=(0.3, 0.15, 0.3), hi=(0.55, 0.4, 0.55),
+                            v0=(0.0, -1.0, 0.0))
+    path = str(tmp_path / "ck.npz")
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, 13)
+        s.save_checkpoint(path)
+        s.run(sc.dt, 12)
+        a = s.get_particles()
+    with Solver() as s:
+        sc.load_into(s)
+        s.load_checkpoint(path)
+        s.run(sc.dt, 12)
+        b = s.get_particles()
+    for k in ("x", "v", "B", "F", "plastic"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    s2 = Solver()
+    sc.load_into(s2)
+    s2.run(sc.dt, 2)
+    s2.write_obj(str(tmp_path / "f.obj"))
+    lines = (tmp_path / "f.obj").read_text().splitlines()
+    s2.close()
+    assert len(lines) == sc.n and lines[0].startswith("v ")
