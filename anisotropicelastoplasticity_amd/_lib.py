@@ -65,6 +65,13 @@ class Tuning(ctypes.Structure):
                 ("p2g_ms", ctypes.c_float), ("grid_ms", ctypes.c_float)]
 
 
+class MeshParams(ctypes.Structure):
+    _fields_ = [("youngs_modulus", ctypes.c_float), ("poisson_ratio", ctypes.c_float),
+                ("shear_stiffness", ctypes.c_float), ("stiffness", ctypes.c_float),
+                ("friction_angle", ctypes.c_float), ("density", ctypes.c_float), ("thickness", ctypes.c_float),
+                ("reference_normal_row", ctypes.c_int32)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("substeps", ctypes.c_int64),
@@ -100,6 +107,9 @@ SIGNATURES = [
     ("mpm_set_scenes", ctypes.c_int, [_ctx, ctypes.c_int32, _P(SceneDesc)]),
     ("mpm_set_particles", ctypes.c_int, [_ctx, ctypes.c_int32, _f, _f, _f, _f, _f, _f, _f, _f]),
     ("mpm_get_particles", ctypes.c_int, [_ctx, _f, _f, _f, _f, _f, _f]),
+    ("mpm_set_mesh", ctypes.c_int, [_ctx, ctypes.c_int32, _f, _f, ctypes.c_int32, _i32, _P(ctypes.c_uint8),
+                                    _P(MeshParams)]),
+    ("mpm_get_mesh", ctypes.c_int, [_ctx, _f, _f, _f]),
     ("mpm_get_stress", ctypes.c_int, [_ctx, _f]),
     ("mpm_set_stress", ctypes.c_int, [_ctx, _f]),
     ("mpm_estimate_volumes", ctypes.c_int, [_ctx]),

@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import LevelSetPrim, Params, SceneDesc, Stats, Tuning
+from ._lib import LevelSetPrim, MeshParams, Params, SceneDesc, Stats, Tuning
 
 
 class MPMError(RuntimeError):
@@ -138,6 +138,31 @@ class Solver:
             cat[k] = np.concatenate(filled, axis=-1)
         self.set_particles(cat["x"], cat["mass"], v=cat["v"], B=cat["B"], F=cat["F"], plastic=cat["plastic"],
                            volume=cat["volume"], phi=cat["phi"])
+
+    def set_mesh(self, x, faces, v=None, fixed=None, youngs_modulus=200.0, poisson_ratio=0.3, shear_stiffness=1e3,
+                 stiffness=1e4, friction_angle=20.0, density=2e3, thickness=0.04, reference_normal_row=False):
+        """Attach a triangle cloth (mpm_set_mesh) after set_particles: x (3, nv), faces (nf, 3)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        nv = x.shape[1]
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 3))
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float32)
+        fx = None if fixed is None else np.ascontiguousarray(np.asarray(fixed, dtype=np.uint8))
+        mp = MeshParams(youngs_modulus, poisson_ratio, shear_stiffness, stiffness, friction_angle, density, thickness,
+                        int(reference_normal_row))
+        self._keep_mesh = (x, f, vv, fx)
+        rc = self.lib.mpm_set_mesh(self.h, nv, _fp(x), _fp(vv), f.shape[0],
+                                   f.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   None if fx is None else fx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                   ctypes.byref(mp))
+        self._check(rc, "mpm_set_mesh")
+        self.nv, self.nf = nv, f.shape[0]
+
+    def get_mesh(self):
+        vx = np.empty((3, self.nv), np.float32)
+        vv = np.empty((3, self.nv), np.float32)
+        D = np.empty((9, self.nf), np.float32)
+        self._check(self.lib.mpm_get_mesh(self.h, _fp(vx), _fp(vv), _fp(D)), "mpm_get_mesh")
+        return dict(vx=vx, vv=vv, D=D)
 
     # ---------------------------------------------------------------- stepping
     def estimate_volumes(self):

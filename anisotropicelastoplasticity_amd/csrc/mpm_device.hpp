@@ -32,7 +32,15 @@ struct DevParams {
   int tiles[3];          // ceil(res/4)
   int nls;
   mpm_levelset ls[16];
+  // cloth: caller ids [mesh_v0, mesh_e0) are mesh vertices, [mesh_e0, mesh_e1) mesh elements
+  int mesh_v0, mesh_e0, mesh_e1;
+  float c_mu, c_lambda, c_gamma, c_k, c_cf;  // in-plane Lame parameters, shear and normal stiffness, tan(friction)
 };
+
+enum ParticleKind : int { KIND_PARTICLE = 0, KIND_VERTEX = 1, KIND_ELEMENT = 2 };
+__device__ __forceinline__ int particle_kind(const DevParams& P, int id) {
+  return id < P.mesh_v0 ? KIND_PARTICLE : (id < P.mesh_e0 ? KIND_VERTEX : KIND_ELEMENT);
+}
 
 // ------------------------------------------------------------------ B-splines
 // Cubic (interpolation.cpp:9-33) and quadratic B-spline value and derivative at r (in cells).
@@ -398,6 +406,146 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
 #pragma unroll
     for (int c = 0; c < 3; ++c) Fh[3 * r + c] = U[3 * r] * Sn[0] * V[3 * c] + U[3 * r + 1] * Sn[1] * V[3 * c + 1] + U[3 * r + 2] * Sn[2] * V[3 * c + 2];
   kirchhoff(P, U, Sn, sgn * Sn[0] * Sn[1] * Sn[2], plastic, vol, K);
+}
+
+// ------------------------------------------------------------------ cloth (LagrangianMesh)
+// Gram-Schmidt QR of D = [d1 d2 d3] (columns; D row-major), GramSchmidtOrthonomalization
+// (geometry.cpp:31-62): Q orthonormal columns, R upper triangular with R(i,i) = column norms.
+template <class T>
+__device__ __forceinline__ void qr3(const T* D, T* Q, T* R) {
+  T q1[3], q2[3], q3[3];
+  const T r11 = sqrt(D[0] * D[0] + D[3] * D[3] + D[6] * D[6]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q1[a] = D[3 * a] / r11;
+  const T r12 = D[1] * q1[0] + D[4] * q1[1] + D[7] * q1[2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q2[a] = D[3 * a + 1] - r12 * q1[a];
+  const T r22 = sqrt(q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q2[a] /= r22;
+  const T r13 = D[2] * q1[0] + D[5] * q1[1] + D[8] * q1[2];
+  const T r23 = D[2] * q2[0] + D[5] * q2[1] + D[8] * q2[2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q3[a] = D[3 * a + 2] - r13 * q1[a] - r23 * q2[a];
+  const T r33 = sqrt(q3[0] * q3[0] + q3[1] * q3[1] + q3[2] * q3[2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    q3[a] /= r33;
+    Q[3 * a] = q1[a];
+    Q[3 * a + 1] = q2[a];
+    Q[3 * a + 2] = q3[a];
+  }
+  R[0] = r11; R[1] = r12; R[2] = r13;
+  R[3] = T(0); R[4] = r22; R[5] = r23;
+  R[6] = T(0); R[7] = T(0); R[8] = r33;
+}
+
+// FP32 QR, redone in FP64 when D is near-degenerate (a column nearly in the span of the previous
+// ones: R(1,1) < ratio |d2| or R(2,2) < ratio |d3|) or the result is not finite.
+__device__ __forceinline__ void qr3_guarded(const float* D, float* Q, float* R, float ratio, int& fallback) {
+  qr3<float>(D, Q, R);
+  const float n2 = sqrtf(D[1] * D[1] + D[4] * D[4] + D[7] * D[7]);
+  const float n3 = sqrtf(D[2] * D[2] + D[5] * D[5] + D[8] * D[8]);
+  const bool finite = isfinite(R[0]) && isfinite(R[4]) && isfinite(R[8]) && isfinite(R[5]);
+  if (!finite || R[4] < ratio * n2 || R[8] < ratio * n3) {
+    double Dd[9], Qd[9], Rd[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Dd[i] = D[i];
+    qr3<double>(Dd, Qd, Rd);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { Q[i] = float(Qd[i]); R[i] = float(Rd[i]); }
+    fallback += 1;
+  }
+}
+
+// Per-element cloth forces from the current directions D (HybridSolver.cpp:370-455 and
+// LagrangianMesh::computeVertexInPlaneForces, LagrangianMesh.cpp:382-460):
+//  - in-plane: A = R_rest^-1 R (upper-left 2x2 blocks of the QR factors), P = 2 mu (A - rot(A)) +
+//    lambda (J - 1) cof(A) with rot the orthogonal polar factor and J the product of A's diagonal;
+//    vertex forces f2 = -(P11 i11 + P12 i12) q1, f3 = -P12 i22 q1 - P22 i22 q2, f1 = -(f2 + f3);
+//  - normal/shear: dR = [[S11 S12 gamma r13], [0 S22 gamma r23], [0 0 -k (1 - r33)^2 or 0 if r33 > 1]]
+//    with S = R_rest^-1 P, K = dR R^T, dF3 = Q sym_upper(K) R^-T cn, element stress V dF3 d3^T.
+// rest = (i11, i12, i22, cn0, cn1, cn2): the inverse rest in-plane R and the normal-force vector.
+__device__ __forceinline__ void cloth_element_forces(const DevParams& P, const float* D, const float* rest, float vol,
+                                                     float* fv /* f1 f2 f3 */, float* M /* 3x3 */, int& fb) {
+  float Q[9], R[9];
+  qr3_guarded(D, Q, R, P.svd_ratio, fb);
+  const float i11 = rest[0], i12 = rest[1], i22 = rest[2];
+  const float a00 = i11 * R[0], a01 = i11 * R[1] + i12 * R[4], a10 = 0.0f, a11 = i22 * R[4];
+  float r00, r01, r10, r11;
+  if (a00 * a11 - a01 * a10 >= 0.0f) {
+    float pp = a00 + a11, qq = a10 - a01;
+    const float nn = sqrtf(pp * pp + qq * qq);
+    if (nn > 0.0f) { pp /= nn; qq /= nn; } else { pp = 1.0f; qq = 0.0f; }
+    r00 = pp; r01 = -qq; r10 = qq; r11 = pp;
+  } else {
+    float pp = a00 - a11, qq = a01 + a10;
+    const float nn = sqrtf(pp * pp + qq * qq);
+    pp /= nn; qq /= nn;
+    r00 = pp; r01 = qq; r10 = qq; r11 = -pp;
+  }
+  const float J = a00 * a11;
+  const float p00 = 2.0f * P.c_mu * (a00 - r00) + P.c_lambda * (J - 1.0f) * a11;
+  const float p01 = 2.0f * P.c_mu * (a01 - r01);
+  const float p10 = 2.0f * P.c_mu * (a10 - r10) + P.c_lambda * (J - 1.0f) * (-a01);
+  const float p11 = 2.0f * P.c_mu * (a11 - r11) + P.c_lambda * (J - 1.0f) * a00;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float q1 = Q[3 * a], q2 = Q[3 * a + 1];
+    const float f2 = -(p00 * i11 + p01 * i12) * q1;
+    const float f3 = -p01 * i22 * q1 - p11 * i22 * q2;
+    fv[3 + a] = f2;
+    fv[6 + a] = f3;
+    fv[a] = -(f2 + f3);
+  }
+  const float s11 = i11 * p00 + i12 * p10, s12 = i11 * p01 + i12 * p11, s22 = i22 * p11;
+  const float dR[9] = {s11, s12, P.c_gamma * R[2], 0.0f, s22, P.c_gamma * R[5], 0.0f, 0.0f,
+                       R[8] > 1.0f ? 0.0f : -P.c_k * (1.0f - R[8]) * (1.0f - R[8])};
+  float K[9];  // dR R^T
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) K[3 * r + c] = dR[3 * r] * R[3 * c] + dR[3 * r + 1] * R[3 * c + 1] + dR[3 * r + 2] * R[3 * c + 2];
+  float Ks[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Ks[3 * r + c] = r <= c ? K[3 * r + c] : K[3 * c + r];
+  // R^-T cn: solve R^T y = cn (R^T lower triangular)
+  float y[3];
+  y[0] = rest[3] / R[0];
+  y[1] = (rest[4] - R[1] * y[0]) / R[4];
+  y[2] = (rest[5] - R[2] * y[0] - R[5] * y[1]) / R[8];
+  float t[3], dF3[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) t[r] = Ks[3 * r] * y[0] + Ks[3 * r + 1] * y[1] + Ks[3 * r + 2] * y[2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) dF3[r] = Q[3 * r] * t[0] + Q[3 * r + 1] * t[1] + Q[3 * r + 2] * t[2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) M[3 * r + c] = vol * dF3[r] * D[3 * c + 2];
+}
+
+// Cloth return mapping on the QR factor of D (HybridSolver.cpp:684-722): r33 > 1 (separation)
+// clears the normal/shear part; otherwise the shear |(r13, r23)| is limited by the friction cone
+// c_F k (r33 - 1)^2 / gamma. d3 <- Q R(:,2).
+__device__ __forceinline__ void cloth_return_map(const DevParams& P, float* D, int& fb) {
+  float Q[9], R[9];
+  qr3_guarded(D, Q, R, P.svd_ratio, fb);
+  if (R[8] > 1.0f) {
+    R[8] = 1.0f;
+    R[2] = R[5] = 0.0f;
+  } else {
+    const float fn = P.c_k * (R[8] - 1.0f) * (R[8] - 1.0f);
+    const float ft = P.c_gamma * sqrtf(R[2] * R[2] + R[5] * R[5]);
+    if (ft > P.c_cf * fn) {
+      R[2] *= P.c_cf * fn / ft;
+      R[5] *= P.c_cf * fn / ft;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) D[3 * a + 2] = Q[3 * a] * R[2] + Q[3 * a + 1] * R[5] + Q[3 * a + 2] * R[8];
 }
 
 // ------------------------------------------------------------------ level sets (LevelSet.cpp:8-42)

@@ -142,6 +142,8 @@ struct P2GArgs {
   float dt;
   unsigned long long* fallback_svd;
   const Clock* clk;
+  const float* vert_force;   // cloth: force per mesh vertex (3), or null without a mesh
+  const float* elem_stress;  // cloth: V dF3 d3^T per mesh element (3x3 row-major)
 };
 
 struct GridArgs {
@@ -238,6 +240,33 @@ void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4);
 void launch_clock_update(hipStream_t s, Clock* c, int init);
 // End of a substep: count it, close a completed frame, stop the batch after a frame or the run.
 void launch_clock_end(hipStream_t s, Clock* c);
+// Cloth (mesh vertices and elements among the particles of scene 0; caller ids from the DevParams).
+struct ClothArgs {
+  DevParams P;
+  float* st;               // particle state (state[cur] before P2G, sout after G2P)
+  int64_t cap;
+  int n;
+  const int* pos;          // caller id -> physical index in st
+  int nv, ne;
+  const int* faces;        // [e][3] vertex caller ids
+  const float* rest;       // [e][6] inverse rest in-plane R (i11 i12 i22) and the normal-force vector
+  const int* vcsr_off;     // [nv+1] incident element corners of each vertex ...
+  const int* vcsr;         // ... as 3 e + corner, ascending e
+  float* elem_fv;          // [e][9] vertex forces f1 f2 f3 of each element
+  float* elem_stress;      // [e][9]
+  float* vert_force;       // [v][3]
+  const int* fixed;        // caller ids of the fixed vertices
+  int nfixed;
+  float4* gvel;
+  uint32_t* keys;
+  const float* thr;
+  int* status;
+  unsigned long long* fallback;
+  const Clock* clk;
+};
+void launch_cloth_forces(hipStream_t s, const ClothArgs& a);   // element forces, then the per-vertex sums
+void launch_cloth_fixed(hipStream_t s, const ClothArgs& a);    // after the grid update
+void launch_cloth_elements(hipStream_t s, const ClothArgs& a); // after G2P: positions, velocities, D, return map
 void surrogate_set_smem_limit();
 void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid);
 

@@ -377,6 +377,7 @@ constexpr int SLAB = FOOT * FOOT * 4;   // nodes of a wave's slab
 constexpr int PY_WX = 0, PY_DWX = 4, PY_WY = 8, PY_DWY = 12, PY_WZ = 16, PY_DWZ = 20;
 constexpr int PY_M = 24, PY_Q0 = 25, PY_Q = 28, PY_S = 37;  // momentum / stress passes
 constexpr int PY_SF = 24;                                    // force pass: 9 floats of the stress matrix
+constexpr int PY_FV = 40;                                    // force pass: force on a cloth vertex (3)
 
 template <int ORDER>
 __device__ __forceinline__ void axis_weights(float f, float* w, float* dw) {
@@ -393,6 +394,13 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const DevParams& P, 
   const int pi = clamp_index(a.src[idx], a.n);
   const float* st = a.state;
   const int64_t cap = a.cap;
+  // particle kind (cloth vertices and elements carry their forces from k_cloth_forces)
+  int kind = KIND_PARTICLE, mid = 0;
+  if (P.mesh_e1 > P.mesh_v0) {
+    const int pid = reinterpret_cast<const int*>(st + (size_t)PID * cap)[pi];
+    kind = particle_kind(P, pid);
+    mid = pid - (kind == KIND_VERTEX ? P.mesh_v0 : P.mesh_e0);
+  }
   float xp[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) xp[d] = st[(PX + d) * cap + pi];
@@ -415,6 +423,9 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const DevParams& P, 
 #pragma unroll
     for (int r = 0; r < 3; ++r)
       pay[PY_Q0 + r] = mp * st[(PV + r) * cap + pi] - (Q[3 * r] * (1.0f + f[0]) + Q[3 * r + 1] * (1.0f + f[1]) + Q[3 * r + 2] * (1.0f + f[2]));
+    if (MODE == P2G_MOM_STRESS && kind == KIND_VERTEX)  // in-plane force impulse (HybridSolver.cpp:378)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) pay[PY_Q0 + r] += dt * a.vert_force[3 * mid + r];
 #pragma unroll
     for (int i = 0; i < 9; ++i) pay[PY_Q + i] = Q[i];
   }
@@ -422,12 +433,29 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const DevParams& P, 
     // stress at F^n: the Kirchhoff stress V P(F) F^T written by the previous G2P (or by
     // k_stress_init), columns scaled by dt/h so that the impulse folds into the momentum
     const int R[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};  // symmetric 3x3 from (xx yy zz xy xz yz)
+    if (kind == KIND_PARTICLE) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pay[PY_S + 3 * r + c] = dt * st[(PK + R[3 * r + c]) * cap + pi] * P.inv_h[c];
+        for (int c = 0; c < 3; ++c) pay[PY_S + 3 * r + c] = dt * st[(PK + R[3 * r + c]) * cap + pi] * P.inv_h[c];
+    } else if (kind == KIND_ELEMENT) {  // element stress of the normal/shear response (:434-454)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pay[PY_S + 3 * r + c] = dt * a.elem_stress[9 * mid + 3 * r + c] * P.inv_h[c];
+    }
   }
-  if constexpr (MODE == P2G_FORCE) {
+  if (MODE == P2G_FORCE && kind != KIND_PARTICLE) {
+    if (kind == KIND_VERTEX) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) pay[PY_FV + r] = a.vert_force[3 * mid + r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pay[PY_SF + 3 * r + c] = a.elem_stress[9 * mid + 3 * r + c] * P.inv_h[c];
+    }
+  } else if constexpr (MODE == P2G_FORCE) {
     // stress matrix V P(F^) F^n^T at the predicted F^ = F + dt (sum_i v_i grad w_ip^T) F from the
     // post-P2G grid velocity (HybridSolver.cpp:269-339), columns scaled by 1/h
     float Fn[9];
@@ -558,9 +586,9 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
               e[rr] = wyz * qrow;
               dv[rr] = wyz * pay[PY_Q + 3 * rr];
             }
-          } else {
+          } else {  // force pass: a cloth vertex's force goes with its weights
 #pragma unroll
-            for (int rr = 0; rr < 3; ++rr) { e[rr] = 0.0f; dv[rr] = 0.0f; }
+            for (int rr = 0; rr < 3; ++rr) { e[rr] = wyz * pay[PY_FV + rr]; dv[rr] = 0.0f; }
           }
           if constexpr (MODE != P2G_MOM) {
 #pragma unroll
@@ -821,7 +849,10 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
         }
       }
     }
-    if (P.damping != 0.0f) {
+    // cloth vertices and elements keep only the skew part of B (damping 1, HybridSolver.cpp:926-932)
+    const int kind = particle_kind(P, pid);
+    const float damping = kind == KIND_PARTICLE ? P.damping : 1.0f;
+    if (damping != 0.0f) {
       float C[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) C[i] = Bn[i];
@@ -830,7 +861,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           float sym = 0.5f * (C[3 * r + c] + C[3 * c + r]);
-          Bn[3 * r + c] = (C[3 * r + c] - sym) + (1.0f - P.damping) * sym;
+          Bn[3 * r + c] = (C[3 * r + c] - sym) + (1.0f - damping) * sym;
         }
     }
     float Fn[9], Fh[9], GF[9];
@@ -849,7 +880,9 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     io.pair_cap = a.pair_cap;
     io.pair_count = a.pair_count;
     io.deferred = false;
-    return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
+    // particles: return mapping. A cloth element's F planes hold D, of which F^ updates d3
+    // (HybridSolver.cpp:599-603); its d1, d2 and return mapping follow in k_cloth_elements.
+    if (kind == KIND_PARTICLE) return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
     if (a.sur_mode == 2) {
       // compact the deferred particles of this wave into the candidate list (one atomic per wave)
       const uint64_t dm = __ballot(io.deferred);
@@ -1024,6 +1057,120 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
   }
 }
 
+// ================================================================== cloth
+// Element forces from the current directions (per element), then each vertex sums the forces of
+// its incident elements in ascending element order (deterministic).
+__global__ __launch_bounds__(256) void k_cloth_element_forces(ClothArgs a) {
+  if (clock_skip(a.clk)) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int fb = 0;
+  if (e < a.ne) {
+    const DevParams& P = a.P;
+    const int pi = clamp_index(a.pos[P.mesh_e0 + e], a.n);
+    float D[9], fv[9], M[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) D[i] = a.st[(PF + i) * a.cap + pi];
+    cloth_element_forces(P, D, a.rest + 6 * (size_t)e, a.st[PVOL * a.cap + pi], fv, M, fb);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      a.elem_fv[9 * (size_t)e + i] = fv[i];
+      a.elem_stress[9 * (size_t)e + i] = M[i];
+    }
+  }
+  if (fb) atomicAdd(a.fallback, (unsigned long long)fb);
+}
+
+__global__ __launch_bounds__(256) void k_cloth_vertex_forces(ClothArgs a) {
+  if (clock_skip(a.clk)) return;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.nv) return;
+  float f[3] = {0.0f, 0.0f, 0.0f};
+  for (int k = a.vcsr_off[v]; k < a.vcsr_off[v + 1]; ++k) {
+    const int ent = a.vcsr[k];  // 3 e + corner
+    const int e = ent / 3, c = ent - 3 * e;
+    if (e < 0 || e >= a.ne) continue;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) f[r] += a.elem_fv[9 * (size_t)e + 3 * c + r];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) a.vert_force[3 * (size_t)v + r] = f[r];
+}
+
+// Fixed vertices (HybridSolver.cpp:513-550): every node within one node (per axis) of a node with
+// positive weight for a fixed vertex is stopped (v = v_BF = 0). Out-of-grid neighbours are skipped.
+template <int ORDER>
+__global__ __launch_bounds__(64) void k_cloth_fixed(ClothArgs a) {
+  if (clock_skip(a.clk)) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nfixed) return;
+  const DevParams& P = a.P;
+  const int pi = clamp_index(a.pos[clamp_index(a.fixed[k], a.n)], a.n);
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float g = (a.st[(PX + d) * a.cap + pi] - P.gmin[d]) * P.inv_h[d];
+    const int base = (ORDER == 2) ? (int)floorf(g - 0.5f) : (int)floorf(g) - 1;
+    lo[d] = base + ORDER + 1;
+    hi[d] = base - 1;
+    for (int o = 0; o <= ORDER; ++o) {
+      float w, dw;
+      spline<ORDER>(g - (float)(base + o), w, dw);
+      if (w > 0.0f) {
+        lo[d] = min(lo[d], base + o);
+        hi[d] = max(hi[d], base + o);
+      }
+    }
+    lo[d] = max(lo[d] - 1, 0);
+    hi[d] = min(hi[d] + 1, P.res[d] - 1);
+  }
+  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int n2 = lo[2]; n2 <= hi[2]; ++n2)
+    for (int n1 = lo[1]; n1 <= hi[1]; ++n1)
+      for (int n0 = lo[0]; n0 <= hi[0]; ++n0) {
+        const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+        a.gvel[2 * node] = z;
+        a.gvel[2 * node + 1] = z;
+      }
+}
+
+// After G2P: element position = centroid and velocity = mean of its vertices' new values
+// (LagrangianMesh::updateElementPositions, HybridSolver.cpp:750-756), d1 = x2 - x1, d2 = x3 - x1
+// (:584-590), d3 as updated by G2P, then the return mapping; the element's next key.
+__global__ __launch_bounds__(256) void k_cloth_elements(ClothArgs a) {
+  if (clock_skip(a.clk)) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int fb = 0, oob = 0;
+  if (e < a.ne) {
+    const DevParams& P = a.P;
+    const int id = P.mesh_e0 + e;
+    const int pe = clamp_index(a.pos[id], a.n);
+    int pv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pv[c] = clamp_index(a.pos[clamp_index(a.faces[3 * e + c], a.n)], a.n);
+    float xv[3][3], D[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xv[c][d] = a.st[(PX + d) * a.cap + pv[c]];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float vm = (a.st[(PV + d) * a.cap + pv[0]] + a.st[(PV + d) * a.cap + pv[1]] + a.st[(PV + d) * a.cap + pv[2]]) / 3.0f;
+      a.st[(PX + d) * a.cap + pe] = (xv[0][d] + xv[1][d] + xv[2][d]) / 3.0f;
+      a.st[(PV + d) * a.cap + pe] = vm;
+      D[3 * d] = xv[1][d] - xv[0][d];
+      D[3 * d + 1] = xv[2][d] - xv[0][d];
+      D[3 * d + 2] = a.st[(PF + 3 * d + 2) * a.cap + pe];
+    }
+    cloth_return_map(P, D, fb);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.st[(PF + i) * a.cap + pe] = D[i];
+    const float xe[3] = {a.st[PX * a.cap + pe], a.st[(PX + 1) * a.cap + pe], a.st[(PX + 2) * a.cap + pe]};
+    a.keys[id] = cell_key(P, a.thr, xe[0], xe[1], xe[2], oob);
+  }
+  if (fb) atomicAdd(a.fallback, (unsigned long long)fb);
+  if (oob) atomicOr(a.status, MPM_STATUS_OUT_OF_DOMAIN);
+}
+
 // ================================================================== auxiliary kernels
 // rho_p = sum_i w_ip m_i / h^3, V_p = m_p / rho_p (HybridSolver.cpp:242-249), at the current positions.
 template <int ORDER>
@@ -1034,6 +1181,8 @@ __global__ __launch_bounds__(256) void k_volumes(SceneTable T, float* __restrict
   if (i >= pb.end) return;
   const DevParams& P = T.params[pb.s];
   const float* gm = gm_all + T.node_off[pb.s];
+  // particles only: a cloth's vertices and elements keep their mesh volumes (HybridSolver.cpp:242-249)
+  if (particle_kind(P, reinterpret_cast<const int*>(st + (size_t)PID * cap)[i]) != KIND_PARTICLE) return;
   float g[3];
   int base[3];
   float w[3][4], dw[3][4];
@@ -1270,6 +1419,22 @@ void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid) {
 
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v) {
   if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_cloth_forces(hipStream_t s, const ClothArgs& a) {
+  if (a.ne <= 0) return;
+  hipLaunchKernelGGL(k_cloth_element_forces, dim3(blocks_for(a.ne, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_cloth_vertex_forces, dim3(blocks_for(a.nv, 256)), dim3(256), 0, s, a);
+}
+
+void launch_cloth_fixed(hipStream_t s, const ClothArgs& a) {
+  if (a.nfixed <= 0) return;
+  if (a.P.order == 2) hipLaunchKernelGGL(k_cloth_fixed<2>, dim3(blocks_for(a.nfixed, 64)), dim3(64), 0, s, a);
+  else hipLaunchKernelGGL(k_cloth_fixed<3>, dim3(blocks_for(a.nfixed, 64)), dim3(64), 0, s, a);
+}
+
+void launch_cloth_elements(hipStream_t s, const ClothArgs& a) {
+  if (a.ne > 0) hipLaunchKernelGGL(k_cloth_elements, dim3(blocks_for(a.ne, 256)), dim3(256), 0, s, a);
 }
 
 }  // namespace mpm

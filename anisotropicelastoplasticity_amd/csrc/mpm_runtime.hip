@@ -37,7 +37,24 @@ struct SceneHost {
   int64_t ng = 0, ntiles = 0;
   mpm_params params{};
   std::vector<mpm_levelset> ls;
-  int n = 0;
+  int n = 0;  // all particles of the scene, mesh vertices and elements included
+  // cloth: caller ids [mesh_v0, mesh_e0) are vertices, [mesh_e0, mesh_e1) elements (none: all equal)
+  int mesh_v0 = INT32_MAX, mesh_e0 = INT32_MAX, mesh_e1 = INT32_MAX;
+  float c_mu = 0, c_lambda = 0, c_gamma = 0, c_k = 0, c_cf = 0;
+};
+
+// Host copy of the cloth of a single-scene context (mpm_set_mesh).
+struct MeshHost {
+  int np = 0, nv = 0, ne = 0;  // particles before the mesh, vertices, elements
+  int* faces = nullptr;        // device: [e][3] vertex caller ids
+  float* rest = nullptr;       // device: [e][6]
+  int* vcsr_off = nullptr;     // device: [nv+1]
+  int* vcsr = nullptr;         // device: [3 ne]
+  float* elem_fv = nullptr;    // device: [e][9]
+  float* elem_stress = nullptr;
+  float* vert_force = nullptr; // device: [v][3]
+  int* fixed = nullptr;        // device: fixed vertex caller ids
+  int nfixed = 0;
 };
 
 struct mpm_ctx {
@@ -71,8 +88,11 @@ struct mpm_ctx {
   float* partials = nullptr;
   int64_t partial_slots = 0;
 
-  // particles
+  // particles (all kinds); n_api: the particles of mpm_set_particles (without a mesh's vertices
+  // and elements), which mpm_get_particles / mpm_get_stress return
   int n = 0;
+  int n_api = 0;
+  MeshHost mesh;
   int64_t cap = 0;
   float* state[2] = {nullptr, nullptr};
   int cur = 0;
@@ -126,6 +146,8 @@ struct mpm_ctx {
   float* cand = nullptr;
   int* cand_slot = nullptr;
 };
+
+static void drop_mesh(mpm_ctx* ctx);
 
 #define CK(call)                                                                     \
   do {                                                                               \
@@ -217,6 +239,14 @@ static DevParams devparams_of(const SceneHost& sc) {
   }
   d.nls = (int)sc.ls.size();
   for (int k = 0; k < d.nls; ++k) d.ls[k] = sc.ls[k];
+  d.mesh_v0 = sc.mesh_v0;
+  d.mesh_e0 = sc.mesh_e0;
+  d.mesh_e1 = sc.mesh_e1;
+  d.c_mu = sc.c_mu;
+  d.c_lambda = sc.c_lambda;
+  d.c_gamma = sc.c_gamma;
+  d.c_k = sc.c_k;
+  d.c_cf = sc.c_cf;
   return d;
 }
 
@@ -327,6 +357,35 @@ static int stage_sort(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
+// Cloth kernel arguments on buffer b (state[b] with its caller-id map pos[b]).
+static ClothArgs cloth_args(mpm_ctx* ctx, int b) {
+  ClothArgs c{};
+  const MeshHost& m = ctx->mesh;
+  c.P = devparams_of(ctx->scenes[0]);
+  c.st = ctx->state[b];
+  c.cap = ctx->cap;
+  c.n = ctx->n;
+  c.pos = ctx->pos[b];
+  c.nv = m.nv;
+  c.ne = m.ne;
+  c.faces = m.faces;
+  c.rest = m.rest;
+  c.vcsr_off = m.vcsr_off;
+  c.vcsr = m.vcsr;
+  c.elem_fv = m.elem_fv;
+  c.elem_stress = m.elem_stress;
+  c.vert_force = m.vert_force;
+  c.fixed = m.fixed;
+  c.nfixed = m.nfixed;
+  c.gvel = reinterpret_cast<float4*>(ctx->grid + GRID_VEL * ctx->ng);
+  c.keys = ctx->keys;
+  c.thr = ctx->thr;
+  c.status = ctx->status;
+  c.fallback = ctx->fallbacks;
+  c.clk = ctx->clk_active;
+  return c;
+}
+
 static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   P2GArgs a;
   a.T = ctx->T;
@@ -343,6 +402,8 @@ static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   a.dt = dt;
   a.fallback_svd = ctx->fallbacks;
   a.clk = ctx->clk_active;
+  a.vert_force = ctx->mesh.vert_force;
+  a.elem_stress = ctx->mesh.elem_stress;
   return a;
 }
 
@@ -366,6 +427,8 @@ static GridArgs grid_args(mpm_ctx* ctx, float dt) {
 static int stage_p2g(mpm_ctx* ctx, float dt) {
   hipStream_t s = ctx->stream;
   P2GArgs a = p2g_args(ctx, dt);
+  // cloth forces from the current directions (HybridSolver.cpp:370-455), read by the P2G staging
+  if (ctx->mesh.ne > 0) launch_cloth_forces(s, cloth_args(ctx, ctx->cur));
   if (!ctx->predicted) {
     if (!ctx->stress_valid) {
       launch_stress_init(s, ctx->T, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
@@ -384,6 +447,8 @@ static int stage_grid(mpm_ctx* ctx, float dt) {
   GridArgs g = grid_args(ctx, dt);
   if (ctx->clk_active) g.vmax_out = &ctx->clk_active->vmax_bits;
   launch_grid(ctx->stream, g, ctx->predicted ? GRID_FORCE_UPDATE : GRID_FUSED, ctx->grid_grid);
+  // fixed cloth vertices stop the nodes around them (part of gridCollisionHandling_, :513-550)
+  if (ctx->mesh.nfixed > 0) launch_cloth_fixed(ctx->stream, cloth_args(ctx, ctx->cur));
   // adaptive step: the new Dt from the updated velocities, used by G2P (HybridSolver.cpp:878)
   if (ctx->clk_active) launch_clock_update(ctx->stream, ctx->clk_active, 0);
   return MPM_OK;
@@ -435,6 +500,8 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     sa.clk = ctx->clk_active;
     launch_surrogate_apply(ctx->stream, sa, ctx->n_cu);
   }
+  // cloth elements follow their vertices (positions, velocities, d1 d2) and are return-mapped
+  if (ctx->mesh.ne > 0) launch_cloth_elements(ctx->stream, cloth_args(ctx, 1 - ctx->cur));
   if (ctx->clk_active) launch_clock_end(ctx->stream, ctx->clk_active);
   ctx->cur = 1 - ctx->cur;
   ctx->keys_valid = true;
@@ -552,6 +619,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graphs(ctx);
   dfree(ctx->table_buf);
+  drop_mesh(ctx);
   dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_end);
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
@@ -732,20 +800,8 @@ int mpm_set_scenes(mpm_ctx* ctx, int32_t nscenes, const mpm_scene_desc* descs) {
   return rc;
 }
 
-int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, const float* B, const float* F,
-                      const float* plastic, const float* mass, const float* volume, const float* phi) {
-  if (!ctx) return MPM_ERR_INVALID;
-  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid or mpm_set_scenes must precede mpm_set_particles");
-  if (n <= 0 || !x || !mass) return fail(ctx, MPM_ERR_INVALID, "n > 0, x and mass are required");
-  if (ctx->batch) {
-    int64_t total = 0;
-    for (const SceneHost& sc : ctx->scenes) total += sc.n;
-    if (total != n) return fail(ctx, MPM_ERR_INVALID, "n must equal the scenes' particle counts summed");
-  } else {
-    ctx->scenes[0].n = n;
-  }
-  for (const SceneHost& sc : ctx->scenes)
-    if (sc.params.per_particle_phi && !phi) return fail(ctx, MPM_ERR_INVALID, "per_particle_phi needs phi");
+// Install n particles given as host planes (NPLANES x n, caller order) as the context's state.
+static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
   CK(hipSetDevice(ctx->device));
   CK(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);
@@ -778,7 +834,45 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
   pick_sorted(ctx);
   int rc = upload_table(ctx);
   if (rc) return rc;
-  const int64_t cap = ctx->cap;
+  CK(hipMemcpyAsync(ctx->state[0], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * n, ctx->stream));
+  CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
+  launch_iota(ctx->stream, ctx->pos[0], n);
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->stress_valid = false;
+  ctx->cur = 0;
+  ctx->sort_buf = 0;
+  ctx->keys_valid = false;
+  ctx->substeps = 0;
+  ctx->next_stage = MPM_STAGE_SORT;
+  return MPM_OK;
+}
+
+static void drop_mesh(mpm_ctx* ctx) {
+  MeshHost& m = ctx->mesh;
+  dfree(m.faces); dfree(m.rest); dfree(m.vcsr_off); dfree(m.vcsr); dfree(m.elem_fv); dfree(m.elem_stress);
+  dfree(m.vert_force); dfree(m.fixed);
+  m = MeshHost{};
+  for (SceneHost& sc : ctx->scenes) sc.mesh_v0 = sc.mesh_e0 = sc.mesh_e1 = INT32_MAX;
+}
+
+int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, const float* B, const float* F,
+                      const float* plastic, const float* mass, const float* volume, const float* phi) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid or mpm_set_scenes must precede mpm_set_particles");
+  if (n <= 0 || !x || !mass) return fail(ctx, MPM_ERR_INVALID, "n > 0, x and mass are required");
+  if (ctx->batch) {
+    int64_t total = 0;
+    for (const SceneHost& sc : ctx->scenes) total += sc.n;
+    if (total != n) return fail(ctx, MPM_ERR_INVALID, "n must equal the scenes' particle counts summed");
+  }
+  for (const SceneHost& sc : ctx->scenes)
+    if (sc.params.per_particle_phi && !phi) return fail(ctx, MPM_ERR_INVALID, "per_particle_phi needs phi");
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  drop_mesh(ctx);  // a mesh is attached after the particles (mpm_set_mesh)
+  if (!ctx->batch) ctx->scenes[0].n = n;
+  const int64_t cap = n;
   std::vector<float> h((size_t)NPLANES * cap, 0.0f);
   int i = 0;
   for (const SceneHost& sc : ctx->scenes) {
@@ -800,17 +894,174 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
       std::memcpy(&h[PID * cap + i], &id, sizeof(int));
     }
   }
-  CK(hipMemcpyAsync(ctx->state[0], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * n, ctx->stream));
-  CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
-  launch_iota(ctx->stream, ctx->pos[0], n);
+  int rc = install_state(ctx, n, h);
+  if (!rc) ctx->n_api = n;
+  return rc;
+}
+
+int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32_t nf, const int32_t* faces,
+                 const uint8_t* fixed, const mpm_mesh_params* mp) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (ctx->batch || ctx->scenes.size() != 1) return fail(ctx, MPM_ERR_UNSUPPORTED, "a mesh needs a single-scene context");
+  if (ctx->n_api <= 0 || ctx->n != ctx->n_api || ctx->substeps != 0 || ctx->keys_valid)
+    return fail(ctx, MPM_ERR_STATE, "mpm_set_mesh follows mpm_set_particles, before the first substep");
+  if (nv < 3 || nf < 1 || !x || !faces || !mp) return fail(ctx, MPM_ERR_INVALID, "a mesh needs vertices, faces and parameters");
+  if (!(mp->poisson_ratio < 0.5f) || !(mp->youngs_modulus > 0.0f) || !(mp->density > 0.0f) || !(mp->thickness > 0.0f))
+    return fail(ctx, MPM_ERR_INVALID, "bad cloth material");
+  for (int64_t k = 0; k < 3 * (int64_t)nf; ++k)
+    if (faces[k] < 0 || faces[k] >= nv) return fail(ctx, MPM_ERR_INVALID, "face vertex index out of range");
+  const int np = ctx->n_api, n = np + nv + nf;
+  CK(hipSetDevice(ctx->device));
   CK(hipStreamSynchronize(ctx->stream));
-  ctx->stress_valid = false;
-  ctx->cur = 0;
-  ctx->sort_buf = 0;
-  ctx->keys_valid = false;
-  ctx->substeps = 0;
-  ctx->next_stage = MPM_STAGE_SORT;
+  // the particles' planes (caller order: no substep has run)
+  std::vector<float> hp((size_t)NPLANES * np);
+  CK(hipMemcpy(hp.data(), ctx->state[ctx->cur], sizeof(float) * hp.size(), hipMemcpyDeviceToHost));
+  std::vector<float> h((size_t)NPLANES * n, 0.0f);
+  for (int p = 0; p < NPLANES; ++p) std::memcpy(&h[(size_t)p * n], &hp[(size_t)p * np], sizeof(float) * np);
+  // LagrangianMesh::ObjMesh (LagrangianMesh.cpp:300-352): element volume 0.25 area thickness, added
+  // to each vertex; masses density x volume; rest directions v2 - v1, v3 - v1, unit normal.
+  std::vector<double> vvol(nv, 0.0);
+  std::vector<float> rest((size_t)6 * nf);
+  auto X = [&](int vtx, int a) { return (double)x[(size_t)a * nv + vtx]; };
+  for (int e = 0; e < nf; ++e) {
+    double p[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int a = 0; a < 3; ++a) p[c][a] = X(faces[3 * e + c], a);
+    auto len = [](const double* a, const double* b) {
+      return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+    };
+    const double la = len(p[0], p[1]), lb = len(p[1], p[2]), lc = len(p[2], p[0]), hp2 = 0.5 * (la + lb + lc);
+    const double area = std::sqrt(std::max(0.0, hp2 * (hp2 - la) * (hp2 - lb) * (hp2 - lc)));  // geometry.cpp:12-23
+    const double vol = 0.25 * area * mp->thickness;
+    for (int c = 0; c < 3; ++c) vvol[faces[3 * e + c]] += vol;
+    double d1[3], d2[3], nr[3];
+    for (int a = 0; a < 3; ++a) {
+      d1[a] = p[1][a] - p[0][a];
+      d2[a] = p[2][a] - p[0][a];
+    }
+    nr[0] = d1[1] * d2[2] - d1[2] * d2[1];
+    nr[1] = d1[2] * d2[0] - d1[0] * d2[2];
+    nr[2] = d1[0] * d2[1] - d1[1] * d2[0];
+    const double nl = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+    if (!(nl > 0.0)) return fail(ctx, MPM_ERR_INVALID, "degenerate triangle");
+    for (int a = 0; a < 3; ++a) nr[a] /= nl;
+    // rest QR (geometry.cpp:31-62) and inverseR of its in-plane block (:67-73)
+    const double r11 = std::sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+    const double q1[3] = {d1[0] / r11, d1[1] / r11, d1[2] / r11};
+    const double r12 = d2[0] * q1[0] + d2[1] * q1[1] + d2[2] * q1[2];
+    double q2[3] = {d2[0] - r12 * q1[0], d2[1] - r12 * q1[1], d2[2] - r12 * q1[2]};
+    const double r22 = std::sqrt(q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2]);
+    rest[6 * e] = float(1.0 / r11);
+    rest[6 * e + 1] = float(-r12 / r11 / r22);
+    rest[6 * e + 2] = float(1.0 / r22);
+    // normal-force vector: the reference's (D_rest^T).col(2) or D_rest^-T e3 = (d1 x d2) / det(D_rest)
+    const double Dr[3][3] = {{d1[0], d2[0], nr[0]}, {d1[1], d2[1], nr[1]}, {d1[2], d2[2], nr[2]}};
+    if (mp->reference_normal_row) {
+      for (int c = 0; c < 3; ++c) rest[6 * e + 3 + c] = float(Dr[2][c]);
+    } else {
+      const double det = Dr[0][0] * (Dr[1][1] * Dr[2][2] - Dr[1][2] * Dr[2][1]) -
+                         Dr[0][1] * (Dr[1][0] * Dr[2][2] - Dr[1][2] * Dr[2][0]) +
+                         Dr[0][2] * (Dr[1][0] * Dr[2][1] - Dr[1][1] * Dr[2][0]);
+      const double cr[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
+      for (int c = 0; c < 3; ++c) rest[6 * e + 3 + c] = float(cr[c] / det);
+    }
+    // element particle
+    const int i = np + nv + e;
+    for (int a = 0; a < 3; ++a) {
+      h[(PX + a) * (size_t)n + i] = float((p[0][a] + p[1][a] + p[2][a]) / 3.0);
+      double vm = 0.0;
+      if (v)
+        for (int c = 0; c < 3; ++c) vm += v[(size_t)a * nv + faces[3 * e + c]];
+      h[(PV + a) * (size_t)n + i] = float(vm / 3.0);
+      h[(PF + 3 * a) * (size_t)n + i] = float(d1[a]);
+      h[(PF + 3 * a + 1) * (size_t)n + i] = float(d2[a]);
+      h[(PF + 3 * a + 2) * (size_t)n + i] = float(nr[a]);
+    }
+    h[PM * (size_t)n + i] = float(mp->density * vol);
+    h[PVOL * (size_t)n + i] = float(vol);
+    std::memcpy(&h[PID * (size_t)n + i], &i, sizeof(int));
+  }
+  for (int k = 0; k < nv; ++k) {
+    const int i = np + k;
+    for (int a = 0; a < 3; ++a) {
+      h[(PX + a) * (size_t)n + i] = x[(size_t)a * nv + k];
+      h[(PV + a) * (size_t)n + i] = v ? v[(size_t)a * nv + k] : 0.0f;
+    }
+    h[(PF + 0) * (size_t)n + i] = h[(PF + 4) * (size_t)n + i] = h[(PF + 8) * (size_t)n + i] = 1.0f;
+    h[PM * (size_t)n + i] = float(mp->density * vvol[k]);
+    h[PVOL * (size_t)n + i] = float(vvol[k]);
+    std::memcpy(&h[PID * (size_t)n + i], &i, sizeof(int));
+  }
+  // vertex -> incident element corners, ascending element (deterministic force sums)
+  std::vector<int> off(nv + 1, 0), ent((size_t)3 * nf), fid((size_t)3 * nf), fixed_ids;
+  for (int64_t k = 0; k < 3 * (int64_t)nf; ++k) off[faces[k] + 1] += 1;
+  for (int k = 0; k < nv; ++k) off[k + 1] += off[k];
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int e = 0; e < nf; ++e)
+    for (int c = 0; c < 3; ++c) ent[fill[faces[3 * e + c]]++] = 3 * e + c;
+  for (int64_t k = 0; k < 3 * (int64_t)nf; ++k) fid[k] = np + faces[k];
+  if (fixed)
+    for (int k = 0; k < nv; ++k)
+      if (fixed[k]) fixed_ids.push_back(np + k);
+  drop_mesh(ctx);
+  MeshHost& m = ctx->mesh;
+  int rc = dalloc(ctx, m.faces, (size_t)3 * nf);
+  if (!rc) rc = dalloc(ctx, m.rest, (size_t)6 * nf);
+  if (!rc) rc = dalloc(ctx, m.vcsr_off, (size_t)nv + 1);
+  if (!rc) rc = dalloc(ctx, m.vcsr, (size_t)3 * nf);
+  if (!rc) rc = dalloc(ctx, m.elem_fv, (size_t)9 * nf);
+  if (!rc) rc = dalloc(ctx, m.elem_stress, (size_t)9 * nf);
+  if (!rc) rc = dalloc(ctx, m.vert_force, (size_t)3 * nv);
+  if (!rc && !fixed_ids.empty()) rc = dalloc(ctx, m.fixed, fixed_ids.size());
+  if (rc) return rc;
+  CK(hipMemcpy(m.faces, fid.data(), sizeof(int) * fid.size(), hipMemcpyHostToDevice));
+  CK(hipMemcpy(m.rest, rest.data(), sizeof(float) * rest.size(), hipMemcpyHostToDevice));
+  CK(hipMemcpy(m.vcsr_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  CK(hipMemcpy(m.vcsr, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice));
+  if (!fixed_ids.empty()) CK(hipMemcpy(m.fixed, fixed_ids.data(), sizeof(int) * fixed_ids.size(), hipMemcpyHostToDevice));
+  m.np = np;
+  m.nv = nv;
+  m.ne = nf;
+  m.nfixed = (int)fixed_ids.size();
+  SceneHost& sc = ctx->scenes[0];
+  sc.n = n;
+  sc.mesh_v0 = np;
+  sc.mesh_e0 = np + nv;
+  sc.mesh_e1 = n;
+  const double E = mp->youngs_modulus, nu = mp->poisson_ratio;
+  sc.c_lambda = float(E * nu / (1.0 + nu) / (1.0 - 2.0 * nu));  // LagrangianMesh.cpp:340-341
+  sc.c_mu = float(E / 2.0 / (1.0 + nu));
+  sc.c_gamma = mp->shear_stiffness;
+  sc.c_k = mp->stiffness;
+  sc.c_cf = float(std::tan(double(mp->friction_angle) * 3.14159265358979323846 / 180.0));
+  rc = install_state(ctx, n, h);
+  return rc;
+}
+
+int mpm_get_mesh(mpm_ctx* ctx, float* vx, float* vv, float* D) {
+  int rc = ready(ctx);
+  if (rc) return rc;
+  const MeshHost& m = ctx->mesh;
+  if (m.ne == 0) return fail(ctx, MPM_ERR_STATE, "no mesh");
+  CK(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  float* tmp = nullptr;
+  rc = dalloc(ctx, tmp, (size_t)9 * n);
+  if (rc) return rc;
+  struct Req { float* dst; int p0; int np; int first; int count; };
+  const Req reqs[3] = {{vx, PX, 3, m.np, m.nv}, {vv, PV, 3, m.np, m.nv}, {D, PF, 9, m.np + m.nv, m.ne}};
+  hipError_t e = hipSuccess;
+  for (const Req& r : reqs) {
+    if (!r.dst) continue;
+    launch_unpermute(ctx->stream, ctx->state[ctx->cur], ctx->cap, n, r.p0, r.np, tmp);
+    for (int p = 0; p < r.np && e == hipSuccess; ++p)
+      e = hipMemcpyAsync(r.dst + (size_t)p * r.count, tmp + (size_t)p * n + r.first, sizeof(float) * r.count,
+                         hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) break;
+  }
+  dfree(tmp);
+  if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_get_mesh: ") + hipGetErrorString(e));
   return MPM_OK;
 }
 
@@ -828,10 +1079,13 @@ int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* F, floa
   for (const Req& r : reqs) {
     if (!r.dst) continue;
     launch_unpermute(ctx->stream, ctx->state[ctx->cur], ctx->cap, n, r.p0, r.np, tmp);
-    e = hipMemcpyAsync(r.dst, tmp, sizeof(float) * r.np * n, hipMemcpyDeviceToHost, ctx->stream);
+    // the particles only (a mesh's vertices and elements follow them in caller order)
+    for (int p = 0; p < r.np && e == hipSuccess; ++p)
+      e = hipMemcpyAsync(r.dst + (size_t)p * ctx->n_api, tmp + (size_t)p * n, sizeof(float) * ctx->n_api,
+                         hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) break;
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   dfree(tmp);
   if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_get_particles: ") + hipGetErrorString(e));
   return MPM_OK;
@@ -851,7 +1105,10 @@ int mpm_get_stress(mpm_ctx* ctx, float* K) {
   rc = dalloc(ctx, tmp, (size_t)6 * ctx->n);
   if (rc) return rc;
   launch_unpermute(ctx->stream, ctx->state[ctx->cur], ctx->cap, ctx->n, PK, 6, tmp);
-  hipError_t e = hipMemcpyAsync(K, tmp, sizeof(float) * 6 * ctx->n, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = hipSuccess;
+  for (int p = 0; p < 6 && e == hipSuccess; ++p)
+    e = hipMemcpyAsync(K + (size_t)p * ctx->n_api, tmp + (size_t)p * ctx->n, sizeof(float) * ctx->n_api,
+                       hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   dfree(tmp);
   if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_get_stress: ") + hipGetErrorString(e));
@@ -868,7 +1125,10 @@ int mpm_set_stress(mpm_ctx* ctx, const float* K) {
   float* tmp = nullptr;
   rc = dalloc(ctx, tmp, (size_t)6 * ctx->n);
   if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(tmp, K, sizeof(float) * 6 * ctx->n, hipMemcpyHostToDevice, ctx->stream);
+  hipError_t e = hipMemsetAsync(tmp, 0, sizeof(float) * 6 * ctx->n, ctx->stream);
+  for (int p = 0; p < 6 && e == hipSuccess; ++p)
+    e = hipMemcpyAsync(tmp + (size_t)p * ctx->n, K + (size_t)p * ctx->n_api, sizeof(float) * ctx->n_api,
+                       hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
     launch_permute_in(ctx->stream, ctx->state[ctx->cur], ctx->cap, ctx->n, PK, 6, tmp, ctx->pos[ctx->cur]);
     e = hipStreamSynchronize(ctx->stream);

@@ -184,3 +184,38 @@ def config5_batch(seed=5, nscenes=64, **kw):
     (64 scenes: about 9.5 million particles), dt = 5e-5, 500 substeps."""
     rng = np.random.default_rng(seed)
     return [batch_scene(k, rng, **kw) for k in range(nscenes)]
+
+
+def grid_mesh(nx, nz, edge, origin, y):
+    """A flat square triangle mesh in the plane y = const: nx x nz vertices with spacing `edge`
+    starting at `origin` (x, z), two triangles per quad (counter-clockwise seen from +y, so the
+    normals point up). Returns x (3, nv) and faces (nf, 3)."""
+    i, k = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    x = np.stack([origin[0] + edge * i.ravel(), np.full(nx * nz, float(y)), origin[1] + edge * k.ravel()]).astype(np.float32)
+    vid = (i * nz + k)
+    a, b = vid[:-1, :-1].ravel(), vid[1:, :-1].ravel()
+    c, d = vid[1:, 1:].ravel(), vid[:-1, 1:].ravel()
+    faces = np.concatenate([np.stack([a, d, b], 1), np.stack([b, d, c], 1)]).astype(np.int32)
+    return x, faces
+
+
+def config3_cloth_sand(seed=3, n_side=256, edge=1.0 / 300.0, sand_cells=(40, 39, 40), substeps=2000):
+    """Cloth-sand coupling: a 256 x 256-vertex cloth (130,050 triangles, edge 1/300) flat at y = 0.7
+    above a static sphere (r = 0.15, centre (0.5, 0.5, 0.5)), E = 200, nu = 0.3, rho = 2e3, thickness
+    0.04 (the reference's main.cpp:78 defaults), normal stiffness k = 1e4, shear stiffness gamma = 1e3,
+    cloth friction angle 20 deg; about 500k Drucker-Prager sand particles (config 2's material) in a
+    block [0.35, 0.65] x [0.72, 0.92] x [0.35, 0.65] falling onto it; grid 128^3, box walls
+    [0.05, 0.95]^3, Coulomb friction 0.4, dt = 2e-5. Returns (scene, mesh dict)."""
+    lo, hi = (0.35, 0.72, 0.35), (0.65, 0.92, 0.65)
+    x = generate_block(seed, lo, hi, sand_cells, 2)
+    vol = float(np.prod(np.subtract(hi, lo)))
+    parts = _particles(x, 2200.0, vol, (0.0, 0.0, 0.0), _lib.SAND)
+    prims = [levelset(_lib.LS_BOX, (0.05, 0.05, 0.05, 0.0), (0.95, 0.95, 0.95, 0.0)),
+             levelset(_lib.LS_SPHERE, (0.5, 0.5, 0.5, 0.15))]
+    side = edge * (n_side - 1)
+    mx, faces = grid_mesh(n_side, n_side, edge, (0.5 - side / 2, 0.5 - side / 2), 0.7)
+    mesh = dict(x=mx, faces=faces, youngs_modulus=200.0, poisson_ratio=0.3, shear_stiffness=1e3, stiffness=1e4,
+                friction_angle=20.0, density=2e3, thickness=0.04)
+    sc = Scene("config3_cloth_sand", sand_params(), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (128, 128, 128), prims, parts,
+               2e-5, substeps, dict(mesh=mesh))
+    return sc, mesh

@@ -194,6 +194,36 @@ int mpm_set_particles(mpm_ctx* ctx, int32_t n, const float* x, const float* v, c
                       const float* F, const float* plastic, const float* mass, const float* volume,
                       const float* phi);
 
+/* Cloth material of a LagrangianMesh (LagrangianMesh.h:66-73, LagrangianMesh::ObjMesh
+ * LagrangianMesh.cpp:300-352): in-plane fixed-corotated Young's modulus and Poisson ratio, the
+ * shear stiffness gamma and normal stiffness k of the normal/shear response, and the friction angle
+ * whose tangent bounds the shear in the return mapping (HybridSolver.cpp:684-722). */
+typedef struct mpm_mesh_params {
+  float youngs_modulus;
+  float poisson_ratio;
+  float shear_stiffness;  /* gamma */
+  float stiffness;        /* k */
+  float friction_angle;   /* degrees */
+  float density;
+  float thickness;        /* element volume = 0.25 x area x thickness (LagrangianMesh.cpp:312) */
+  int32_t reference_normal_row; /* 1: the normal-force direction vector is (D_rest^T).col(2) as in
+                                   HybridSolver.cpp:432 (equal to D_rest^-T e3 for a mesh in the
+                                   z = const plane only); 0: D_rest^-T e3 */
+} mpm_mesh_params;
+
+/* Attach a triangle cloth to a single-scene context after mpm_set_particles and before the first
+ * substep (replaces HybridSolver::setLagrangianMesh, HybridSolver.h:88, and the LagrangianMesh
+ * constructor, LagrangianMesh.cpp:127-195). x and v are planar 3 x nv (v may be NULL), faces are
+ * nf x 3 vertex indices, fixed (nv bytes, may be NULL) marks vertices whose surrounding nodes are
+ * stopped (LagrangianMesh::bindConstraints, HybridSolver.cpp:513-550). The mesh's vertices and
+ * triangles move with the particles in every substep; mpm_get_particles keeps returning the
+ * particles only. */
+int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32_t nf, const int32_t* faces,
+                 const uint8_t* fixed, const mpm_mesh_params* params);
+/* Vertex positions and velocities (3 x nv planar) and the element directions D = [d1 d2 d3]
+ * (9 x nf, row-major per element); any pointer may be NULL. */
+int mpm_get_mesh(mpm_ctx* ctx, float* vx, float* vv, float* D);
+
 /* Copy particle state out in caller order; any pointer may be NULL. */
 int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* F, float* plastic,
                       float* volume);

@@ -32,6 +32,14 @@ def load():
         "orc_stage": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
         "orc_substep": (ctypes.c_int, [vp, ctypes.c_double]),
         "orc_run": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
+        "orc_set_mesh": (ctypes.c_int, [vp, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]),
+        "orc_get_mesh": (ctypes.c_int, [vp, _d, _d, _d]),
+        "orc_cloth_element": (None, [_d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, _d, _d]),
+        "orc_cloth_return_map": (None, [_d, ctypes.c_double, ctypes.c_double, ctypes.c_double, _d]),
         "orc_solve": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d,
                                      ctypes.POINTER(ctypes.c_int), _d]),
         "orc_estimate_volumes": (ctypes.c_int, [vp]),
@@ -107,6 +115,25 @@ class Oracle:
 
     def run(self, dt, n):
         self.lib.orc_run(self.h, float(dt), int(n))
+
+    def set_mesh(self, x, faces, v=None, fixed=None, density=2e3, thickness=0.04, E=200.0, nu=0.3, gamma=1e3, k=1e4,
+                 angle=20.0, reference_normal_row=False):
+        x = _f64(x)
+        nv = x.shape[1]
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 3))
+        v = None if v is None else _f64(v)
+        fx = None if fixed is None else np.ascontiguousarray(np.asarray(fixed, dtype=np.uint8))
+        self._mesh = (x, f, v, fx)
+        self.lib.orc_set_mesh(self.h, nv, _dp(x), _dp(v), f.shape[0], f.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              density, thickness, E, nu, gamma, k, angle,
+                              None if fx is None else fx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                              int(reference_normal_row))
+        self.nv, self.nf = nv, f.shape[0]
+
+    def get_mesh(self):
+        vx, vv, D = np.empty((3, self.nv)), np.empty((3, self.nv)), np.empty((9, self.nf))
+        self.lib.orc_get_mesh(self.h, _dp(vx), _dp(vv), _dp(D))
+        return dict(vx=vx, vv=vv, D=D)
 
     def solve(self, cfl, maxt, max_substeps=1000000):
         """The reference's adaptive loop; returns (dts, frame_end flags, t)."""
@@ -208,6 +235,25 @@ def dp_project(params, phi0, q, S):
     dq = np.empty(1)
     lib.orc_dp_project(ctypes.addressof(params), float(phi0), float(q), _dp(S), _dp(out), _dp(dq))
     return out, float(dq[0])
+
+
+def cloth_element(D, Dr, vol, E, nu, gamma, k, reference_normal_row=False):
+    """Vertex forces (3, 3: rows f1 f2 f3) and element stress (3, 3) of one cloth element."""
+    lib = load()
+    D = _f64(D, (9,))
+    Dr = _f64(Dr, (9,))
+    f, M = np.empty(9), np.empty(9)
+    lib.orc_cloth_element(_dp(D), _dp(Dr), float(vol), float(E), float(nu), float(gamma), float(k),
+                          int(reference_normal_row), _dp(f), _dp(M))
+    return f.reshape(3, 3), M.reshape(3, 3)
+
+
+def cloth_return_map(D, gamma, k, angle_deg):
+    lib = load()
+    D = _f64(D, (9,))
+    out = np.empty(9)
+    lib.orc_cloth_return_map(_dp(D), float(gamma), float(k), float(angle_deg), _dp(out))
+    return out.reshape(3, 3)
 
 
 def levelset_eval(prims, x):

@@ -300,6 +300,91 @@ template <class T> void levelset(const std::vector<mpm_levelset>& prims, const T
   }
 }
 
+// Forces of one cloth element from its current and rest directions (HybridSolver.cpp:370-455,
+// LagrangianMesh::computeVertexInPlaneForces LagrangianMesh.cpp:382-460): the in-plane vertex forces
+// f (f1, f2, f3 at f[0..2], f[3..5], f[6..8]) and the element stress Ms = V dF3 d3^T.
+template <class T>
+void cloth_element(const M3<T>& D, const M3<T>& Dr, T vol, T c_mu, T c_lambda, T c_gamma, T c_k, bool ref_normal,
+                   T* fout, M3<T>& Ms_) {
+  M3<T> Q, R, rQ, rR;
+  gram_schmidt_qr(D, Q, R);
+  gram_schmidt_qr(Dr, rQ, rR);
+  // inverseR of the rest in-plane R (geometry.cpp:67-73)
+  const T i11 = T(1) / rR(0, 0), i12 = -rR(0, 1) / rR(0, 0) / rR(1, 1), i22 = T(1) / rR(1, 1);
+  // A = invRest * inPlaneR (both upper triangular)
+  const T a00 = i11 * R(0, 0), a01 = i11 * R(0, 1) + i12 * R(1, 1), a10 = T(0), a11 = i22 * R(1, 1);
+  // polar rotation U V^T of A (orthogonal factor of its SVD)
+  T r00, r01, r10, r11;
+  const T dA = a00 * a11 - a01 * a10;
+  if (dA >= T(0)) {
+    T pp = a00 + a11, qq = a10 - a01, nn = std::sqrt(pp * pp + qq * qq);
+    if (nn > T(0)) { pp /= nn; qq /= nn; } else { pp = T(1); qq = T(0); }
+    r00 = pp; r01 = -qq; r10 = qq; r11 = pp;
+  } else {
+    T pp = a00 - a11, qq = a01 + a10, nn = std::sqrt(pp * pp + qq * qq);
+    pp /= nn; qq /= nn;
+    r00 = pp; r01 = qq; r10 = qq; r11 = -pp;
+  }
+  const T J = a00 * a11;  // product of the diagonal (:443)
+  // P = 2 mu (A - R) + lambda (J - 1) [a11 -a01; 0 a00]^T (:445-446)
+  const T p00 = T(2) * c_mu * (a00 - r00) + c_lambda * (J - T(1)) * a11;
+  const T p01 = T(2) * c_mu * (a01 - r01);
+  const T p10 = T(2) * c_mu * (a10 - r10) + c_lambda * (J - T(1)) * (-a01);
+  const T p11 = T(2) * c_mu * (a11 - r11) + c_lambda * (J - T(1)) * a00;
+  // vertex forces (:450-458)
+  for (int a = 0; a < 3; ++a) {
+    const T q1 = Q(a, 0), q2 = Q(a, 1);
+    const T f2 = -(p00 * i11 + p01 * i12) * q1;
+    const T f3 = -p01 * i22 * q1 - p11 * i22 * q2;
+    fout[3 + a] = f2;
+    fout[6 + a] = f3;
+    fout[a] = -(f2 + f3);
+  }
+  // in-plane PK of the element: invRest * P (:448)
+  const T d11 = i11 * p00 + i12 * p10, d12 = i11 * p01 + i12 * p11, d22 = i22 * p11;
+  // normal and shear response (HybridSolver.cpp:406-430)
+  M3<T> dR = M3<T>::zero();
+  dR(0, 0) = d11; dR(0, 1) = d12; dR(0, 2) = c_gamma * R(0, 2);
+  dR(1, 1) = d22; dR(1, 2) = c_gamma * R(1, 2);
+  dR(2, 2) = R(2, 2) > T(1) ? T(0) : -c_k * (T(1) - R(2, 2)) * (T(1) - R(2, 2));
+  M3<T> K = mul(dR, transpose(R));
+  M3<T> Ks;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Ks(i, j) = K(std::min(i, j), std::max(i, j));
+  M3<T> Rit = inv_transpose(R);
+  T cn[3];
+  if (ref_normal) {
+    for (int i = 0; i < 3; ++i) cn[i] = Dr(2, i);  // (D_rest^T).col(2) (:432)
+  } else {
+    M3<T> Drit = inv_transpose(Dr);  // D_rest^-T e3
+    for (int i = 0; i < 3; ++i) cn[i] = Drit(i, 2);
+  }
+  M3<T> QK = mul(mul(Q, Ks), Rit);
+  T dF3[3];
+  for (int i = 0; i < 3; ++i) dF3[i] = QK(i, 0) * cn[0] + QK(i, 1) * cn[1] + QK(i, 2) * cn[2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Ms_(i, j) = vol * dF3[i] * D(j, 2);  // V_e dF3 d3^T (:434)
+}
+
+// Cloth return mapping on the QR factor of D (HybridSolver.cpp:684-722); returns the new D.
+template <class T> M3<T> cloth_return_map(M3<T> D, T c_gamma, T c_k, T c_cf) {
+  M3<T> Q, R;
+  gram_schmidt_qr(D, Q, R);
+  if (R(2, 2) > T(1)) {
+    R(2, 2) = T(1);
+    R(0, 2) = R(1, 2) = T(0);
+  } else {
+    const T fn = c_k * (R(2, 2) - T(1)) * (R(2, 2) - T(1));
+    const T ft = c_gamma * std::sqrt(R(0, 2) * R(0, 2) + R(1, 2) * R(1, 2));
+    if (ft > c_cf * fn) {
+      R(0, 2) *= c_cf * fn / ft;
+      R(1, 2) *= c_cf * fn / ft;
+    }
+  }
+  for (int a = 0; a < 3; ++a) D(a, 2) = Q(a, 0) * R(0, 2) + Q(a, 1) * R(1, 2) + Q(a, 2) * R(2, 2);
+  return D;
+}
+
 // ---------------------------------------------------------------- the simulation
 struct NodeW {
   int idx;
@@ -318,6 +403,18 @@ template <class T> struct Sim {
   std::vector<T> gm, gp, gf, gv, gvbf;
   std::vector<T> gnode;  // node positions (3 planes)
   uint32_t status = 0;
+
+  // cloth: LagrangianMesh (LagrangianMesh.h:13-117). Vertices and elements (triangles) are MPM
+  // particles; element e carries the directions D = [d1 d2 d3] (row-major M3 per element) and its
+  // rest directions; planes are [component][index].
+  int nv = 0, ne = 0;
+  std::vector<T> vx, vv, vB, vm;      // vertices: x, v, B, m
+  std::vector<T> ex, ev, eB, em, eV;  // elements: x (centroid), v, B, m, volume
+  std::vector<T> eD, eDr;             // directions, rest directions (9 planes each)
+  std::vector<int> faces;             // [e][3] vertex indices
+  std::vector<uint8_t> vfixed;
+  T c_mu = 0, c_lambda = 0, c_gamma = 0, c_k = 0, c_cf = 0;
+  bool c_ref_normal = false;  // true: the reference's (D_rest^T).col(2); false: D_rest^-T e3
 
   void set_grid(const double mn[3], const double mx[3], const int r[3]) {
     for (int a = 0; a < 3; ++a) {
@@ -344,8 +441,12 @@ template <class T> struct Sim {
 
   // evaluateInterpolationWeights_ for one particle (HybridSolver.cpp:30-79)
   void weights(int p, std::vector<NodeW>& out) const {
+    const T xp[3] = {x[p], x[n + p], x[2 * (size_t)n + p]};
+    weights_at(xp, out);
+  }
+  void weights_at(const T xp[3], std::vector<NodeW>& out) const {
     out.clear();
-    T pr[3] = {x[p] - gmin[0], x[n + p] - gmin[1], x[2 * (size_t)n + p] - gmin[2]};
+    T pr[3] = {xp[0] - gmin[0], xp[1] - gmin[1], xp[2] - gmin[2]};
     int fl[3];
     for (int a = 0; a < 3; ++a) fl[a] = static_cast<int>(pr[a] / h[a]);
     for (int i = fl[0] - 2; i <= fl[0] + 2; ++i)
@@ -397,9 +498,29 @@ template <class T> struct Sim {
         }
       }
     }
+    // cloth vertices and elements (:121-125, 137-141, 216-230)
+    for (int k = 0; k < nv; ++k) p2g_one(vx, vv, vB, vm, nv, k, Di, nw);
+    for (int k = 0; k < ne; ++k) p2g_one(ex, ev, eB, em, ne, k, Di, nw);
     for (int c = 0; c < ng; ++c)
       for (int a = 0; a < 3; ++a)
         gv[(size_t)a * ng + c] = gm[c] > T(0) ? gp[(size_t)a * ng + c] / gm[c] : T(0);  // :233-240
+  }
+  // mass and APIC momentum of one particle of a planar set (cloth vertices or elements)
+  void p2g_one(const std::vector<T>& X_, const std::vector<T>& V_, const std::vector<T>& B_, const std::vector<T>& M_,
+               int cnt, int k, T Di, std::vector<NodeW>& nw) {
+    const T xp[3] = {X_[k], X_[cnt + k], X_[2 * (size_t)cnt + k]};
+    weights_at(xp, nw);
+    for (const auto& q : nw) {
+      T w = T(q.w);
+      gm[q.idx] += w * M_[k];
+      T dx[3];
+      for (int a = 0; a < 3; ++a) dx[a] = gnode[(size_t)a * ng + q.idx] - xp[a];
+      for (int a = 0; a < 3; ++a) {
+        T aff = B_[(size_t)(3 * a) * cnt + k] * dx[0] + B_[(size_t)(3 * a + 1) * cnt + k] * dx[1] +
+                B_[(size_t)(3 * a + 2) * cnt + k] * dx[2];
+        gp[(size_t)a * ng + q.idx] += w * M_[k] * (V_[(size_t)a * cnt + k] + Di * aff);
+      }
+    }
   }
 
   // Particle part of computeGridForces_ (HybridSolver.cpp:252-368); gravity is added by grid_update.
@@ -426,8 +547,44 @@ template <class T> struct Sim {
         for (int a = 0; a < 3; ++a)
           gf[(size_t)a * ng + q.idx] -= st(a, 0) * T(q.dw[0]) + st(a, 1) * T(q.dw[1]) + st(a, 2) * T(q.dw[2]);
     }
+    cloth_forces();
   }
   T jp_of(int p) const { return det(getM(FP, p)); }  // :283
+
+  M3<T> elemM(const std::vector<T>& arr, int e) const {
+    M3<T> M;
+    for (int c = 0; c < 9; ++c) M.a[c] = arr[(size_t)c * ne + e];
+    return M;
+  }
+  // Cloth forces (HybridSolver.cpp:370-455): in-plane fixed-corotated forces on the vertices
+  // (LagrangianMesh::computeVertexInPlaneForces, LagrangianMesh.cpp:382-460) spread by the vertex
+  // weights, and the element stress V_e dF3 d3^T of the normal/shear response spread by the element
+  // weight gradients.
+  void cloth_forces() {
+    if (ne == 0) return;
+    std::vector<T> fv(3 * (size_t)nv, T(0));
+    std::vector<M3<T>> Ms(ne);
+    for (int e = 0; e < ne; ++e) {
+      T f[9];
+      cloth_element<T>(elemM(eD, e), elemM(eDr, e), eV[e], c_mu, c_lambda, c_gamma, c_k, c_ref_normal, f, Ms[e]);
+      for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) fv[(size_t)a * nv + faces[3 * e + c]] += f[3 * c + a];
+    }
+    std::vector<NodeW> nw;
+    for (int k = 0; k < nv; ++k) {
+      const T xp[3] = {vx[k], vx[nv + k], vx[2 * (size_t)nv + k]};
+      weights_at(xp, nw);
+      for (const auto& q : nw)
+        for (int a = 0; a < 3; ++a) gf[(size_t)a * ng + q.idx] += T(q.w) * fv[(size_t)a * nv + k];
+    }
+    for (int e = 0; e < ne; ++e) {
+      const T xp[3] = {ex[e], ex[ne + e], ex[2 * (size_t)ne + e]};
+      weights_at(xp, nw);
+      for (const auto& q : nw)
+        for (int a = 0; a < 3; ++a)
+          gf[(size_t)a * ng + q.idx] -= Ms[e](a, 0) * T(q.dw[0]) + Ms[e](a, 1) * T(q.dw[1]) + Ms[e](a, 2) * T(q.dw[2]);
+    }
+  }
 
   // gravity (:457), updateGridVelocities_ (:725-737), gridCollisionHandling_ (:460-511)
   void grid_update(T dt) {
@@ -482,6 +639,28 @@ template <class T> struct Sim {
       for (int a = 0; a < 3; ++a) {
         gvbf[(size_t)a * ng + c] = vt[a];
         gv[(size_t)a * ng + c] = out[a];
+      }
+    }
+    fix_vertices();
+  }
+  // Fixed cloth vertices (HybridSolver.cpp:513-550): every node within one node (per axis) of a node
+  // that a fixed vertex touches is stopped (v = v_BF = 0). Out-of-grid neighbours are skipped per
+  // axis.
+  void fix_vertices() {
+    std::vector<NodeW> nw;
+    for (int k = 0; k < nv; ++k) {
+      if (!vfixed[k]) continue;
+      const T xp[3] = {vx[k], vx[nv + k], vx[2 * (size_t)nv + k]};
+      weights_at(xp, nw);
+      for (const auto& q : nw) {
+        const int ri = q.idx % res[0], rj = (q.idx / res[0]) % res[1], rk = q.idx / (res[0] * res[1]);
+        for (int i = ri - 1; i <= ri + 1; ++i)
+          for (int j = rj - 1; j <= rj + 1; ++j)
+            for (int kk = rk - 1; kk <= rk + 1; ++kk) {
+              if (i < 0 || j < 0 || kk < 0 || i >= res[0] || j >= res[1] || kk >= res[2]) continue;
+              const int id = to_index(i, j, kk);
+              for (int a = 0; a < 3; ++a) gv[(size_t)a * ng + id] = gvbf[(size_t)a * ng + id] = T(0);
+            }
       }
     }
   }
@@ -544,6 +723,139 @@ template <class T> struct Sim {
       for (int a = 0; a < 3; ++a) x[(size_t)a * n + p] = xnew[a];
       return_map(p, Fh);
     }
+    g2p_mesh(dt);
+  }
+
+  // Velocity, affine momentum (damping 1: the symmetric part is dropped, HybridSolver.cpp:808-824)
+  // and the weight-gradient product sum_i v_BF,i grad w^T of one cloth particle at position xp.
+  void g2p_one(const T xp[3], T dt, T vp[3], M3<T>& Bn, M3<T>& Gv, T xnew[3], std::vector<NodeW>& nw) {
+    weights_at(xp, nw);
+    T xs[3] = {0, 0, 0}, vbfs[3] = {0, 0, 0};
+    M3<T> VX = M3<T>::zero();
+    Bn = M3<T>::zero();
+    Gv = M3<T>::zero();
+    for (int a = 0; a < 3; ++a) vp[a] = T(0);
+    for (const auto& q : nw) {
+      T w = T(q.w);
+      for (int a = 0; a < 3; ++a) {
+        const T vi = gv[(size_t)a * ng + q.idx], vb = gvbf[(size_t)a * ng + q.idx];
+        const T xi = gnode[(size_t)a * ng + q.idx];
+        vp[a] += w * vi;
+        vbfs[a] += w * vb;
+        xs[a] += w * (xi + dt * vb);
+        for (int b = 0; b < 3; ++b) {
+          const T xb = gnode[(size_t)b * ng + q.idx];
+          VX(a, b) += w * vi * xb;
+          Bn(a, b) += w * vi * (xb - xp[b]);
+          Gv(a, b) += vb * T(q.dw[b]);
+        }
+      }
+    }
+    if (reference_forms)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Bn(a, b) = VX(a, b) - vp[a] * xp[b];
+    M3<T> C = Bn;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) Bn(a, b) = C(a, b) - T(0.5) * (C(a, b) + C(b, a));
+    for (int a = 0; a < 3; ++a) xnew[a] = reference_forms ? xs[a] : xp[a] + dt * vbfs[a];
+  }
+
+  // G2P of the cloth (HybridSolver.cpp:739-758, 760-825, 942-950, 578-609) and the element return
+  // mapping (:684-722).
+  void g2p_mesh(T dt) {
+    if (nv == 0 && ne == 0) return;
+    std::vector<NodeW> nw;
+    T vp[3], xn[3];
+    M3<T> Bn, Gv;
+    for (int k = 0; k < nv; ++k) {
+      const T xp[3] = {vx[k], vx[nv + k], vx[2 * (size_t)nv + k]};
+      g2p_one(xp, dt, vp, Bn, Gv, xn, nw);
+      for (int c = 0; c < 9; ++c) vB[(size_t)c * nv + k] = Bn.a[c];
+      for (int a = 0; a < 3; ++a) {
+        vv[(size_t)a * nv + k] = vp[a];
+        vx[(size_t)a * nv + k] = xn[a];
+      }
+    }
+    for (int e = 0; e < ne; ++e) {
+      const T xp[3] = {ex[e], ex[ne + e], ex[2 * (size_t)ne + e]};
+      g2p_one(xp, dt, vp, Bn, Gv, xn, nw);
+      for (int c = 0; c < 9; ++c) eB[(size_t)c * ne + e] = Bn.a[c];
+      const int v0 = faces[3 * e], v1 = faces[3 * e + 1], v2 = faces[3 * e + 2];
+      M3<T> D = elemM(eD, e);
+      T d3[3];
+      for (int a = 0; a < 3; ++a) d3[a] = D(a, 2) + dt * (Gv(a, 0) * D(0, 2) + Gv(a, 1) * D(1, 2) + Gv(a, 2) * D(2, 2));
+      for (int a = 0; a < 3; ++a) {
+        const T p0 = vx[(size_t)a * nv + v0], p1 = vx[(size_t)a * nv + v1], p2 = vx[(size_t)a * nv + v2];
+        ex[(size_t)a * ne + e] = (p0 + p1 + p2) / T(3);
+        ev[(size_t)a * ne + e] = (vv[(size_t)a * nv + v0] + vv[(size_t)a * nv + v1] + vv[(size_t)a * nv + v2]) / T(3);
+        D(a, 0) = p1 - p0;
+        D(a, 1) = p2 - p0;
+        D(a, 2) = d3[a];
+      }
+      // return mapping on the QR factor of D (:684-722)
+      D = cloth_return_map<T>(D, c_gamma, c_k, c_cf);
+      for (int c = 0; c < 9; ++c) eD[(size_t)c * ne + e] = D.a[c];
+    }
+  }
+
+  // LagrangianMesh::ObjMesh (LagrangianMesh.cpp:300-352) from given vertices and faces: element
+  // volume 0.25 area thickness, added to each of its vertices; masses density x volume; rest and
+  // current directions v2 - v1, v3 - v1 and the unit normal; element positions at the centroids.
+  void set_mesh(int nv_, const double* x_, const double* v_, int ne_, const int* f_, double density,
+                double thickness, double E, double nu, double gamma, double k, double angle_deg,
+                const uint8_t* fixed, int ref_normal) {
+    nv = nv_;
+    ne = ne_;
+    faces.assign(f_, f_ + 3 * (size_t)ne);
+    vx.assign(x_, x_ + 3 * (size_t)nv);
+    vv.assign(3 * (size_t)nv, T(0));
+    if (v_) for (size_t i = 0; i < 3 * (size_t)nv; ++i) vv[i] = T(v_[i]);
+    vB.assign(9 * (size_t)nv, T(0));
+    std::vector<double> vvol(nv, 0.0);
+    ex.assign(3 * (size_t)ne, T(0)); ev.assign(3 * (size_t)ne, T(0)); eB.assign(9 * (size_t)ne, T(0));
+    em.assign(ne, T(0)); eV.assign(ne, T(0)); eD.assign(9 * (size_t)ne, T(0)); eDr.assign(9 * (size_t)ne, T(0));
+    for (int e = 0; e < ne; ++e) {
+      double p[3][3];
+      for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) p[c][a] = x_[(size_t)a * nv + faces[3 * e + c]];
+      auto len = [](const double* a, const double* b) {
+        return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+      };
+      const double la = len(p[0], p[1]), lb = len(p[1], p[2]), lc = len(p[2], p[0]), hp = 0.5 * (la + lb + lc);
+      const double area = std::sqrt(hp * (hp - la) * (hp - lb) * (hp - lc));  // geometry.cpp:12-23
+      const double vol = 0.25 * area * thickness;
+      eV[e] = T(vol);
+      em[e] = T(density * vol);
+      for (int c = 0; c < 3; ++c) vvol[faces[3 * e + c]] += vol;
+      double d1[3], d2[3], nrm[3];
+      for (int a = 0; a < 3; ++a) {
+        d1[a] = p[1][a] - p[0][a];
+        d2[a] = p[2][a] - p[0][a];
+      }
+      nrm[0] = d1[1] * d2[2] - d1[2] * d2[1];
+      nrm[1] = d1[2] * d2[0] - d1[0] * d2[2];
+      nrm[2] = d1[0] * d2[1] - d1[1] * d2[0];
+      const double nl = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+      for (int a = 0; a < 3; ++a) {
+        const double col[3] = {d1[a], d2[a], nrm[a] / nl};
+        for (int c = 0; c < 3; ++c) eDr[(size_t)(3 * a + c) * ne + e] = eD[(size_t)(3 * a + c) * ne + e] = T(col[c]);
+        ex[(size_t)a * ne + e] = T((p[0][a] + p[1][a] + p[2][a]) / 3.0);
+      }
+    }
+    vm.assign(nv, T(0));
+    for (int k2 = 0; k2 < nv; ++k2) vm[k2] = T(density * vvol[k2]);
+    for (int e = 0; e < ne; ++e)
+      for (int a = 0; a < 3; ++a)
+        ev[(size_t)a * ne + e] = (vv[(size_t)a * nv + faces[3 * e]] + vv[(size_t)a * nv + faces[3 * e + 1]] +
+                                  vv[(size_t)a * nv + faces[3 * e + 2]]) / T(3);
+    c_lambda = T(E * nu / (1.0 + nu) / (1.0 - 2.0 * nu));
+    c_mu = T(E / 2.0 / (1.0 + nu));
+    c_gamma = T(gamma);
+    c_k = T(k);
+    c_cf = T(std::tan(angle_deg * 3.14159265358979323846 / 180.0));
+    c_ref_normal = ref_normal != 0;
+    vfixed.assign(nv, 0);
+    if (fixed) for (int i = 0; i < nv; ++i) vfixed[i] = fixed[i];
   }
 
   // updatePlasticity_ particle branch (HybridSolver.cpp:616-681)
@@ -677,6 +989,17 @@ template <class T> int stage(orc::Sim<T>& s, int st, double dt) {
 }
 }  // namespace
 
+namespace {
+template <class T> static void get_mesh(const orc::Sim<T>& s, double* vx, double* vv, double* D) {
+  for (size_t i = 0; i < 3 * (size_t)s.nv; ++i) {
+    if (vx) vx[i] = double(s.vx[i]);
+    if (vv) vv[i] = double(s.vv[i]);
+  }
+  if (D)
+    for (size_t i = 0; i < 9 * (size_t)s.ne; ++i) D[i] = double(s.eD[i]);
+}
+}  // namespace
+
 extern "C" {
 
 void* orc_create(int precision, const mpm_params* P, const double mn[3], const double mx[3], const int32_t res[3],
@@ -728,6 +1051,37 @@ int orc_substep(void* hp, double dt) {
   return 0;
 }
 
+// Cloth of the oracle (LagrangianMesh::ObjMesh from given vertices and faces).
+int orc_set_mesh(void* hp, int nv, const double* x, const double* v, int nf, const int* faces, double density,
+                 double thickness, double E, double nu, double gamma, double k, double angle_deg, const uint8_t* fixed,
+                 int ref_normal) {
+  Handle* h = static_cast<Handle*>(hp);
+  h->d.set_mesh(nv, x, v, nf, faces, density, thickness, E, nu, gamma, k, angle_deg, fixed, ref_normal);
+  h->f.set_mesh(nv, x, v, nf, faces, density, thickness, E, nu, gamma, k, angle_deg, fixed, ref_normal);
+  return 0;
+}
+// One cloth element in FP64: vertex forces f[9] and element stress M[9] (row-major) from D and the
+// rest directions Dr (both row-major, columns d1 d2 d3).
+void orc_cloth_element(const double* D, const double* Dr, double vol, double E, double nu, double gamma, double k,
+                       int ref_normal, double* f, double* M) {
+  orc::M3<double> A, B, Ms;
+  for (int i = 0; i < 9; ++i) { A.a[i] = D[i]; B.a[i] = Dr[i]; }
+  const double lambda = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu), mu = E / 2.0 / (1.0 + nu);
+  orc::cloth_element<double>(A, B, vol, mu, lambda, gamma, k, ref_normal != 0, f, Ms);
+  for (int i = 0; i < 9; ++i) M[i] = Ms.a[i];
+}
+void orc_cloth_return_map(const double* D, double gamma, double k, double angle_deg, double* out) {
+  orc::M3<double> A;
+  for (int i = 0; i < 9; ++i) A.a[i] = D[i];
+  orc::M3<double> B = orc::cloth_return_map<double>(A, gamma, k, std::tan(angle_deg * 3.14159265358979323846 / 180.0));
+  for (int i = 0; i < 9; ++i) out[i] = B.a[i];
+}
+int orc_get_mesh(void* hp, double* vx, double* vv, double* D) {
+  Handle* h = static_cast<Handle*>(hp);
+  if (h->prec == 64) get_mesh(h->d, vx, vv, D);
+  else get_mesh(h->f, vx, vv, D);
+  return 0;
+}
 int orc_solve(void* hp, double cfl, double maxt, int max_substeps, double* dts, int* frame_end, double* t_out) {
   Handle* h = static_cast<Handle*>(hp);
   return h->prec == 64 ? solve(h->d, cfl, maxt, max_substeps, dts, frame_end, t_out)
