@@ -411,8 +411,11 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
 // ------------------------------------------------------------------ cloth (LagrangianMesh)
 // Gram-Schmidt QR of D = [d1 d2 d3] (columns; D row-major), GramSchmidtOrthonomalization
 // (geometry.cpp:31-62): Q orthonormal columns, R upper triangular with R(i,i) = column norms.
+// No FMA contraction: the rest factors and the per-substep factors are computed in different kernels
+// and must agree bit for bit on an undeformed element.
 template <class T>
 __device__ __forceinline__ void qr3(const T* D, T* Q, T* R) {
+#pragma clang fp contract(off)
   T q1[3], q2[3], q3[3];
   const T r11 = sqrt(D[0] * D[0] + D[3] * D[3] + D[6] * D[6]);
 #pragma unroll
@@ -443,6 +446,7 @@ __device__ __forceinline__ void qr3(const T* D, T* Q, T* R) {
 // FP32 QR, redone in FP64 when D is near-degenerate (a column nearly in the span of the previous
 // ones: R(1,1) < ratio |d2| or R(2,2) < ratio |d3|) or the result is not finite.
 __device__ __forceinline__ void qr3_guarded(const float* D, float* Q, float* R, float ratio, int& fallback) {
+#pragma clang fp contract(off)
   qr3<float>(D, Q, R);
   const float n2 = sqrtf(D[1] * D[1] + D[4] * D[4] + D[7] * D[7]);
   const float n3 = sqrtf(D[2] * D[2] + D[5] * D[5] + D[8] * D[8]);
@@ -465,30 +469,34 @@ __device__ __forceinline__ void qr3_guarded(const float* D, float* Q, float* R, 
 //    vertex forces f2 = -(P11 i11 + P12 i12) q1, f3 = -P12 i22 q1 - P22 i22 q2, f1 = -(f2 + f3);
 //  - normal/shear: dR = [[S11 S12 gamma r13], [0 S22 gamma r23], [0 0 -k (1 - r33)^2 or 0 if r33 > 1]]
 //    with S = R_rest^-1 P, K = dR R^T, dF3 = Q sym_upper(K) R^-T cn, element stress V dF3 d3^T.
-// rest = (i11, i12, i22, cn0, cn1, cn2): the inverse rest in-plane R and the normal-force vector.
+// rest = (i11, i12, i22, cn0, cn1, cn2, R11, R12, R22): the inverse rest in-plane R, the normal-force
+// vector, and the rest in-plane R as this file's qr3 computes it from the element's rest directions.
+// A is formed as I + E with E = R_rest^-1 (R - R_rest), and A - rot(A), J - 1 from E without
+// cancellation. This equals the reference's A in exact arithmetic; in FP32 it makes an undeformed
+// element exactly force-free (A = I bit for bit), where R_rest^-1 R carries a rounding strain of
+// about 1e-7 that, for fine meshes, the unscaled in-plane force turns into accelerations of m/s^2.
 __device__ __forceinline__ void cloth_element_forces(const DevParams& P, const float* D, const float* rest, float vol,
                                                      float* fv /* f1 f2 f3 */, float* M /* 3x3 */, int& fb) {
   float Q[9], R[9];
   qr3_guarded(D, Q, R, P.svd_ratio, fb);
   const float i11 = rest[0], i12 = rest[1], i22 = rest[2];
-  const float a00 = i11 * R[0], a01 = i11 * R[1] + i12 * R[4], a10 = 0.0f, a11 = i22 * R[4];
-  float r00, r01, r10, r11;
-  if (a00 * a11 - a01 * a10 >= 0.0f) {
-    float pp = a00 + a11, qq = a10 - a01;
-    const float nn = sqrtf(pp * pp + qq * qq);
-    if (nn > 0.0f) { pp /= nn; qq /= nn; } else { pp = 1.0f; qq = 0.0f; }
-    r00 = pp; r01 = -qq; r10 = qq; r11 = pp;
-  } else {
-    float pp = a00 - a11, qq = a01 + a10;
-    const float nn = sqrtf(pp * pp + qq * qq);
-    pp /= nn; qq /= nn;
-    r00 = pp; r01 = qq; r10 = qq; r11 = -pp;
+  const float d11 = R[0] - rest[6], d12 = R[1] - rest[7], d22 = R[4] - rest[8];
+  const float e00 = i11 * d11, e01 = i11 * d12 + i12 * d22, e11 = i22 * d22;
+  // A = [[1 + e00, e01], [0, 1 + e11]] has a non-negative diagonal (R and R_rest^-1 do), so det A >= 0
+  // and rot(A) = [[c, -s], [s, c]] with (c, s) = (2 + e00 + e11, -e01) / norm.
+  const float pp = 2.0f + e00 + e11, qq = -e01;
+  const float nn = sqrtf(pp * pp + qq * qq);
+  float s = 0.0f, omc = 0.0f;  // sin and 1 - cos of the polar rotation
+  if (nn > 0.0f) {
+    s = qq / nn;
+    omc = qq * qq / (nn * (nn + pp));
   }
-  const float J = a00 * a11;
-  const float p00 = 2.0f * P.c_mu * (a00 - r00) + P.c_lambda * (J - 1.0f) * a11;
-  const float p01 = 2.0f * P.c_mu * (a01 - r01);
-  const float p10 = 2.0f * P.c_mu * (a10 - r10) + P.c_lambda * (J - 1.0f) * (-a01);
-  const float p11 = 2.0f * P.c_mu * (a11 - r11) + P.c_lambda * (J - 1.0f) * a00;
+  const float m00 = e00 + omc, m01 = e01 + s, m10 = -s, m11 = e11 + omc;  // A - rot(A)
+  const float Jm1 = e00 + e11 + e00 * e11;
+  const float p00 = 2.0f * P.c_mu * m00 + P.c_lambda * Jm1 * (1.0f + e11);
+  const float p01 = 2.0f * P.c_mu * m01;
+  const float p10 = 2.0f * P.c_mu * m10 + P.c_lambda * Jm1 * (-e01);
+  const float p11 = 2.0f * P.c_mu * m11 + P.c_lambda * Jm1 * (1.0f + e00);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float q1 = Q[3 * a], q2 = Q[3 * a + 1];

@@ -249,7 +249,7 @@ struct ClothArgs {
   const int* pos;          // caller id -> physical index in st
   int nv, ne;
   const int* faces;        // [e][3] vertex caller ids
-  const float* rest;       // [e][6] inverse rest in-plane R (i11 i12 i22) and the normal-force vector
+  const float* rest;       // [e][9] inverse rest in-plane R (i11 i12 i22), normal-force vector, rest R (R11 R12 R22)
   const int* vcsr_off;     // [nv+1] incident element corners of each vertex ...
   const int* vcsr;         // ... as 3 e + corner, ascending e
   float* elem_fv;          // [e][9] vertex forces f1 f2 f3 of each element
@@ -264,6 +264,7 @@ struct ClothArgs {
   unsigned long long* fallback;
   const Clock* clk;
 };
+void launch_cloth_rest(hipStream_t s, const ClothArgs& a, float* rest);  // rest in-plane R from the directions
 void launch_cloth_forces(hipStream_t s, const ClothArgs& a);   // element forces, then the per-vertex sums
 void launch_cloth_fixed(hipStream_t s, const ClothArgs& a);    // after the grid update
 void launch_cloth_elements(hipStream_t s, const ClothArgs& a); // after G2P: positions, velocities, D, return map

@@ -75,11 +75,13 @@ __global__ __launch_bounds__(256) void k_stress_init(SceneTable T, float* __rest
   const DevParams& P = T.params[b.s];
   const int i = b.j;
   if (i >= b.end) return;
-  float F[9], K[6];
+  float F[9], K[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int c = 0; c < 9; ++c) F[c] = st[(PF + c) * cap + i];
   int fb = 0;
-  kirchhoff_of_F(P, F, st[PPL * cap + i], st[PVOL * cap + i], K, fb);
+  // cloth vertices and elements carry no particle stress (their F planes hold D for elements)
+  const int pid = reinterpret_cast<const int*>(st + (size_t)PID * cap)[i];
+  if (particle_kind(P, pid) == KIND_PARTICLE) kirchhoff_of_F(P, F, st[PPL * cap + i], st[PVOL * cap + i], K, fb);
 #pragma unroll
   for (int e = 0; e < 6; ++e) st[(PK + e) * cap + i] = K[e];
   if (fb) atomicAdd(fallback, (unsigned long long)fb);
@@ -1070,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_cloth_element_forces(ClothArgs a) {
     float D[9], fv[9], M[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) D[i] = a.st[(PF + i) * a.cap + pi];
-    cloth_element_forces(P, D, a.rest + 6 * (size_t)e, a.st[PVOL * a.cap + pi], fv, M, fb);
+    cloth_element_forces(P, D, a.rest + 9 * (size_t)e, a.st[PVOL * a.cap + pi], fv, M, fb);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       a.elem_fv[9 * (size_t)e + i] = fv[i];
@@ -1078,6 +1080,21 @@ __global__ __launch_bounds__(256) void k_cloth_element_forces(ClothArgs a) {
     }
   }
   if (fb) atomicAdd(a.fallback, (unsigned long long)fb);
+}
+
+// The rest in-plane QR factor of each element, from its undeformed directions (set-up only).
+__global__ __launch_bounds__(256) void k_cloth_rest(ClothArgs a, float* __restrict__ rest) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.ne) return;
+  const int pi = clamp_index(a.pos[a.P.mesh_e0 + e], a.n);
+  float D[9], Q[9], R[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) D[i] = a.st[(PF + i) * a.cap + pi];
+  int fb = 0;
+  qr3_guarded(D, Q, R, a.P.svd_ratio, fb);
+  rest[9 * (size_t)e + 6] = R[0];
+  rest[9 * (size_t)e + 7] = R[1];
+  rest[9 * (size_t)e + 8] = R[4];
 }
 
 __global__ __launch_bounds__(256) void k_cloth_vertex_forces(ClothArgs a) {
@@ -1419,6 +1436,10 @@ void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid) {
 
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v) {
   if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_cloth_rest(hipStream_t s, const ClothArgs& a, float* rest) {
+  if (a.ne > 0) hipLaunchKernelGGL(k_cloth_rest, dim3(blocks_for(a.ne, 256)), dim3(256), 0, s, a, rest);
 }
 
 void launch_cloth_forces(hipStream_t s, const ClothArgs& a) {

@@ -47,7 +47,7 @@ struct SceneHost {
 struct MeshHost {
   int np = 0, nv = 0, ne = 0;  // particles before the mesh, vertices, elements
   int* faces = nullptr;        // device: [e][3] vertex caller ids
-  float* rest = nullptr;       // device: [e][6]
+  float* rest = nullptr;       // device: [e][9]
   int* vcsr_off = nullptr;     // device: [nv+1]
   int* vcsr = nullptr;         // device: [3 ne]
   float* elem_fv = nullptr;    // device: [e][9]
@@ -921,7 +921,7 @@ int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32
   // LagrangianMesh::ObjMesh (LagrangianMesh.cpp:300-352): element volume 0.25 area thickness, added
   // to each vertex; masses density x volume; rest directions v2 - v1, v3 - v1, unit normal.
   std::vector<double> vvol(nv, 0.0);
-  std::vector<float> rest((size_t)6 * nf);
+  std::vector<float> rest((size_t)9 * nf, 0.0f);  // the rest R (entries 6..8) is filled on the device
   auto X = [&](int vtx, int a) { return (double)x[(size_t)a * nv + vtx]; };
   for (int e = 0; e < nf; ++e) {
     double p[3][3];
@@ -951,19 +951,19 @@ int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32
     const double r12 = d2[0] * q1[0] + d2[1] * q1[1] + d2[2] * q1[2];
     double q2[3] = {d2[0] - r12 * q1[0], d2[1] - r12 * q1[1], d2[2] - r12 * q1[2]};
     const double r22 = std::sqrt(q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2]);
-    rest[6 * e] = float(1.0 / r11);
-    rest[6 * e + 1] = float(-r12 / r11 / r22);
-    rest[6 * e + 2] = float(1.0 / r22);
+    rest[9 * e] = float(1.0 / r11);
+    rest[9 * e + 1] = float(-r12 / r11 / r22);
+    rest[9 * e + 2] = float(1.0 / r22);
     // normal-force vector: the reference's (D_rest^T).col(2) or D_rest^-T e3 = (d1 x d2) / det(D_rest)
     const double Dr[3][3] = {{d1[0], d2[0], nr[0]}, {d1[1], d2[1], nr[1]}, {d1[2], d2[2], nr[2]}};
     if (mp->reference_normal_row) {
-      for (int c = 0; c < 3; ++c) rest[6 * e + 3 + c] = float(Dr[2][c]);
+      for (int c = 0; c < 3; ++c) rest[9 * e + 3 + c] = float(Dr[2][c]);
     } else {
       const double det = Dr[0][0] * (Dr[1][1] * Dr[2][2] - Dr[1][2] * Dr[2][1]) -
                          Dr[0][1] * (Dr[1][0] * Dr[2][2] - Dr[1][2] * Dr[2][0]) +
                          Dr[0][2] * (Dr[1][0] * Dr[2][1] - Dr[1][1] * Dr[2][0]);
       const double cr[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
-      for (int c = 0; c < 3; ++c) rest[6 * e + 3 + c] = float(cr[c] / det);
+      for (int c = 0; c < 3; ++c) rest[9 * e + 3 + c] = float(cr[c] / det);
     }
     // element particle
     const int i = np + nv + e;
@@ -1006,7 +1006,7 @@ int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32
   drop_mesh(ctx);
   MeshHost& m = ctx->mesh;
   int rc = dalloc(ctx, m.faces, (size_t)3 * nf);
-  if (!rc) rc = dalloc(ctx, m.rest, (size_t)6 * nf);
+  if (!rc) rc = dalloc(ctx, m.rest, (size_t)9 * nf);
   if (!rc) rc = dalloc(ctx, m.vcsr_off, (size_t)nv + 1);
   if (!rc) rc = dalloc(ctx, m.vcsr, (size_t)3 * nf);
   if (!rc) rc = dalloc(ctx, m.elem_fv, (size_t)9 * nf);
@@ -1035,7 +1035,10 @@ int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32
   sc.c_k = mp->stiffness;
   sc.c_cf = float(std::tan(double(mp->friction_angle) * 3.14159265358979323846 / 180.0));
   rc = install_state(ctx, n, h);
-  return rc;
+  if (rc) return rc;
+  launch_cloth_rest(ctx->stream, cloth_args(ctx, ctx->cur), m.rest);
+  CK(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
 }
 
 int mpm_get_mesh(mpm_ctx* ctx, float* vx, float* vv, float* D) {

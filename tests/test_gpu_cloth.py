@@ -1,15 +1,16 @@
 """GPU parity of the cloth (LagrangianMesh) path through the C ABI against the FP64 oracle.
 
 Tolerances (FP32 device state against the FP64 restatement):
-  one substep, per stage    grid m, momentum (cloth forces included), v, v_BF: relative L2 <= 5e-5;
-                            mesh vertices |dx| <= 1e-5 h, vertex velocity and element directions D
-                            relative L2 <= 5e-5; sand particles as in the particle parity tests
+  one substep, per stage    grid m, momentum (cloth forces included), v, v_BF, mesh vertices and
+                            directions D, sand: each error at most 2x the FP32 oracle's own error
+                            against the FP64 oracle, plus 1e-6 (the in-plane forces cancel to small
+                            strains, which FP32 positions resolve to a few percent)
   60 substeps, sphere       mesh vertices |dx| <= 1e-4 h, D relative L2 <= 1e-3, with the return map
                             active (some elements changed by it)
   fixed vertices            fixed vertices keep their positions exactly; parity as above
-  full size (256^2 cloth, about 500k sand particles, 128^3 grid), 10 substeps: before any contact the
-                            cloth falls as a rigid body (v_y = 10 dt g within 1e-4 relative) and its
-                            directions keep their rest lengths within 1e-4
+  full size (256^2 cloth, about 500k sand particles, 128^3 grid), 200 substeps: before any contact
+                            the cloth falls as a rigid body (v_y = n dt g within 1e-4 relative, no
+                            in-plane velocity) and its directions keep their rest values within 1e-4
 """
 import sys
 
@@ -67,41 +68,61 @@ def _both(sc, x, v, faces, fixed):
     return s, o
 
 
+def _oracle_substep_then_stages(sc, x, v, faces, fixed, precision):
+    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims, precision=precision)
+    o.load_scene(sc)
+    o.set_mesh(x, faces, v=v, fixed=fixed)
+    o.substep(sc.dt)
+    o.stage(_lib.STAGE_P2G, sc.dt)
+    o.stage(_lib.STAGE_GRID, sc.dt)
+    g = o.get_grid()
+    o.stage(_lib.STAGE_G2P, sc.dt)
+    return g, o.get_mesh(), o.get_particles()
+
+
+def _errors(g, m, q, g64, m64, q64, dt, h):
+    on = g64["m"] > 0
+    mom = g64["p"] + dt * g64["f"]
+    gm = g["mom"] if "mom" in g else g["p"] + dt * g["f"]
+    return dict(m=rel(g["m"], g64["m"]), mom=rel(gm[:, on], mom[:, on]), v=rel(g["v"][:, on], g64["v"][:, on]),
+                vbf=rel(g["vbf"][:, on], g64["vbf"][:, on]), mesh_x=np.max(np.abs(m["vx"] - m64["vx"])) / h,
+                mesh_v=rel(m["vv"], m64["vv"]), D=rel(m["D"], m64["D"]),
+                sand_x=np.max(np.abs(q["x"] - q64["x"])) / h, sand_v=rel(q["v"], q64["v"]))
+
+
 def test_cloth_one_substep_stages():
+    """The in-plane forces come from F - R with strains of order 1e-4, formed from FP32 vertex
+    positions: FP32 rounding alone moves them by a few percent and the grid velocity by about 1e-4
+    relative. The bar is therefore the FP32 oracle's own distance from the FP64 oracle (same
+    algorithm, FP32 arithmetic): the GPU may be at most twice as far, plus 1e-6."""
     sc, x, v, faces, fixed = cloth_scene()
     dt = sc.dt
+    h = 1.0 / sc.res[0]
     say(f"[cloth stages] particles {sc.n}, vertices {x.shape[1]}, faces {len(faces)}")
-    s, o = _both(sc, x, v, faces, fixed)
+    g64, m64, q64 = _oracle_substep_then_stages(sc, x, v, faces, fixed, 64)
+    g32, m32, q32 = _oracle_substep_then_stages(sc, x, v, faces, fixed, 32)
+    s = Solver()
+    sc.load_into(s)
+    s.set_mesh(x, faces, v=v, fixed=fixed)
     with s:
         # one full substep first, so the directions D carry deformation into the next forces
         s.substep(dt)
-        o.substep(dt)
         for st in range(4):
             s.stage(st, dt)
-            if st > 0:
-                o.stage(st, dt)
             if st == _lib.STAGE_GRID:
-                g, r = s.get_grid(), o.get_grid()
-                on = r["m"] > 0
-                assert np.array_equal(g["m"] > 0, on), "active node sets differ"
-                mom_ref = r["p"] + dt * r["f"]
-                assert np.max(np.abs(r["f"][:, on])) > 1.0  # cloth forces are present
-                assert rel(g["m"], r["m"]) < 5e-5
-                assert rel(g["mom"][:, on], mom_ref[:, on]) < 5e-5
-                assert rel(g["v"][:, on], r["v"][:, on]) < 5e-5
-                assert rel(g["vbf"][:, on], r["vbf"][:, on]) < 5e-5
-        m, mr = s.get_mesh(), o.get_mesh()
-        q, qr = s.get_particles(), o.get_particles()
-        st = s.stats()
-    h = 1.0 / sc.res[0]
-    say(f"  mesh |dx| {np.max(np.abs(m['vx'] - mr['vx'])):.3e}, v rel {rel(m['vv'], mr['vv']):.3e}, "
-        f"D rel {rel(m['D'], mr['D']):.3e}")
-    assert np.max(np.abs(m["vx"] - mr["vx"])) <= 1e-5 * h
-    assert rel(m["vv"], mr["vv"]) < 5e-5
-    assert rel(m["D"], mr["D"]) < 5e-5
-    assert np.max(np.abs(q["x"] - qr["x"])) <= 1e-5 * h
-    assert rel(q["v"], qr["v"]) < 5e-5
-    assert st["status"] == 0, st
+                g = s.get_grid()
+        m, q = s.get_mesh(), s.get_particles()
+        stats = s.stats()
+    on = g64["m"] > 0
+    assert np.array_equal(g["m"] > 0, on), "active node sets differ"
+    assert np.max(np.abs(g64["f"][:, on])) > 1.0  # cloth forces are present
+    e_gpu = _errors(g, m, q, g64, m64, q64, dt, h)
+    e_f32 = _errors(g32, m32, q32, g64, m64, q64, dt, h)
+    for k in e_gpu:
+        say(f"  {k:7s} gpu {e_gpu[k]:.3e}  fp32 oracle {e_f32[k]:.3e}")
+    for k in e_gpu:
+        assert e_gpu[k] <= 2.0 * e_f32[k] + 1e-6, k
+    assert stats["status"] == 0, stats
 
 
 @pytest.mark.parametrize("fixed_corners", [False, True])
@@ -135,7 +156,7 @@ def test_cloth_drape_60_substeps(fixed_corners):
 
 def test_config3_full_size_free_fall():
     sc, mesh = scenes.config3_cloth_sand()
-    n = 10
+    n = 200
     say(f"[config3] sand {sc.n}, vertices {mesh['x'].shape[1]}, faces {len(mesh['faces'])}, substeps {n}")
     with Solver() as s:
         sc.load_into(s)
