@@ -30,6 +30,8 @@ constexpr int SORT_IPT = 16;
 constexpr int SORT_TILE = 256 * SORT_IPT;  // keys per sort pass block
 constexpr int HIST_IPT = 16;
 constexpr int HIST_TILE = 256 * HIST_IPT;  // keys per histogram block
+constexpr int SCAN_IPT = 16;
+constexpr int SCAN_TILE = 256 * SCAN_IPT;  // cells per cell-scan block
 constexpr int TILE = 4;                    // cells per tile edge
 constexpr int FOOT = TILE + 3;             // footprint nodes per edge (cell window c-1 .. c+2)
 constexpr int FOOT3 = FOOT * FOOT * FOOT;
@@ -104,20 +106,24 @@ __device__ __forceinline__ float clock_dt(const Clock* c, float dt) { return c !
 struct SortBufs {
   uint32_t *k1, *k2;
   int *v1, *v2;
-  uint32_t* hist;     // [3][256]
-  uint32_t* tickets;  // [4]
-  uint32_t* status;   // [3][nblocks][256]
-  int* counters;      // zeroed by the histogram kernel
+  uint32_t* hist;     // [4][256]
+  uint32_t* tickets;  // [4] digit passes, [4] cell scan
+  uint32_t* status;   // [passes][nblocks][256] digit-pass look-back words
+  unsigned long long* scan_status;  // [scan blocks] cell-scan look-back words
+  int* cell_count;    // [cells] zero between substeps (the bin pass clears the occupied cells)
+  int* cell_start;    // [cells + 1] exclusive prefix of the counts; cell c holds [start[c], start[c+1])
+  int64_t ncells;
+  SceneTable T;       // the counting sort also writes src and flags the occupied tiles
+  int* tile_slot;
+  const int* pos;
+  int* src;
+  int* counters;      // zeroed by the first sort kernel
   const Clock* clk;
 };
 
 struct ResetArgs {
   uint32_t* zero_words;
   int64_t n_zero;
-  const uint32_t* prev_keys;
-  int n;
-  int* cell_start;
-  int* cell_end;
   const int* counters;
   const int* active_tiles;
   int* tile_slot;
@@ -215,12 +221,17 @@ void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* s
 void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* st, int64_t cap, int n,
                         unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
-// `passes` 8-bit digit passes (1..4); the result is in k1/v1 for an odd count, k2/v2 for an even one.
-void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n, int passes);
-// Also writes src[j] = pos[perm[j]]: the physical index of the particle at sorted position j.
-void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
-                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
-                   const int* perm, const int* pos, int* src, const Clock* clk);
+// Stable sort of the cell keys by caller id. Both modes count the particles of every cell and scan
+// the counts into cell_start. SORT_COUNTING then places each caller id at its stable rank inside
+// its cell; SORT_RADIX runs `passes` 8-bit LSD digit passes (1..4). Both give the same order. The
+// result is in k1/v1 for an odd `passes`, k2/v2 for an even one.
+enum SortMode : int { SORT_COUNTING = 0, SORT_RADIX = 1 };
+void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n, int passes, int mode);
+// Active tiles and node blocks from the sorted keys; clears the counts of the occupied cells and
+// writes src[j] = pos[perm[j]]: the physical index of the particle at sorted position j.
+void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles,
+                   int* cell_count, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks,
+                   int* counters, const int* perm, const int* pos, int* src, const Clock* clk, int mode);
 // order: B-spline order of every scene (2 or 3); ranks: particles per cell staged per chunk (4 or 8)
 void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);

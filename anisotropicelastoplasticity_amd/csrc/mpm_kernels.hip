@@ -95,8 +95,6 @@ __global__ __launch_bounds__(256) void k_stress_init(SceneTable T, float* __rest
 constexpr uint32_t FLAG_AGG = 1u << 30, FLAG_PREFIX = 2u << 30, VALUE_MASK = (1u << 30) - 1u;
 
 __global__ __launch_bounds__(256) void k_reset(uint32_t* __restrict__ zero_words, int64_t n_zero,
-                                               const uint32_t* __restrict__ prev_keys, int n,
-                                               int* __restrict__ cell_start, int* __restrict__ cell_end,
                                                const int* __restrict__ counters, const int* __restrict__ active_tiles,
                                                int* __restrict__ tile_slot, const int* __restrict__ active_blocks,
                                                int* __restrict__ block_slot, const Clock* clk) {
@@ -104,10 +102,6 @@ __global__ __launch_bounds__(256) void k_reset(uint32_t* __restrict__ zero_words
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = t0; i < n_zero; i += stride) zero_words[i] = 0u;
-  for (int64_t j = t0; j < n; j += stride) {
-    uint32_t k = prev_keys[j];
-    if (j == 0 || prev_keys[j - 1] != k) { cell_start[k] = 0; cell_end[k] = 0; }
-  }
   int nt = counters[CNT_TILES], nb = counters[CNT_BLOCKS];
   for (int64_t j = t0; j < nt; j += stride) tile_slot[active_tiles[j]] = -1;
   for (int64_t j = t0; j < nb; j += stride) block_slot[active_blocks[j]] = -1;
@@ -126,12 +120,10 @@ __device__ __forceinline__ uint64_t digit_peers(bool valid, uint32_t d) {
 }
 
 __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ keys, int n, int passes,
-                                                   uint32_t* __restrict__ hist, int* __restrict__ counters,
-                                                   const Clock* clk) {
+                                                   uint32_t* __restrict__ hist, const Clock* clk) {
   if (clock_skip(clk)) return;
   __shared__ uint32_t h[4][256];
   for (int i = threadIdx.x; i < 1024; i += 256) (&h[0][0])[i] = 0u;
-  if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
   __syncthreads();
   const int base = blockIdx.x * HIST_TILE;
   for (int k = 0; k < HIST_IPT; ++k) {
@@ -290,13 +282,196 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------------ one-digit counting sort
+// The cell key itself is the single digit: count the particles of every cell (atomic ranks, in
+// no particular order), scan the counts into cell offsets, scatter the caller ids into their
+// cells, then order each cell by caller id (the rank of id i in its cell is the number of smaller
+// ids there). The result equals the LSD passes' stable sort bit for bit.
+//
+// Counting: the lanes of a wave that share a key are grouped first (leader = the lowest such lane);
+// then every leader adds its group's size with one atomic, all leaders in the same instruction, so
+// a wave waits for one atomic round trip.
+__global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__ keys, int n, int* __restrict__ cnt,
+                                                    uint32_t* __restrict__ rank, int* __restrict__ counters,
+                                                    const Clock* clk) {
+  if (clock_skip(clk)) return;
+  if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < n;
+  const uint32_t key = valid ? keys[i] : 0u;
+  const int lane = (int)lane_id();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint64_t todo = __ballot(valid);
+  int leader = lane, group = 0, within = 0;
+  while (todo) {
+    const int first = __ffsll((long long)todo) - 1;
+    const uint32_t k0 = (uint32_t)__shfl((int)key, first);
+    const uint64_t peers = __ballot(valid && key == k0);
+    if (valid && key == k0) {
+      leader = first;
+      group = __popcll(peers);
+      within = __popcll(peers & lt);
+    }
+    todo &= ~peers;
+  }
+  int base = 0;
+  if (valid && leader == lane) base = atomicAdd(&cnt[key], group);
+  base = __shfl(base, leader);
+  if (valid) rank[i] = (uint32_t)(base + within);
+}
+
+constexpr unsigned long long SCAN_AGG = 1ull << 62, SCAN_PREFIX = 2ull << 62, SCAN_VALUE = 0xFFFFFFFFull;
+
+// Exclusive scan of the cell counts into cell_start[0 .. ncells] (single pass, decoupled look-back:
+// blocks take tickets in start order and publish their aggregate before looking back, so a block
+// only waits on running blocks).
+__global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, int64_t ncells, int* __restrict__ cell_start,
+                                                   uint32_t* __restrict__ ticket, unsigned long long* __restrict__ status,
+                                                   const Clock* clk) {
+  if (clock_skip(clk)) return;
+  __shared__ int s_b;
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_excl;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_b = (int)atomicAdd(ticket, 1u);
+  __syncthreads();
+  const int b = s_b;
+  const int64_t base = (int64_t)b * SCAN_TILE + (int64_t)tid * SCAN_IPT;
+  int c[SCAN_IPT];
+  if (base + SCAN_IPT <= ncells) {
+    const int4* p4 = reinterpret_cast<const int4*>(cnt + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; ++q) {
+      const int4 v = p4[q];
+      c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT; ++q) c[q] = base + q < ncells ? cnt[base + q] : 0;
+  }
+  uint32_t tsum = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_IPT; ++q) tsum += (uint32_t)c[q];
+  uint32_t x = tsum;  // inclusive scan over the wave's threads
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t wpre = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t v = s_wsum[k];
+    wpre += k < w ? v : 0u;
+    total += v;
+  }
+  if (w == 0) {
+    uint32_t excl = 0;
+    if (b == 0) {
+      if (lane == 0) __hip_atomic_store(status, SCAN_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0) __hip_atomic_store(status + b, SCAN_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // lane l reads predecessor b-1-l-64m; an index below 0 counts as an empty prefix
+      int bb = b - 1;
+      while (true) {
+        const int idx = bb - lane;
+        const unsigned long long sv =
+            idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SCAN_PREFIX;
+        const unsigned long long fl = sv >> 62;
+        const uint64_t pm = __ballot(fl == 2ull), zm = __ballot(fl == 0ull);
+        const int fp = pm ? __ffsll((long long)pm) - 1 : 64;
+        const uint64_t upto = fp >= 63 ? ~0ull : ((2ull << fp) - 1ull);
+        if (zm & upto) continue;  // a predecessor before the nearest prefix has not published yet
+        uint32_t v = lane <= fp ? (uint32_t)(sv & SCAN_VALUE) : 0u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        excl += v;
+        if (fp < 64) break;
+        bb -= 64;
+      }
+      if (lane == 0)
+        __hip_atomic_store(status + b, SCAN_PREFIX | (unsigned long long)(excl + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_excl = excl;
+  }
+  __syncthreads();
+  uint32_t run = s_excl + wpre + (x - tsum);
+  if (base + SCAN_IPT <= ncells) {
+    int4* ps = reinterpret_cast<int4*>(cell_start + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; ++q) {
+      int4 vs;
+      vs.x = (int)run; run += (uint32_t)c[4 * q];
+      vs.y = (int)run; run += (uint32_t)c[4 * q + 1];
+      vs.z = (int)run; run += (uint32_t)c[4 * q + 2];
+      vs.w = (int)run; run += (uint32_t)c[4 * q + 3];
+      ps[q] = vs;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT; ++q) {
+      if (base + q < ncells) {
+        cell_start[base + q] = (int)run;
+        run += (uint32_t)c[q];
+      }
+    }
+  }
+  if (base < ncells && base + SCAN_IPT >= ncells) cell_start[ncells] = (int)run;  // the total
+}
+
+__global__ __launch_bounds__(256) void k_cell_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rank,
+                                                      int n, const int* __restrict__ cell_start, int* __restrict__ tmp,
+                                                      const Clock* clk) {
+  if (clock_skip(clk)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int j = cell_start[keys[i]] + (int)rank[i];
+  if ((unsigned)j < (unsigned)n) tmp[j] = i;  // always true for consistent counts
+}
+
+// Thread per sorted position: the caller id placed there by the scatter moves to its stable rank
+// within the cell (a cell of c particles costs c comparisons per particle). It also does the bin
+// pass's work (k_bounds) for its new position: src = pos of the caller id, and the cell's first
+// particle clears the cell's count and flags the cell's tile.
+__global__ __launch_bounds__(256) void k_cell_rank(SceneTable T, const uint32_t* __restrict__ keys,
+                                                   const int* __restrict__ tmp, int n, const int* __restrict__ cell_start,
+                                                   uint32_t* __restrict__ ks, int* __restrict__ vs, int* __restrict__ cnt,
+                                                   int* __restrict__ tile_slot, const int* __restrict__ pos,
+                                                   int* __restrict__ src, const Clock* clk) {
+  if (clock_skip(clk)) return;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int i = clamp_index(tmp[j], n);
+  const uint32_t k = keys[i];
+  const int s0 = cell_start[k], e0 = min(cell_start[k + 1], n);
+  int r = 0;
+  for (int m = s0; m < e0; ++m) r += tmp[m] < i ? 1 : 0;
+  const int d = s0 + r;
+  if ((unsigned)d >= (unsigned)n) return;  // never, for consistent counts
+  vs[d] = i;
+  ks[d] = k;
+  src[d] = clamp_index(pos[i], n);
+  if (r != 0) return;
+  cnt[k] = 0;
+  const int sc = scene_of(T.part_off, T.nscenes, i);
+  const DevParams& P = T.params[sc];
+  const uint32_t lk = k - (uint32_t)T.node_off[sc];
+  const int r01 = P.res[0] * P.res[1];
+  const int c2 = (int)(lk / (uint32_t)r01), rem = (int)(lk % (uint32_t)r01);
+  const int c1 = rem / P.res[0], c0 = rem % P.res[0];
+  tile_slot[T.tile_off[sc] + ((c2 >> 2) * P.tiles[1] + (c1 >> 2)) * P.tiles[0] + (c0 >> 2)] = -2;
+}
+
 // ================================================================== bins and tiles
-// Thread per sorted position: the first position of each run of equal keys writes the cell's bin
-// start and flags the cell's tile (plain stores of one value), the last position writes the end.
+// Thread per sorted position: the first position of each run of equal keys clears the cell's count
+// for the next substep and flags the cell's tile (plain stores of one value).
 __global__ __launch_bounds__(256) void k_bounds(SceneTable T, const uint32_t* __restrict__ ks, int n,
-                                                int* __restrict__ cell_start, int* __restrict__ cell_end,
-                                                int* __restrict__ tile_slot, const int* __restrict__ perm,
-                                                const int* __restrict__ pos, int* __restrict__ src, const Clock* clk) {
+                                                int* __restrict__ cnt, int* __restrict__ tile_slot,
+                                                const int* __restrict__ perm, const int* __restrict__ pos,
+                                                int* __restrict__ src, const Clock* clk) {
   if (clock_skip(clk)) return;
   const PBlock b = particle_block(T);  // sorted positions are scene-major as well
   const int j = b.j;
@@ -304,9 +479,8 @@ __global__ __launch_bounds__(256) void k_bounds(SceneTable T, const uint32_t* __
   const DevParams& P = T.params[b.s];
   src[j] = clamp_index(pos[clamp_index(perm[j], n)], n);
   const uint32_t k = ks[j];
-  if (j == n - 1 || ks[j + 1] != k) cell_end[k] = j + 1;
   if (j != 0 && ks[j - 1] == k) return;
-  cell_start[k] = j;
+  cnt[k] = 0;
   const uint32_t lk = k - (uint32_t)T.node_off[b.s];
   const int r01 = P.res[0] * P.res[1];
   const int c2 = (int)(lk / (uint32_t)r01), rem = (int)(lk % (uint32_t)r01);
@@ -375,136 +549,231 @@ constexpr int PAY = 48;                 // payload floats per staged particle (1
 template <int RANKS>
 constexpr int pay_cell() { return RANKS * PAY + 4; }
 constexpr int SLAB = FOOT * FOOT * 4;   // nodes of a wave's slab
-// payload layout: weights and derivatives (per cell width) of window nodes c-1 .. c+2 on each axis
-constexpr int PY_WX = 0, PY_DWX = 4, PY_WY = 8, PY_DWY = 12, PY_WZ = 16, PY_DWZ = 20;
-constexpr int PY_M = 24, PY_Q0 = 25, PY_Q = 28, PY_S = 37;  // momentum / stress passes
-constexpr int PY_SF = 24;                                    // force pass: 9 floats of the stress matrix
-constexpr int PY_FV = 40;                                    // force pass: force on a cloth vertex (3)
+// payload layout (floats): the x weights of window nodes c-1 .. c+2, the y and z weights and
+// derivatives (per cell width), the x-column terms g[ox][r] = ox wx[ox] Q[r][0] - dwx[ox] S[r][0]
+// that every lane of the cell adds with weight wy wz, the mass, q0 (a cloth vertex's force in the
+// force pass), and the y and z columns of Q and of the scaled stress matrix S.
+constexpr int PY_WX = 0, PY_WY = 4, PY_DWY = 8, PY_WZ = 12, PY_DWZ = 16, PY_G = 20;
+constexpr int PY_M = 32, PY_Q0 = 33, PY_Q12 = 36, PY_S12 = 42;
 
+// Weights and derivatives (per cell width) of the window nodes c-1 .. c+2 of a particle at fraction
+// f of cell c. Quadratic: the three nodes from base = floor(c + f - 0.5) (c - 1 for f < 0.5, else c),
+// fx = c + f - base in [0.5, 1.5): N = 0.5 (1.5 - fx)^2, 0.75 - (fx - 1)^2, 0.5 (fx - 0.5)^2, in
+// closed form and without branches (HybridSolver.cpp:18-97, interpolation.cpp:9-33).
 template <int ORDER>
-__device__ __forceinline__ void axis_weights(float f, float* w, float* dw) {
+__device__ __forceinline__ void window_weights(float f, float* w, float* dw) {
+  if constexpr (ORDER == 2) {
+    const bool hi = f >= 0.5f;
+    const float fx = hi ? f : f + 1.0f;
+    const float a = 1.5f - fx, b = fx - 1.0f, c = fx - 0.5f;
+    const float w0 = 0.5f * a * a, w1 = 0.75f - b * b, w2 = 0.5f * c * c;
+    const float d0 = -a, d1 = -2.0f * b, d2 = c;
+    w[0] = hi ? 0.0f : w0;
+    w[1] = hi ? w0 : w1;
+    w[2] = hi ? w1 : w2;
+    w[3] = hi ? w2 : 0.0f;
+    dw[0] = hi ? 0.0f : d0;
+    dw[1] = hi ? d0 : d1;
+    dw[2] = hi ? d1 : d2;
+    dw[3] = hi ? d2 : 0.0f;
+  } else {
 #pragma unroll
-  for (int o = 0; o < 4; ++o) spline<ORDER>(f + 1.0f - (float)o, w[o], dw[o]);
+    for (int o = 0; o < 4; ++o) spline<ORDER>(f + 1.0f - (float)o, w[o], dw[o]);
+  }
 }
 
-template <int ORDER, int MODE>
-__device__ __forceinline__ void p2g_stage(const P2GArgs& a, const DevParams& P, int64_t noff, float dt, int idx, int g0,
-                                          int g1, int g2, float4* dst) {
-  float pay[PAY];
+// Scene constants of the tile being transferred, held in registers for the whole tile.
+struct P2GScene {
+  float gmin[3], inv_h[3], h[3], dinv;
+  int res[3];
+  int mesh_v0, mesh_e0, mesh_e1;
+};
+
+// wave-uniform float held in a scalar register (the readfirstlane builtin works on 32-bit integers)
+__device__ __forceinline__ float uniform_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
+__device__ __forceinline__ P2GScene p2g_scene(const DevParams& P) {
+  P2GScene S;
 #pragma unroll
-  for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
+  for (int d = 0; d < 3; ++d) {
+    S.gmin[d] = uniform_f(P.gmin[d]);
+    S.inv_h[d] = uniform_f(P.inv_h[d]);
+    S.h[d] = uniform_f(P.h[d]);
+    S.res[d] = __builtin_amdgcn_readfirstlane(P.res[d]);
+  }
+  S.dinv = uniform_f(P.dinv);
+  S.mesh_v0 = __builtin_amdgcn_readfirstlane(P.mesh_v0);
+  S.mesh_e0 = __builtin_amdgcn_readfirstlane(P.mesh_e0);
+  S.mesh_e1 = __builtin_amdgcn_readfirstlane(P.mesh_e1);
+  return S;
+}
+
+// Particle data a staging lane needs, loaded one chunk ahead of its use (the loads of the next
+// chunk are in flight while the wave accumulates the current one). The force pass of the
+// predicted-stress mode loads inside p2g_stage instead.
+struct StageIn {
+  int pi, kind, mid;
+  float x[3], m, v[3], B[9], K[6];
+};
+
+template <int MODE>
+__device__ __forceinline__ void p2g_load(const P2GArgs& a, const P2GScene& S, int idx, StageIn& in) {
   const int pi = clamp_index(a.src[idx], a.n);
   const float* st = a.state;
   const int64_t cap = a.cap;
-  // particle kind (cloth vertices and elements carry their forces from k_cloth_forces)
-  int kind = KIND_PARTICLE, mid = 0;
-  if (P.mesh_e1 > P.mesh_v0) {
+  in.pi = pi;
+  in.kind = KIND_PARTICLE;
+  in.mid = 0;
+  if (S.mesh_e1 > S.mesh_v0) {
     const int pid = reinterpret_cast<const int*>(st + (size_t)PID * cap)[pi];
-    kind = particle_kind(P, pid);
-    mid = pid - (kind == KIND_VERTEX ? P.mesh_v0 : P.mesh_e0);
+    in.kind = pid < S.mesh_v0 ? KIND_PARTICLE : (pid < S.mesh_e0 ? KIND_VERTEX : KIND_ELEMENT);
+    in.mid = pid - (in.kind == KIND_VERTEX ? S.mesh_v0 : S.mesh_e0);
   }
-  float xp[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) xp[d] = st[(PX + d) * cap + pi];
+  for (int d = 0; d < 3; ++d) in.x[d] = st[(PX + d) * cap + pi];
+  if constexpr (MODE != P2G_FORCE) {
+    in.m = st[PM * cap + pi];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) in.v[d] = st[(PV + d) * cap + pi];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) in.B[i] = st[(PB + i) * cap + pi];
+  }
+  if constexpr (MODE == P2G_MOM_STRESS) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) in.K[i] = st[(PK + i) * cap + pi];
+  }
+}
+
+template <int ORDER, int MODE>
+__device__ __forceinline__ void p2g_stage(const P2GArgs& a, const P2GScene& S, const DevParams& P, int64_t noff,
+                                          float dt, const StageIn& in, int g0, int g1, int g2, float4* dst) {
+  float pay[PAY];
+#pragma unroll
+  for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
+  const int pi = in.pi, kind = in.kind, mid = in.mid;
+  const float* st = a.state;
+  const int64_t cap = a.cap;
   const int cc[3] = {g0, g1, g2};
   float f[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) f[d] = (xp[d] - P.gmin[d]) * P.inv_h[d] - (float)cc[d];
-  axis_weights<ORDER>(f[0], &pay[PY_WX], &pay[PY_DWX]);
-  axis_weights<ORDER>(f[1], &pay[PY_WY], &pay[PY_DWY]);
-  axis_weights<ORDER>(f[2], &pay[PY_WZ], &pay[PY_DWZ]);
+  for (int d = 0; d < 3; ++d) f[d] = (in.x[d] - S.gmin[d]) * S.inv_h[d] - (float)cc[d];
+  float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+  window_weights<ORDER>(f[0], wx, dwx);
+  window_weights<ORDER>(f[1], wy, dwy);
+  window_weights<ORDER>(f[2], wz, dwz);
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    pay[PY_WX + o] = wx[o];
+    pay[PY_WY + o] = wy[o];
+    pay[PY_DWY + o] = dwy[o];
+    pay[PY_WZ + o] = wz[o];
+    pay[PY_DWZ + o] = dwz[o];
+  }
+  float Q0[3] = {0.0f, 0.0f, 0.0f}, S0[3] = {0.0f, 0.0f, 0.0f};  // x columns of Q and of S
   if constexpr (MODE != P2G_FORCE) {
     // momentum w*(q0 + Q o): Q = m D^-1 B diag(h), q0 = m v - Q (1 + f)  (HybridSolver.cpp:134,169-170)
-    const float mp = st[PM * cap + pi];
+    const float mp = in.m;
     float Q[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) Q[3 * r + c] = mp * P.dinv * st[(PB + 3 * r + c) * cap + pi] * P.h[c];
+      for (int c = 0; c < 3; ++c) Q[3 * r + c] = mp * S.dinv * in.B[3 * r + c] * S.h[c];
     pay[PY_M] = mp;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-      pay[PY_Q0 + r] = mp * st[(PV + r) * cap + pi] - (Q[3 * r] * (1.0f + f[0]) + Q[3 * r + 1] * (1.0f + f[1]) + Q[3 * r + 2] * (1.0f + f[2]));
-    if (MODE == P2G_MOM_STRESS && kind == KIND_VERTEX)  // in-plane force impulse (HybridSolver.cpp:378)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) pay[PY_Q0 + r] += dt * a.vert_force[3 * mid + r];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) pay[PY_Q + i] = Q[i];
+    for (int r = 0; r < 3; ++r) {
+      float q0 = mp * in.v[r] - (Q[3 * r] * (1.0f + f[0]) + Q[3 * r + 1] * (1.0f + f[1]) + Q[3 * r + 2] * (1.0f + f[2]));
+      if (MODE == P2G_MOM_STRESS && kind == KIND_VERTEX) q0 += dt * a.vert_force[3 * mid + r];  // in-plane impulse (:378)
+      pay[PY_Q0 + r] = q0;
+      Q0[r] = Q[3 * r];
+      pay[PY_Q12 + 2 * r] = Q[3 * r + 1];
+      pay[PY_Q12 + 2 * r + 1] = Q[3 * r + 2];
+    }
   }
+  float Sm[9];  // stress matrix, columns scaled by 1/h (and by dt when folded into the momentum)
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Sm[i] = 0.0f;
   if constexpr (MODE == P2G_MOM_STRESS) {
-    // stress at F^n: the Kirchhoff stress V P(F) F^T written by the previous G2P (or by
-    // k_stress_init), columns scaled by dt/h so that the impulse folds into the momentum
+    // stress at F^n: the Kirchhoff stress V P(F) F^T written by the previous G2P (or by k_stress_init)
     const int R[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};  // symmetric 3x3 from (xx yy zz xy xz yz)
     if (kind == KIND_PARTICLE) {
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pay[PY_S + 3 * r + c] = dt * st[(PK + R[3 * r + c]) * cap + pi] * P.inv_h[c];
+        for (int c = 0; c < 3; ++c) Sm[3 * r + c] = dt * in.K[R[3 * r + c]] * S.inv_h[c];
     } else if (kind == KIND_ELEMENT) {  // element stress of the normal/shear response (:434-454)
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pay[PY_S + 3 * r + c] = dt * a.elem_stress[9 * mid + 3 * r + c] * P.inv_h[c];
+        for (int c = 0; c < 3; ++c) Sm[3 * r + c] = dt * a.elem_stress[9 * mid + 3 * r + c] * S.inv_h[c];
     }
   }
-  if (MODE == P2G_FORCE && kind != KIND_PARTICLE) {
+  if constexpr (MODE == P2G_FORCE) {
     if (kind == KIND_VERTEX) {
 #pragma unroll
-      for (int r = 0; r < 3; ++r) pay[PY_FV + r] = a.vert_force[3 * mid + r];
-    } else {
+      for (int r = 0; r < 3; ++r) pay[PY_Q0 + r] = a.vert_force[3 * mid + r];
+    } else if (kind == KIND_ELEMENT) {
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pay[PY_SF + 3 * r + c] = a.elem_stress[9 * mid + 3 * r + c] * P.inv_h[c];
-    }
-  } else if constexpr (MODE == P2G_FORCE) {
-    // stress matrix V P(F^) F^n^T at the predicted F^ = F + dt (sum_i v_i grad w_ip^T) F from the
-    // post-P2G grid velocity (HybridSolver.cpp:269-339), columns scaled by 1/h
-    float Fn[9];
+        for (int c = 0; c < 3; ++c) Sm[3 * r + c] = a.elem_stress[9 * mid + 3 * r + c] * S.inv_h[c];
+    } else {
+      // stress matrix V P(F^) F^n^T at the predicted F^ = F + dt (sum_i v_i grad w_ip^T) F from the
+      // post-P2G grid velocity (HybridSolver.cpp:269-339)
+      float Fn[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
-    const float* wx = &pay[PY_WX]; const float* dx = &pay[PY_DWX];
-    const float* wy = &pay[PY_WY]; const float* dy = &pay[PY_DWY];
-    const float* wz = &pay[PY_WZ]; const float* dz = &pay[PY_DWZ];
-    float G[9];
+      for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+      float G[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) G[i] = 0.0f;
-    for (int oz = 0; oz < 4; ++oz) {
-      int n2 = g2 - 1 + oz;
-      if (n2 < 0 || n2 >= P.res[2]) continue;
-      for (int oy = 0; oy < 4; ++oy) {
-        int n1 = g1 - 1 + oy;
-        if (n1 < 0 || n1 >= P.res[1]) continue;
-        for (int ox = 0; ox < 4; ++ox) {
-          int n0 = g0 - 1 + ox;
-          if (n0 < 0 || n0 >= P.res[0]) continue;
-          float gw[3] = {dx[ox] * wy[oy] * wz[oz] * P.inv_h[0], wx[ox] * dy[oy] * wz[oz] * P.inv_h[1],
-                         wx[ox] * wy[oy] * dz[oz] * P.inv_h[2]};
-          int64_t node = noff + ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
-          const float4 gv4 = a.gvel[2 * node];
-          const float vr[3] = {gv4.x, gv4.y, gv4.z};
+      for (int i = 0; i < 9; ++i) G[i] = 0.0f;
+      for (int oz = 0; oz < 4; ++oz) {
+        int n2 = g2 - 1 + oz;
+        if (n2 < 0 || n2 >= S.res[2]) continue;
+        for (int oy = 0; oy < 4; ++oy) {
+          int n1 = g1 - 1 + oy;
+          if (n1 < 0 || n1 >= S.res[1]) continue;
+          for (int ox = 0; ox < 4; ++ox) {
+            int n0 = g0 - 1 + ox;
+            if (n0 < 0 || n0 >= S.res[0]) continue;
+            float gw[3] = {dwx[ox] * wy[oy] * wz[oz] * S.inv_h[0], wx[ox] * dwy[oy] * wz[oz] * S.inv_h[1],
+                           wx[ox] * wy[oy] * dwz[oz] * S.inv_h[2]};
+            int64_t node = noff + ((int64_t)n2 * S.res[1] + n1) * S.res[0] + n0;
+            const float4 gv4 = a.gvel[2 * node];
+            const float vr[3] = {gv4.x, gv4.y, gv4.z};
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
+            for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) G[3 * r + c] += vr[r] * gw[c];
+              for (int c = 0; c < 3; ++c) G[3 * r + c] += vr[r] * gw[c];
+          }
         }
       }
+      float GF[9], Fh[9];
+      mat_mul(G, Fn, GF);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + dt * GF[i];
+      float Jp = st[PPL * cap + pi];
+      float vol = st[PVOL * cap + pi];
+      float Sp[9];
+      int fb = 0;
+      stress_matrix(P, Fh, Fn, Jp, vol, Sp, fb);
+      if (fb) atomicAdd(a.fallback_svd, (unsigned long long)fb);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Sm[3 * r + c] = Sp[3 * r + c] * S.inv_h[c];
     }
-    float GF[9], Fh[9];
-    mat_mul(G, Fn, GF);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + dt * GF[i];
-    float Jp = st[PPL * cap + pi];
-    float vol = st[PVOL * cap + pi];
-    float S[9];
-    int fb = 0;
-    stress_matrix(P, Fh, Fn, Jp, vol, S, fb);
-    if (fb) atomicAdd(a.fallback_svd, (unsigned long long)fb);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) pay[PY_SF + 3 * r + c] = S[3 * r + c] * P.inv_h[c];
   }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    S0[r] = Sm[3 * r];
+    pay[PY_S12 + 2 * r] = Sm[3 * r + 1];
+    pay[PY_S12 + 2 * r + 1] = Sm[3 * r + 2];
+  }
+#pragma unroll
+  for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) pay[PY_G + 3 * ox + r] = (float)ox * wx[ox] * Q0[r] - dwx[ox] * S0[r];
 #pragma unroll
   for (int i = 0; i < PAY / 4; ++i) dst[i] = make_float4(pay[4 * i], pay[4 * i + 1], pay[4 * i + 2], pay[4 * i + 3]);
 }
@@ -520,113 +789,135 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int ntile = a.counters[CNT_TILES];
   constexpr int NQ = (MODE == P2G_FORCE) ? 3 : 4;
-  // staging role: lane (cx, sr), sr < 8; accumulation role: lane (cx, zo, oy)
+  // staging role: lane (cx, sr), sr < RANKS; accumulation role: lane (cx, zo, oy)
   const int cx = lane >> 4, sr = lane & 15;
   const int zo = (lane >> 2) & 3, oy = lane & 3;
+  const float foy = (float)oy, fzo = (float)zo;
   float* slab = &s_slab[w][0][0];
   for (int i = lane; i < 4 * SLAB; i += 64) slab[i] = 0.0f;
   for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
     const int gtile = a.active_tiles[slot];
     const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
     const DevParams& P = a.T.params[sc];
+    const P2GScene S = p2g_scene(P);
     const int64_t noff = a.T.node_off[sc];
     const int tile = gtile - a.T.tile_off[sc];
-    const int t0 = tile % P.tiles[0], t1 = (tile / P.tiles[0]) % P.tiles[1], t2 = tile / (P.tiles[0] * P.tiles[1]);
+    const int nt0 = __builtin_amdgcn_readfirstlane(P.tiles[0]), nt1 = __builtin_amdgcn_readfirstlane(P.tiles[1]);
+    const int t0 = tile % nt0, t1 = (tile / nt0) % nt1, t2 = tile / (nt0 * nt1);
     const int g2 = 4 * t2 + w;
+    const int g0 = 4 * t0 + cx;  // this lane's cell of each row (both roles)
+    // the bins of this lane's cell in the wave's four rows, and each row's largest count
+    int rstart[4], rcnt[4], rmax[4];
+#pragma unroll
     for (int cy = 0; cy < 4; ++cy) {
       const int g1 = 4 * t1 + cy;
-      const int g0 = 4 * t0 + cx;  // this lane's cell of the row (both roles)
-      int start = 0, cnt = 0;
-      if (g0 < P.res[0] && g1 < P.res[1] && g2 < P.res[2]) {
-        const int64_t key = noff + (g2 * P.res[1] + g1) * P.res[0] + g0;
-        start = a.cell_start[key];
-        cnt = a.cell_end[key] - start;
+      rstart[cy] = 0;
+      rcnt[cy] = 0;
+      if (g0 < S.res[0] && g1 < S.res[1] && g2 < S.res[2]) {
+        const int64_t key = noff + (g2 * S.res[1] + g1) * S.res[0] + g0;
+        rstart[cy] = a.cell_start[key];
+        rcnt[cy] = a.cell_end[key] - rstart[cy];
       }
-      int cmax = max(max(__builtin_amdgcn_readlane(cnt, 0), __builtin_amdgcn_readlane(cnt, 16)),
-                     max(__builtin_amdgcn_readlane(cnt, 32), __builtin_amdgcn_readlane(cnt, 48)));
-      if (cmax == 0) continue;
-      float acc[4][NQ];
+      rmax[cy] = max(max(__builtin_amdgcn_readlane(rcnt[cy], 0), __builtin_amdgcn_readlane(rcnt[cy], 16)),
+                     max(__builtin_amdgcn_readlane(rcnt[cy], 32), __builtin_amdgcn_readlane(rcnt[cy], 48)));
+    }
+    // walk the (row, chunk) steps in order, loading the particles of the next step ahead
+    auto next_row = [&](int cy) {
+      int r = cy;
+      while (r < 4 && rmax[r] == 0) ++r;
+      return r;
+    };
+    int cy = next_row(0), k = 0;
+    StageIn in;
+    if (MODE != P2G_FORCE && cy < 4 && sr < P2G_RANKS && sr < rcnt[cy]) p2g_load<MODE>(a, S, rstart[cy] + sr, in);
+    float acc[4][NQ];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0f;
-      for (int k = 0; k * P2G_RANKS < cmax; ++k) {
-        // ---- stage ranks 8k .. 8k+7 of each cell of the row
-        if (sr < P2G_RANKS) {
-          const int rank = P2G_RANKS * k + sr;
-          if (rank < cnt)
-            p2g_stage<ORDER, MODE>(a, P, noff, dt, start + rank, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
+      for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0f;
+    while (cy < 4) {
+      const int g1 = 4 * t1 + cy;
+      const int cnt = rcnt[cy];
+      // ---- stage ranks RANKS k .. RANKS k + RANKS - 1 of each cell of the row
+      if (sr < P2G_RANKS) {
+        const int rank = P2G_RANKS * k + sr;
+        if (rank < cnt) {
+          if constexpr (MODE == P2G_FORCE) p2g_load<MODE>(a, S, rstart[cy] + rank, in);
+          p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // ---- accumulate this lane's cell, ascending rank
-        const int nr = min(P2G_RANKS, cnt - P2G_RANKS * k);
-        for (int r = 0; r < nr; ++r) {
-          const float4* src = &s_pay[w][(cx * PAY_CELL + r * PAY) / 4];
-          float pay[PAY];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      // ---- the next step, and its particles' loads
+      int ncy = cy, nk = k + 1;
+      if (P2G_RANKS * nk >= rmax[cy]) {
+        ncy = next_row(cy + 1);
+        nk = 0;
+      }
+      if (MODE != P2G_FORCE && ncy < 4 && sr < P2G_RANKS) {
+        const int nrank = P2G_RANKS * nk + sr;
+        if (nrank < rcnt[ncy]) p2g_load<MODE>(a, S, rstart[ncy] + nrank, in);
+      }
+      // ---- accumulate this lane's cell, ascending rank
+      const int nr = min(P2G_RANKS, cnt - P2G_RANKS * k);
+      for (int r = 0; r < nr; ++r) {
+        const float4* src = &s_pay[w][(cx * PAY_CELL + r * PAY) / 4];
+        const float* pf = reinterpret_cast<const float*>(src);
+        const float4 WX = src[0];
+        const float wxs[4] = {WX.x, WX.y, WX.z, WX.w};
+        const float wy = pf[PY_WY + oy], dwy = pf[PY_DWY + oy], wz = pf[PY_WZ + zo], dwz = pf[PY_DWZ + zo];
+        float g[12];
 #pragma unroll
-          for (int i = 0; i < 2; ++i) {  // x weights and derivatives
-            const float4 t = src[i];
-            pay[4 * i] = t.x; pay[4 * i + 1] = t.y; pay[4 * i + 2] = t.z; pay[4 * i + 3] = t.w;
-          }
-#pragma unroll
-          for (int i = 6; i < PAY / 4; ++i) {  // mass, momentum and stress terms
-            const float4 t = src[i];
-            pay[4 * i] = t.x; pay[4 * i + 1] = t.y; pay[4 * i + 2] = t.z; pay[4 * i + 3] = t.w;
-          }
-          // this lane's y and z node weights
-          const float* pf = reinterpret_cast<const float*>(src);
-          const float wy = pf[PY_WY + oy], dwy = pf[PY_DWY + oy], wz = pf[PY_WZ + zo], dwz = pf[PY_DWZ + zo];
-          const float wyz = wy * wz, dyz = dwy * wz, ydz = wy * dwz;
-          const float* S = (MODE == P2G_FORCE) ? &pay[PY_SF] : &pay[PY_S];
-          float e[3], dv[3], av[3], mw = 0.0f;
-          if constexpr (MODE != P2G_FORCE) {
-            mw = pay[PY_M] * wyz;
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-              const float qrow = pay[PY_Q0 + rr] + pay[PY_Q + 3 * rr + 1] * (float)oy + pay[PY_Q + 3 * rr + 2] * (float)zo;
-              e[rr] = wyz * qrow;
-              dv[rr] = wyz * pay[PY_Q + 3 * rr];
-            }
-          } else {  // force pass: a cloth vertex's force goes with its weights
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) { e[rr] = wyz * pay[PY_FV + rr]; dv[rr] = 0.0f; }
-          }
-          if constexpr (MODE != P2G_MOM) {
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-              e[rr] -= S[3 * rr + 1] * dyz + S[3 * rr + 2] * ydz;
-              av[rr] = S[3 * rr] * wyz;
-            }
-          } else {
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) av[rr] = 0.0f;
-          }
-#pragma unroll
-          for (int ox = 0; ox < 4; ++ox) {
-            const float wx = pay[PY_WX + ox], dwx = pay[PY_DWX + ox], wxo = wx * (float)ox;
-            if constexpr (MODE != P2G_FORCE) {
-              acc[ox][0] += wx * mw;
-#pragma unroll
-              for (int rr = 0; rr < 3; ++rr) acc[ox][1 + rr] += wx * e[rr] + wxo * dv[rr] - dwx * av[rr];
-            } else {
-#pragma unroll
-              for (int rr = 0; rr < 3; ++rr) acc[ox][rr] += wx * e[rr] - dwx * av[rr];
-            }
-          }
+        for (int i = 0; i < 3; ++i) {
+          const float4 t = src[PY_G / 4 + i];
+          g[4 * i] = t.x; g[4 * i + 1] = t.y; g[4 * i + 2] = t.z; g[4 * i + 3] = t.w;
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      }
-      // ---- row sums into the wave's slab: phase ox, node (cx + ox, cy + oy, zo)
+        float qs[16];  // m, q0(3), Q12(6), S12(6)
 #pragma unroll
-      for (int ox = 0; ox < 4; ++ox) {
-        const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * zo;
+        for (int i = 0; i < 4; ++i) {
+          const float4 t = src[PY_M / 4 + i];
+          qs[4 * i] = t.x; qs[4 * i + 1] = t.y; qs[4 * i + 2] = t.z; qs[4 * i + 3] = t.w;
+        }
+        const float wyz = wy * wz, dyz = dwy * wz, ydz = wy * dwz;
+        float e[3];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) atomicAdd(&slab[q * SLAB + ln], acc[ox][q]);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int rr = 0; rr < 3; ++rr) {
+          float qrow = qs[1 + rr];
+          if constexpr (MODE != P2G_FORCE)
+            qrow = __builtin_fmaf(qs[4 + 2 * rr + 1], fzo, __builtin_fmaf(qs[4 + 2 * rr], foy, qrow));
+          e[rr] = wyz * qrow;
+          if constexpr (MODE != P2G_MOM)
+            e[rr] = __builtin_fmaf(-qs[10 + 2 * rr + 1], ydz, __builtin_fmaf(-qs[10 + 2 * rr], dyz, e[rr]));
+        }
+        const float mw = qs[0] * wyz;
+#pragma unroll
+        for (int ox = 0; ox < 4; ++ox) {
+          const float wx = wxs[ox];
+          constexpr int q1 = (MODE != P2G_FORCE) ? 1 : 0;
+          if constexpr (MODE != P2G_FORCE) acc[ox][0] = __builtin_fmaf(wx, mw, acc[ox][0]);
+#pragma unroll
+          for (int rr = 0; rr < 3; ++rr)
+            acc[ox][q1 + rr] = __builtin_fmaf(wyz, g[3 * ox + rr], __builtin_fmaf(wx, e[rr], acc[ox][q1 + rr]));
+        }
       }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (ncy != cy) {
+        // ---- row sums into the wave's slab: phase ox, node (cx + ox, cy + oy, zo)
+#pragma unroll
+        for (int ox = 0; ox < 4; ++ox) {
+          const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * zo;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            atomicAdd(&slab[q * SLAB + ln], acc[ox][q]);
+            acc[ox][q] = 0.0f;
+          }
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+      }
+      cy = ncy;
+      k = nk;
     }
     __syncthreads();
     // ---- tile partial: node (X, Y, Z) = sum over waves w, 0 <= Z - w <= 3, ascending; slabs cleared
@@ -1313,10 +1604,29 @@ void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* 
   if (n > 0) hipLaunchKernelGGL(k_stress_init, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, fallback);
 }
 
-void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n, int passes) {
+void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int n, int passes, int mode) {
+  if (n <= 0) return;
+  {
+    // bins: counts (the rank is used by the counting sort) and their scan, in both modes
+    uint32_t* rank = (passes & 1) ? b.k2 : b.k1;
+    hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, n, b.cell_count, rank,
+                       b.counters, b.clk);
+    hipLaunchKernelGGL(k_cell_scan, dim3(blocks_for(b.ncells, SCAN_TILE)), dim3(256), 0, s, b.cell_count, b.ncells,
+                       b.cell_start, b.tickets + 4, b.scan_status, b.clk);
+  }
+  if (mode == SORT_COUNTING) {
+    uint32_t* kf = (passes & 1) ? b.k1 : b.k2;  // the result goes where the digit passes put it
+    int* vf = (passes & 1) ? b.v1 : b.v2;
+    const uint32_t* rank = (passes & 1) ? b.k2 : b.k1;
+    int* tmp = (passes & 1) ? b.v2 : b.v1;
+    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, rank, n, b.cell_start, tmp,
+                       b.clk);
+    hipLaunchKernelGGL(k_cell_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.T, keys_in, tmp, n, b.cell_start, kf,
+                       vf, b.cell_count, b.tile_slot, b.pos, b.src, b.clk);
+    return;
+  }
   const int nb = blocks_for(n, SORT_TILE);
-  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, passes, b.hist,
-                     b.counters, b.clk);
+  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, passes, b.hist, b.clk);
   const uint32_t* kin = keys_in;
   const int* vin = nullptr;
   for (int p = 0; p < passes; ++p) {
@@ -1330,21 +1640,21 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
 }
 
 void launch_reset(hipStream_t s, const ResetArgs& r) {
-  int64_t work = r.n_zero > r.n ? r.n_zero : r.n;
+  int64_t work = r.n_zero;
   if (work < r.max_tiles) work = r.max_tiles;
   int grid = blocks_for(work, 256);
   if (grid > 4096) grid = 4096;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, r.zero_words, r.n_zero, r.prev_keys, r.n, r.cell_start,
-                     r.cell_end, r.counters, r.active_tiles, r.tile_slot, r.active_blocks, r.block_slot, r.clk);
+  hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, s, r.zero_words, r.n_zero, r.counters, r.active_tiles,
+                     r.tile_slot, r.active_blocks, r.block_slot, r.clk);
 }
 
-void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles, int* cell_start,
-                   int* cell_end, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks, int* counters,
-                   const int* perm, const int* pos, int* src, const Clock* clk) {
+void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32_t* ks, int n, int64_t ntiles,
+                   int* cell_count, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks,
+                   int* counters, const int* perm, const int* pos, int* src, const Clock* clk, int mode) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_bounds, dim3(nblocks), dim3(256), 0, s, T, ks, n, cell_start, cell_end, tile_slot, perm, pos,
-                     src, clk);
+  if (mode != SORT_COUNTING)  // the counting sort's rank kernel has done this pass
+    hipLaunchKernelGGL(k_bounds, dim3(nblocks), dim3(256), 0, s, T, ks, n, cell_count, tile_slot, perm, pos, src, clk);
   hipLaunchKernelGGL(k_claim<0>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, tile_slot, active_tiles,
                      counters + CNT_TILES, block_slot, clk);
   hipLaunchKernelGGL(k_claim<1>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, block_slot, active_blocks,

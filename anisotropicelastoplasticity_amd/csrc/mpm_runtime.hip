@@ -74,13 +74,15 @@ struct mpm_ctx {
   SceneTable T{};
   int nblocks_p = 0;    // 256-particle blocks over all scenes
   int sort_passes = 3;  // 8-bit digit passes covering the largest global cell key
+  int sort_mode = SORT_COUNTING;  // SORT_COUNTING or SORT_RADIX (mpm_autotune / mpm_set_tuning)
   uint32_t* ks_sorted = nullptr;  // sorted keys / ids of the last sort (k1/v1 or k2/v2)
   int* vs_sorted = nullptr;
 
   float* thr = nullptr;      // per-axis cell thresholds
   float* grid = nullptr;     // m, mom(3), v(3), vbf(3)
   int* cell_start = nullptr;
-  int* cell_end = nullptr;
+  int* cell_count = nullptr;  // per-cell particle counts of the counting sort (zero between substeps)
+  int* cell_end = nullptr;  // cell_start + 1 (not a separate allocation)
   int* tile_slot = nullptr;
   int* active_tiles = nullptr;
   int* block_slot = nullptr;
@@ -315,8 +317,12 @@ static float threshold(double gmin, double h, int c) {
   return f;
 }
 
-// Radix-sort workspace words: 4 x 256 histogram words, 4 pass tickets, then the look-back words.
-static constexpr int64_t SORTWS_TICKETS = 1024, SORTWS_STATUS = 1028;
+// Sort workspace words: 4 x 256 digit histograms, 4 digit-pass tickets and 4 cell-scan tickets, the
+// cell-scan look-back words (64-bit, one per scan block), then the digit passes' look-back words.
+static constexpr int64_t SORTWS_TICKETS = 1024, SORTWS_SCAN = 1032;
+static int64_t sortws_radix_status(const mpm_ctx* ctx) {
+  return SORTWS_SCAN + 2 * ((ctx->ng + SCAN_TILE - 1) / SCAN_TILE);
+}
 
 // ------------------------------------------------------------------ substep stages
 static int stage_sort(mpm_ctx* ctx) {
@@ -325,12 +331,13 @@ static int stage_sort(mpm_ctx* ctx) {
   if (!ctx->keys_valid)
     launch_keys(s, ctx->T, ctx->nblocks_p, st, ctx->cap, ctx->n, ctx->pos[ctx->cur], ctx->thr, ctx->keys, ctx->status);
   ResetArgs r;
-  r.zero_words = ctx->sortws;
-  r.n_zero = ctx->sortws_words;
-  r.prev_keys = ctx->ks_sorted;
-  r.n = ctx->n;
-  r.cell_start = ctx->cell_start;
-  r.cell_end = ctx->cell_end;
+  if (ctx->sort_mode == SORT_COUNTING) {  // only the tickets and the scan's look-back words
+    r.zero_words = ctx->sortws + SORTWS_TICKETS;
+    r.n_zero = sortws_radix_status(ctx) - SORTWS_TICKETS;
+  } else {
+    r.zero_words = ctx->sortws;
+    r.n_zero = ctx->sortws_words;
+  }
   r.counters = ctx->counters;
   r.active_tiles = ctx->active_tiles;
   r.tile_slot = ctx->tile_slot;
@@ -346,13 +353,21 @@ static int stage_sort(mpm_ctx* ctx) {
   b.v2 = ctx->v2;
   b.hist = ctx->sortws;
   b.tickets = ctx->sortws + SORTWS_TICKETS;
-  b.status = ctx->sortws + SORTWS_STATUS;
+  b.scan_status = reinterpret_cast<unsigned long long*>(ctx->sortws + SORTWS_SCAN);
+  b.status = ctx->sortws + sortws_radix_status(ctx);
+  b.cell_count = ctx->cell_count;
+  b.cell_start = ctx->cell_start;
+  b.ncells = ctx->ng;
+  b.T = ctx->T;
+  b.tile_slot = ctx->tile_slot;
+  b.pos = ctx->pos[ctx->cur];
+  b.src = ctx->src;
   b.counters = ctx->counters;
   b.clk = ctx->clk_active;
-  launch_sort(s, b, ctx->keys, ctx->n, ctx->sort_passes);
-  launch_bounds(s, ctx->T, ctx->nblocks_p, ctx->ks_sorted, ctx->n, ctx->ntiles, ctx->cell_start, ctx->cell_end,
+  launch_sort(s, b, ctx->keys, ctx->n, ctx->sort_passes, ctx->sort_mode);
+  launch_bounds(s, ctx->T, ctx->nblocks_p, ctx->ks_sorted, ctx->n, ctx->ntiles, ctx->cell_count,
                 ctx->tile_slot, ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters, ctx->vs_sorted,
-                ctx->pos[ctx->cur], ctx->src, ctx->clk_active);
+                ctx->pos[ctx->cur], ctx->src, ctx->clk_active, ctx->sort_mode);
   ctx->sort_buf = ctx->cur;
   return MPM_OK;
 }
@@ -620,7 +635,8 @@ int mpm_destroy(mpm_ctx* ctx) {
   drop_graphs(ctx);
   dfree(ctx->table_buf);
   drop_mesh(ctx);
-  dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_end);
+  dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_count);
+  ctx->cell_end = nullptr;
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
   dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
@@ -650,7 +666,7 @@ static int alloc_partials(mpm_ctx* ctx) {
 static int alloc_sortws(mpm_ctx* ctx) {
   if (ctx->n <= 0) return MPM_OK;
   ctx->nb_sort = (ctx->n + SORT_TILE - 1) / SORT_TILE;
-  const int64_t words = SORTWS_STATUS + (int64_t)ctx->sort_passes * ctx->nb_sort * 256;
+  const int64_t words = sortws_radix_status(ctx) + (int64_t)ctx->sort_passes * ctx->nb_sort * 256;
   if (words != ctx->sortws_words || !ctx->sortws) {
     int rc = dalloc(ctx, ctx->sortws, words);
     if (rc) return rc;
@@ -718,8 +734,8 @@ static int layout(mpm_ctx* ctx) {
   while ((int64_t(1) << bits) < ng) ++bits;  // keys lie in [0, ng)
   ctx->sort_passes = std::max(1, (bits + 7) / 8);
   int rc = dalloc(ctx, ctx->grid, (size_t)GRID_FLOATS * ng);
-  if (!rc) rc = dalloc(ctx, ctx->cell_start, ng);
-  if (!rc) rc = dalloc(ctx, ctx->cell_end, ng);
+  if (!rc) rc = dalloc(ctx, ctx->cell_start, ng + 1);
+  if (!rc) rc = dalloc(ctx, ctx->cell_count, ng);
   if (!rc) rc = dalloc(ctx, ctx->tile_slot, nt);
   if (!rc) rc = dalloc(ctx, ctx->active_tiles, nt);
   if (!rc) rc = dalloc(ctx, ctx->block_slot, nt);
@@ -727,8 +743,9 @@ static int layout(mpm_ctx* ctx) {
   if (!rc) rc = dalloc(ctx, ctx->thr, (size_t)nthr);
   if (rc) return rc;
   CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * GRID_FLOATS * ng, ctx->stream));
-  CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * ng, ctx->stream));
-  CK(hipMemsetAsync(ctx->cell_end, 0, sizeof(int) * ng, ctx->stream));
+  ctx->cell_end = ctx->cell_start + 1;  // cell c holds sorted positions [cell_start[c], cell_start[c + 1])
+  CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * (ng + 1), ctx->stream));
+  CK(hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * ng, ctx->stream));
   launch_fill_int(ctx->stream, ctx->tile_slot, nt, -1);
   launch_fill_int(ctx->stream, ctx->block_slot, nt, -1);
   CK(hipMemsetAsync(ctx->counters, 0, CNT_NUM * sizeof(int), ctx->stream));
@@ -1354,8 +1371,22 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t reps, mpm_tuning* chosen) {
   }
   const int p2g_mode = ctx->predicted ? P2G_MOM : P2G_MOM_STRESS;
   const int grid_mode = ctx->predicted ? GRID_NORMALIZE : GRID_FUSED;
-  const P2GArgs a = p2g_args(ctx, dt);
   mpm_tuning best{};
+  best.sort_ms = 1e30f;
+  const int keep_mode = ctx->sort_mode;
+  for (int m : {SORT_COUNTING, SORT_RADIX}) {
+    ctx->sort_mode = m;
+    const float ms = time_launches(ctx, reps, [&] { (void)stage_sort(ctx); });
+    CK(hipGetLastError());
+    if (ms >= 0.0f && ms < best.sort_ms) {
+      best.sort_ms = ms;
+      best.sort_mode = m;
+    }
+  }
+  ctx->sort_mode = best.sort_ms < 1e30f ? best.sort_mode : keep_mode;
+  rc = stage_sort(ctx);  // the bins of the chosen mode (identical for both) for the P2G timings
+  if (rc) return rc;
+  const P2GArgs a = p2g_args(ctx, dt);
   best.p2g_ms = 1e30f;
   const int ranks[2] = {4, 8};
   const int mults[5] = {2, 3, 4, 5, 6};
@@ -1399,6 +1430,7 @@ int mpm_get_tuning(mpm_ctx* ctx, mpm_tuning* t) {
   t->p2g_ranks = ctx->p2g_ranks;
   t->p2g_blocks = ctx->p2g_grid;
   t->grid_blocks = ctx->grid_grid;
+  t->sort_mode = ctx->sort_mode;
   return MPM_OK;
 }
 
@@ -1406,11 +1438,14 @@ int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* t) {
   if (!ctx || !t) return MPM_ERR_INVALID;
   if ((t->p2g_ranks != 4 && t->p2g_ranks != 8) || t->p2g_blocks < 1 || t->grid_blocks < 1)
     return fail(ctx, MPM_ERR_INVALID, "p2g_ranks must be 4 or 8 and the block counts positive");
+  if (t->sort_mode != MPM_SORT_COUNTING && t->sort_mode != MPM_SORT_RADIX)
+    return fail(ctx, MPM_ERR_INVALID, "sort_mode must be MPM_SORT_COUNTING or MPM_SORT_RADIX");
   CK(hipSetDevice(ctx->device));
   CK(hipStreamSynchronize(ctx->stream));
   ctx->p2g_ranks = t->p2g_ranks;
   ctx->p2g_grid = t->p2g_blocks;
   ctx->grid_grid = t->grid_blocks;
+  ctx->sort_mode = t->sort_mode;
   drop_graphs(ctx);
   return MPM_OK;
 }
@@ -1437,14 +1472,7 @@ int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins) {
   CK(hipStreamSynchronize(ctx->stream));
   if (keys) CK(hipMemcpy(keys, ctx->ks_sorted, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
   if (ids) CK(hipMemcpy(ids, ctx->vs_sorted, sizeof(int) * n, hipMemcpyDeviceToHost));
-  if (bins) {
-    const int64_t ng = ctx->ng;
-    std::vector<int> cs(ng), ce(ng);
-    CK(hipMemcpy(cs.data(), ctx->cell_start, sizeof(int) * ng, hipMemcpyDeviceToHost));
-    CK(hipMemcpy(ce.data(), ctx->cell_end, sizeof(int) * ng, hipMemcpyDeviceToHost));
-    bins[ng] = n;
-    for (int64_t c = ng - 1; c >= 0; --c) bins[c] = (ce[c] > cs[c]) ? cs[c] : bins[c + 1];
-  }
+  if (bins) CK(hipMemcpy(bins, ctx->cell_start, sizeof(int) * (ctx->ng + 1), hipMemcpyDeviceToHost));
   return MPM_OK;
 }
 

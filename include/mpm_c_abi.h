@@ -253,12 +253,16 @@ typedef void (*mpm_frame_callback)(mpm_ctx* ctx, int32_t frame, double t, void* 
 int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void* user);
 /* Launch shape of the P2G and grid kernels. None of the shapes changes an order of summation, so
  * results are bitwise identical for every valid shape. */
+#define MPM_SORT_COUNTING 0 /* one-digit counting sort over the cells, stable rank inside each cell */
+#define MPM_SORT_RADIX 1    /* LSD radix sort, 8-bit digits */
 typedef struct mpm_tuning {
   int32_t p2g_ranks;   /* particles per cell staged per P2G chunk (the bin tile): 4 or 8 */
   int32_t p2g_blocks;  /* P2G workgroups, persistent over the active 4x4x4-cell tiles */
   int32_t grid_blocks; /* grid-update workgroups, persistent over the active node blocks */
   float p2g_ms;        /* times of the chosen shapes measured by mpm_autotune (0 otherwise) */
   float grid_ms;
+  int32_t sort_mode;   /* MPM_SORT_*: both give the same order, bit for bit */
+  float sort_ms;
 } mpm_tuning;
 /* Autotuner (no counterpart in the reference): time every candidate shape `reps` times on the
  * current particle state (sort, P2G and grid update are idempotent, the state does not advance),

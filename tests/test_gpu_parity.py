@@ -36,13 +36,23 @@ def user_keys(x, sc):
 
 
 # ------------------------------------------------------------------ sort
+def set_sort_mode(s, mode):
+    t = s.tuning()
+    s.set_tuning(t["p2g_ranks"], t["p2g_blocks"], t["grid_blocks"], mode)
+
+
+SORT_MODES = [("counting", _lib.SORT_COUNTING), ("radix", _lib.SORT_RADIX)]
+
+
+@pytest.mark.parametrize("mode_name,mode", SORT_MODES)
 @pytest.mark.parametrize("which", ["config1", "config2"])
-def test_sort_bit_exact(which):
+def test_sort_bit_exact(which, mode_name, mode):
     sc = scenes.config1_snow_block() if which == "config1" else scenes.config2_sand_column()
-    say(f"[sort] {which} n={sc.n} res={sc.res}")
+    say(f"[sort] {which} {mode_name} n={sc.n} res={sc.res}")
     ncell = int(np.prod(sc.res))
     with Solver() as s:
         sc.load_into(s)
+        set_sort_mode(s, mode)
         for rnd in range(3):
             s.stage(_lib.STAGE_SORT, sc.dt)
             keys, ids, bins = s.get_sorted()
@@ -57,18 +67,20 @@ def test_sort_bit_exact(which):
             s.run(sc.dt, 7)
 
 
-def test_sort_random_order_and_ties():
+@pytest.mark.parametrize("mode_name,mode", SORT_MODES)
+def test_sort_random_order_and_ties(mode_name, mode):
     # caller order is random and many particles share a cell: ties must follow caller order
     rng = np.random.default_rng(9)
     n = 70000
     sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB, res=(32, 32, 32), cells=(8, 8, 8), ppa=2)
     x = rng.uniform(0.3, 0.32, size=(3, n)).astype(np.float32)  # ~1 cell: long runs of equal keys
     x[:, ::3] = rng.uniform(0.1, 0.9, size=(3, x[:, ::3].shape[1])).astype(np.float32)
-    say(f"[sort ties] n={n}")
+    say(f"[sort ties] {mode_name} n={n}")
     with Solver() as s:
         s.set_grid(sc.gmin, sc.gmax, sc.res)
         s.set_params(sc.params)
         s.set_particles(x, np.full(n, 1e-6, np.float32), volume=np.full(n, 1e-9, np.float32))
+        set_sort_mode(s, mode)
         s.stage(_lib.STAGE_SORT, 1e-5)
         keys, ids, bins = s.get_sorted()
     k_user = user_keys(x, sc)
