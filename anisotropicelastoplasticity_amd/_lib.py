@@ -61,12 +61,13 @@ class SceneDesc(ctypes.Structure):
 
 
 SORT_COUNTING, SORT_RADIX = 0, 1
+P2G_ROWS, P2G_LAYERS = 0, 1
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("p2g_ranks", ctypes.c_int32), ("p2g_blocks", ctypes.c_int32), ("grid_blocks", ctypes.c_int32),
                 ("p2g_ms", ctypes.c_float), ("grid_ms", ctypes.c_float), ("sort_mode", ctypes.c_int32),
-                ("sort_ms", ctypes.c_float)]
+                ("sort_ms", ctypes.c_float), ("p2g_pass", ctypes.c_int32)]
 
 
 class MeshParams(ctypes.Structure):

@@ -204,8 +204,8 @@ class Solver:
         self._check(self.lib.mpm_get_tuning(self.h, ctypes.byref(t)), "mpm_get_tuning")
         return {name: getattr(t, name) for name, _ in Tuning._fields_}
 
-    def set_tuning(self, p2g_ranks, p2g_blocks, grid_blocks, sort_mode=_lib.SORT_COUNTING):
-        t = Tuning(int(p2g_ranks), int(p2g_blocks), int(grid_blocks), 0.0, 0.0, int(sort_mode), 0.0)
+    def set_tuning(self, p2g_ranks, p2g_blocks, grid_blocks, sort_mode=_lib.SORT_COUNTING, p2g_pass=_lib.P2G_ROWS):
+        t = Tuning(int(p2g_ranks), int(p2g_blocks), int(grid_blocks), 0.0, 0.0, int(sort_mode), 0.0, int(p2g_pass))
         self._check(self.lib.mpm_set_tuning(self.h, ctypes.byref(t)), "mpm_set_tuning")
 
     def clock(self):

@@ -233,7 +233,10 @@ void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32
                    int* cell_count, int* tile_slot, int* active_tiles, int* block_slot, int* active_blocks,
                    int* counters, const int* perm, const int* pos, int* src, const Clock* clk, int mode);
 // order: B-spline order of every scene (2 or 3); ranks: particles per cell staged per chunk (4 or 8)
-void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks);
+// pass P2G_ROWS (8-wave workgroups, ranks 4 or 8) or P2G_LAYERS (4-wave workgroups, ranks 2 or 4):
+// two schedules of the same sums in the same order, so every choice gives the same bits.
+enum P2GPass : int { P2G_ROWS = 0, P2G_LAYERS = 1 };
+void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks, int pass);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
 void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks);
 void launch_volumes(hipStream_t s, const SceneTable& T, int order, int nblocks, float* st, int64_t cap, int n,

@@ -525,23 +525,26 @@ __global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int*
 }
 
 // ================================================================== P2G
-// One workgroup (4 waves) per active 4x4x4-cell tile; wave w owns the cell layer z = w and walks
-// its four cell rows (y) one after another, with no workgroup barrier until the tile is done.
+// One workgroup (8 waves) per active 4x4x4-cell tile; wave w owns the cell layer z = w & 3 and walks
+// rows (y) 2 (w >> 2) and 2 (w >> 2) + 1 one after another, with no workgroup barrier until the
+// tile is done. (Two row halves per layer halve a tile's latency, which evens out the last round
+// when the active tiles do not divide evenly over the resident workgroups.)
 //
 // Row pass. The four cells of a row (x = 0..3) are contiguous in the sorted order. In chunk k,
-// lane (cx, r), r < 8, stages the particle of rank 8k + r of cell cx: it loads the particle once
-// and writes a 48-float payload (B-spline weights and derivatives of the window's 4 nodes per axis,
-// mass, affine momentum terms, stress impulse) into the wave's LDS slots. Then lane (cx, zo, oy) adds the chunk's
-// particles of cell cx, in ascending rank, to its 4 register sums: nodes (cx-1+ox, cy-1+oy,
-// w-1+zo), ox = 0..3, of the cell's 4x4x4 node window c-1 .. c+2 (a superset of the quadratic
-// stencil and exactly the cubic one). At the end of the row the sums are added into the wave's
-// own LDS node slab (7 x 7 x 4 nodes) in four phases ox = 0..3; within a phase every lane has a
-// different node, so the slab receives the row sums in a fixed order.
+// lane (cx, r), r < RANKS, stages the particle of rank RANKS k + r of cell cx: it loads the
+// particle (one chunk ahead) and writes a 48-float payload (B-spline weights of the window's 4 nodes
+// per axis, mass, affine momentum terms, stress impulse, and the x-column terms every lane of the
+// cell needs) into the wave's LDS slots. Then lane (cx, zo, oy) adds the chunk's particles of cell
+// cx, in ascending rank, to its 4 register sums: nodes (cx-1+ox, cy-1+oy, z-1+zo), ox = 0..3, of the
+// cell's 4x4x4 node window c-1 .. c+2 (a superset of the quadratic stencil and exactly the cubic
+// one). At the end of the row the sums are added into the wave's own LDS node slab (7 x 7 x 4 nodes)
+// in four phases ox = 0..3; within a phase every lane has a different node, so the slab receives
+// the row sums in a fixed order.
 //
 // When all waves are done, node (X, Y, Z) of the tile's 7x7x7 footprint is the sum of the slabs of
-// the waves w with 0 <= Z - w <= 3, in ascending w, and is stored as the tile's partial. k_grid sums
-// the covering tiles' partials in ascending tile order: every node value is a sum in a fixed order,
-// without float atomics on global memory, and reruns are bitwise identical.
+// the waves of layers l with 0 <= Z - l <= 3, in ascending l and row half, and is stored as the
+// tile's partial. k_grid sums the covering tiles' partials in ascending tile order: every node value
+// is a sum in a fixed order, without float atomics on global memory, and reruns are bitwise identical.
 constexpr int PAY = 48;                 // payload floats per staged particle (12 x float4)
 // RANKS (4 or 8, chosen by the autotuner) = ranks per cell staged per chunk; one cell's staged
 // payloads take RANKS * PAY + 4 floats (+1 float4 against bank conflicts). The chunk size does not
@@ -549,6 +552,7 @@ constexpr int PAY = 48;                 // payload floats per staged particle (1
 template <int RANKS>
 constexpr int pay_cell() { return RANKS * PAY + 4; }
 constexpr int SLAB = FOOT * FOOT * 4;   // nodes of a wave's slab
+constexpr int P2G_WAVES = 8;            // waves per P2G workgroup: 4 cell layers x 2 row halves
 // payload layout (floats): the x weights of window nodes c-1 .. c+2, the y and z weights and
 // derivatives (per cell width), the x-column terms g[ox][r] = ox wx[ox] Q[r][0] - dwx[ox] S[r][0]
 // that every lane of the cell adds with weight wy wz, the mass, q0 (a cloth vertex's force in the
@@ -618,9 +622,13 @@ struct StageIn {
   float x[3], m, v[3], B[9], K[6];
 };
 
+// The physical slot of sorted position idx, loaded a chunk before the particle's own data so that
+// the two dependent loads do not stall the wave back to back; it is clamped where it is used.
+__device__ __forceinline__ int p2g_slot(const P2GArgs& a, int idx) { return a.src[idx]; }
+
 template <int MODE>
-__device__ __forceinline__ void p2g_load(const P2GArgs& a, const P2GScene& S, int idx, StageIn& in) {
-  const int pi = clamp_index(a.src[idx], a.n);
+__device__ __forceinline__ void p2g_load_at(const P2GArgs& a, const P2GScene& S, int slot, StageIn& in) {
+  const int pi = clamp_index(slot, a.n);
   const float* st = a.state;
   const int64_t cap = a.cap;
   in.pi = pi;
@@ -644,6 +652,11 @@ __device__ __forceinline__ void p2g_load(const P2GArgs& a, const P2GScene& S, in
 #pragma unroll
     for (int i = 0; i < 6; ++i) in.K[i] = st[(PK + i) * cap + pi];
   }
+}
+
+template <int MODE>
+__device__ __forceinline__ void p2g_load(const P2GArgs& a, const P2GScene& S, int idx, StageIn& in) {
+  p2g_load_at<MODE>(a, S, p2g_slot(a, idx), in);
 }
 
 template <int ORDER, int MODE>
@@ -779,14 +792,15 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const P2GScene& S, c
 }
 
 template <int ORDER, int MODE, int RANKS>
-__global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
+__global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
   if (clock_skip(a.clk)) return;
   const float dt = clock_dt(a.clk, a.dt);
   constexpr int P2G_RANKS = RANKS;
   constexpr int PAY_CELL = pay_cell<RANKS>();
-  __shared__ float4 s_pay[4][4 * PAY_CELL / 4];
-  __shared__ float s_slab[4][4][SLAB];  // [wave][quantity][node]
+  __shared__ float4 s_pay[P2G_WAVES][4 * PAY_CELL / 4];
+  __shared__ float s_slab[P2G_WAVES][4][SLAB];  // [wave][quantity][node]
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int layer = w & 3, half = w >> 2;  // wave w owns cell layer z = w & 3, rows 2 (w >> 2) and 2 (w >> 2) + 1
   const int ntile = a.counters[CNT_TILES];
   constexpr int NQ = (MODE == P2G_FORCE) ? 3 : 4;
   // staging role: lane (cx, sr), sr < RANKS; accumulation role: lane (cx, zo, oy)
@@ -804,60 +818,79 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
     const int tile = gtile - a.T.tile_off[sc];
     const int nt0 = __builtin_amdgcn_readfirstlane(P.tiles[0]), nt1 = __builtin_amdgcn_readfirstlane(P.tiles[1]);
     const int t0 = tile % nt0, t1 = (tile / nt0) % nt1, t2 = tile / (nt0 * nt1);
-    const int g2 = 4 * t2 + w;
+    const int g2 = 4 * t2 + layer;
     const int g0 = 4 * t0 + cx;  // this lane's cell of each row (both roles)
-    // the bins of this lane's cell in the wave's four rows, and each row's largest count
-    int rstart[4], rcnt[4], rmax[4];
-#pragma unroll
-    for (int cy = 0; cy < 4; ++cy) {
-      const int g1 = 4 * t1 + cy;
-      rstart[cy] = 0;
-      rcnt[cy] = 0;
-      if (g0 < S.res[0] && g1 < S.res[1] && g2 < S.res[2]) {
-        const int64_t key = noff + (g2 * S.res[1] + g1) * S.res[0] + g0;
-        rstart[cy] = a.cell_start[key];
-        rcnt[cy] = a.cell_end[key] - rstart[cy];
+    // the bins of this lane's cell in the wave's two rows, and each row's largest count (uniform)
+    int rstart0 = 0, rcnt0 = 0, rstart1 = 0, rcnt1 = 0;
+    {
+      const int g1a = 4 * t1 + 2 * half, g1b = g1a + 1;
+      if (g0 < S.res[0] && g2 < S.res[2]) {
+        if (g1a < S.res[1]) {
+          const int64_t key = noff + (g2 * S.res[1] + g1a) * S.res[0] + g0;
+          rstart0 = a.cell_start[key];
+          rcnt0 = a.cell_end[key] - rstart0;
+        }
+        if (g1b < S.res[1]) {
+          const int64_t key = noff + (g2 * S.res[1] + g1b) * S.res[0] + g0;
+          rstart1 = a.cell_start[key];
+          rcnt1 = a.cell_end[key] - rstart1;
+        }
       }
-      rmax[cy] = max(max(__builtin_amdgcn_readlane(rcnt[cy], 0), __builtin_amdgcn_readlane(rcnt[cy], 16)),
-                     max(__builtin_amdgcn_readlane(rcnt[cy], 32), __builtin_amdgcn_readlane(rcnt[cy], 48)));
     }
-    // walk the (row, chunk) steps in order, loading the particles of the next step ahead
-    auto next_row = [&](int cy) {
-      int r = cy;
-      while (r < 4 && rmax[r] == 0) ++r;
-      return r;
+    const int rmax0 = max(max(__builtin_amdgcn_readlane(rcnt0, 0), __builtin_amdgcn_readlane(rcnt0, 16)),
+                          max(__builtin_amdgcn_readlane(rcnt0, 32), __builtin_amdgcn_readlane(rcnt0, 48)));
+    const int rmax1 = max(max(__builtin_amdgcn_readlane(rcnt1, 0), __builtin_amdgcn_readlane(rcnt1, 16)),
+                          max(__builtin_amdgcn_readlane(rcnt1, 32), __builtin_amdgcn_readlane(rcnt1, 48)));
+    auto rmax_of = [&](int r) { return r == 0 ? rmax0 : rmax1; };
+    auto rcnt_of = [&](int r) { return r == 0 ? rcnt0 : rcnt1; };
+    auto rstart_of = [&](int r) { return r == 0 ? rstart0 : rstart1; };
+    // walk the (row, chunk) steps in order (row 2 = done), loading the particles ahead
+    auto next_row = [&](int r) { return (r == 0 && rmax0 > 0) ? 0 : ((r <= 1 && rmax1 > 0) ? 1 : 2); };
+    auto advance = [&](int& r, int& kk) {
+      if (r >= 2) return;
+      if (P2G_RANKS * (kk + 1) >= rmax_of(r)) {
+        r = next_row(r + 1);
+        kk = 0;
+      } else {
+        ++kk;
+      }
     };
-    int cy = next_row(0), k = 0;
+    int cr = next_row(0), k = 0;
     StageIn in;
-    if (MODE != P2G_FORCE && cy < 4 && sr < P2G_RANKS && sr < rcnt[cy]) p2g_load<MODE>(a, S, rstart[cy] + sr, in);
+    // slot of this lane's particle two steps ahead (step after the next)
+    int cr2 = cr, k2 = k;
+    advance(cr2, k2);
+    int pi2 = 0;
+    if (MODE != P2G_FORCE && cr < 2 && sr < P2G_RANKS && sr < rcnt_of(cr)) p2g_load<MODE>(a, S, rstart_of(cr) + sr, in);
+    if (MODE != P2G_FORCE && cr2 < 2 && sr < P2G_RANKS && P2G_RANKS * k2 + sr < rcnt_of(cr2))
+      pi2 = p2g_slot(a, rstart_of(cr2) + P2G_RANKS * k2 + sr);
     float acc[4][NQ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0f;
-    while (cy < 4) {
+    while (cr < 2) {
+      const int cy = 2 * half + cr;
       const int g1 = 4 * t1 + cy;
-      const int cnt = rcnt[cy];
+      const int cnt = rcnt_of(cr), rstart = rstart_of(cr);
       // ---- stage ranks RANKS k .. RANKS k + RANKS - 1 of each cell of the row
       if (sr < P2G_RANKS) {
         const int rank = P2G_RANKS * k + sr;
         if (rank < cnt) {
-          if constexpr (MODE == P2G_FORCE) p2g_load<MODE>(a, S, rstart[cy] + rank, in);
+          if constexpr (MODE == P2G_FORCE) p2g_load<MODE>(a, S, rstart + rank, in);
           p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
         }
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      // ---- the next step, and its particles' loads
-      int ncy = cy, nk = k + 1;
-      if (P2G_RANKS * nk >= rmax[cy]) {
-        ncy = next_row(cy + 1);
-        nk = 0;
-      }
-      if (MODE != P2G_FORCE && ncy < 4 && sr < P2G_RANKS) {
-        const int nrank = P2G_RANKS * nk + sr;
-        if (nrank < rcnt[ncy]) p2g_load<MODE>(a, S, rstart[ncy] + nrank, in);
-      }
+      // ---- the next step: its particles' data (their slots arrived during this step), and the
+      // slots of the step after it
+      const int ncr = cr2, nk = k2;
+      if (MODE != P2G_FORCE && ncr < 2 && sr < P2G_RANKS && P2G_RANKS * nk + sr < rcnt_of(ncr))
+        p2g_load_at<MODE>(a, S, pi2, in);
+      advance(cr2, k2);
+      if (MODE != P2G_FORCE && cr2 < 2 && sr < P2G_RANKS && P2G_RANKS * k2 + sr < rcnt_of(cr2))
+        pi2 = p2g_slot(a, rstart_of(cr2) + P2G_RANKS * k2 + sr);
       // ---- accumulate this lane's cell, ascending rank
       const int nr = min(P2G_RANKS, cnt - P2G_RANKS * k);
       for (int r = 0; r < nr; ++r) {
@@ -902,44 +935,208 @@ __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (ncy != cy) {
+      if (ncr != cr) {
         // ---- row sums into the wave's slab: phase ox, node (cx + ox, cy + oy, zo)
 #pragma unroll
         for (int ox = 0; ox < 4; ++ox) {
           const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * zo;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            atomicAdd(&slab[q * SLAB + ln], acc[ox][q]);
+          for (int q = 0; q < NQ; ++q) {  // the wave's own slab, a different node per lane: no atomics
+            slab[q * SLAB + ln] += acc[ox][q];
             acc[ox][q] = 0.0f;
           }
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
       }
-      cy = ncy;
+      cr = ncr;
       k = nk;
     }
     __syncthreads();
-    // ---- tile partial: node (X, Y, Z) = sum over waves w, 0 <= Z - w <= 3, ascending; slabs cleared
+    // ---- tile partial: node (X, Y, Z) = sum over layers l, 0 <= Z - l <= 3, ascending, and within a
+    // layer over the row halves in order; slabs cleared
     float* part = a.partials + (size_t)slot * PART_STRIDE + (MODE == P2G_FORCE ? 4 * FOOT3 : 0);
-    for (int i = tid; i < FOOT3; i += 256) {
+    for (int i = tid; i < FOOT3; i += 64 * P2G_WAVES) {
       const int X = i % FOOT, Y = (i / FOOT) % FOOT, Z = i / (FOOT * FOOT);
       float s[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) s[q] = 0.0f;
 #pragma unroll
-      for (int ww = 0; ww < 4; ++ww) {
-        const int z = Z - ww;
+      for (int ww = 0; ww < P2G_WAVES; ++ww) {
+        const int l = ww >> 1, hh = ww & 1, wv = l + 4 * hh;
+        const int z = Z - l;
         if (z < 0 || z > 3) continue;
         const int ln = X + FOOT * Y + FOOT * FOOT * z;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          s[q] += s_slab[ww][q][ln];
-          s_slab[ww][q][ln] = 0.0f;
+          s[q] += s_slab[wv][q][ln];
+          s_slab[wv][q][ln] = 0.0f;
         }
       }
 #pragma unroll
       for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = s[q];
+    }
+    __syncthreads();
+  }
+}
+
+// Layer pass (the alternative P2G schedule, chosen by the autotuner). One workgroup (4 waves) per
+// active tile; wave w owns cell layer z = w and all 16 cells of it at once. Staging lane (c, r),
+// c = lane >> 2 (cell x = c & 3, y = c >> 2), r < RANKS, stages rank RANKS k + r of cell c; then
+// lane (c, zo) adds the chunk's particles of cell c, in ascending rank, to 16 x 4 register sums:
+// nodes (x-1+ox, y-1+oy, z-1+zo) of the cell's 4x4x4 window. Four lanes read each payload, instead
+// of the row pass's sixteen. The arithmetic per particle and node, the split of a layer into two
+// row halves with a slab each, the order in which a slab node receives the cell sums (rows
+// ascending, then ox ascending: phases oy = 3..0, ox = 0..3) and the order of the slab sums are
+// those of the row pass, so both passes give the same bits.
+template <int ORDER, int MODE, int RANKS>
+__global__ __launch_bounds__(256) void k_p2g_layer(P2GArgs a) {
+  if (clock_skip(a.clk)) return;
+  const float dt = clock_dt(a.clk, a.dt);
+  constexpr int PAY_CELL = pay_cell<RANKS>();
+  __shared__ float4 s_pay[4][16 * PAY_CELL / 4];
+  __shared__ float s_slab[8][4][SLAB];  // [layer + 4 row half][quantity][node], as the row pass's waves
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int ntile = a.counters[CNT_TILES];
+  constexpr int NQ = (MODE == P2G_FORCE) ? 3 : 4;
+  const int c = lane >> 2, sr = lane & 3, zo = lane & 3;  // staging (c, sr); accumulation (c, zo)
+  const int cx = c & 3, cy = c >> 2;
+  const float fzo = (float)zo;
+  for (int i = tid; i < 8 * 4 * SLAB; i += 256) (&s_slab[0][0][0])[i] = 0.0f;
+  __syncthreads();
+  float* slab = &s_slab[w + 4 * (cy >> 1)][0][0];
+  for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
+    const int gtile = a.active_tiles[slot];
+    const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
+    const DevParams& P = a.T.params[sc];
+    const P2GScene S = p2g_scene(P);
+    const int64_t noff = a.T.node_off[sc];
+    const int tile = gtile - a.T.tile_off[sc];
+    const int nt0 = __builtin_amdgcn_readfirstlane(P.tiles[0]), nt1 = __builtin_amdgcn_readfirstlane(P.tiles[1]);
+    const int t0 = tile % nt0, t1 = (tile / nt0) % nt1, t2 = tile / (nt0 * nt1);
+    const int g0 = 4 * t0 + cx, g1 = 4 * t1 + cy, g2 = 4 * t2 + w;
+    int start = 0, cnt = 0;
+    if (g0 < S.res[0] && g1 < S.res[1] && g2 < S.res[2]) {
+      const int64_t key = noff + (g2 * S.res[1] + g1) * S.res[0] + g0;
+      start = a.cell_start[key];
+      cnt = a.cell_end[key] - start;
+    }
+    int cmax = cnt;
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) cmax = max(cmax, __shfl_xor(cmax, off));
+    cmax = __builtin_amdgcn_readfirstlane(cmax);
+    if (cmax > 0) {
+      float acc[4][4][NQ];  // [oy][ox][quantity]
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) acc[i][j][q] = 0.0f;
+      StageIn in;
+      if (MODE != P2G_FORCE && sr < RANKS && sr < cnt) p2g_load<MODE>(a, S, start + sr, in);
+      int pi2 = 0;  // slot of this lane's particle in the chunk after the next
+      if (MODE != P2G_FORCE && sr < RANKS && RANKS + sr < cnt) pi2 = p2g_slot(a, start + RANKS + sr);
+      for (int k = 0; k * RANKS < cmax; ++k) {
+        if (sr < RANKS) {
+          const int rank = RANKS * k + sr;
+          if (rank < cnt) {
+            if constexpr (MODE == P2G_FORCE) p2g_load<MODE>(a, S, start + rank, in);
+            p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(c * PAY_CELL + sr * PAY) / 4]);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (MODE != P2G_FORCE && sr < RANKS) {  // the next chunk's data and the slots of the one after
+          if (RANKS * (k + 1) + sr < cnt) p2g_load_at<MODE>(a, S, pi2, in);
+          if (RANKS * (k + 2) + sr < cnt) pi2 = p2g_slot(a, start + RANKS * (k + 2) + sr);
+        }
+        const int nr = min(RANKS, cnt - RANKS * k);
+        for (int r = 0; r < nr; ++r) {
+          const float4* src = &s_pay[w][(c * PAY_CELL + r * PAY) / 4];
+          const float* pf = reinterpret_cast<const float*>(src);
+          const float4 WX = src[PY_WX / 4], WY = src[PY_WY / 4], DWY = src[PY_DWY / 4];
+          const float wxs[4] = {WX.x, WX.y, WX.z, WX.w};
+          const float wys[4] = {WY.x, WY.y, WY.z, WY.w};
+          const float dwys[4] = {DWY.x, DWY.y, DWY.z, DWY.w};
+          const float wz = pf[PY_WZ + zo], dwz = pf[PY_DWZ + zo];
+          float g[12];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const float4 t = src[PY_G / 4 + i];
+            g[4 * i] = t.x; g[4 * i + 1] = t.y; g[4 * i + 2] = t.z; g[4 * i + 3] = t.w;
+          }
+          float qs[16];  // m, q0(3), Q12(6), S12(6)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float4 t = src[PY_M / 4 + i];
+            qs[4 * i] = t.x; qs[4 * i + 1] = t.y; qs[4 * i + 2] = t.z; qs[4 * i + 3] = t.w;
+          }
+#pragma unroll
+          for (int oy = 0; oy < 4; ++oy) {
+            const float wy = wys[oy], dwy = dwys[oy];
+            const float wyz = wy * wz, dyz = dwy * wz, ydz = wy * dwz;
+            float e[3];
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) {
+              float qrow = qs[1 + rr];
+              if constexpr (MODE != P2G_FORCE)
+                qrow = __builtin_fmaf(qs[4 + 2 * rr + 1], fzo, __builtin_fmaf(qs[4 + 2 * rr], (float)oy, qrow));
+              e[rr] = wyz * qrow;
+              if constexpr (MODE != P2G_MOM)
+                e[rr] = __builtin_fmaf(-qs[10 + 2 * rr + 1], ydz, __builtin_fmaf(-qs[10 + 2 * rr], dyz, e[rr]));
+            }
+            const float mw = qs[0] * wyz;
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox) {
+              const float wx = wxs[ox];
+              constexpr int q1 = (MODE != P2G_FORCE) ? 1 : 0;
+              if constexpr (MODE != P2G_FORCE) acc[oy][ox][0] = __builtin_fmaf(wx, mw, acc[oy][ox][0]);
+#pragma unroll
+              for (int rr = 0; rr < 3; ++rr)
+                acc[oy][ox][q1 + rr] =
+                    __builtin_fmaf(wyz, g[3 * ox + rr], __builtin_fmaf(wx, e[rr], acc[oy][ox][q1 + rr]));
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+      // ---- the cells' sums into the slab of their row half: phase (oy descending, ox ascending),
+      // node (cx + ox, cy + oy, zo); a node receives its cells in ascending row, then ascending ox
+#pragma unroll
+      for (int oy = 3; oy >= 0; --oy)
+#pragma unroll
+        for (int ox = 0; ox < 4; ++ox) {
+          const int ln = (cx + ox) + FOOT * (cy + oy) + FOOT * FOOT * zo;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) slab[q * SLAB + ln] += acc[oy][ox][q];  // own slab, a node per lane
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+    }
+    __syncthreads();
+    // ---- tile partial: as the row pass (layers l ascending, row halves in order); slabs cleared
+    float* part = a.partials + (size_t)slot * PART_STRIDE + (MODE == P2G_FORCE ? 4 * FOOT3 : 0);
+    for (int i = tid; i < FOOT3; i += 256) {
+      const int X = i % FOOT, Y = (i / FOOT) % FOOT, Z = i / (FOOT * FOOT);
+      float sum[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sum[q] = 0.0f;
+#pragma unroll
+      for (int ww = 0; ww < 8; ++ww) {
+        const int l = ww >> 1, hh = ww & 1, wv = l + 4 * hh;
+        const int z = Z - l;
+        if (z < 0 || z > 3) continue;
+        const int ln = X + FOOT * Y + FOOT * FOOT * z;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          sum[q] += s_slab[wv][q][ln];
+          s_slab[wv][q][ln] = 0.0f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = sum[q];
     }
     __syncthreads();
   }
@@ -1664,20 +1861,39 @@ void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32
 template <int RANKS>
 static void launch_p2g_r(hipStream_t s, const P2GArgs& a, int order, int mode, int grid) {
   if (mode == P2G_MOM_STRESS) {
-    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(64 * P2G_WAVES), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(64 * P2G_WAVES), 0, s, a);
   } else if (mode == P2G_MOM) {
-    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM, RANKS>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_MOM, RANKS>), dim3(grid), dim3(64 * P2G_WAVES), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_MOM, RANKS>), dim3(grid), dim3(64 * P2G_WAVES), 0, s, a);
   } else {
-    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_FORCE, RANKS>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p2g<3, P2G_FORCE, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    if (order == 2) hipLaunchKernelGGL((k_p2g<2, P2G_FORCE, RANKS>), dim3(grid), dim3(64 * P2G_WAVES), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g<3, P2G_FORCE, RANKS>), dim3(grid), dim3(64 * P2G_WAVES), 0, s, a);
   }
 }
 
-void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks) {
-  if (ranks == 4) launch_p2g_r<4>(s, a, order, mode, grid);
-  else launch_p2g_r<8>(s, a, order, mode, grid);
+template <int RANKS>
+static void launch_p2g_l(hipStream_t s, const P2GArgs& a, int order, int mode, int grid) {
+  if (mode == P2G_MOM_STRESS) {
+    if (order == 2) hipLaunchKernelGGL((k_p2g_layer<2, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g_layer<3, P2G_MOM_STRESS, RANKS>), dim3(grid), dim3(256), 0, s, a);
+  } else if (mode == P2G_MOM) {
+    if (order == 2) hipLaunchKernelGGL((k_p2g_layer<2, P2G_MOM, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g_layer<3, P2G_MOM, RANKS>), dim3(grid), dim3(256), 0, s, a);
+  } else {
+    if (order == 2) hipLaunchKernelGGL((k_p2g_layer<2, P2G_FORCE, RANKS>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p2g_layer<3, P2G_FORCE, RANKS>), dim3(grid), dim3(256), 0, s, a);
+  }
+}
+
+void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks, int pass) {
+  if (pass == P2G_LAYERS) {
+    if (ranks == 2) launch_p2g_l<2>(s, a, order, mode, grid);
+    else launch_p2g_l<4>(s, a, order, mode, grid);
+  } else {
+    if (ranks == 4) launch_p2g_r<4>(s, a, order, mode, grid);
+    else launch_p2g_r<8>(s, a, order, mode, grid);
+  }
 }
 
 void launch_clock_update(hipStream_t s, Clock* c, int init) {

@@ -121,8 +121,9 @@ struct mpm_ctx {
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   float graph_dt[2] = {0.0f, 0.0f};
 
-  int p2g_grid = 1024;
+  int p2g_grid = 512;
   int p2g_ranks = 8;  // P2G staging chunk (mpm_autotune / mpm_set_tuning)
+  int p2g_pass = P2G_ROWS;  // P2G schedule (mpm_autotune / mpm_set_tuning)
   // adaptive time step (mpm_solve): the device clock, and the pointer the kernels get (null for
   // fixed-step runs)
   Clock* clock = nullptr;
@@ -449,11 +450,11 @@ static int stage_p2g(mpm_ctx* ctx, float dt) {
       launch_stress_init(s, ctx->T, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->fallbacks);
       ctx->stress_valid = true;
     }
-    launch_p2g(s, a, ctx->order, P2G_MOM_STRESS, ctx->p2g_grid, ctx->p2g_ranks);
+    launch_p2g(s, a, ctx->order, P2G_MOM_STRESS, ctx->p2g_grid, ctx->p2g_ranks, ctx->p2g_pass);
   } else {
-    launch_p2g(s, a, ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks);
+    launch_p2g(s, a, ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks, ctx->p2g_pass);
     launch_grid(s, grid_args(ctx, dt), GRID_NORMALIZE, ctx->grid_grid);
-    launch_p2g(s, a, ctx->order, P2G_FORCE, ctx->p2g_grid, ctx->p2g_ranks);
+    launch_p2g(s, a, ctx->order, P2G_FORCE, ctx->p2g_grid, ctx->p2g_ranks, ctx->p2g_pass);
   }
   return MPM_OK;
 }
@@ -621,7 +622,7 @@ int mpm_create(mpm_ctx** out, int32_t device) {
   (void)hipMemsetAsync(ctx->sur_counts, 0, 4 * sizeof(unsigned long long), ctx->stream);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
-    ctx->p2g_grid = std::max(256, prop.multiProcessorCount * 4);
+    ctx->p2g_grid = std::max(128, prop.multiProcessorCount * 2);  // 8-wave workgroups, two per CU
     ctx->n_cu = std::max(1, prop.multiProcessorCount);
     ctx->grid_grid = std::max(512, prop.multiProcessorCount * 8);
   }
@@ -1168,7 +1169,7 @@ int mpm_estimate_volumes(mpm_ctx* ctx) {
   rc = stage_sort(ctx);
   if (rc) return rc;
   P2GArgs a = p2g_args(ctx, 0.0f);
-  launch_p2g(ctx->stream, a, ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks);
+  launch_p2g(ctx->stream, a, ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks, ctx->p2g_pass);
   launch_grid(ctx->stream, grid_args(ctx, 0.0f), GRID_NORMALIZE, ctx->grid_grid);
   launch_volumes(ctx->stream, ctx->T, ctx->order, ctx->nblocks_p, ctx->state[ctx->cur], ctx->cap, ctx->n, ctx->grid);
   ctx->stress_valid = false;
@@ -1278,7 +1279,7 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
   // first Dt from the velocities p/m of a P2G without forces
   rc = stage_sort(ctx);
   if (rc) return rc;
-  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks);
+  launch_p2g(ctx->stream, p2g_args(ctx, 0.0f), ctx->order, P2G_MOM, ctx->p2g_grid, ctx->p2g_ranks, ctx->p2g_pass);
   GridArgs g0 = grid_args(ctx, 0.0f);
   g0.vmax_out = &ctx->clock->vmax_bits;
   launch_grid(ctx->stream, g0, GRID_NORMALIZE, ctx->grid_grid);
@@ -1373,6 +1374,7 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t reps, mpm_tuning* chosen) {
   const int grid_mode = ctx->predicted ? GRID_NORMALIZE : GRID_FUSED;
   mpm_tuning best{};
   best.sort_ms = 1e30f;
+  best.p2g_pass = -1;
   const int keep_mode = ctx->sort_mode;
   for (int m : {SORT_COUNTING, SORT_RADIX}) {
     ctx->sort_mode = m;
@@ -1388,20 +1390,22 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t reps, mpm_tuning* chosen) {
   if (rc) return rc;
   const P2GArgs a = p2g_args(ctx, dt);
   best.p2g_ms = 1e30f;
-  const int ranks[2] = {4, 8};
-  const int mults[5] = {2, 3, 4, 5, 6};
-  for (int r : ranks)
-    for (int m : mults) {
-      const int blocks = std::max(64, m * ctx->n_cu);
-      const float ms = time_launches(ctx, reps, [&] { launch_p2g(s, a, ctx->order, p2g_mode, blocks, r); });
-      CK(hipGetLastError());
-      if (ms >= 0.0f && ms < best.p2g_ms) {
-        best.p2g_ms = ms;
-        best.p2g_ranks = r;
-        best.p2g_blocks = blocks;
-      }
+  struct P2GCand { int pass, ranks, mult; };
+  const P2GCand cands[] = {{P2G_ROWS, 4, 1}, {P2G_ROWS, 4, 2}, {P2G_ROWS, 4, 4}, {P2G_ROWS, 8, 1},
+                           {P2G_ROWS, 8, 2}, {P2G_ROWS, 8, 4}, {P2G_LAYERS, 2, 2}, {P2G_LAYERS, 2, 3},
+                           {P2G_LAYERS, 2, 6}, {P2G_LAYERS, 4, 2}, {P2G_LAYERS, 4, 4}};
+  for (const P2GCand& cd : cands) {
+    const int blocks = std::max(64, cd.mult * ctx->n_cu);
+    const float ms = time_launches(ctx, reps, [&] { launch_p2g(s, a, ctx->order, p2g_mode, blocks, cd.ranks, cd.pass); });
+    CK(hipGetLastError());
+    if (ms >= 0.0f && ms < best.p2g_ms) {
+      best.p2g_ms = ms;
+      best.p2g_ranks = cd.ranks;
+      best.p2g_blocks = blocks;
+      best.p2g_pass = cd.pass;
     }
-  launch_p2g(s, a, ctx->order, p2g_mode, best.p2g_blocks, best.p2g_ranks);
+  }
+  launch_p2g(s, a, ctx->order, p2g_mode, best.p2g_blocks, best.p2g_ranks, best.p2g_pass);
   const GridArgs g = grid_args(ctx, dt);
   best.grid_ms = 1e30f;
   const int gmults[4] = {2, 4, 8, 16};
@@ -1418,6 +1422,7 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t reps, mpm_tuning* chosen) {
   if (best.p2g_ranks == 0 || best.grid_blocks == 0) return fail(ctx, MPM_ERR_HIP, "autotune: no timing succeeded");
   ctx->p2g_ranks = best.p2g_ranks;
   ctx->p2g_grid = best.p2g_blocks;
+  ctx->p2g_pass = best.p2g_pass;
   ctx->grid_grid = best.grid_blocks;
   drop_graphs(ctx);
   if (chosen) *chosen = best;
@@ -1431,13 +1436,18 @@ int mpm_get_tuning(mpm_ctx* ctx, mpm_tuning* t) {
   t->p2g_blocks = ctx->p2g_grid;
   t->grid_blocks = ctx->grid_grid;
   t->sort_mode = ctx->sort_mode;
+  t->p2g_pass = ctx->p2g_pass;
   return MPM_OK;
 }
 
 int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* t) {
   if (!ctx || !t) return MPM_ERR_INVALID;
-  if ((t->p2g_ranks != 4 && t->p2g_ranks != 8) || t->p2g_blocks < 1 || t->grid_blocks < 1)
-    return fail(ctx, MPM_ERR_INVALID, "p2g_ranks must be 4 or 8 and the block counts positive");
+  if (t->p2g_pass != MPM_P2G_ROWS && t->p2g_pass != MPM_P2G_LAYERS)
+    return fail(ctx, MPM_ERR_INVALID, "p2g_pass must be MPM_P2G_ROWS or MPM_P2G_LAYERS");
+  const bool ranks_ok = t->p2g_pass == MPM_P2G_ROWS ? (t->p2g_ranks == 4 || t->p2g_ranks == 8)
+                                                     : (t->p2g_ranks == 2 || t->p2g_ranks == 4);
+  if (!ranks_ok || t->p2g_blocks < 1 || t->grid_blocks < 1)
+    return fail(ctx, MPM_ERR_INVALID, "p2g_ranks must be 4 or 8 (rows) or 2 or 4 (layers), block counts positive");
   if (t->sort_mode != MPM_SORT_COUNTING && t->sort_mode != MPM_SORT_RADIX)
     return fail(ctx, MPM_ERR_INVALID, "sort_mode must be MPM_SORT_COUNTING or MPM_SORT_RADIX");
   CK(hipSetDevice(ctx->device));
@@ -1446,6 +1456,7 @@ int mpm_set_tuning(mpm_ctx* ctx, const mpm_tuning* t) {
   ctx->p2g_grid = t->p2g_blocks;
   ctx->grid_grid = t->grid_blocks;
   ctx->sort_mode = t->sort_mode;
+  ctx->p2g_pass = t->p2g_pass;
   drop_graphs(ctx);
   return MPM_OK;
 }

@@ -6,7 +6,11 @@ mapping. Inputs are synthetic (seeded jittered lattice), resident in HBM before 
 
 `value` is the exact-return-map path; the `surrogate` object reports the same workload with the
 learned return map (trained on the GPU from pairs the exact path logs during the run), its
-throughput and its position error against the exact run after the same substeps.
+throughput and its position error against the exact run after the same substeps. The `batched`
+object reports the batched many-scene workload (64 independent snow/sand scenes of 20k-500k
+particles on 64^3-128^3 grids, one context, one launch per kernel per substep, autotuned launch
+shapes) against the same scenes run one after another with the plain exact path, with and without
+the learned return map.
 
 Prints one JSON line. `roofline` is for the dominant kernel (its own algorithmic bytes over its
 HIP-event-timed duration); `substep_roofline` is the whole substep against 208 B/particle +
@@ -32,13 +36,14 @@ METRIC = "particle-substeps/sec, surrogate vs exact return map at ≤1e-3 rel po
 #   GRID reads m, momentum (16 B), writes v and v_BF (24 B) per active node
 #   G2P  reads x, F, plastic (13 floats), writes x, v, B, F, plastic (25 floats) per particle,
 #        reads v and v_BF (24 B) per active node
-#   SORT reads the 4-byte key and writes key + index (8 B) per particle and digit pass (3 passes)
+#   SORT (counting sort) reads the key twice and the rank once, writes rank, scattered id, sorted key,
+#        id and slot index: 36 B per particle, plus the cell counts scanned into the bins
 STAGES = ["sort", "p2g", "grid", "g2p"]
-ALG_PER_PARTICLE = {"sort": 3 * 12, "p2g": 104, "grid": 0, "g2p": 152}
+ALG_PER_PARTICLE = {"sort": 36, "p2g": 104, "grid": 0, "g2p": 152}
 ALG_PER_NODE = {"sort": 0, "p2g": 16, "grid": 40, "g2p": 24}
 SUBSTEP_PER_PARTICLE, SUBSTEP_PER_NODE = 208, 56
 KERNEL_NAMES = {"p2g": "k_p2g<2, P2G_MOM_STRESS>", "g2p": "k_g2p<2>", "grid": "k_grid<GRID_FUSED>",
-                "sort": "k_sort_pass (x3) + hist + reset + segments"}
+                "sort": "k_cell_count + k_cell_scan + k_cell_scatter + k_cell_rank + k_claim x2"}
 
 
 def load_traffic(kernel):
@@ -133,7 +138,86 @@ def surrogate_leg(sc, warmup, steps, x_exact, wall_exact, gate_tol=1e-5, pair_ch
         "acceptance": (st["surrogate_accepted"] / cand) if cand else 0.0,
         "pairs_trained": int(len(pairs)), "train_epochs": epochs, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
         "train_s": train_s, "status": st["status"],
-    }
+    }, w
+
+
+def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=10, pairs_per_chunk=400000,
+                epochs=8, batch_size=4096, lr=1e-3):
+    """64 scenes in one context against the same scenes one context at a time (exact path), and the
+    batch with the learned return map (trained on the GPU from pairs the exact batch logs, exact
+    fallback) against the exact batch after the same substeps."""
+    from anisotropicelastoplasticity_amd import Solver, scenes
+    batch = scenes.config5_batch()
+    dt = batch[0].dt
+    ntot = sum(sc.n for sc in batch)
+    weights = None
+    if surrogate:
+        total = warmup + steps
+        pairs = []
+        with Solver(0) as s:
+            s.load_batch(batch)
+            every = max(1, total // pair_chunks)
+            done = 0
+            while done < total:
+                s.set_pair_logging(pairs_per_chunk)
+                s.run(dt, 1)
+                pairs.append(s.get_pairs())
+                s.set_pair_logging(0)
+                k = min(every - 1, total - done - 1)
+                if k > 0:
+                    s.run(dt, k)
+                done += 1 + max(k, 0)
+            pairs = np.concatenate(pairs)
+            s.surrogate_init(1)
+            s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch_size, lr=lr, seed=2)
+            weights = s.surrogate_get_weights()
+
+    def run_batch(sur):
+        with Solver(0) as s:
+            s.load_batch(batch)
+            tuning = s.autotune(dt, reps=3)
+            if sur:
+                s.surrogate_set_weights(weights)
+                s.set_surrogate(True, gate_tol)
+            s.run(dt, warmup)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run(dt, steps)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            x = s.get_particles()["x"]
+            st = s.stats()
+        return wall, x, st, tuning
+
+    wall_b, x_b, st_b, tuning = run_batch(False)
+    seq_wall = 0.0
+    for sc in batch:  # the plain path: one context per scene, default launch shapes
+        with Solver(0) as s:
+            sc.load_into(s)
+            s.run(dt, warmup)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run(dt, seq_steps)
+            s.synchronize()
+            seq_wall += time.perf_counter() - t0
+    seq_value = ntot * seq_steps / seq_wall
+    out = {"workload": "config5 batched scenes", "scenes": len(batch), "particles": ntot, "dt": dt,
+           "substeps": warmup + steps, "value": ntot * steps / wall_b, "ms_per_step": 1e3 * wall_b / steps,
+           "sequential_value": seq_value, "sequential_substeps_per_scene": seq_steps,
+           "speedup_vs_sequential": (ntot * steps / wall_b) / seq_value,
+           "tuning": {k: tuning[k] for k in ("sort_mode", "p2g_ranks", "p2g_blocks", "grid_blocks")},
+           "status": st_b["status"]}
+    if weights is not None:
+        wall_s, x_s, st_s, _ = run_batch(True)
+        dx = x_s.astype(np.float64) - x_b.astype(np.float64)
+        cand = st_s["surrogate_candidates"]
+        out["surrogate"] = {"value": ntot * steps / wall_s, "ms_per_step": 1e3 * wall_s / steps,
+                            "speedup_vs_sequential_exact": (ntot * steps / wall_s) / seq_value,
+                            "speedup_vs_batched_exact": wall_b / wall_s,
+                            "rel_pos_err": float(np.linalg.norm(dx) / np.linalg.norm(x_b.astype(np.float64))),
+                            "acceptance": (st_s["surrogate_accepted"] / cand) if cand else 0.0,
+                            "pairs_trained": int(len(pairs)), "gate_tol": gate_tol, "status": st_s["status"]}
+    return out
 
 
 def main():
@@ -145,6 +229,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned return-map leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
+    ap.add_argument("--batch-steps", type=int, default=480, help="timed substeps of the batched leg (after 20 warm-up)")
+    ap.add_argument("--seq-steps", type=int, default=40, help="timed substeps per scene of the one-at-a-time baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -180,7 +267,8 @@ def main():
             copy = {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "launches": reps,
                     "kernel": "k_copy", "avg_ms": s.bench_copy(nbytes, reps)}
 
-    sur = None if args.no_surrogate else surrogate_leg(sc, args.warmup, args.steps, x_exact, wall)
+    sur, weights = (None, None) if args.no_surrogate else surrogate_leg(sc, args.warmup, args.steps, x_exact, wall)
+    bat = None if args.no_batch else batched_leg(not args.no_surrogate, args.batch_steps, 20, args.seq_steps)
     value = n * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
     n_nodes = st["n_active_nodes"]
@@ -223,6 +311,8 @@ def main():
         # the same workload with the learned return map (exact fallback behind the yield-residual
         # gate), compared with the exact run above after the same number of substeps
         out["surrogate"] = sur
+    if bat is not None:
+        out["batched"] = bat
     if copy is not None:
         out["copy_calibration"] = copy
     if not args.no_cpu_baseline:

@@ -255,14 +255,17 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
  * results are bitwise identical for every valid shape. */
 #define MPM_SORT_COUNTING 0 /* one-digit counting sort over the cells, stable rank inside each cell */
 #define MPM_SORT_RADIX 1    /* LSD radix sort, 8-bit digits */
+#define MPM_P2G_ROWS 0      /* P2G: a wave walks cell rows, 16 lanes per cell */
+#define MPM_P2G_LAYERS 1    /* P2G: a wave holds a whole cell layer, 4 lanes per cell */
 typedef struct mpm_tuning {
-  int32_t p2g_ranks;   /* particles per cell staged per P2G chunk (the bin tile): 4 or 8 */
+  int32_t p2g_ranks;   /* particles per cell staged per P2G chunk (the bin tile): 4 or 8 (rows), 2 or 4 (layers) */
   int32_t p2g_blocks;  /* P2G workgroups, persistent over the active 4x4x4-cell tiles */
   int32_t grid_blocks; /* grid-update workgroups, persistent over the active node blocks */
   float p2g_ms;        /* times of the chosen shapes measured by mpm_autotune (0 otherwise) */
   float grid_ms;
   int32_t sort_mode;   /* MPM_SORT_*: both give the same order, bit for bit */
   float sort_ms;
+  int32_t p2g_pass;    /* MPM_P2G_*: the same sums in the same order, bit for bit */
 } mpm_tuning;
 /* Autotuner (no counterpart in the reference): time every candidate shape `reps` times on the
  * current particle state (sort, P2G and grid update are idempotent, the state does not advance),

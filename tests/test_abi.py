@@ -34,7 +34,7 @@ def test_struct_sizes_match_header():
     # sizes follow from the C declarations in include/mpm_c_abi.h
     assert ctypes.sizeof(_lib.Params) == 4 * (4 + 3 + 5 + 4 + 1 + 4 + 4)
     assert ctypes.sizeof(_lib.LevelSetPrim) == 36
-    assert ctypes.sizeof(_lib.Tuning) == 7 * 4
+    assert ctypes.sizeof(_lib.Tuning) == 8 * 4
     assert ctypes.sizeof(_lib.Stats) == 8 + 4 * 4 + 8 * 6
     assert ctypes.sizeof(_lib.SceneDesc) == 6 * 8 + 4 * 4 + ctypes.sizeof(_lib.Params) + 4 + 16 * 36
 

@@ -116,13 +116,14 @@ def test_batch_surrogate_all_rejected_is_exact():
 
 
 def test_launch_shapes_give_identical_bits_and_autotune():
-    # the autotuner's candidates (sort algorithm, P2G staging chunk, P2G and grid workgroup counts) do not change
+    # the autotuner's candidates (sort algorithm, P2G schedule and staging chunk, P2G and grid workgroup counts) do not change
     # any order of summation: every shape gives the same bits, and tuning does not advance the state
     batch = _small_batch(14, 3, (5000, 30000), (32, 48))
     dt, nsteps = batch[0].dt, 11
     outs = []
-    for shape in [(8, 1024, 2048, _lib.SORT_COUNTING), (4, 512, 512, _lib.SORT_RADIX),
-                  (4, 1536, 4096, _lib.SORT_COUNTING), (8, 300, 700, _lib.SORT_RADIX)]:
+    for shape in [(8, 512, 2048, _lib.SORT_COUNTING, _lib.P2G_ROWS), (4, 256, 512, _lib.SORT_RADIX, _lib.P2G_ROWS),
+                  (2, 768, 4096, _lib.SORT_COUNTING, _lib.P2G_LAYERS), (4, 300, 700, _lib.SORT_RADIX, _lib.P2G_LAYERS),
+                  (4, 1536, 4096, _lib.SORT_COUNTING, _lib.P2G_ROWS)]:
         with Solver() as s:
             s.load_batch(batch)
             s.set_tuning(*shape)
@@ -134,8 +135,9 @@ def test_launch_shapes_give_identical_bits_and_autotune():
         s.load_batch(batch)
         t = s.autotune(dt, reps=3)
         say(f"[autotune] {t}")
-        assert t["p2g_ranks"] in (4, 8) and t["p2g_blocks"] > 0 and t["grid_blocks"] > 0
+        assert t["p2g_ranks"] in (2, 4, 8) and t["p2g_blocks"] > 0 and t["grid_blocks"] > 0
         assert t["sort_mode"] in (_lib.SORT_COUNTING, _lib.SORT_RADIX)
+        assert t["p2g_pass"] in (_lib.P2G_ROWS, _lib.P2G_LAYERS)
         assert s.tuning()["p2g_ranks"] == t["p2g_ranks"]
         s.run(dt, nsteps)
         _assert_bitwise(s.get_particles(), outs[0], "after autotune")

@@ -38,7 +38,7 @@ def user_keys(x, sc):
 # ------------------------------------------------------------------ sort
 def set_sort_mode(s, mode):
     t = s.tuning()
-    s.set_tuning(t["p2g_ranks"], t["p2g_blocks"], t["grid_blocks"], mode)
+    s.set_tuning(t["p2g_ranks"], t["p2g_blocks"], t["grid_blocks"], mode, t["p2g_pass"])
 
 
 SORT_MODES = [("counting", _lib.SORT_COUNTING), ("radix", _lib.SORT_RADIX)]
