@@ -75,6 +75,18 @@ __device__ __forceinline__ void spline(T r, T& w, T& dw) {
   }
 }
 
+// The three quadratic B-spline weights and derivatives (per cell) of a particle at fx = g - base in
+// [0.5, 1.5), base = floor(g - 0.5): N(fx), N(fx - 1), N(fx - 2) in closed form, without branches.
+__device__ __forceinline__ void quad_weights(float fx, float* w, float* dw) {
+  const float a = 1.5f - fx, b = fx - 1.0f, c = fx - 0.5f;
+  w[0] = 0.5f * a * a;
+  w[1] = 0.75f - b * b;
+  w[2] = 0.5f * c * c;
+  dw[0] = -a;
+  dw[1] = -2.0f * b;
+  dw[2] = c;
+}
+
 // ------------------------------------------------------------------ 3x3 helpers (row-major)
 template <class T>
 __device__ __forceinline__ void mat_mul(const T* A, const T* B, T* C) {
@@ -87,6 +99,20 @@ template <class T>
 __device__ __forceinline__ T mat_det(const T* A) {
 #pragma clang fp contract(off)
   return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+// Reciprocal and reciprocal square root for the rotation parameters of the Jacobi SVD: the hardware
+// approximations (1 ulp) in FP32, exact operations in FP64. Any rotation angle is valid for the
+// iteration; only c^2 + s^2 = 1 matters, which holds to 2 ulp with c = rsqrt(1 + t^2), s = c t.
+template <class T>
+__device__ __forceinline__ T jac_rcp(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+template <class T>
+__device__ __forceinline__ T jac_rsqrt(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rsqf(x);
+  else return T(1) / sqrt(x);
 }
 
 // One-sided (Hestenes) Jacobi SVD of a 3x3 matrix: A = U diag(S) V^T with S >= 0 in decreasing
@@ -110,13 +136,12 @@ __device__ __forceinline__ bool svd3(const T* A, T* U, T* S, T* V, int max_sweep
       T alpha = w[p] * w[p] + w[3 + p] * w[3 + p] + w[6 + p] * w[6 + p];
       T beta = w[q] * w[q] + w[3 + q] * w[3 + q] + w[6 + q] * w[6 + q];
       T gamma = w[p] * w[q] + w[3 + p] * w[3 + q] + w[6 + p] * w[6 + q];
-      T ag = gamma < T(0) ? -gamma : gamma;
-      if (ag > eps * sqrt(alpha * beta)) {
+      if (gamma * gamma > eps * eps * (alpha * beta)) {  // |gamma| > eps sqrt(alpha beta)
         rotated = true;
-        T zeta = (beta - alpha) / (T(2) * gamma);
+        T zeta = (beta - alpha) * jac_rcp(T(2) * gamma);
         T az = zeta < T(0) ? -zeta : zeta;
-        T t = (zeta >= T(0) ? T(1) : T(-1)) / (az + sqrt(T(1) + zeta * zeta));
-        T c = T(1) / sqrt(T(1) + t * t);
+        T t = (zeta >= T(0) ? T(1) : T(-1)) * jac_rcp(az + sqrt(T(1) + zeta * zeta));
+        T c = jac_rsqrt(T(1) + t * t);
         T s = c * t;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -316,6 +341,10 @@ struct SurrogateIO {
   float sigma[3];
   float U[9];
   float V[9];
+  // FP64 apex projections: deferred to a kernel of their own (defer_apex), which then calls
+  // return_map again on the unchanged trial state with defer_apex off and gets the same bits
+  bool defer_apex;
+  bool apex_deferred;
 };
 
 // Friction angle (degrees) and K/mu of one particle: phi(q) = h0 + (h1 q - h3) exp(-h2 q)
@@ -376,6 +405,10 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
       // close to the cone apex redo it in FP64
       float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
       if (!(l0 + l1 + l2 > 0.0f) && l0 * l0 + l1 * l1 + l2 * l2 < P.apex_radius * P.apex_radius) {
+        if (io.defer_apex) {  // F^, q and the stress are left as they are for the apex kernel
+          io.apex_deferred = true;
+          return;
+        }
         double Sd[3] = {S[0], S[1], S[2]}, So[3], dqd;
         dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, Sd, So, dqd);
         Sn[0] = float(So[0]); Sn[1] = float(So[1]); Sn[2] = float(So[2]);

@@ -37,7 +37,7 @@ constexpr int FOOT = TILE + 3;             // footprint nodes per edge (cell win
 constexpr int FOOT3 = FOOT * FOOT * FOOT;
 constexpr int PART_STRIDE = 7 * FOOT3;     // m, p(3), f(3) per tile slot
 
-enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_CAND = 2, CNT_NUM = 4 };
+enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_CAND = 2, CNT_APEX = 3, CNT_NUM = 4 };
 enum P2GMode : int { P2G_MOM_STRESS = 0, P2G_MOM = 1, P2G_FORCE = 2 };
 enum GridMode : int { GRID_FUSED = 0, GRID_NORMALIZE = 1, GRID_FORCE_UPDATE = 2 };
 
@@ -192,8 +192,29 @@ struct G2PArgs {
   float* cand;      // candidate records, 21 planes (sigma 3, U 9, V 9) of n floats
   int* cand_slot;   // sorted slot of each candidate
   int* cand_count;
+  int* apex_slot;   // sorted slots of the particles whose FP64 apex projection is deferred
+  int* apex_count;
   const Clock* clk;
 };
+
+// The deferred FP64 apex projections (after G2P): the same return map on the trial state.
+struct ApexArgs {
+  SceneTable T;
+  float* st;   // the G2P output buffer
+  int64_t cap;
+  int n;
+  const int* apex_slot;
+  const int* apex_count;
+  int write_stress;
+  int log_pairs;
+  float* pair_log;
+  int64_t pair_cap;
+  unsigned long long* pair_count;
+  unsigned long long* fallback_svd;
+  unsigned long long* fallback_apex;
+  const Clock* clk;
+};
+void launch_apex_finish(hipStream_t s, const ApexArgs& a, int grid);
 
 // Candidate planes of the surrogate return map.
 constexpr int CAND_SIGMA = 0, CAND_U = 3, CAND_V = 12, CAND_PLANES = 21;

@@ -1295,19 +1295,28 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       base[d] = (ORDER == 2) ? (int)floorf(g[d] - 0.5f) : (int)floorf(g[d]) - 1;
+      if constexpr (ORDER == 2) {
+        quad_weights(g[d] - (float)base[d], w[d], dw[d]);
+      } else {
 #pragma unroll
-      for (int o = 0; o < ORDER + 1; ++o) spline<ORDER>(g[d] - (float)(base[d] + o), w[d][o], dw[d][o]);
+        for (int o = 0; o < ORDER + 1; ++o) spline<ORDER>(g[d] - (float)(base[d] + o), w[d][o], dw[d][o]);
+      }
     }
+    // the whole stencil inside the grid (the common case): no per-node bounds tests
+    const bool inside = base[0] >= 0 && base[1] >= 0 && base[2] >= 0 && base[0] + ORDER < P.res[0] &&
+                        base[1] + ORDER < P.res[1] && base[2] + ORDER < P.res[2];
     float vp[3] = {0.0f, 0.0f, 0.0f}, vb[3] = {0.0f, 0.0f, 0.0f};
     float Bn[9], Gv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) { Bn[i] = 0.0f; Gv[i] = 0.0f; }
+#pragma unroll
     for (int oz = 0; oz < ORDER + 1; ++oz) {
       const int n2 = base[2] + oz;
-      if (n2 < 0 || n2 >= P.res[2]) continue;
+      if (!inside && (n2 < 0 || n2 >= P.res[2])) continue;
+#pragma unroll
       for (int oy = 0; oy < ORDER + 1; ++oy) {
         const int n1 = base[1] + oy;
-        if (n1 < 0 || n1 >= P.res[1]) continue;
+        if (!inside && (n1 < 0 || n1 >= P.res[1])) continue;
         const float wyz = w[1][oy] * w[2][oz];
         const float dyz = dw[1][oy] * w[2][oz] * P.inv_h[1];
         const float ydz = w[1][oy] * dw[2][oz] * P.inv_h[2];
@@ -1316,7 +1325,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
 #pragma unroll
         for (int ox = 0; ox < ORDER + 1; ++ox) {
           const int n0 = base[0] + ox;
-          if (n0 < 0 || n0 >= P.res[0]) continue;
+          if (!inside && (n0 < 0 || n0 >= P.res[0])) continue;
           const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
           const float4 q0 = gvel[2 * node];
           const float4 q1 = gvel[2 * node + 1];
@@ -1370,6 +1379,8 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     io.pair_cap = a.pair_cap;
     io.pair_count = a.pair_count;
     io.deferred = false;
+    io.defer_apex = a.apex_slot != nullptr;
+    io.apex_deferred = false;
     // particles: return mapping. A cloth element's F planes hold D, of which F^ updates d3
     // (HybridSolver.cpp:599-603); its d1, d2 and return mapping follow in k_cloth_elements.
     if (kind == KIND_PARTICLE) return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
@@ -1394,6 +1405,19 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
             a.cand[(CAND_V + k) * n + c] = io.V[k];
           }
         }
+      }
+    }
+    {
+      // compact the deferred FP64 apex projections of this wave into their list (one atomic per wave)
+      const uint64_t am = __ballot(io.apex_deferred);
+      if (am) {
+        const int lane = (int)lane_id();
+        const int first = __ffsll((long long)am) - 1;
+        int base = 0;
+        if (lane == first) base = atomicAdd(a.apex_count, __popcll(am));
+        base = __shfl(base, first);
+        const int c = base + __popcll(am & ((1ull << lane) - 1ull));
+        if (io.apex_deferred && c < a.n) a.apex_slot[c] = j;
       }
     }
     float xn[3];
@@ -1431,7 +1455,62 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   if (__ballot(flags != 0)) {
     if (flags) atomicOr(a.status, flags);
   }
-  if (fb_svd) atomicAdd(a.fallback_svd, (unsigned long long)fb_svd);
+  // fallback counts: one atomic per wave
+  if (__ballot(fb_svd != 0)) {
+    unsigned long long c = (unsigned long long)fb_svd;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane_id() == 0) atomicAdd(a.fallback_svd, c);
+  }
+  if (__ballot(fb_apex != 0)) {
+    unsigned long long c = (unsigned long long)fb_apex;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane_id() == 0) atomicAdd(a.fallback_apex, c);
+  }
+}
+
+// The FP64 apex projections deferred by k_g2p, gathered into full waves: the particle's trial F^
+// (left in the output buffer), q, phi and V go through return_map again, now with the FP64
+// projection, and F, q and the stress are stored. The result equals computing it inside k_g2p.
+__global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
+  if (clock_skip(a.clk)) return;
+  const int count = min(*a.apex_count, a.n);
+  int fb_svd = 0, fb_apex = 0;
+  for (int c0 = blockIdx.x * 256; c0 < count; c0 += gridDim.x * 256) {
+    const int c = c0 + (int)threadIdx.x;
+    if (c < count) {
+      const int j = clamp_index(a.apex_slot[c], a.n);
+      const int sc = scene_of(a.T.part_off, a.T.nscenes, j);
+      const DevParams& P = a.T.params[sc];
+      float* st = a.st;
+      const int64_t cap = a.cap;
+      float Fh[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fh[i] = st[(PF + i) * cap + j];
+      float plastic = st[PPL * cap + j];
+      const float phi_p = st[PPHI * cap + j];
+      const float vol = st[PVOL * cap + j];
+      float K[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      SurrogateIO io;
+      io.mode = a.log_pairs ? 1 : 0;
+      io.pair_log = a.pair_log;
+      io.pair_cap = a.pair_cap;
+      io.pair_count = a.pair_count;
+      io.deferred = false;
+      io.defer_apex = false;
+      io.apex_deferred = false;
+      return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) st[(PF + i) * cap + j] = Fh[i];
+      st[PPL * cap + j] = plastic;
+      if (a.write_stress) {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) st[(PK + e) * cap + j] = K[e];
+      }
+    }
+  }
+  (void)fb_svd;  // the SVD redos of these particles were counted by k_g2p
   if (fb_apex) atomicAdd(a.fallback_apex, (unsigned long long)fb_apex);
 }
 
@@ -1958,6 +2037,10 @@ void surrogate_set_smem_limit() {
 
 void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid) {
   hipLaunchKernelGGL(k_surrogate_apply, dim3(grid), dim3(256), surrogate_smem(), s, a);
+}
+
+void launch_apex_finish(hipStream_t s, const ApexArgs& a, int grid) {
+  hipLaunchKernelGGL(k_apex_finish, dim3(grid), dim3(256), 0, s, a);
 }
 
 void launch_fill_int(hipStream_t s, int* p, int64_t n, int v) {

@@ -148,6 +148,7 @@ struct mpm_ctx {
   float gate_tol = 1e-5f;
   float* cand = nullptr;
   int* cand_slot = nullptr;
+  int* apex_slot = nullptr;  // sorted slots of the deferred FP64 apex projections (n)
 };
 
 static void drop_mesh(mpm_ctx* ctx);
@@ -496,8 +497,28 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.cand = ctx->cand;
   a.cand_slot = ctx->cand_slot;
   a.cand_count = ctx->counters + CNT_CAND;
+  a.apex_slot = ctx->any_sand ? ctx->apex_slot : nullptr;
+  a.apex_count = ctx->counters + CNT_APEX;
   a.clk = ctx->clk_active;
   launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p);
+  if (a.apex_slot) {
+    ApexArgs x;
+    x.T = ctx->T;
+    x.st = ctx->state[1 - ctx->cur];
+    x.cap = ctx->cap;
+    x.n = ctx->n;
+    x.apex_slot = ctx->apex_slot;
+    x.apex_count = ctx->counters + CNT_APEX;
+    x.write_stress = a.write_stress;
+    x.log_pairs = a.sur_mode == 1;
+    x.pair_log = ctx->pair_log;
+    x.pair_cap = ctx->pair_cap;
+    x.pair_count = a.pair_count;
+    x.fallback_svd = ctx->fallbacks;
+    x.fallback_apex = ctx->fallbacks + 1;
+    x.clk = ctx->clk_active;
+    launch_apex_finish(ctx->stream, x, 2 * ctx->n_cu);
+  }
   if (surrogate) {
     SurrogateArgs sa;
     sa.T = ctx->T;
@@ -641,7 +662,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
   dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
-  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws);
+  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws); dfree(ctx->apex_slot);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   dfree(ctx->clock);
   dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
@@ -834,6 +855,7 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
     if (!rc) rc = dalloc(ctx, ctx->v1, n);
     if (!rc) rc = dalloc(ctx, ctx->v2, n);
     if (!rc) rc = dalloc(ctx, ctx->src, n);
+    if (!rc) rc = dalloc(ctx, ctx->apex_slot, n);
     if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
     if (rc) return rc;

@@ -142,7 +142,7 @@ def surrogate_leg(sc, warmup, steps, x_exact, wall_exact, gate_tol=1e-5, pair_ch
 
 
 def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=10, pairs_per_chunk=400000,
-                epochs=8, batch_size=4096, lr=1e-3):
+                epochs=40, batch_size=4096, lr=1e-3):
     """64 scenes in one context against the same scenes one context at a time (exact path), and the
     batch with the learned return map (trained on the GPU from pairs the exact batch logs, exact
     fallback) against the exact batch after the same substeps."""
