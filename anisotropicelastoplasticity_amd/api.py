@@ -141,7 +141,8 @@ class Solver:
 
     def set_mesh(self, x, faces, v=None, fixed=None, youngs_modulus=200.0, poisson_ratio=0.3, shear_stiffness=1e3,
                  stiffness=1e4, friction_angle=20.0, density=2e3, thickness=0.04, reference_normal_row=False):
-        """Attach a triangle cloth (mpm_set_mesh) after set_particles: x (3, nv), faces (nf, 3)."""
+        """Attach a triangle cloth (mpm_set_mesh) after set_particles, or alone in a context without
+        particles (the reference's main.cpp:77-84): x (3, nv), faces (nf, 3)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         nv = x.shape[1]
         f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 3))
@@ -245,6 +246,11 @@ class Solver:
         """Particle positions as an OBJ point list, one `v x y z` line per particle in caller order
         (the frame files of HybridSolver.cpp:996-1005)."""
         write_obj(path, self.get_particles()["x"])
+
+    def write_mesh_obj(self, path):
+        """The cloth as the reference's mesh frame file (HybridSolver.cpp:1006-1030): `v x y z` per
+        vertex, then `f a b c` per face with 1-based indices."""
+        write_mesh_obj(path, self.get_mesh()["vx"], self._keep_mesh[1])
 
     def save_checkpoint(self, path):
         """Resumable snapshot (numpy .npz, no pickles): particle state in caller order, masses,
@@ -395,3 +401,44 @@ def levelset(kind, a=(0, 0, 0, 0), b=(0, 0, 0, 0)):
         p.a[i] = float(a[i])
         p.b[i] = float(b[i])
     return p
+
+
+def write_mesh_obj(path, x, faces):
+    """`v x y z` per vertex, then `f a b c` (1-based) per face (HybridSolver.cpp:1006-1030)."""
+    x = np.asarray(x)
+    faces = np.asarray(faces).reshape(-1, 3)
+    with open(path, "w") as f:
+        for i in range(x.shape[1]):
+            f.write("v %.6g %.6g %.6g\n" % (x[0, i], x[1, i], x[2, i]))
+        for a, b, c in faces:
+            f.write("f %d %d %d\n" % (a + 1, b + 1, c + 1))
+
+
+def read_mesh_obj(path):
+    """LagrangianMesh::ObjMesh's reader (LagrangianMesh.cpp:198-280): `v` and `f` lines, 1-based
+    indices (`a/b/c` keeps a; negative indices count from the end), polygons split into fans.
+    Returns x (3, nv) float32 and faces (nf, 3) int32."""
+    pos, faces = [], []
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                pos.append([float(t) for t in tok[1:4]])
+            elif tok[0] == "f":
+                idx = []
+                for t in tok[1:]:
+                    k = int(t.split("/")[0])
+                    if k == 0:
+                        raise MPMError("%s:%d: face index 0" % (path, lineno))
+                    idx.append(k - 1 if k > 0 else len(pos) + k)
+                if len(idx) < 3:
+                    raise MPMError("%s:%d: face with < 3 corners" % (path, lineno))
+                for c in range(1, len(idx) - 1):
+                    faces.append([idx[0], idx[c], idx[c + 1]])
+    x = np.asarray(pos, np.float32).reshape(-1, 3).T.copy()
+    fa = np.asarray(faces, np.int32).reshape(-1, 3)
+    if fa.size and (fa.min() < 0 or fa.max() >= x.shape[1]):
+        raise MPMError("%s: face index out of range" % path)
+    return x, fa

@@ -11,8 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpm_hip.so")
 SOURCES = ["mpm_kernels.hip", "mpm_mlp.hip", "mpm_runtime.hip"]
-EXAMPLE = os.path.join(HERE, "sand_column_example")
-EXAMPLE_SRC = os.path.join(CSRC, "host", "sand_column_example.cpp")
+# programs over the C++ host API (include/mpm/*.h), linked against the library
+EXAMPLES = ["sand_column_example", "cloth_example"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -43,14 +43,15 @@ def build(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libmpm_hip.so")
     os.replace(tmp, LIB)
-    # the C++ host API (include/mpm/*.h) exercised by a small program linked against the library
-    cmd = ["g++", "-std=c++17", "-O2", "-I/opt/rocm/include", EXAMPLE_SRC, "-L", HERE, "-lmpm_hip",
-           "-Wl,-rpath,$ORIGIN", "-o", EXAMPLE + ".tmp"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("g++ failed building the host API example")
-    os.replace(EXAMPLE + ".tmp", EXAMPLE)
+    for name in EXAMPLES:
+        exe = os.path.join(HERE, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I/opt/rocm/include", os.path.join(CSRC, "host", name + ".cpp"),
+               "-L", HERE, "-lmpm_hip", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("g++ failed building " + name)
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -943,8 +943,11 @@ int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32
                  const uint8_t* fixed, const mpm_mesh_params* mp) {
   if (!ctx) return MPM_ERR_INVALID;
   if (ctx->batch || ctx->scenes.size() != 1) return fail(ctx, MPM_ERR_UNSUPPORTED, "a mesh needs a single-scene context");
-  if (ctx->n_api <= 0 || ctx->n != ctx->n_api || ctx->substeps != 0 || ctx->keys_valid)
-    return fail(ctx, MPM_ERR_STATE, "mpm_set_mesh follows mpm_set_particles, before the first substep");
+  // a cloth after the particles, before their first substep, or a cloth alone (no particles, the
+  // reference's main.cpp:77-84), which replaces any cloth the context held
+  if (!ctx->have_grid) return fail(ctx, MPM_ERR_STATE, "mpm_set_grid must precede mpm_set_mesh");
+  if (ctx->n_api > 0 && (ctx->n != ctx->n_api || ctx->substeps != 0 || ctx->keys_valid))
+    return fail(ctx, MPM_ERR_STATE, "mpm_set_mesh follows mpm_set_particles (if any), before the first substep");
   if (nv < 3 || nf < 1 || !x || !faces || !mp) return fail(ctx, MPM_ERR_INVALID, "a mesh needs vertices, faces and parameters");
   if (!(mp->poisson_ratio < 0.5f) || !(mp->youngs_modulus > 0.0f) || !(mp->density > 0.0f) || !(mp->thickness > 0.0f))
     return fail(ctx, MPM_ERR_INVALID, "bad cloth material");
@@ -955,7 +958,7 @@ int mpm_set_mesh(mpm_ctx* ctx, int32_t nv, const float* x, const float* v, int32
   CK(hipStreamSynchronize(ctx->stream));
   // the particles' planes (caller order: no substep has run)
   std::vector<float> hp((size_t)NPLANES * np);
-  CK(hipMemcpy(hp.data(), ctx->state[ctx->cur], sizeof(float) * hp.size(), hipMemcpyDeviceToHost));
+  if (np > 0) CK(hipMemcpy(hp.data(), ctx->state[ctx->cur], sizeof(float) * hp.size(), hipMemcpyDeviceToHost));
   std::vector<float> h((size_t)NPLANES * n, 0.0f);
   for (int p = 0; p < NPLANES; ++p) std::memcpy(&h[(size_t)p * n], &hp[(size_t)p * np], sizeof(float) * np);
   // LagrangianMesh::ObjMesh (LagrangianMesh.cpp:300-352): element volume 0.25 area thickness, added

@@ -13,13 +13,12 @@
 #include <vector>
 
 #include "../mpm_c_abi.h"
+#include "LagrangianMesh.h"
 #include "LevelSet.h"
 #include "ParticleSystem.h"
 #include "RegularGrid.h"
 
 namespace mpm {
-
-class LagrangianMesh;  // cloth (LagrangianMesh.h:13-117): not available in this build
 
 class HybridSolver {
  public:
@@ -33,7 +32,9 @@ class HybridSolver {
 
   void setParticleSystem(ParticleSystem* ps) { ps_ = ps; dirty_ = true; }
   void setRegularGrid(RegularGrid* rg) { rg_ = rg; dirty_ = true; }
-  void setLagrangianMesh(LagrangianMesh*) { throw std::runtime_error("cloth (LagrangianMesh) is not built yet"); }
+  // HybridSolver::setLagrangianMesh (HybridSolver.h:88): a cloth alone (no ParticleSystem, as in
+  // main.cpp:77-84) or beside the particles; null detaches it.
+  void setLagrangianMesh(LagrangianMesh* mesh) { mesh_ = mesh; dirty_ = true; }
   void setLevelSet(const LevelSet& ls) { ls_ = ls; dirty_ = true; }
   // The constants the reference hard-codes (HybridSolver.cpp:264-267, 457, 465, 641-644) and the
   // switches the configurations need (kernel order, stress at F^n, friction model, gravity).
@@ -44,11 +45,15 @@ class HybridSolver {
   // (:854, when estimateVolumes), then the adaptive loop Dt = CFL / max(300, max|v_i| / h_min)
   // clamped to 1/60 s frames until t > maxt. After every frame the particle positions go to
   // <outputDirectory>/particle/particle_<frame>.obj as in the reference (:990-1005), if an output
-  // directory is set. `alpha` is unused, as in the reference.
+  // directory is set, and the cloth's vertices and faces to <outputDirectory>/mesh/mesh_<frame>.obj
+  // (:1006-1030) when a mesh is bound. `alpha` is unused, as in the reference.
   void solve(double CFL, double maxt, double /*alpha*/ = 0.0, bool estimateVolumes = true) {
     upload();
     if (estimateVolumes) check(mpm_estimate_volumes(ctx_), "mpm_estimate_volumes");
-    if (!outdir_.empty()) std::filesystem::create_directories(std::filesystem::path(outdir_) / "particle");
+    if (!outdir_.empty()) {
+      std::filesystem::create_directories(std::filesystem::path(outdir_) / "particle");
+      if (mesh_) std::filesystem::create_directories(std::filesystem::path(outdir_) / "mesh");
+    }
     check(mpm_solve(ctx_, CFL, maxt, &HybridSolver::onFrame, this), "mpm_solve");
     syncToHost();
   }
@@ -70,6 +75,7 @@ class HybridSolver {
   // restores it into the bound ParticleSystem and the device, and the following substeps repeat the
   // original run bit for bit.
   void saveCheckpoint(const std::string& path) {
+    if (mesh_) throw std::runtime_error("saveCheckpoint covers particle systems without a cloth");
     syncToHost();
     const size_t n = ps_->size();
     std::vector<float> K(6 * n, 0.0f);
@@ -138,9 +144,19 @@ class HybridSolver {
   void substep(double dt) { upload(); check(mpm_substep(ctx_, float(dt)), "mpm_substep"); }
   void run(double dt, int n) { upload(); check(mpm_run(ctx_, float(dt), n), "mpm_run"); }
 
-  // Copy particle state (and the grid's masses and velocities) back to the host objects.
+  // Copy particle state, the cloth's vertices and element directions, and the grid's masses and
+  // velocities back to the host objects.
   void syncToHost() {
-    if (ps_) {
+    if (mesh_) {
+      const size_t nv = mesh_->vertexCount(), nf = mesh_->faceCount();
+      mesh_->vertexPositions.resize(3 * nv);
+      mesh_->vertexVelocities.resize(3 * nv);
+      mesh_->elementDirections.resize(9 * nf);
+      check(mpm_get_mesh(ctx_, mesh_->vertexPositions.data(), mesh_->vertexVelocities.data(),
+                         mesh_->elementDirections.data()),
+            "mpm_get_mesh");
+    }
+    if (ps_ && ps_->size() > 0) {
       const size_t n = ps_->size();
       ps_->positions.resize(3 * n);
       ps_->velocities.resize(3 * n);
@@ -173,27 +189,44 @@ class HybridSolver {
   }
   void upload() {
     if (!dirty_) return;
-    if (!rg_ || !ps_) throw std::runtime_error("HybridSolver needs a RegularGrid and a ParticleSystem");
+    if (!rg_ || (!ps_ && !mesh_)) throw std::runtime_error("HybridSolver needs a RegularGrid and a ParticleSystem or a LagrangianMesh");
     const double mn[3] = {rg_->minBound()[0], rg_->minBound()[1], rg_->minBound()[2]};
     const double mx[3] = {rg_->maxBound()[0], rg_->maxBound()[1], rg_->maxBound()[2]};
     const int32_t res[3] = {rg_->resolution()[0], rg_->resolution()[1], rg_->resolution()[2]};
     check(mpm_set_grid(ctx_, mn, mx, res), "mpm_set_grid");
     mpm_params p = params_;
-    p.material = ps_->material;
-    p.youngs_modulus = float(ps_->youngsModulus);
-    p.poisson_ratio = float(ps_->poissonRatio);
-    p.critical_compression = float(ps_->criticalCompression);
-    p.critical_stretch = float(ps_->criticalStretch);
-    p.per_particle_phi = ps_->frictionAngles.size() == ps_->size() ? 1 : 0;
+    const bool haveParticles = ps_ && ps_->size() > 0;
+    if (haveParticles) {
+      p.material = ps_->material;
+      p.youngs_modulus = float(ps_->youngsModulus);
+      p.poisson_ratio = float(ps_->poissonRatio);
+      p.critical_compression = float(ps_->criticalCompression);
+      p.critical_stretch = float(ps_->criticalStretch);
+      p.per_particle_phi = ps_->frictionAngles.size() == ps_->size() ? 1 : 0;
+    } else {
+      p.per_particle_phi = 0;
+    }
     check(mpm_set_params(ctx_, &p), "mpm_set_params");
     check(mpm_set_levelset(ctx_, ls_.prims.data(), (int32_t)ls_.prims.size()), "mpm_set_levelset");
-    const int32_t n = (int32_t)ps_->size();
-    check(mpm_set_particles(ctx_, n, ps_->positions.data(), ps_->velocities.data(),
-                            ps_->affineMomenta.empty() ? nullptr : ps_->affineMomenta.data(),
-                            ps_->elasticDeformationGradients.empty() ? nullptr : ps_->elasticDeformationGradients.data(),
-                            ps_->plastic.empty() ? nullptr : ps_->plastic.data(), ps_->masses.data(),
-                            ps_->volumes.data(), p.per_particle_phi ? ps_->frictionAngles.data() : nullptr),
-          "mpm_set_particles");
+    if (haveParticles) {
+      const int32_t n = (int32_t)ps_->size();
+      check(mpm_set_particles(ctx_, n, ps_->positions.data(), ps_->velocities.data(),
+                              ps_->affineMomenta.empty() ? nullptr : ps_->affineMomenta.data(),
+                              ps_->elasticDeformationGradients.empty() ? nullptr : ps_->elasticDeformationGradients.data(),
+                              ps_->plastic.empty() ? nullptr : ps_->plastic.data(), ps_->masses.data(),
+                              ps_->volumes.data(), p.per_particle_phi ? ps_->frictionAngles.data() : nullptr),
+            "mpm_set_particles");
+    }
+    if (mesh_) {
+      mesh_->validate();
+      const mpm_mesh_params mp = mesh_->params();
+      check(mpm_set_mesh(ctx_, (int32_t)mesh_->vertexCount(), mesh_->vertexPositions.data(),
+                         mesh_->vertexVelocities.size() == mesh_->vertexPositions.size() ? mesh_->vertexVelocities.data()
+                                                                                         : nullptr,
+                         (int32_t)mesh_->faceCount(), mesh_->faces.data(),
+                         mesh_->vertexIsFixed.empty() ? nullptr : mesh_->vertexIsFixed.data(), &mp),
+            "mpm_set_mesh");
+    }
     dirty_ = false;
   }
 
@@ -201,11 +234,14 @@ class HybridSolver {
     HybridSolver* self = static_cast<HybridSolver*>(user);
     if (self->outdir_.empty()) return;
     self->syncToHost();
-    const std::string path = self->outdir_ + "/particle/particle_" + std::to_string(frame) + ".obj";
-    std::ofstream out(path);
-    const size_t n = self->ps_->size();
-    const std::vector<float>& x = self->ps_->positions;  // planar: x0[n], x1[n], x2[n]
-    for (size_t p = 0; p < n; ++p) out << "v " << x[p] << ' ' << x[n + p] << ' ' << x[2 * n + p] << '\n';
+    if (self->ps_) {
+      const std::string path = self->outdir_ + "/particle/particle_" + std::to_string(frame) + ".obj";
+      std::ofstream out(path);
+      const size_t n = self->ps_->size();
+      const std::vector<float>& x = self->ps_->positions;  // planar: x0[n], x1[n], x2[n]
+      for (size_t p = 0; p < n; ++p) out << "v " << x[p] << ' ' << x[n + p] << ' ' << x[2 * n + p] << '\n';
+    }
+    if (self->mesh_) self->mesh_->writeObj(self->outdir_ + "/mesh/mesh_" + std::to_string(frame) + ".obj");
     ++self->frames_;
   }
 
@@ -214,6 +250,7 @@ class HybridSolver {
   mpm_ctx* ctx_ = nullptr;
   ParticleSystem* ps_ = nullptr;
   RegularGrid* rg_ = nullptr;
+  LagrangianMesh* mesh_ = nullptr;
   LevelSet ls_;
   mpm_params params_{};
   bool dirty_ = true;

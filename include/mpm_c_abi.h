@@ -212,8 +212,9 @@ typedef struct mpm_mesh_params {
 } mpm_mesh_params;
 
 /* Attach a triangle cloth to a single-scene context after mpm_set_particles and before the first
- * substep (replaces HybridSolver::setLagrangianMesh, HybridSolver.h:88, and the LagrangianMesh
- * constructor, LagrangianMesh.cpp:127-195). x and v are planar 3 x nv (v may be NULL), faces are
+ * substep, or to a context without particles (a cloth alone, as in the reference's main.cpp:77-84;
+ * a later call then replaces that cloth) (replaces HybridSolver::setLagrangianMesh,
+ * HybridSolver.h:88, and the LagrangianMesh constructor, LagrangianMesh.cpp:127-195). x and v are planar 3 x nv (v may be NULL), faces are
  * nf x 3 vertex indices, fixed (nv bytes, may be NULL) marks vertices whose surrounding nodes are
  * stopped (LagrangianMesh::bindConstraints, HybridSolver.cpp:513-550). The mesh's vertices and
  * triangles move with the particles in every substep; mpm_get_particles keeps returning the

@@ -8,6 +8,8 @@ Tolerances (FP32 device state against the FP64 restatement):
   60 substeps, sphere       mesh vertices |dx| <= 1e-4 h, D relative L2 <= 1e-3, with the return map
                             active (some elements changed by it)
   fixed vertices            fixed vertices keep their positions exactly; parity as above
+  cloth alone               no particles in the context: parity as above; a second mpm_set_mesh
+                            replaces the cloth and repeats the run bit for bit
   full size (256^2 cloth, about 500k sand particles, 128^3 grid), 200 substeps: before any contact
                             the cloth falls as a rigid body (v_y = n dt g within 1e-4 relative, no
                             in-plane velocity) and its directions keep their rest values within 1e-4
@@ -17,7 +19,7 @@ import sys
 import numpy as np
 import pytest
 
-from anisotropicelastoplasticity_amd import Solver, _lib, levelset, scenes
+from anisotropicelastoplasticity_amd import Solver, _lib, api, levelset, scenes
 from oracle.bind import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -174,3 +176,60 @@ def test_config3_full_size_free_fall():
     assert np.max(np.abs(m["vv"][[0, 2]])) < 1e-6
     # rigid fall: the element directions keep their rest values
     assert np.max(np.abs(m["D"] - m0["D"])) < 1e-4 * np.max(np.abs(m0["D"]))
+
+
+def test_cloth_alone_60_substeps():
+    """A cloth in a context without particles (the reference's main.cpp:77-84 binds no
+    ParticleSystem): same parity bar as the drape test."""
+    sc, x, v, faces, fixed = cloth_scene(fixed_corners=True, stretch=True)
+    n = 60
+    say(f"[cloth alone] vertices {x.shape[1]}, faces {len(faces)}, substeps {n}")
+    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
+    o.set_mesh(x, faces, v=v, fixed=fixed)
+    o.run(sc.dt, n)
+    mr = o.get_mesh()
+    with Solver() as s:
+        s.set_grid(sc.gmin, sc.gmax, sc.res)
+        s.set_params(sc.params)
+        s.set_levelset(sc.prims)
+        s.set_mesh(x, faces, v=v, fixed=fixed)
+        s.run(sc.dt, n)
+        m = s.get_mesh()
+        st = s.stats()
+        # a second cloth replaces the first in a context without particles
+        s.set_mesh(x, faces, v=v, fixed=fixed)
+        s.run(sc.dt, n)
+        m2 = s.get_mesh()
+    h = 1.0 / sc.res[0]
+    dxm = np.max(np.abs(m["vx"] - mr["vx"]))
+    say(f"  mesh |dx|/h {dxm / h:.3e}, D rel {rel(m['D'], mr['D']):.3e}")
+    assert np.max(np.abs(mr["vx"] - x)) > 10 * h * 1e-3  # it moved
+    assert dxm <= 1e-4 * h
+    assert rel(m["D"], mr["D"]) < 1e-3
+    idx = np.flatnonzero(fixed)
+    assert np.array_equal(m["vx"][:, idx], x[:, idx])
+    for k in ("vx", "vv", "D"):
+        assert np.array_equal(m[k], m2[k]), k
+    assert st["status"] == 0, st
+
+
+def test_cpp_cloth_example(tmp_path):
+    """The C++ host API's cloth pattern (LagrangianMesh::ObjMesh, bindConstraints, a solver with no
+    ParticleSystem, adaptive solve writing mesh/mesh_N.obj)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "anisotropicelastoplasticity_amd",
+                       "cloth_example")
+    say("[c++ host api] cloth_example")
+    res = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    say(res.stdout.strip())
+    assert res.returncode == 0, res.stdout + res.stderr
+    nv, nf = 24 * 24, 2 * 23 * 23
+    for f in range(2):
+        lines = (tmp_path / "mesh" / f"mesh_{f}.obj").read_text().splitlines()
+        assert len(lines) == nv + nf
+        assert all(l.startswith("v ") for l in lines[:nv]) and all(l.startswith("f ") for l in lines[nv:])
+        x, faces = api.read_mesh_obj(str(tmp_path / "mesh" / f"mesh_{f}.obj"))
+        assert x.shape == (3, nv) and faces.shape == (nf, 3) and np.all(np.isfinite(x))
+    assert not (tmp_path / "mesh" / "mesh_2.obj").exists()
+    assert not (tmp_path / "particle" / "particle_0.obj").exists()
