@@ -61,6 +61,7 @@ class SceneDesc(ctypes.Structure):
 
 
 SORT_COUNTING, SORT_RADIX = 0, 1
+SHAPE_BALL, SHAPE_BLOCK, SHAPE_CYLINDER = 0, 1, 2
 P2G_ROWS, P2G_LAYERS = 0, 1
 
 
@@ -147,6 +148,7 @@ SIGNATURES = [
     ("mpm_bench_copy", ctypes.c_int, [_ctx, ctypes.c_int64, ctypes.c_int32, _f]),
     ("mpm_generate_block", ctypes.c_int,
      [ctypes.c_uint64, _d, _d, _i32, ctypes.c_int32, ctypes.c_int64, _f]),
+    ("mpm_generate_shape", ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, _d, ctypes.c_int64, _f]),
 ]
 
 # void (*)(mpm_ctx*, int32_t frame, double t, void* user)

@@ -394,6 +394,21 @@ def generate_block(seed, lo, hi, cells, ppa):
     return x
 
 
+def generate_shape(seed, shape, a, n):
+    """Seeded rejection sampler of the reference's shape factories (mpm_generate_shape): shape is
+    _lib.SHAPE_BALL (a = cx, cy, cz, r), SHAPE_BLOCK (a = min xyz, max xyz, hole radius) or
+    SHAPE_CYLINDER (a = base xyz, r, height, axis). Returns x with shape (3, n)."""
+    lib = _lib.load()
+    n = int(n)
+    x = np.empty((3, n), np.float32)
+    a = [float(v) for v in a]
+    a_ = (ctypes.c_double * 8)(*(a + [0.0] * (8 - len(a))))
+    rc = lib.mpm_generate_shape(ctypes.c_uint64(int(seed)), int(shape), a_, n, _fp(x))
+    if rc != 0:
+        raise MPMError(f"mpm_generate_shape failed ({rc})")
+    return x
+
+
 def levelset(kind, a=(0, 0, 0, 0), b=(0, 0, 0, 0)):
     p = LevelSetPrim()
     p.type = int(kind)

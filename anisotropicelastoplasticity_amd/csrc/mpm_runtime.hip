@@ -1947,4 +1947,41 @@ int mpm_generate_block(uint64_t seed, const double lo[3], const double hi[3], co
   return MPM_OK;
 }
 
+int mpm_generate_shape(uint64_t seed, int32_t shape, const double* a, int64_t n, float* x) {
+  if (!a || !x || n < 1) return MPM_ERR_INVALID;
+  if (shape == MPM_SHAPE_BALL && !(a[3] > 0.0)) return MPM_ERR_INVALID;
+  if (shape == MPM_SHAPE_BLOCK &&
+      (!(a[3] > a[0]) || !(a[4] > a[1]) || !(a[5] > a[2]) || !(a[6] >= 0.0) || !(2.0 * a[6] < a[5] - a[2])))
+    return MPM_ERR_INVALID;
+  if (shape == MPM_SHAPE_CYLINDER && (!(a[3] > 0.0) || !(a[4] > 0.0) || (a[5] != 0.0 && a[5] != 1.0 && a[5] != 2.0)))
+    return MPM_ERR_INVALID;
+  if (shape < MPM_SHAPE_BALL || shape > MPM_SHAPE_CYLINDER) return MPM_ERR_INVALID;
+  const uint64_t s = splitmix64((seed * 0x632BE59BD9B4E019ull) ^ (uint64_t)shape);
+  const int64_t limit = 64 * n + 4096;
+  int64_t i = 0;
+  for (int64_t k = 0; k < limit && i < n; ++k) {
+    double u[3];
+    for (int c = 0; c < 3; ++c) u[c] = (double)(splitmix64(s ^ (uint64_t)(4 * k + c)) >> 40) * (1.0 / 16777216.0);
+    double p[3];
+    if (shape == MPM_SHAPE_BALL) {
+      const double q[3] = {2.0 * u[0] - 1.0, 2.0 * u[1] - 1.0, 2.0 * u[2] - 1.0};
+      if (q[0] * q[0] + q[1] * q[1] + q[2] * q[2] > 1.0) continue;
+      for (int c = 0; c < 3; ++c) p[c] = a[c] + a[3] * q[c];
+    } else if (shape == MPM_SHAPE_BLOCK) {
+      for (int c = 0; c < 3; ++c) p[c] = a[c] + (a[3 + c] - a[c]) * u[c];
+      const double hc[3] = {a[0], a[1], 0.5 * (a[5] + a[2])};
+      const double d2 = (p[0] - hc[0]) * (p[0] - hc[0]) + (p[1] - hc[1]) * (p[1] - hc[1]) + (p[2] - hc[2]) * (p[2] - hc[2]);
+      if (std::sqrt(d2) < a[6]) continue;
+    } else {
+      const double d[2] = {2.0 * u[0] - 1.0, 2.0 * u[1] - 1.0};
+      if (d[0] * d[0] + d[1] * d[1] > 1.0) continue;
+      const int ax = (int)a[5];
+      for (int c = 0, j = 0; c < 3; ++c) p[c] = c == ax ? a[c] + a[4] * u[2] : a[c] + a[3] * d[j++];
+    }
+    for (int c = 0; c < 3; ++c) x[(size_t)c * n + i] = (float)p[c];
+    ++i;
+  }
+  return i == n ? MPM_OK : MPM_ERR_INVALID;
+}
+
 }  // extern "C"

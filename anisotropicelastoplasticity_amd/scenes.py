@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .api import generate_block, levelset
+from .api import generate_block, generate_shape, levelset
 
 
 @dataclass
@@ -51,6 +51,50 @@ def _particles(x, rho, vol_total, v0, material, phi=None):
                 plastic=np.full(n, 1.0 if material == _lib.SNOW else 0.0, np.float32),
                 mass=np.full(n, rho * vp, np.float32), volume=np.full(n, vp, np.float32),
                 phi=phi)
+
+
+def _shape_particles(x, total_mass, total_volume, v0, material):
+    """The reference factories' per-particle mass total_mass / n (ParticleSystem.cpp:149-153);
+    their volumes are placeholders (ones) that the first P2G's density estimate replaces
+    (HybridSolver.cpp:242-249), so here V = total_volume / n until mpm_estimate_volumes."""
+    n = x.shape[1]
+    p = _particles(x, 0.0, total_volume, v0, material)
+    p["mass"] = np.full(n, total_mass / n, np.float32)
+    return p
+
+
+def snow_ball(seed, center, radius, n):
+    """ParticleSystem::SnowBall (ParticleSystem.cpp:119-178): n particles uniform in the ball, at
+    rest, total mass 100 * 3.14 * r^3 (the reference's constant), snow E = 1.4e5, nu = 0.2."""
+    x = generate_shape(seed, _lib.SHAPE_BALL, list(center) + [radius], n)
+    return _shape_particles(x, 100.0 * 3.14 * radius ** 3, 4.0 / 3.0 * np.pi * radius ** 3, (0.0, 0.0, 0.0), _lib.SNOW)
+
+
+def sand_ball(seed, center, radius, n):
+    """ParticleSystem::SandBall (ParticleSystem.cpp:180-238): as snow_ball with total mass
+    1300 * 3.14 * r^3; sand E = 3.537e5, nu = 0.3."""
+    x = generate_shape(seed, _lib.SHAPE_BALL, list(center) + [radius], n)
+    return _shape_particles(x, 1300.0 * 3.14 * radius ** 3, 4.0 / 3.0 * np.pi * radius ** 3, (0.0, 0.0, 0.0), _lib.SAND)
+
+
+def sand_block(seed, bmin, bmax, hole_radius, n):
+    """ParticleSystem::SandBlock (ParticleSystem.cpp:240-323): n particles uniform in the box minus
+    the ball of radius hole_radius around (min x, min y, mid z); total mass
+    1300 * (box volume - 0.25 pi r^3), as the reference computes it."""
+    x = generate_shape(seed, _lib.SHAPE_BLOCK, list(bmin) + list(bmax) + [hole_radius], n)
+    vol = float(np.prod(np.subtract(bmax, bmin))) - 0.25 * np.pi * hole_radius ** 3
+    return _shape_particles(x, 1300.0 * vol, vol, (0.0, 0.0, 0.0), _lib.SAND)
+
+
+def sand_cylinder(seed, base, radius, height, n, axis=2):
+    """ParticleSystem::SandCylinder (ParticleSystem.cpp:325-401): n particles uniform in the
+    cylinder over the disk of radius r at `base`, height along `axis` (z in the reference), initial
+    velocity -1 along the axis, total mass 1300 pi r^2 height."""
+    x = generate_shape(seed, _lib.SHAPE_CYLINDER, list(base) + [radius, height, axis], n)
+    v0 = [0.0, 0.0, 0.0]
+    v0[axis] = -1.0
+    vol = np.pi * radius ** 2 * height
+    return _shape_particles(x, 1300.0 * vol, vol, v0, _lib.SAND)
 
 
 def snow_params(order=_lib.KERNEL_QUADRATIC, gravity=(0.0, -9.8, 0.0)):

@@ -4,6 +4,7 @@
 // of B, kept here as one 9-plane row-major array; plastic holds J_P = det F_P for snow (the only use
 // of plasticDeformationGradients, HybridSolver.cpp:283) and plasticAmount q for sand.
 #pragma once
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <vector>
@@ -65,6 +66,46 @@ class ParticleSystem {
       vol[p] = float(V);
       for (int a = 0; a < 3; ++a) v[a * n + p] = v0[a];
     }
+    return ParticleSystem(v, x, m, vol, material, E, nu);
+  }
+
+  // Seeded equivalents of the reference's shape factories (ParticleSystem.cpp:119-401), drawn by
+  // mpm_generate_shape. Masses are the reference's totals divided by n (its constants kept,
+  // e.g. 100 * 3.14 r^3 for SnowBall); its volumes are placeholders for the first P2G's density
+  // estimate (HybridSolver::solve with estimateVolumes), so here V = shape volume / n until then.
+  static ParticleSystem SnowBall(const double center[3], double radius, int64_t n, uint64_t seed = 1) {
+    const double a[4] = {center[0], center[1], center[2], radius};
+    return Shape(seed, MPM_SHAPE_BALL, a, n, 100.0 * 3.14 * radius * radius * radius,
+                 4.0 / 3.0 * M_PI * radius * radius * radius, -1, MPM_SNOW, 1.4e5, 0.2);
+  }
+  static ParticleSystem SandBall(const double center[3], double radius, int64_t n, uint64_t seed = 1) {
+    const double a[4] = {center[0], center[1], center[2], radius};
+    return Shape(seed, MPM_SHAPE_BALL, a, n, 1300.0 * 3.14 * radius * radius * radius,
+                 4.0 / 3.0 * M_PI * radius * radius * radius, -1, MPM_SAND, 3.537e5, 0.3);
+  }
+  static ParticleSystem SandBlock(const double bmin[3], const double bmax[3], double holeRadius, int64_t n,
+                                  uint64_t seed = 1) {
+    const double a[7] = {bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], holeRadius};
+    const double vol = (bmax[0] - bmin[0]) * (bmax[1] - bmin[1]) * (bmax[2] - bmin[2]) -
+                       0.25 * M_PI * holeRadius * holeRadius * holeRadius;
+    return Shape(seed, MPM_SHAPE_BLOCK, a, n, 1300.0 * vol, vol, -1, MPM_SAND, 3.537e5, 0.3);
+  }
+  // Height along `axis` (2 = z, the reference's); initial velocity -1 along the axis.
+  static ParticleSystem SandCylinder(const double baseCenter[3], double radius, double height, int64_t n,
+                                     uint64_t seed = 1, int axis = 2) {
+    const double a[6] = {baseCenter[0], baseCenter[1], baseCenter[2], radius, height, double(axis)};
+    const double vol = M_PI * radius * radius * height;
+    return Shape(seed, MPM_SHAPE_CYLINDER, a, n, 1300.0 * vol, vol, axis, MPM_SAND, 3.537e5, 0.3);
+  }
+
+ private:
+  static ParticleSystem Shape(uint64_t seed, int shape, const double* a, int64_t n, double totalMass,
+                              double totalVolume, int velocityAxis, int material, double E, double nu) {
+    if (n < 1) throw std::invalid_argument("ParticleSystem: n must be positive");
+    std::vector<float> x(3 * n), v(3 * n, 0.0f), m(n, float(totalMass / double(n))), vol(n, float(totalVolume / double(n)));
+    if (mpm_generate_shape(seed, shape, a, n, x.data()) != MPM_OK) throw std::runtime_error("mpm_generate_shape failed");
+    if (velocityAxis >= 0)
+      for (int64_t p = 0; p < n; ++p) v[velocityAxis * n + p] = -1.0f;
     return ParticleSystem(v, x, m, vol, material, E, nu);
   }
 };

@@ -345,6 +345,25 @@ int mpm_bench_copy(mpm_ctx* ctx, int64_t nbytes, int32_t reps, float* ms);
 int mpm_generate_block(uint64_t seed, const double lo[3], const double hi[3], const int32_t cells[3],
                        int32_t ppa, int64_t n, float* x);
 
+/* Seeded rejection samplers of the reference's shape factories (ParticleSystem.cpp:119-401), which
+ * draw from an unseeded Eigen Random / default_random_engine(time(0)). Candidate k (k = 0, 1, ...)
+ * has the uniforms u_c = (splitmix64(s ^ (4*k + c)) >> 40) * 2^-24, c = 0, 1, 2, with
+ * s = splitmix64(seed*0x632BE59BD9B4E019 ^ shape); the accepted candidates fill x (3 x n planar)
+ * in order. Arithmetic is in double, rounded to float at the end.
+ *   MPM_SHAPE_BALL      a = {cx, cy, cz, r}: p = 2u - 1, accepted if |p|^2 <= 1, x = c + r p
+ *                       (SnowBall :119-178, SandBall :180-238)
+ *   MPM_SHAPE_BLOCK     a = {min x, y, z, max x, y, z, hole radius}: x = min + (max - min) u,
+ *                       rejected if |x - (min x, min y, (min z + max z)/2)| < hole radius; the hole
+ *                       diameter must be below max z - min z (SandBlock :240-323)
+ *   MPM_SHAPE_CYLINDER  a = {base x, y, z, r, height, axis}: d = (2u_0 - 1, 2u_1 - 1) accepted if
+ *                       |d|^2 <= 1; x = base + r d on the two other axes in ascending order and
+ *                       base + height u_2 along `axis` (2 = z as in SandCylinder :325-401)
+ * Fails with MPM_ERR_INVALID on bad arguments or when 64 n + 4096 candidates do not fill n. */
+#define MPM_SHAPE_BALL 0
+#define MPM_SHAPE_BLOCK 1
+#define MPM_SHAPE_CYLINDER 2
+int mpm_generate_shape(uint64_t seed, int32_t shape, const double* a, int64_t n, float* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,8 +1,11 @@
-"""The seeded particle generator (mpm_generate_block, host code) against an independent numpy
-restatement of its specification, and the configuration inputs it produces."""
+"""The seeded particle generators (mpm_generate_block, mpm_generate_shape; host code) against
+independent numpy restatements of their specifications, geometric properties of the shapes
+(uniform density in the ball, the excluded hole, the cylinder's extent), and the configuration
+inputs they produce."""
 import numpy as np
+import pytest
 
-from anisotropicelastoplasticity_amd import generate_block
+from anisotropicelastoplasticity_amd import MPMError, _lib, generate_block, generate_shape
 from anisotropicelastoplasticity_amd import scenes
 
 M64 = (1 << 64) - 1
@@ -64,3 +67,91 @@ def test_config_scenes():
     assert s2.n == 8 * 16 * 8 * 8
     full = 32 * 128 * 32 * 8
     assert full == 1 << 20
+
+
+def reference_shape(seed, shape, a, n):
+    """Independent restatement of mpm_generate_shape's specification (include/mpm_c_abi.h)."""
+    s = splitmix64(((seed * 0x632BE59BD9B4E019) & M64) ^ shape)
+    x = np.empty((3, n), np.float32)
+    i = k = 0
+    while i < n:
+        u = [(splitmix64(s ^ (4 * k + c)) >> 40) * (1.0 / 16777216.0) for c in range(3)]
+        k += 1
+        if shape == _lib.SHAPE_BALL:
+            q = [2.0 * v - 1.0 for v in u]
+            if q[0] * q[0] + q[1] * q[1] + q[2] * q[2] > 1.0:
+                continue
+            p = [a[c] + a[3] * q[c] for c in range(3)]
+        elif shape == _lib.SHAPE_BLOCK:
+            p = [a[c] + (a[3 + c] - a[c]) * u[c] for c in range(3)]
+            hc = (a[0], a[1], 0.5 * (a[5] + a[2]))
+            if np.sqrt(sum((p[c] - hc[c]) ** 2 for c in range(3))) < a[6]:
+                continue
+        else:
+            d = [2.0 * u[0] - 1.0, 2.0 * u[1] - 1.0]
+            if d[0] * d[0] + d[1] * d[1] > 1.0:
+                continue
+            ax = int(a[5])
+            others = [c for c in range(3) if c != ax]
+            p = [0.0] * 3
+            p[ax] = a[ax] + a[4] * u[2]
+            for j, c in enumerate(others):
+                p[c] = a[c] + a[3] * d[j]
+        x[:, i] = np.float32(p)
+        i += 1
+    return x
+
+
+@pytest.mark.parametrize("shape,a", [
+    (_lib.SHAPE_BALL, (0.5, 0.4, 0.3, 0.1)),
+    (_lib.SHAPE_BLOCK, (0.2, 0.1, 0.3, 0.6, 0.5, 0.7, 0.15)),
+    (_lib.SHAPE_CYLINDER, (0.5, 0.1, 0.5, 0.08, 0.3, 1)),
+    (_lib.SHAPE_CYLINDER, (0.5, 0.5, 0.1, 0.08, 0.3, 2)),
+])
+def test_shape_generator_matches_spec(shape, a):
+    got = generate_shape(7, shape, a, 500)
+    assert np.array_equal(got, reference_shape(7, shape, a, 500))
+
+
+def test_shape_generator_geometry():
+    n = 20000
+    c, r = np.array([0.5, 0.4, 0.3]), 0.1
+    x = generate_shape(3, _lib.SHAPE_BALL, tuple(c) + (r,), n).astype(np.float64)
+    d = np.linalg.norm(x - c[:, None], axis=0)
+    assert np.all(d <= r * (1 + 1e-6))
+    # uniform in the ball: P(|x - c| < r/2) = 1/8 and the mean is the centre
+    assert abs(np.mean(d < r / 2) - 0.125) < 0.01
+    assert np.all(np.abs(x.mean(axis=1) - c) < 0.005)
+    lo, hi, hr = np.array([0.2, 0.1, 0.3]), np.array([0.6, 0.5, 0.7]), 0.15
+    x = generate_shape(3, _lib.SHAPE_BLOCK, tuple(lo) + tuple(hi) + (hr,), n).astype(np.float64)
+    assert np.all(x >= lo[:, None] - 1e-7) and np.all(x <= hi[:, None] + 1e-7)
+    hc = np.array([lo[0], lo[1], 0.5 * (lo[2] + hi[2])])
+    assert np.all(np.linalg.norm(x - hc[:, None], axis=0) >= hr * (1 - 1e-6))
+    base = np.array([0.5, 0.1, 0.5])
+    x = generate_shape(3, _lib.SHAPE_CYLINDER, tuple(base) + (0.08, 0.3, 1), n).astype(np.float64)
+    assert np.all(np.hypot(x[0] - 0.5, x[2] - 0.5) <= 0.08 * (1 + 1e-6))
+    assert np.all((x[1] >= 0.1 - 1e-7) & (x[1] <= 0.4 + 1e-7))
+    assert not np.array_equal(generate_shape(3, _lib.SHAPE_BALL, (0, 0, 0, 1), 100),
+                              generate_shape(4, _lib.SHAPE_BALL, (0, 0, 0, 1), 100))
+
+
+@pytest.mark.parametrize("shape,a", [
+    (_lib.SHAPE_BALL, (0, 0, 0, 0.0)),
+    (_lib.SHAPE_BLOCK, (0, 0, 0, 1, 1, 1, 0.5)),  # hole diameter = block depth (the reference exits)
+    (_lib.SHAPE_CYLINDER, (0, 0, 0, 0.1, 0.2, 3)),
+    (7, (0, 0, 0, 1)),
+])
+def test_shape_generator_rejects_bad_arguments(shape, a):
+    with pytest.raises(MPMError):
+        generate_shape(1, shape, a, 10)
+
+
+def test_shape_factories_mass_and_material():
+    p = scenes.snow_ball(1, (0.5, 0.5, 0.5), 0.1, 1000)
+    assert np.isclose(p["mass"].sum(dtype=np.float64), 100 * 3.14 * 1e-3, rtol=1e-5)
+    assert np.all(p["plastic"] == 1.0)
+    p = scenes.sand_block(1, (0.2, 0.1, 0.3), (0.6, 0.5, 0.7), 0.15, 1000)
+    assert np.isclose(p["mass"].sum(dtype=np.float64), 1300 * (0.4 ** 3 - 0.25 * np.pi * 0.15 ** 3), rtol=1e-5)
+    p = scenes.sand_cylinder(1, (0.5, 0.1, 0.5), 0.08, 0.3, 1000, axis=1)
+    assert np.all(p["v"][1] == -1.0) and np.all(p["v"][[0, 2]] == 0.0)
+    assert np.all(p["plastic"] == 0.0)
