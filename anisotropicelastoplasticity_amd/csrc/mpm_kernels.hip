@@ -1,20 +1,25 @@
 // mpm_kernels.hip -- gfx950 kernels of the explicit MPM substep and their host launchers.
 //
-// Data layout in HBM (one context = one scene):
+// Data layout in HBM (a context holds one scene, or several scenes side by side):
 //   particles   two ping-pong buffers of NPLANES float planes of `cap` floats each (SoA):
-//               x(3) v(3) B(9) F(9) plastic m V phi id(int bits). After the first substep the
-//               planes are kept physically sorted by cell key, so every particle pass streams them.
+//               x(3) v(3) B(9) F(9) plastic m V phi id(int bits) V*tau(6). After the first substep
+//               the planes are kept physically sorted by cell key, so every particle pass streams
+//               them. A cloth's vertices and elements are particles of the same buffers.
 //   keys        uint32 cell key per particle in the current physical order (written by G2P).
-//   sort        K1/K2 keys and V1/V2 index ping-pong buffers, per-pass lookback status words.
-//   grid        dense planes m, mom(3), v(3), vbf(3) over res0*res1*res2 nodes; only the 4x4x4 node
-//               blocks touched by particles are written each substep.
-//   cells       cell_start/cell_end per cell (reset sparsely from the previous substep's keys).
+//   sort        counting sort: per-cell counts (zero between substeps), cell_start over all cells
+//               plus one, scan status words and tickets; LSD radix: K1/K2 keys and V1/V2 index
+//               ping-pong buffers and per-pass look-back status words.
+//   grid        dense planes m, mom(3) and interleaved float4 pairs (v, v_BF) over the nodes; only
+//               the 4x4x4 node blocks around active tiles are written each substep.
 //   tiles       4x4x4-cell tiles: slot per tile (-1 = inactive), active list, per-slot P2G partials
 //               over the tile's 7x7x7 node footprint.
 //
-// Substep (stress at F^n): reset -> hist -> 3 radix passes -> segments -> p2g -> grid -> g2p.
+// Substep (stress at F^n): reset -> cell count -> cell scan -> cell scatter -> cell rank (bins,
+// tile flags) -> claim tiles -> claim node blocks -> p2g -> grid -> g2p -> apex finish.
 // Reference-compatible substep (stress at predicted F^): p2g(momentum) -> grid(normalise) ->
-// p2g(force, gathers grad v) -> grid(force update) replace the p2g -> grid pair.
+// p2g(force, gathers grad v) -> grid(force update) replace the p2g -> grid pair. A cloth adds its
+// element and vertex forces before P2G, the fixed-vertex node stops after the grid update, and the
+// element update (directions, QR return map) after G2P.
 #include <hip/hip_runtime.h>
 
 #include "mpm_device.hpp"
