@@ -22,6 +22,10 @@ struct DevParams {
   float theta_c, theta_s, xi;
   float h0, h1, h2, h3;  // DP hardening, degrees
   int per_particle_phi;
+  // alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) when phi is the same for every particle and every q
+  // (h1 = h3 = 0 and no per-particle angle), evaluated once in double on the host; -1 otherwise
+  float alpha_f;
+  double alpha_d;
   float friction;
   float damping;
   float svd_ratio;
@@ -252,32 +256,64 @@ __device__ __forceinline__ void stress_matrix(const DevParams& P, const float* F
       out[3 * r + c] = vol * (Pk[3 * r] * G[3 * c] + Pk[3 * r + 1] * G[3 * c + 1] + Pk[3 * r + 2] * G[3 * c + 2]);
 }
 
-// Drucker-Prager projection of principal stretches (HybridSolver.cpp:634-673).
+// Drucker-Prager projection of principal stretches (HybridSolver.cpp:634-673). alpha_c >= 0 is the
+// scene's constant alpha (phi independent of q and of the particle); otherwise alpha follows from
+// phi(q) = phi0 + (h1 q - h3) exp(-h2 q). lnS receives ln Sin (the trial Hencky strains) and lnSout
+// ln Sout: the trial strains (elastic), 0 (tip) or H = eps - dgamma dev(eps)/|dev(eps)| (cone).
 template <class T>
-__device__ __forceinline__ void dp_project(T lambda, T mu, T phi0, T h1, T h2, T h3, T q, const T* Sin, T* Sout, T& dq) {
+__device__ __forceinline__ void dp_project(T lambda, T mu, T phi0, T h1, T h2, T h3, T q, T alpha_c, const T* Sin,
+                                           T* Sout, T* lnS, T* lnSout, T& dq) {
 #pragma clang fp contract(off)
-  const T pi = T(3.14159265358979323846);
-  T fa = (phi0 + (h1 * q - h3) * exp(-h2 * q)) * pi / T(180);
-  T sf = sin(fa);
-  T alpha = sqrt(T(2) / T(3)) * T(2) * sf / (T(3) - sf);
+  T alpha;
+  if (alpha_c >= T(0)) {
+    alpha = alpha_c;
+  } else {
+    const T pi = T(3.14159265358979323846);
+    T fa = (phi0 + (h1 * q - h3) * exp(-h2 * q)) * pi / T(180);
+    T sf = sin(fa);
+    alpha = sqrt(T(2) / T(3)) * T(2) * sf / (T(3) - sf);
+  }
   T l0 = log(Sin[0]), l1 = log(Sin[1]), l2 = log(Sin[2]);
+  lnS[0] = l0; lnS[1] = l1; lnS[2] = l2;
   T tr = l0 + l1 + l2;
   T e0 = l0 - tr / T(3), e1 = l1 - tr / T(3), e2 = l2 - tr / T(3);
   T en = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
   T dg = en + (T(3) * lambda + T(2) * mu) / T(2) / mu * tr * alpha;
   dq = T(0);
   Sout[0] = Sin[0]; Sout[1] = Sin[1]; Sout[2] = Sin[2];
+  lnSout[0] = l0; lnSout[1] = l1; lnSout[2] = l2;
   if (dg <= T(0)) {
   } else if (en == T(0) || tr > T(0)) {
     Sout[0] = Sout[1] = Sout[2] = T(1);
+    lnSout[0] = lnSout[1] = lnSout[2] = T(0);
     dq = sqrt(l0 * l0 + l1 * l1 + l2 * l2);
   } else {
     T k = dg / en;
-    Sout[0] = exp(l0 - k * e0);
-    Sout[1] = exp(l1 - k * e1);
-    Sout[2] = exp(l2 - k * e2);
+    lnSout[0] = l0 - k * e0;
+    lnSout[1] = l1 - k * e1;
+    lnSout[2] = l2 - k * e2;
+    Sout[0] = exp(lnSout[0]);
+    Sout[1] = exp(lnSout[1]);
+    Sout[2] = exp(lnSout[2]);
     dq = dg;
   }
+}
+
+// K = U diag(d) U^T as its six entries (xx, yy, zz, xy, xz, yz).
+__device__ __forceinline__ void kirchhoff_assemble(const float* U, const float* d, float* K) {
+#pragma clang fp contract(off)
+  const int R[6] = {0, 1, 2, 0, 0, 1}, C[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int e = 0; e < 6; ++e)
+    K[e] = U[3 * R[e]] * d[0] * U[3 * C[e]] + U[3 * R[e] + 1] * d[1] * U[3 * C[e] + 1] + U[3 * R[e] + 2] * d[2] * U[3 * C[e] + 2];
+}
+
+// Sand: V tau = V U diag(2 mu ln s_i + lambda sum ln s) U^T from the Hencky strains ln s_i.
+__device__ __forceinline__ void kirchhoff_sand_ln(const DevParams& P, const float* U, const float* l, float vol, float* K) {
+#pragma clang fp contract(off)
+  const float tr = P.lambda0 * (l[0] + l[1] + l[2]);
+  const float d[3] = {vol * (2.0f * P.mu0 * l[0] + tr), vol * (2.0f * P.mu0 * l[1] + tr), vol * (2.0f * P.mu0 * l[2] + tr)};
+  kirchhoff_assemble(U, d, K);
 }
 
 // Volume-weighted Kirchhoff stress K = V_p P(F) F^T of F = U diag(S) V^T, stored as the six entries
@@ -296,16 +332,11 @@ __device__ __forceinline__ void kirchhoff(const DevParams& P, const float* U, co
 #pragma unroll
     for (int k = 0; k < 3; ++k) d[k] = vol * (2.0f * mu * (S[k] * S[k] - S[k]) + iso);
   } else {
-    float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
-    float tr = P.lambda0 * (l0 + l1 + l2);
-    d[0] = vol * (2.0f * P.mu0 * l0 + tr);
-    d[1] = vol * (2.0f * P.mu0 * l1 + tr);
-    d[2] = vol * (2.0f * P.mu0 * l2 + tr);
+    const float l[3] = {logf(S[0]), logf(S[1]), logf(S[2])};
+    kirchhoff_sand_ln(P, U, l, vol, K);
+    return;
   }
-  const int R[6] = {0, 1, 2, 0, 0, 1}, C[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int e = 0; e < 6; ++e)
-    K[e] = U[3 * R[e]] * d[0] * U[3 * C[e]] + U[3 * R[e] + 1] * d[1] * U[3 * C[e] + 1] + U[3 * R[e] + 2] * d[2] * U[3 * C[e] + 2];
+  kirchhoff_assemble(U, d, K);
 }
 
 // K of a given F (used once after the caller sets the state; afterwards G2P produces K).
@@ -315,18 +346,18 @@ __device__ __forceinline__ void kirchhoff_of_F(const DevParams& P, const float* 
   kirchhoff(P, U, S, mat_det(F), Jp, vol, K);
 }
 
-// The sand return map after the SVD, as in return_map (used by the surrogate kernel's fallback so that
-// a rejected prediction gives bitwise the exact result): Sn, dq from S, then F_E and V tau.
-__device__ __forceinline__ void sand_finish(const DevParams& P, const float* U, const float* S, const float* V,
-                                            const float* Sn, float dq, float& plastic, float vol, float* Fe, float* K) {
+// The sand return map after the SVD and the projection, as in return_map (used by the surrogate
+// kernel's fallback so that a rejected prediction gives bitwise the exact result): F_E = U Sn V^T and
+// V tau from the projected Hencky strains ln Sn.
+__device__ __forceinline__ void sand_finish(const DevParams& P, const float* U, const float* V, const float* Sn,
+                                            const float* lnSn, float dq, float& plastic, float vol, float* Fe, float* K) {
 #pragma clang fp contract(off)
-  const float sgn = mat_det(U) * mat_det(V) < 0.0f ? -1.0f : 1.0f;
   plastic += dq;
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) Fe[3 * r + c] = U[3 * r] * Sn[0] * V[3 * c] + U[3 * r + 1] * Sn[1] * V[3 * c + 1] + U[3 * r + 2] * Sn[2] * V[3 * c + 2];
-  kirchhoff(P, U, Sn, sgn * Sn[0] * Sn[1] * Sn[2], plastic, vol, K);
+  kirchhoff_sand_ln(P, U, lnSn, vol, K);
 }
 
 // Surrogate hooks of the return map: pair logging (exact path) or deferral of cone-branch
@@ -356,8 +387,9 @@ __device__ __forceinline__ float bulk_over_shear(const DevParams& P) { return P.
 
 // FP32 Drucker-Prager projection as called by the return map; the surrogate's exact fallback calls
 // the same function on the same operands (no reassociation without fast-math), giving the same bits.
-__device__ __forceinline__ void sand_project(const DevParams& P, float phi0, float q, const float* S, float* Sn, float& dq) {
-  dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, q, S, Sn, dq);
+__device__ __forceinline__ void sand_project(const DevParams& P, float phi0, float q, const float* S, float* Sn,
+                                             float* lnS, float* lnSn, float& dq) {
+  dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, q, P.alpha_f, S, Sn, lnS, lnSn, dq);
 }
 
 // Return mapping of one particle (HybridSolver.cpp:616-681). Fh is replaced by the elastic part
@@ -398,40 +430,42 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
     plastic = plastic * ((S[0] / Sn[0]) * (S[1] / Sn[1]) * (S[2] / Sn[2]));
   } else {
     float phi0 = P.per_particle_phi ? phi_p : P.h0;
-    float dq;
-    sand_project(P, phi0, plastic, S, Sn, dq);
+    float dq, l[3], lnSn[3];
+    sand_project(P, phi0, plastic, S, Sn, l, lnSn, dq);
     if (dq > 0.0f) {
       // a projection onto the cone happened (the tip branch, tr > 0, gives Sigma = 1 exactly);
       // close to the cone apex redo it in FP64
-      float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
-      if (!(l0 + l1 + l2 > 0.0f) && l0 * l0 + l1 * l1 + l2 * l2 < P.apex_radius * P.apex_radius) {
+      if (!(l[0] + l[1] + l[2] > 0.0f) && l[0] * l[0] + l[1] * l[1] + l[2] * l[2] < P.apex_radius * P.apex_radius) {
         if (io.defer_apex) {  // F^, q and the stress are left as they are for the apex kernel
           io.apex_deferred = true;
           return;
         }
-        double Sd[3] = {S[0], S[1], S[2]}, So[3], dqd;
-        dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, Sd, So, dqd);
-        Sn[0] = float(So[0]); Sn[1] = float(So[1]); Sn[2] = float(So[2]);
+        double Sd[3] = {S[0], S[1], S[2]}, So[3], lSd[3], lSo[3], dqd;
+        dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, P.alpha_d, Sd, So, lSd, lSo, dqd);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          Sn[k] = float(So[k]);
+          lnSn[k] = float(lSo[k]);
+        }
         dq = float(dqd);
         fb_apex += 1;
       }
     }
     if (io.mode == 1 && dq > 0.0f) {
       // log (trial, projected) cone pairs: the training data of the surrogate
-      float l0 = logf(S[0]), l1 = logf(S[1]), l2 = logf(S[2]);
-      const float tr = l0 + l1 + l2;
+      const float tr = l[0] + l[1] + l[2];
       if (tr <= 0.0f && Sn[0] != 1.0f) {
         const unsigned long long idx = atomicAdd(io.pair_count, 1ull);
         if ((int64_t)idx < io.pair_cap) {
           float* rec = io.pair_log + 8 * idx;
-          rec[0] = l0; rec[1] = l1; rec[2] = l2;
+          rec[0] = l[0]; rec[1] = l[1]; rec[2] = l[2];
           rec[3] = dp_phi(P, phi_p, plastic);
           rec[4] = bulk_over_shear(P);
-          rec[5] = logf(Sn[0]); rec[6] = logf(Sn[1]); rec[7] = logf(Sn[2]);
+          rec[5] = lnSn[0]; rec[6] = lnSn[1]; rec[7] = lnSn[2];
         }
       }
     }
-    sand_finish(P, U, S, V, Sn, dq, plastic, vol, Fh, K);
+    sand_finish(P, U, V, Sn, lnSn, dq, plastic, vol, Fh, K);
     return;
   }
 #pragma unroll

@@ -35,6 +35,62 @@ __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 // produce wrong values, never an out-of-bounds access.
 __device__ __forceinline__ int clamp_index(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
+// wave-uniform float held in a scalar register (the readfirstlane builtin works on 32-bit integers)
+__device__ __forceinline__ float uniform_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
+// The scene parameters of a workgroup-uniform scene, read once and held in scalar registers for the
+// whole kernel (the table lives in global memory, which the compiler reads with a vector load at
+// every use otherwise). The level-set primitives are not copied: kernels that use this copy do not
+// evaluate level sets.
+__device__ __forceinline__ DevParams uniform_params(const DevParams& G) {
+  DevParams P;
+  const auto ui = [](int x) { return __builtin_amdgcn_readfirstlane(x); };
+  P.order = ui(G.order);
+  P.material = ui(G.material);
+  P.predicted = ui(G.predicted);
+  P.friction_model = ui(G.friction_model);
+  P.lambda0 = uniform_f(G.lambda0);
+  P.mu0 = uniform_f(G.mu0);
+  P.theta_c = uniform_f(G.theta_c);
+  P.theta_s = uniform_f(G.theta_s);
+  P.xi = uniform_f(G.xi);
+  P.h0 = uniform_f(G.h0);
+  P.h1 = uniform_f(G.h1);
+  P.h2 = uniform_f(G.h2);
+  P.h3 = uniform_f(G.h3);
+  P.per_particle_phi = ui(G.per_particle_phi);
+  P.alpha_f = uniform_f(G.alpha_f);
+  const long long ad = __builtin_bit_cast(long long, G.alpha_d);
+  P.alpha_d = __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)ui((int)(ad >> 32)) << 32) |
+                                                    (unsigned)ui((int)ad)));
+  P.friction = uniform_f(G.friction);
+  P.damping = uniform_f(G.damping);
+  P.svd_ratio = uniform_f(G.svd_ratio);
+  P.apex_radius = uniform_f(G.apex_radius);
+  P.dinv = uniform_f(G.dinv);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    P.gravity[d] = uniform_f(G.gravity[d]);
+    P.gmin[d] = uniform_f(G.gmin[d]);
+    P.h[d] = uniform_f(G.h[d]);
+    P.inv_h[d] = uniform_f(G.inv_h[d]);
+    P.res[d] = ui(G.res[d]);
+    P.tiles[d] = ui(G.tiles[d]);
+  }
+  P.nls = 0;
+  P.mesh_v0 = ui(G.mesh_v0);
+  P.mesh_e0 = ui(G.mesh_e0);
+  P.mesh_e1 = ui(G.mesh_e1);
+  P.c_mu = uniform_f(G.c_mu);
+  P.c_lambda = uniform_f(G.c_lambda);
+  P.c_gamma = uniform_f(G.c_gamma);
+  P.c_k = uniform_f(G.c_k);
+  P.c_cf = uniform_f(G.c_cf);
+  return P;
+}
+
 // Exact cell index along one axis: the number of thresholds thr[c] <= x, where thr[c] is the
 // smallest float f with (double(f) - min)/h >= c. This equals static_cast<int>((x - min)/h) of the
 // reference evaluated in double (HybridSolver.cpp:33-36) for x >= min, using FP32 compares only.
@@ -46,6 +102,18 @@ __device__ __forceinline__ int axis_cell(const float* thr, int res, float x, flo
   while (c > 0 && x < thr[c]) --c;
   while (c < res - 1 && x >= thr[c + 1]) ++c;
   return c;
+}
+
+// Lowest stencil node along one axis of a particle at grid coordinate g: floor(g - 0.5) for the
+// quadratic kernel, floor(g) - 1 for the cubic one. The floor is clamped to [-8, res + 8] before the
+// conversion, so that a non-finite or far out-of-grid position (an unstable run) gives a stencil whose
+// nodes all lie outside the grid instead of an integer overflow: every index formed from it as
+// base + o (o <= 3) stays far inside the int range, and every node is then dropped as out of grid.
+template <int ORDER>
+__device__ __forceinline__ int stencil_base(float g, int res) {
+  float b = ORDER == 2 ? floorf(g - 0.5f) : floorf(g) - 1.0f;
+  b = fminf(fmaxf(b, -8.0f), (float)res + 8.0f);  // NaN gives -8
+  return (int)b;
 }
 
 __device__ __forceinline__ uint32_t cell_key(const DevParams& P, const float* thr, float x0, float x1, float x2, int& oob) {
@@ -598,10 +666,6 @@ struct P2GScene {
   int mesh_v0, mesh_e0, mesh_e1;
 };
 
-// wave-uniform float held in a scalar register (the readfirstlane builtin works on 32-bit integers)
-__device__ __forceinline__ float uniform_f(float x) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
-}
 
 __device__ __forceinline__ P2GScene p2g_scene(const DevParams& P) {
   P2GScene S;
@@ -1279,7 +1343,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   if (clock_skip(a.clk)) return;
   const float dt = clock_dt(a.clk, a.dt);
   const PBlock pb = particle_block(a.T);
-  const DevParams& P = a.T.params[pb.s];
+  const DevParams P = uniform_params(a.T.params[pb.s]);
   const int64_t noff = a.T.node_off[pb.s];
   const float4* gvel = a.gvel + 2 * noff;
   const int j = pb.j;
@@ -1295,11 +1359,19 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
       xp[d] = st[(PX + d) * cap + pi];
       g[d] = (xp[d] - P.gmin[d]) * P.inv_h[d];
     }
+    // the rest of the particle's state is loaded now, so that it arrives during the gather
+    float Fn[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+    float plastic = st[PPL * cap + pi];
+    const float phi_p = st[PPHI * cap + pi];
+    const float vol = st[PVOL * cap + pi];
+    const float mass = st[PM * cap + pi];
     int base[3];
     float w[3][4], dw[3][4];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      base[d] = (ORDER == 2) ? (int)floorf(g[d] - 0.5f) : (int)floorf(g[d]) - 1;
+      base[d] = stencil_base<ORDER>(g[d], P.res[d]);
       if constexpr (ORDER == 2) {
         quad_weights(g[d] - (float)base[d], w[d], dw[d]);
       } else {
@@ -1307,49 +1379,97 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
         for (int o = 0; o < ORDER + 1; ++o) spline<ORDER>(g[d] - (float)(base[d] + o), w[d][o], dw[d][o]);
       }
     }
-    // the whole stencil inside the grid (the common case): no per-node bounds tests
-    const bool inside = base[0] >= 0 && base[1] >= 0 && base[2] >= 0 && base[0] + ORDER < P.res[0] &&
-                        base[1] + ORDER < P.res[1] && base[2] + ORDER < P.res[2];
+    // Out-of-grid nodes are dropped (HybridSolver.cpp:44-47): they get zero weight and an in-range
+    // (clamped) index, so every load stays inside the stencil's active node blocks.
+    int nid[3][ORDER + 1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int o = 0; o < ORDER + 1; ++o) {
+        const int nn = base[d] + o;
+        const bool in = nn >= 0 && nn < P.res[d];
+        w[d][o] = in ? w[d][o] : 0.0f;
+        dw[d][o] = in ? dw[d][o] : 0.0f;
+        nid[d][o] = nn < 0 ? 0 : (nn >= P.res[d] ? P.res[d] - 1 : nn);
+      }
+    // The stencil sums factor over the axes: along each x row of nodes (fixed y, z) the row sums
+    //   A = sum wx v, Ax = sum wx (x_i - x_p) v, C = sum wx v_BF, Cd = sum wx' v_BF
+    // are formed first, then weighted by the row's y and z factors:
+    //   v_p += wyz A, B[:,0] += wyz Ax, B[:,1] += wyz dy A, B[:,2] += wyz dz A,
+    //   sum w v_BF += wyz C, grad v_BF[:,0] += wyz/h Cd, [:,1] += wy' wz/h C, [:,2] += wy wz'/h C.
+    float wdx[ORDER + 1];
+#pragma unroll
+    for (int ox = 0; ox < ORDER + 1; ++ox) wdx[ox] = w[0][ox] * (((float)(base[0] + ox) - g[0]) * P.h[0]);
     float vp[3] = {0.0f, 0.0f, 0.0f}, vb[3] = {0.0f, 0.0f, 0.0f};
     float Bn[9], Gv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) { Bn[i] = 0.0f; Gv[i] = 0.0f; }
-#pragma unroll
+    const int64_t res0 = P.res[0], res01 = (int64_t)P.res[0] * P.res[1];
+    // one z plane of rows per iteration (not unrolled): its loads are in flight together without
+    // holding the whole stencil in registers
+#pragma unroll 1
     for (int oz = 0; oz < ORDER + 1; ++oz) {
-      const int n2 = base[2] + oz;
-      if (!inside && (n2 < 0 || n2 >= P.res[2])) continue;
+      float wz = w[2][0], dwz = dw[2][0];
+      int nz = nid[2][0];
+#pragma unroll
+      for (int o = 1; o < ORDER + 1; ++o) {
+        wz = oz == o ? w[2][o] : wz;
+        dwz = oz == o ? dw[2][o] : dwz;
+        nz = oz == o ? nid[2][o] : nz;
+      }
+      const float dz = ((float)(base[2] + oz) - g[2]) * P.h[2];
+      const float4* pp = gvel + 2 * (nz * res01);
+      // the plane's (ORDER+1)^2 nodes are loaded together and waited for once (v and v_BF of a
+      // node are its two float4, of which the second holds v_BF y, z in x, y)
+      float4 qa[ORDER + 1][ORDER + 1];
+      float2 qb[ORDER + 1][ORDER + 1];
 #pragma unroll
       for (int oy = 0; oy < ORDER + 1; ++oy) {
-        const int n1 = base[1] + oy;
-        if (!inside && (n1 < 0 || n1 >= P.res[1])) continue;
-        const float wyz = w[1][oy] * w[2][oz];
-        const float dyz = dw[1][oy] * w[2][oz] * P.inv_h[1];
-        const float ydz = w[1][oy] * dw[2][oz] * P.inv_h[2];
-        const float dy = ((float)n1 - g[1]) * P.h[1];
-        const float dz = ((float)n2 - g[2]) * P.h[2];
+        const float4* rp = pp + 2 * (nid[1][oy] * res0);
 #pragma unroll
         for (int ox = 0; ox < ORDER + 1; ++ox) {
-          const int n0 = base[0] + ox;
-          if (!inside && (n0 < 0 || n0 >= P.res[0])) continue;
-          const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
-          const float4 q0 = gvel[2 * node];
-          const float4 q1 = gvel[2 * node + 1];
+          qa[oy][ox] = rp[2 * nid[0][ox]];
+          qb[oy][ox] = *reinterpret_cast<const float2*>(rp + 2 * nid[0][ox] + 1);
+        }
+      }
+      // keep the plane's loads ahead of their uses: issue all of them, then the arithmetic
+      __builtin_amdgcn_sched_group_barrier(0x0020, 2 * (ORDER + 1) * (ORDER + 1), 0);
+      __builtin_amdgcn_sched_group_barrier(0x0002, 1000, 0);
+#pragma unroll
+      for (int oy = 0; oy < ORDER + 1; ++oy) {
+        const float dy = ((float)(base[1] + oy) - g[1]) * P.h[1];
+        float A[3], Ax[3], C[3], Cd[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) { A[r] = 0.0f; Ax[r] = 0.0f; C[r] = 0.0f; Cd[r] = 0.0f; }
+#pragma unroll
+        for (int ox = 0; ox < ORDER + 1; ++ox) {
+          const float4 q0 = qa[oy][ox];
+          const float2 q1 = qb[oy][ox];
           const float vi[3] = {q0.x, q0.y, q0.z};
           const float bi[3] = {q0.w, q1.x, q1.y};
-          const float wt = w[0][ox] * wyz;
-          const float gw[3] = {dw[0][ox] * wyz * P.inv_h[0], w[0][ox] * dyz, w[0][ox] * ydz};
-          const float dxn[3] = {((float)n0 - g[0]) * P.h[0], dy, dz};
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            const float wv = wt * vi[r];
-            vp[r] += wv;
-            vb[r] += wt * bi[r];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              Bn[3 * r + c] += wv * dxn[c];
-              Gv[3 * r + c] += bi[r] * gw[c];
-            }
+            A[r] += w[0][ox] * vi[r];
+            Ax[r] += wdx[ox] * vi[r];
+            C[r] += w[0][ox] * bi[r];
+            Cd[r] += dw[0][ox] * bi[r];
           }
+        }
+        const float wyz = w[1][oy] * wz;
+        const float wyzy = wyz * dy, wyzz = wyz * dz;
+        const float gx = wyz * P.inv_h[0];
+        const float gy = dw[1][oy] * wz * P.inv_h[1];
+        const float gz = w[1][oy] * dwz * P.inv_h[2];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          vp[r] += wyz * A[r];
+          Bn[3 * r] += wyz * Ax[r];
+          Bn[3 * r + 1] += wyzy * A[r];
+          Bn[3 * r + 2] += wyzz * A[r];
+          vb[r] += wyz * C[r];
+          Gv[3 * r] += gx * Cd[r];
+          Gv[3 * r + 1] += gy * C[r];
+          Gv[3 * r + 2] += gz * C[r];
         }
       }
     }
@@ -1368,15 +1488,10 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
           Bn[3 * r + c] = (C[3 * r + c] - sym) + (1.0f - damping) * sym;
         }
     }
-    float Fn[9], Fh[9], GF[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+    float Fh[9], GF[9];
     mat_mul(Gv, Fn, GF);
 #pragma unroll
     for (int i = 0; i < 9; ++i) Fh[i] = Fn[i] + dt * GF[i];
-    float plastic = st[PPL * cap + pi];
-    const float phi_p = st[PPHI * cap + pi];
-    const float vol = st[PVOL * cap + pi];
     float K[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     SurrogateIO io;
     io.mode = a.sur_mode;
@@ -1447,7 +1562,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
       for (int e = 0; e < 6; ++e) so[(PK + e) * cap + j] = K[e];
     }
     so[PPL * cap + j] = plastic;
-    so[PM * cap + j] = st[PM * cap + pi];
+    so[PM * cap + j] = mass;
     so[PVOL * cap + j] = vol;
     so[PPHI * cap + j] = phi_p;
     reinterpret_cast<int*>(so + (size_t)PID * cap)[j] = pid;
@@ -1601,7 +1716,7 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
       const float tol = a.gate_tol;
       accepted = fabsf(nH + c_yield * alpha * trH) <= tol && fabsf(trH - trE) <= tol && along > 0.0f &&
                  sqrtf(ortho2) <= tol;
-      float U[9], V[9], Sn[3], dq;
+      float U[9], V[9], Sn[3], lnSn[3], dq;
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
         U[k] = a.cand[(CAND_U + k) * n + row];
@@ -1609,15 +1724,19 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
       }
       if (accepted) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) Sn[k] = expf(H[k]);
+        for (int k = 0; k < 3; ++k) {
+          Sn[k] = expf(H[k]);
+          lnSn[k] = H[k];
+        }
         dq = sqrtf((eps[0] - H[0]) * (eps[0] - H[0]) + (eps[1] - H[1]) * (eps[1] - H[1]) + (eps[2] - H[2]) * (eps[2] - H[2]));
       } else {
         const float phi0 = P.per_particle_phi ? phi_p : P.h0;
-        sand_project(P, phi0, q, S, Sn, dq);
+        float lnS[3];
+        sand_project(P, phi0, q, S, Sn, lnS, lnSn, dq);
       }
       float Fe[9], K[6];
       const float vol = a.st[PVOL * cap + j];
-      sand_finish(P, U, S, V, Sn, dq, q, vol, Fe, K);
+      sand_finish(P, U, V, Sn, lnSn, dq, q, vol, Fe, K);
 #pragma unroll
       for (int k = 0; k < 9; ++k) a.st[(PF + k) * cap + j] = Fe[k];
       a.st[PPL * cap + j] = q;
@@ -1698,7 +1817,7 @@ __global__ __launch_bounds__(64) void k_cloth_fixed(ClothArgs a) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float g = (a.st[(PX + d) * a.cap + pi] - P.gmin[d]) * P.inv_h[d];
-    const int base = (ORDER == 2) ? (int)floorf(g - 0.5f) : (int)floorf(g) - 1;
+    const int base = stencil_base<ORDER>(g, P.res[d]);
     lo[d] = base + ORDER + 1;
     hi[d] = base - 1;
     for (int o = 0; o <= ORDER; ++o) {
@@ -1778,7 +1897,7 @@ __global__ __launch_bounds__(256) void k_volumes(SceneTable T, float* __restrict
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     g[d] = (st[(PX + d) * cap + i] - P.gmin[d]) * P.inv_h[d];
-    base[d] = (ORDER == 2) ? (int)floorf(g[d] - 0.5f) : (int)floorf(g[d]) - 1;
+    base[d] = stencil_base<ORDER>(g[d], P.res[d]);
 #pragma unroll
     for (int o = 0; o < ORDER + 1; ++o) spline<ORDER>(g[d] - (float)(base[d] + o), w[d][o], dw[d][o]);
   }

@@ -228,6 +228,14 @@ static DevParams devparams_of(const SceneHost& sc) {
   d.h2 = p.dp_h[2];
   d.h3 = p.dp_h[3];
   d.per_particle_phi = p.per_particle_phi;
+  d.alpha_f = -1.0f;
+  d.alpha_d = -1.0;
+  if (p.dp_h[1] == 0.0f && p.dp_h[3] == 0.0f && !p.per_particle_phi) {
+    // phi(q) = h0 + (0 q - 0) exp(-h2 q) = h0 for every q (HybridSolver.cpp:646-647)
+    const double sf = std::sin(double(p.dp_h[0]) * 3.14159265358979323846 / 180.0);
+    d.alpha_d = std::sqrt(2.0 / 3.0) * 2.0 * sf / (3.0 - sf);
+    d.alpha_f = float(d.alpha_d);
+  }
   d.friction = p.friction;
   d.damping = p.apic_damping;
   d.svd_ratio = p.fp64_svd_ratio;

@@ -274,6 +274,19 @@ def test_status_flags():
         sc.load_into(s)
         s.substep(sc.dt)
         assert s.stats()["status"] & _lib.STATUS_OUT_OF_DOMAIN
+    # positions whose grid coordinate is beyond the int range or not finite: the G2P stencil base
+    # is clamped before the integer conversion, so these particles gather nothing and are flagged
+    sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB)
+    x = sc.particles["x"]
+    x[0, 1], x[1, 2], x[2, 3], x[0, 4], x[1, 5] = 1e30, -1e30, 3e9, np.inf, np.nan
+    say("[flags] far and non-finite positions")
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, 2)
+        st = s.stats()["status"]
+        q = s.get_particles()
+    assert st & _lib.STATUS_OUT_OF_DOMAIN
+    assert np.all(np.isfinite(q["x"][:, 10:]))
 
 
 def test_fp64_svd_fallback():
