@@ -392,6 +392,44 @@ __device__ __forceinline__ void sand_project(const DevParams& P, float phi0, flo
   dp_project<float>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, q, P.alpha_f, S, Sn, lnS, lnSn, dq);
 }
 
+// Training pairs of the surrogate (mode 1): the trial Hencky strains l of a cone projection and its
+// result ln Sn (the tip branch, Sigma = 1, and the tensile side are not logged).
+__device__ __forceinline__ void sand_log_pair(const DevParams& P, SurrogateIO& io, const float* l, const float* Sn,
+                                              const float* lnSn, float dq, float phi_p, float q) {
+  if (io.mode != 1 || !(dq > 0.0f)) return;
+  const float tr = l[0] + l[1] + l[2];
+  if (tr <= 0.0f && Sn[0] != 1.0f) {
+    const unsigned long long idx = atomicAdd(io.pair_count, 1ull);
+    if ((int64_t)idx < io.pair_cap) {
+      float* rec = io.pair_log + 8 * idx;
+      rec[0] = l[0]; rec[1] = l[1]; rec[2] = l[2];
+      rec[3] = dp_phi(P, phi_p, q);
+      rec[4] = bulk_over_shear(P);
+      rec[5] = lnSn[0]; rec[6] = lnSn[1]; rec[7] = lnSn[2];
+    }
+  }
+}
+
+// A sand particle whose projection lies within the apex radius: the FP64 projection from its trial
+// SVD, then F_E and V tau (the apex kernel calls this with the SVD that k_g2p stored, return_map
+// with its own, so both give the same bits). l = ln S in FP32 (for the pair log).
+__device__ __forceinline__ void sand_apex_finish(const DevParams& P, const float* U, const float* S, const float* V,
+                                                 const float* l, float phi0, float phi_p, float& plastic, float vol,
+                                                 float* Fe, float* K, int& fb_apex, SurrogateIO& io) {
+  double Sd[3] = {S[0], S[1], S[2]}, So[3], lSd[3], lSo[3], dqd;
+  dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, P.alpha_d, Sd, So, lSd, lSo, dqd);
+  float Sn[3], lnSn[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Sn[k] = float(So[k]);
+    lnSn[k] = float(lSo[k]);
+  }
+  const float dq = float(dqd);
+  fb_apex += 1;
+  sand_log_pair(P, io, l, Sn, lnSn, dq, phi_p, plastic);
+  sand_finish(P, U, V, Sn, lnSn, dq, plastic, vol, Fe, K);
+}
+
 // Return mapping of one particle (HybridSolver.cpp:616-681). Fh is replaced by the elastic part
 // U Sigma V^T; plastic holds J_P (snow, whose update det(F_P') = J_P * prod(Sigma)/prod(Sigma') is
 // the determinant of F_P' = V Sigma'^-1 U^T F^ F_P, :676-677) or plasticAmount q (sand). K receives
@@ -432,39 +470,21 @@ __device__ __forceinline__ void return_map(const DevParams& P, float* Fh, float&
     float phi0 = P.per_particle_phi ? phi_p : P.h0;
     float dq, l[3], lnSn[3];
     sand_project(P, phi0, plastic, S, Sn, l, lnSn, dq);
-    if (dq > 0.0f) {
-      // a projection onto the cone happened (the tip branch, tr > 0, gives Sigma = 1 exactly);
-      // close to the cone apex redo it in FP64
-      if (!(l[0] + l[1] + l[2] > 0.0f) && l[0] * l[0] + l[1] * l[1] + l[2] * l[2] < P.apex_radius * P.apex_radius) {
-        if (io.defer_apex) {  // F^, q and the stress are left as they are for the apex kernel
-          io.apex_deferred = true;
-          return;
-        }
-        double Sd[3] = {S[0], S[1], S[2]}, So[3], lSd[3], lSo[3], dqd;
-        dp_project<double>(P.lambda0, P.mu0, phi0, P.h1, P.h2, P.h3, plastic, P.alpha_d, Sd, So, lSd, lSo, dqd);
+    // a projection onto the cone happened (the tip branch, tr > 0, gives Sigma = 1 exactly): close
+    // to the cone apex it is redone in FP64
+    if (dq > 0.0f && !(l[0] + l[1] + l[2] > 0.0f) &&
+        l[0] * l[0] + l[1] * l[1] + l[2] * l[2] < P.apex_radius * P.apex_radius) {
+      if (io.defer_apex) {  // F^, q and the stress are left as they are; the SVD goes to the apex kernel
+        io.apex_deferred = true;
+        io.sigma[0] = S[0]; io.sigma[1] = S[1]; io.sigma[2] = S[2];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          Sn[k] = float(So[k]);
-          lnSn[k] = float(lSo[k]);
-        }
-        dq = float(dqd);
-        fb_apex += 1;
+        for (int i = 0; i < 9; ++i) { io.U[i] = U[i]; io.V[i] = V[i]; }
+        return;
       }
+      sand_apex_finish(P, U, S, V, l, phi0, phi_p, plastic, vol, Fh, K, fb_apex, io);
+      return;
     }
-    if (io.mode == 1 && dq > 0.0f) {
-      // log (trial, projected) cone pairs: the training data of the surrogate
-      const float tr = l[0] + l[1] + l[2];
-      if (tr <= 0.0f && Sn[0] != 1.0f) {
-        const unsigned long long idx = atomicAdd(io.pair_count, 1ull);
-        if ((int64_t)idx < io.pair_cap) {
-          float* rec = io.pair_log + 8 * idx;
-          rec[0] = l[0]; rec[1] = l[1]; rec[2] = l[2];
-          rec[3] = dp_phi(P, phi_p, plastic);
-          rec[4] = bulk_over_shear(P);
-          rec[5] = lnSn[0]; rec[6] = lnSn[1]; rec[7] = lnSn[2];
-        }
-      }
-    }
+    sand_log_pair(P, io, l, Sn, lnSn, dq, phi_p, plastic);
     sand_finish(P, U, V, Sn, lnSn, dq, plastic, vol, Fh, K);
     return;
   }

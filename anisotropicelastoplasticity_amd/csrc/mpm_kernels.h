@@ -103,6 +103,18 @@ struct Clock {
 __device__ __forceinline__ bool clock_skip(const Clock* c) { return c != nullptr && c->skip != 0; }
 __device__ __forceinline__ float clock_dt(const Clock* c, float dt) { return c != nullptr ? c->dt_f : dt; }
 
+struct ResetArgs {
+  uint32_t* zero_words;
+  int64_t n_zero;
+  const int* counters;
+  const int* active_tiles;
+  int* tile_slot;
+  const int* active_blocks;
+  int* block_slot;
+  int64_t max_tiles;
+  const Clock* clk;
+};
+
 struct SortBufs {
   uint32_t *k1, *k2;
   int *v1, *v2;
@@ -117,21 +129,11 @@ struct SortBufs {
   int* tile_slot;
   const int* pos;
   int* src;
-  int* counters;      // zeroed by the first sort kernel
+  int* counters;      // zeroed by the cell scan (after the counting kernel has read them)
+  ResetArgs reset;    // the counting sort's reset work, done by its counting kernel
   const Clock* clk;
 };
 
-struct ResetArgs {
-  uint32_t* zero_words;
-  int64_t n_zero;
-  const int* counters;
-  const int* active_tiles;
-  int* tile_slot;
-  const int* active_blocks;
-  int* block_slot;
-  int64_t max_tiles;
-  const Clock* clk;
-};
 
 struct P2GArgs {
   SceneTable T;
@@ -193,17 +195,19 @@ struct G2PArgs {
   int* cand_slot;   // sorted slot of each candidate
   int* cand_count;
   int* apex_slot;   // sorted slots of the particles whose FP64 apex projection is deferred
+  float* apex_usv;  // their trial SVD, 21 planes (sigma 3, U 9, V 9) of n floats
   int* apex_count;
   const Clock* clk;
 };
 
-// The deferred FP64 apex projections (after G2P): the same return map on the trial state.
+// The deferred FP64 apex projections (after G2P): the rest of the return map from the stored SVD.
 struct ApexArgs {
   SceneTable T;
   float* st;   // the G2P output buffer
   int64_t cap;
   int n;
   const int* apex_slot;
+  const float* apex_usv;
   const int* apex_count;
   int write_stress;
   int log_pairs;

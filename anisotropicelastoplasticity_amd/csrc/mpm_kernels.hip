@@ -364,11 +364,22 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 // Counting: the lanes of a wave that share a key are grouped first (leader = the lowest such lane);
 // then every leader adds its group's size with one atomic, all leaders in the same instruction, so
 // a wave waits for one atomic round trip.
+//
+// With `reset` set (the counting sort, which has no k_reset launch) the kernel also does k_reset's
+// work: it zeroes the scan's tickets and look-back words and returns the previous substep's active
+// tiles and node blocks to -1. The counters those lists are read with are zeroed by k_cell_scan.
 __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__ keys, int n, int* __restrict__ cnt,
-                                                    uint32_t* __restrict__ rank, int* __restrict__ counters,
+                                                    uint32_t* __restrict__ rank, ResetArgs r, int reset,
                                                     const Clock* clk) {
   if (clock_skip(clk)) return;
-  if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
+  if (reset) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t q = t0; q < r.n_zero; q += stride) r.zero_words[q] = 0u;
+    const int nt = r.counters[CNT_TILES], nb = r.counters[CNT_BLOCKS];
+    for (int64_t q = t0; q < nt; q += stride) r.tile_slot[r.active_tiles[q]] = -1;
+    for (int64_t q = t0; q < nb; q += stride) r.block_slot[r.active_blocks[q]] = -1;
+  }
   const int i = blockIdx.x * 256 + threadIdx.x;
   const bool valid = i < n;
   const uint32_t key = valid ? keys[i] : 0u;
@@ -398,10 +409,13 @@ constexpr unsigned long long SCAN_AGG = 1ull << 62, SCAN_PREFIX = 2ull << 62, SC
 // Exclusive scan of the cell counts into cell_start[0 .. ncells] (single pass, decoupled look-back:
 // blocks take tickets in start order and publish their aggregate before looking back, so a block
 // only waits on running blocks).
+// Block 0 also zeroes the substep's counters (active tiles and node blocks, candidates, apex
+// projections), which k_cell_count has read.
 __global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, int64_t ncells, int* __restrict__ cell_start,
                                                    uint32_t* __restrict__ ticket, unsigned long long* __restrict__ status,
-                                                   const Clock* clk) {
+                                                   int* __restrict__ counters, const Clock* clk) {
   if (clock_skip(clk)) return;
+  if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
   __shared__ int s_b;
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_excl;
@@ -1537,7 +1551,17 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
         if (lane == first) base = atomicAdd(a.apex_count, __popcll(am));
         base = __shfl(base, first);
         const int c = base + __popcll(am & ((1ull << lane) - 1ull));
-        if (io.apex_deferred && c < a.n) a.apex_slot[c] = j;
+        if (io.apex_deferred && c < a.n) {
+          a.apex_slot[c] = j;
+          const int64_t n = a.n;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) a.apex_usv[(CAND_SIGMA + k) * n + c] = io.sigma[k];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            a.apex_usv[(CAND_U + k) * n + c] = io.U[k];
+            a.apex_usv[(CAND_V + k) * n + c] = io.V[k];
+          }
+        }
       }
     }
     float xn[3];
@@ -1590,13 +1614,14 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   }
 }
 
-// The FP64 apex projections deferred by k_g2p, gathered into full waves: the particle's trial F^
-// (left in the output buffer), q, phi and V go through return_map again, now with the FP64
-// projection, and F, q and the stress are stored. The result equals computing it inside k_g2p.
+// The FP64 apex projections deferred by k_g2p, gathered into full waves: from the trial SVD that
+// k_g2p stored and the particle's q, phi and V, the FP64 projection and F_E and V tau (the same
+// function return_map calls, on the same operands), stored over the trial F^ left in the output
+// buffer. The result equals computing it inside k_g2p.
 __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
   if (clock_skip(a.clk)) return;
   const int count = min(*a.apex_count, a.n);
-  int fb_svd = 0, fb_apex = 0;
+  int fb_apex = 0;
   for (int c0 = blockIdx.x * 256; c0 < count; c0 += gridDim.x * 256) {
     const int c = c0 + (int)threadIdx.x;
     if (c < count) {
@@ -1604,25 +1629,34 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
       const int sc = scene_of(a.T.part_off, a.T.nscenes, j);
       const DevParams& P = a.T.params[sc];
       float* st = a.st;
-      const int64_t cap = a.cap;
-      float Fh[9];
+      const int64_t cap = a.cap, n = a.n;
+      float S[3], U[9], V[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) Fh[i] = st[(PF + i) * cap + j];
+      for (int k = 0; k < 3; ++k) S[k] = a.apex_usv[(CAND_SIGMA + k) * n + c];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        U[k] = a.apex_usv[(CAND_U + k) * n + c];
+        V[k] = a.apex_usv[(CAND_V + k) * n + c];
+      }
       float plastic = st[PPL * cap + j];
       const float phi_p = st[PPHI * cap + j];
       const float vol = st[PVOL * cap + j];
-      float K[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
       SurrogateIO io;
       io.mode = a.log_pairs ? 1 : 0;
       io.pair_log = a.pair_log;
       io.pair_cap = a.pair_cap;
       io.pair_count = a.pair_count;
-      io.deferred = false;
-      io.defer_apex = false;
-      io.apex_deferred = false;
-      return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
+      // the FP32 trial strains, as the return map computed them (used by the pair log only)
+      float l[3] = {0.0f, 0.0f, 0.0f};
+      if (io.mode == 1) {
+        l[0] = logf(S[0]);
+        l[1] = logf(S[1]);
+        l[2] = logf(S[2]);
+      }
+      float Fe[9], K[6];
+      sand_apex_finish(P, U, S, V, l, P.per_particle_phi ? phi_p : P.h0, phi_p, plastic, vol, Fe, K, fb_apex, io);
 #pragma unroll
-      for (int i = 0; i < 9; ++i) st[(PF + i) * cap + j] = Fh[i];
+      for (int i = 0; i < 9; ++i) st[(PF + i) * cap + j] = Fe[i];
       st[PPL * cap + j] = plastic;
       if (a.write_stress) {
 #pragma unroll
@@ -1630,7 +1664,6 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
       }
     }
   }
-  (void)fb_svd;  // the SVD redos of these particles were counted by k_g2p
   if (fb_apex) atomicAdd(a.fallback_apex, (unsigned long long)fb_apex);
 }
 
@@ -2010,9 +2043,9 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
     // bins: counts (the rank is used by the counting sort) and their scan, in both modes
     uint32_t* rank = (passes & 1) ? b.k2 : b.k1;
     hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, n, b.cell_count, rank,
-                       b.counters, b.clk);
+                       b.reset, mode == SORT_COUNTING ? 1 : 0, b.clk);
     hipLaunchKernelGGL(k_cell_scan, dim3(blocks_for(b.ncells, SCAN_TILE)), dim3(256), 0, s, b.cell_count, b.ncells,
-                       b.cell_start, b.tickets + 4, b.scan_status, b.clk);
+                       b.cell_start, b.tickets + 4, b.scan_status, b.counters, b.clk);
   }
   if (mode == SORT_COUNTING) {
     uint32_t* kf = (passes & 1) ? b.k1 : b.k2;  // the result goes where the digit passes put it

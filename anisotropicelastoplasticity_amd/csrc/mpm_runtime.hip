@@ -149,6 +149,7 @@ struct mpm_ctx {
   float* cand = nullptr;
   int* cand_slot = nullptr;
   int* apex_slot = nullptr;  // sorted slots of the deferred FP64 apex projections (n)
+  float* apex_usv = nullptr;  // their trial SVD (CAND_PLANES x n)
 };
 
 static void drop_mesh(mpm_ctx* ctx);
@@ -355,8 +356,10 @@ static int stage_sort(mpm_ctx* ctx) {
   r.block_slot = ctx->block_slot;
   r.max_tiles = ctx->ntiles;
   r.clk = ctx->clk_active;
-  launch_reset(s, r);
+  // the counting sort's first kernel does the reset work itself (one launch fewer)
+  if (ctx->sort_mode != SORT_COUNTING) launch_reset(s, r);
   SortBufs b;
+  b.reset = r;
   b.k1 = ctx->k1;
   b.k2 = ctx->k2;
   b.v1 = ctx->v1;
@@ -506,6 +509,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.cand_slot = ctx->cand_slot;
   a.cand_count = ctx->counters + CNT_CAND;
   a.apex_slot = ctx->any_sand ? ctx->apex_slot : nullptr;
+  a.apex_usv = ctx->apex_usv;
   a.apex_count = ctx->counters + CNT_APEX;
   a.clk = ctx->clk_active;
   launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p);
@@ -516,6 +520,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     x.cap = ctx->cap;
     x.n = ctx->n;
     x.apex_slot = ctx->apex_slot;
+    x.apex_usv = ctx->apex_usv;
     x.apex_count = ctx->counters + CNT_APEX;
     x.write_stress = a.write_stress;
     x.log_pairs = a.sur_mode == 1;
@@ -670,7 +675,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
   dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
-  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws); dfree(ctx->apex_slot);
+  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws); dfree(ctx->apex_slot); dfree(ctx->apex_usv);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   dfree(ctx->clock);
   dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
@@ -864,6 +869,7 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
     if (!rc) rc = dalloc(ctx, ctx->v2, n);
     if (!rc) rc = dalloc(ctx, ctx->src, n);
     if (!rc) rc = dalloc(ctx, ctx->apex_slot, n);
+    if (!rc) rc = dalloc(ctx, ctx->apex_usv, (size_t)CAND_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
     if (rc) return rc;
