@@ -119,9 +119,9 @@ struct SortBufs {
   uint32_t *k1, *k2;
   int *v1, *v2;
   uint32_t* hist;     // [4][256]
-  uint32_t* tickets;  // [4] digit passes, [4] cell scan
+  uint32_t* tickets;  // [4] digit passes
   uint32_t* status;   // [passes][nblocks][256] digit-pass look-back words
-  unsigned long long* scan_status;  // [scan blocks] cell-scan look-back words
+  uint32_t* scan_agg;   // [scan blocks] particles counted in each scan block's cells
   int* cell_count;    // [cells] zero between substeps (the bin pass clears the occupied cells)
   int* cell_start;    // [cells + 1] exclusive prefix of the counts; cell c holds [start[c], start[c+1])
   int64_t ncells;

@@ -366,8 +366,8 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 // a wave waits for one atomic round trip.
 //
 // With `reset` set (the counting sort, which has no k_reset launch) the kernel also does k_reset's
-// work: it zeroes the scan's tickets and look-back words and returns the previous substep's active
-// tiles and node blocks to -1. The counters those lists are read with are zeroed by k_cell_scan.
+// work: it returns the previous substep's active tiles and node blocks to -1. The counters those
+// lists are read with are zeroed by k_cell_scan.
 __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__ keys, int n, int* __restrict__ cnt,
                                                     uint32_t* __restrict__ rank, ResetArgs r, int reset,
                                                     const Clock* clk) {
@@ -404,25 +404,53 @@ __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__
   if (valid) rank[i] = (uint32_t)(base + within);
 }
 
-constexpr unsigned long long SCAN_AGG = 1ull << 62, SCAN_PREFIX = 2ull << 62, SCAN_VALUE = 0xFFFFFFFFull;
+// Total of each scan block's SCAN_TILE cell counts (the blocks are independent).
+__global__ __launch_bounds__(256) void k_scan_sums(const int* __restrict__ cnt, int64_t ncells, uint32_t* __restrict__ agg,
+                                                   const Clock* clk) {
+  if (clock_skip(clk)) return;
+  __shared__ uint32_t s_wsum[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)tid * SCAN_IPT;
+  uint32_t t = 0;
+  if (base + SCAN_IPT <= ncells) {
+    const int4* p4 = reinterpret_cast<const int4*>(cnt + base);
+#pragma unroll
+    for (int q = 0; q < SCAN_IPT / 4; ++q) {
+      const int4 v = p4[q];
+      t += (uint32_t)v.x + (uint32_t)v.y + (uint32_t)v.z + (uint32_t)v.w;
+    }
+  } else {
+    for (int q = 0; q < SCAN_IPT; ++q)
+      if (base + q < ncells) t += (uint32_t)cnt[base + q];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+  if (lane == 0) s_wsum[w] = t;
+  __syncthreads();
+  if (tid == 0) agg[blockIdx.x] = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+}
 
-// Exclusive scan of the cell counts into cell_start[0 .. ncells] (single pass, decoupled look-back:
-// blocks take tickets in start order and publish their aggregate before looking back, so a block
-// only waits on running blocks).
+// Exclusive scan of the cell counts into cell_start[0 .. ncells]: block b starts at the sum of the
+// totals of the blocks before it (read and summed by the block itself) and scans its SCAN_TILE
+// cells; the blocks are independent. The last block writes cell_start[ncells].
 // Block 0 also zeroes the substep's counters (active tiles and node blocks, candidates, apex
 // projections), which k_cell_count has read.
 __global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, int64_t ncells, int* __restrict__ cell_start,
-                                                   uint32_t* __restrict__ ticket, unsigned long long* __restrict__ status,
-                                                   int* __restrict__ counters, const Clock* clk) {
+                                                   const uint32_t* __restrict__ agg, int* __restrict__ counters,
+                                                   const Clock* clk) {
   if (clock_skip(clk)) return;
   if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
-  __shared__ int s_b;
   __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_excl;
+  __shared__ uint32_t s_boff[4];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  if (tid == 0) s_b = (int)atomicAdd(ticket, 1u);
-  __syncthreads();
-  const int b = s_b;
+  const int b = blockIdx.x;
+  {
+    uint32_t p = 0;
+    for (int q = tid; q < b; q += 256) p += agg[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
+    if (lane == 0) s_boff[w] = p;
+  }
   const int64_t base = (int64_t)b * SCAN_TILE + (int64_t)tid * SCAN_IPT;
   int c[SCAN_IPT];
   if (base + SCAN_IPT <= ncells) {
@@ -447,45 +475,11 @@ __global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, 
   }
   if (lane == 63) s_wsum[w] = x;
   __syncthreads();
-  uint32_t wpre = 0, total = 0;
+  const uint32_t boff = s_boff[0] + s_boff[1] + s_boff[2] + s_boff[3];
+  uint32_t wpre = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t v = s_wsum[k];
-    wpre += k < w ? v : 0u;
-    total += v;
-  }
-  if (w == 0) {
-    uint32_t excl = 0;
-    if (b == 0) {
-      if (lane == 0) __hip_atomic_store(status, SCAN_PREFIX | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0) __hip_atomic_store(status + b, SCAN_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // lane l reads predecessor b-1-l-64m; an index below 0 counts as an empty prefix
-      int bb = b - 1;
-      while (true) {
-        const int idx = bb - lane;
-        const unsigned long long sv =
-            idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SCAN_PREFIX;
-        const unsigned long long fl = sv >> 62;
-        const uint64_t pm = __ballot(fl == 2ull), zm = __ballot(fl == 0ull);
-        const int fp = pm ? __ffsll((long long)pm) - 1 : 64;
-        const uint64_t upto = fp >= 63 ? ~0ull : ((2ull << fp) - 1ull);
-        if (zm & upto) continue;  // a predecessor before the nearest prefix has not published yet
-        uint32_t v = lane <= fp ? (uint32_t)(sv & SCAN_VALUE) : 0u;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        excl += v;
-        if (fp < 64) break;
-        bb -= 64;
-      }
-      if (lane == 0)
-        __hip_atomic_store(status + b, SCAN_PREFIX | (unsigned long long)(excl + total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (lane == 0) s_excl = excl;
-  }
-  __syncthreads();
-  uint32_t run = s_excl + wpre + (x - tsum);
+  for (int k = 0; k < 4; ++k) wpre += k < w ? s_wsum[k] : 0u;
+  uint32_t run = boff + wpre + (x - tsum);
   if (base + SCAN_IPT <= ncells) {
     int4* ps = reinterpret_cast<int4*>(cell_start + base);
 #pragma unroll
@@ -534,8 +528,16 @@ __global__ __launch_bounds__(256) void k_cell_rank(SceneTable T, const uint32_t*
   const int i = clamp_index(tmp[j], n);
   const uint32_t k = keys[i];
   const int s0 = cell_start[k], e0 = min(cell_start[k + 1], n);
+  // 8 ids per step, loaded together (past the cell's end the loads read its last entry, in
+  // bounds, and are not counted)
   int r = 0;
-  for (int m = s0; m < e0; ++m) r += tmp[m] < i ? 1 : 0;
+  for (int m0 = s0; m0 < e0; m0 += 8) {
+    int t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = tmp[min(m0 + u, e0 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r += (m0 + u < e0 && t[u] < i) ? 1 : 0;
+  }
   const int d = s0 + r;
   if ((unsigned)d >= (unsigned)n) return;  // never, for consistent counts
   vs[d] = i;
@@ -1242,32 +1244,58 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
     const DevParams& P = a.T.params[sc];
     const int toff = a.T.tile_off[sc];
     const int blk = gblk - toff;
-    const int b0 = blk % P.tiles[0], b1 = (blk / P.tiles[0]) % P.tiles[1], b2 = blk / (P.tiles[0] * P.tiles[1]);
+    const int nt0 = __builtin_amdgcn_readfirstlane(P.tiles[0]), nt1 = __builtin_amdgcn_readfirstlane(P.tiles[1]),
+              nt2 = __builtin_amdgcn_readfirstlane(P.tiles[2]);
+    const int b0 = blk % nt0, b1 = (blk / nt0) % nt1, b2 = blk / (nt0 * nt1);
+    // the slots of the 27 tiles around the block (b - 1 .. b + 1 per axis), one lane each; every
+    // node of the block is covered by tiles among them
+    int slot27 = -1;
+    {
+      const int q = (int)threadIdx.x;
+      const int u0 = b0 - 1 + q % 3, u1 = b1 - 1 + (q / 3) % 3, u2 = b2 - 1 + q / 9;
+      if (q < 27 && u0 >= 0 && u1 >= 0 && u2 >= 0 && u0 < nt0 && u1 < nt1 && u2 < nt2)
+        slot27 = a.tile_slot[toff + (u2 * nt1 + u1) * nt0 + u0];
+    }
+    const int r0 = __builtin_amdgcn_readfirstlane(P.res[0]), r1 = __builtin_amdgcn_readfirstlane(P.res[1]),
+              r2 = __builtin_amdgcn_readfirstlane(P.res[2]);
     const int n0 = 4 * b0 + lx, n1 = 4 * b1 + ly, n2 = 4 * b2 + lz;
-    if (n0 >= P.res[0] || n1 >= P.res[1] || n2 >= P.res[2]) continue;
-    const int64_t node = a.T.node_off[sc] + ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
-    const int nn[3] = {n0, n1, n2};
+    const bool in_grid = n0 < r0 && n1 < r1 && n2 < r2;
+    const int64_t node = a.T.node_off[sc] + ((int64_t)n2 * r1 + n1) * r0 + n0;
+    const int nn[3] = {n0, n1, n2}, ntl[3] = {nt0, nt1, nt2};
+    // covering tiles of this node along each axis: tlo and tlo + 1 (if <= thi), both in b - 1 .. b + 1
     int tlo[3], thi[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       int lo = (nn[d] - 5 + 3) / 4;  // ceil((n-5)/4) for n >= 2; clamped below
       if (nn[d] - 5 < 0) lo = 0;
       int hi = (nn[d] + 1) / 4;
-      thi[d] = hi < P.tiles[d] - 1 ? hi : P.tiles[d] - 1;
+      thi[d] = hi < ntl[d] - 1 ? hi : ntl[d] - 1;
       tlo[d] = lo;
     }
-    float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const int qoff = (MODE == GRID_FORCE_UPDATE) ? 4 : 0;
-    const int nq = (MODE == GRID_FORCE_UPDATE) ? 3 : 4;
-    for (int t2 = tlo[2]; t2 <= thi[2]; ++t2)
-      for (int t1 = tlo[1]; t1 <= thi[1]; ++t1)
-        for (int t0 = tlo[0]; t0 <= thi[0]; ++t0) {
-          const int sl = a.tile_slot[toff + (t2 * P.tiles[1] + t1) * P.tiles[0] + t0];
-          if (sl < 0) continue;
-          const int ln = (n0 - (4 * t0 - 1)) + FOOT * (n1 - (4 * t1 - 1)) + FOOT * FOOT * (n2 - (4 * t2 - 1));
-          const float* part = a.partials + (size_t)sl * PART_STRIDE + qoff * FOOT3 + ln;
-          for (int q = 0; q < nq; ++q) sum[q] += part[q * FOOT3];
-        }
+    constexpr int NQ = (MODE == GRID_FORCE_UPDATE) ? 3 : 4;
+    // all partial loads of the node first (at most 2 x 2 x 2 tiles), then the sums in ascending tile
+    // order (t2, t1, t0), as a loop over the covering tiles would add them
+    float val[8][NQ];
+    bool use[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int t0 = tlo[0] + (k & 1), t1 = tlo[1] + ((k >> 1) & 1), t2 = tlo[2] + (k >> 2);
+      const int q27 = (t2 - (b2 - 1)) * 9 + (t1 - (b1 - 1)) * 3 + (t0 - (b0 - 1));
+      const int sl = __shfl(slot27, q27);  // q27 is in [0, 27): lanes 0..26 hold the slots
+      use[k] = in_grid && t0 <= thi[0] && t1 <= thi[1] && t2 <= thi[2] && q27 >= 0 && q27 < 27 && sl >= 0;
+      const int ln = (n0 - (4 * t0 - 1)) + FOOT * (n1 - (4 * t1 - 1)) + FOOT * FOOT * (n2 - (4 * t2 - 1));
+      const float* part = a.partials + (size_t)(use[k] ? sl : 0) * PART_STRIDE + qoff * FOOT3 + (use[k] ? ln : 0);
+      // unconditional loads (an unused tile reads slot 0, node 0: in bounds) keep them all in flight
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) val[k][q] = part[q * FOOT3];
+    }
+    float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sum[q] = use[k] ? sum[q] + val[k][q] : sum[q];
+    if (!in_grid) continue;
     float v[3], vbf[3];
     if constexpr (MODE == GRID_FORCE_UPDATE) {
       const float m = a.gm[node];
@@ -2042,10 +2070,12 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
   {
     // bins: counts (the rank is used by the counting sort) and their scan, in both modes
     uint32_t* rank = (passes & 1) ? b.k2 : b.k1;
+    const int nscan = blocks_for(b.ncells, SCAN_TILE);
     hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, n, b.cell_count, rank,
                        b.reset, mode == SORT_COUNTING ? 1 : 0, b.clk);
-    hipLaunchKernelGGL(k_cell_scan, dim3(blocks_for(b.ncells, SCAN_TILE)), dim3(256), 0, s, b.cell_count, b.ncells,
-                       b.cell_start, b.tickets + 4, b.scan_status, b.counters, b.clk);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nscan), dim3(256), 0, s, b.cell_count, b.ncells, b.scan_agg, b.clk);
+    hipLaunchKernelGGL(k_cell_scan, dim3(nscan), dim3(256), 0, s, b.cell_count, b.ncells, b.cell_start, b.scan_agg,
+                       b.counters, b.clk);
   }
   if (mode == SORT_COUNTING) {
     uint32_t* kf = (passes & 1) ? b.k1 : b.k2;  // the result goes where the digit passes put it

@@ -331,9 +331,9 @@ static float threshold(double gmin, double h, int c) {
 // Sort workspace words: 4 x 256 digit histograms, 4 digit-pass tickets and 4 cell-scan tickets, the
 // cell-scan look-back words (64-bit, one per scan block), then the digit passes' look-back words.
 static constexpr int64_t SORTWS_TICKETS = 1024, SORTWS_SCAN = 1032;
-static int64_t sortws_radix_status(const mpm_ctx* ctx) {
-  return SORTWS_SCAN + 2 * ((ctx->ng + SCAN_TILE - 1) / SCAN_TILE);
-}
+// [SORTWS_SCAN, radix status): the scan blocks' totals
+static int64_t scan_blocks(const mpm_ctx* ctx) { return (ctx->ng + SCAN_TILE - 1) / SCAN_TILE; }
+static int64_t sortws_radix_status(const mpm_ctx* ctx) { return SORTWS_SCAN + scan_blocks(ctx) + 1; }
 
 // ------------------------------------------------------------------ substep stages
 static int stage_sort(mpm_ctx* ctx) {
@@ -342,9 +342,9 @@ static int stage_sort(mpm_ctx* ctx) {
   if (!ctx->keys_valid)
     launch_keys(s, ctx->T, ctx->nblocks_p, st, ctx->cap, ctx->n, ctx->pos[ctx->cur], ctx->thr, ctx->keys, ctx->status);
   ResetArgs r;
-  if (ctx->sort_mode == SORT_COUNTING) {  // only the tickets and the scan's look-back words
+  if (ctx->sort_mode == SORT_COUNTING) {  // nothing to zero: the scan totals are overwritten
     r.zero_words = ctx->sortws + SORTWS_TICKETS;
-    r.n_zero = sortws_radix_status(ctx) - SORTWS_TICKETS;
+    r.n_zero = 0;
   } else {
     r.zero_words = ctx->sortws;
     r.n_zero = ctx->sortws_words;
@@ -366,7 +366,7 @@ static int stage_sort(mpm_ctx* ctx) {
   b.v2 = ctx->v2;
   b.hist = ctx->sortws;
   b.tickets = ctx->sortws + SORTWS_TICKETS;
-  b.scan_status = reinterpret_cast<unsigned long long*>(ctx->sortws + SORTWS_SCAN);
+  b.scan_agg = ctx->sortws + SORTWS_SCAN;
   b.status = ctx->sortws + sortws_radix_status(ctx);
   b.cell_count = ctx->cell_count;
   b.cell_start = ctx->cell_start;
