@@ -635,11 +635,16 @@ __global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int*
 // tile's partial. k_grid sums the covering tiles' partials in ascending tile order: every node value
 // is a sum in a fixed order, without float atomics on global memory, and reruns are bitwise identical.
 constexpr int PAY = 48;                 // payload floats per staged particle (12 x float4)
-// RANKS (4 or 8, chosen by the autotuner) = ranks per cell staged per chunk; one cell's staged
-// payloads take RANKS * PAY + 4 floats (+1 float4 against bank conflicts). The chunk size does not
-// change the order in which a lane adds its cell's particles, so results do not depend on it.
+// LDS floats between consecutive ranks' payloads: 52 = 20 (mod 32), so the 128-bit payload writes of
+// the eight staging lanes of a cell fall on distinct groups of the 32 banks (48 would give 4-way
+// bank conflicts).
+constexpr int PAY_STRIDE = 52;
+// RANKS (2, 4 or 8, chosen by the autotuner) = ranks per cell staged per chunk; one cell's staged
+// payloads take RANKS * PAY_STRIDE + 4 floats (the 4 shift the cells of a row onto different banks).
+// The chunk size does not change the order in which a lane adds its cell's particles, so results do
+// not depend on it.
 template <int RANKS>
-constexpr int pay_cell() { return RANKS * PAY + 4; }
+constexpr int pay_cell() { return RANKS * PAY_STRIDE + 4; }
 constexpr int SLAB = FOOT * FOOT * 4;   // nodes of a wave's slab
 constexpr int P2G_WAVES = 8;            // waves per P2G workgroup: 4 cell layers x 2 row halves
 // payload layout (floats): the x weights of window nodes c-1 .. c+2, the y and z weights and
@@ -963,7 +968,7 @@ __global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
         const int rank = P2G_RANKS * k + sr;
         if (rank < cnt) {
           if constexpr (MODE == P2G_FORCE) p2g_load<MODE>(a, S, rstart + rank, in);
-          p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY) / 4]);
+          p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(cx * PAY_CELL + sr * PAY_STRIDE) / 4]);
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -979,7 +984,7 @@ __global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
       // ---- accumulate this lane's cell, ascending rank
       const int nr = min(P2G_RANKS, cnt - P2G_RANKS * k);
       for (int r = 0; r < nr; ++r) {
-        const float4* src = &s_pay[w][(cx * PAY_CELL + r * PAY) / 4];
+        const float4* src = &s_pay[w][(cx * PAY_CELL + r * PAY_STRIDE) / 4];
         const float* pf = reinterpret_cast<const float*>(src);
         const float4 WX = src[0];
         const float wxs[4] = {WX.x, WX.y, WX.z, WX.w};
@@ -1127,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_p2g_layer(P2GArgs a) {
           const int rank = RANKS * k + sr;
           if (rank < cnt) {
             if constexpr (MODE == P2G_FORCE) p2g_load<MODE>(a, S, start + rank, in);
-            p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(c * PAY_CELL + sr * PAY) / 4]);
+            p2g_stage<ORDER, MODE>(a, S, P, noff, dt, in, g0, g1, g2, &s_pay[w][(c * PAY_CELL + sr * PAY_STRIDE) / 4]);
           }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1138,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_p2g_layer(P2GArgs a) {
         }
         const int nr = min(RANKS, cnt - RANKS * k);
         for (int r = 0; r < nr; ++r) {
-          const float4* src = &s_pay[w][(c * PAY_CELL + r * PAY) / 4];
+          const float4* src = &s_pay[w][(c * PAY_CELL + r * PAY_STRIDE) / 4];
           const float* pf = reinterpret_cast<const float*>(src);
           const float4 WX = src[PY_WX / 4], WY = src[PY_WY / 4], DWY = src[PY_DWY / 4];
           const float wxs[4] = {WX.x, WX.y, WX.z, WX.w};
