@@ -899,14 +899,29 @@ __global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
   const float foy = (float)oy, fzo = (float)zo;
   float* slab = &s_slab[w][0][0];
   for (int i = lane; i < 4 * SLAB; i += 64) slab[i] = 0.0f;
+  // a context with one scene reads its constants once; the next tile's index is loaded a tile ahead
+  const bool one_scene = a.T.nscenes == 1;
+  P2GScene S1{};
+  int64_t noff1 = 0;
+  int toff1 = 0, nt0_1 = 1, nt1_1 = 1;
+  if (one_scene) {
+    S1 = p2g_scene(a.T.params[0]);
+    noff1 = a.T.node_off[0];
+    toff1 = a.T.tile_off[0];
+    nt0_1 = __builtin_amdgcn_readfirstlane(a.T.params[0].tiles[0]);
+    nt1_1 = __builtin_amdgcn_readfirstlane(a.T.params[0].tiles[1]);
+  }
+  int gtile_next = (int)blockIdx.x < ntile ? a.active_tiles[blockIdx.x] : 0;
   for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
-    const int gtile = a.active_tiles[slot];
-    const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
+    const int gtile = __builtin_amdgcn_readfirstlane(gtile_next);
+    if (slot + (int)gridDim.x < ntile) gtile_next = a.active_tiles[slot + gridDim.x];
+    const int sc = one_scene ? 0 : __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
     const DevParams& P = a.T.params[sc];
-    const P2GScene S = p2g_scene(P);
-    const int64_t noff = a.T.node_off[sc];
-    const int tile = gtile - a.T.tile_off[sc];
-    const int nt0 = __builtin_amdgcn_readfirstlane(P.tiles[0]), nt1 = __builtin_amdgcn_readfirstlane(P.tiles[1]);
+    const P2GScene S = one_scene ? S1 : p2g_scene(P);
+    const int64_t noff = one_scene ? noff1 : a.T.node_off[sc];
+    const int tile = gtile - (one_scene ? toff1 : a.T.tile_off[sc]);
+    const int nt0 = one_scene ? nt0_1 : __builtin_amdgcn_readfirstlane(P.tiles[0]);
+    const int nt1 = one_scene ? nt1_1 : __builtin_amdgcn_readfirstlane(P.tiles[1]);
     const int t0 = tile % nt0, t1 = (tile / nt0) % nt1, t2 = tile / (nt0 * nt1);
     const int g2 = 4 * t2 + layer;
     const int g0 = 4 * t0 + cx;  // this lane's cell of each row (both roles)
