@@ -371,6 +371,16 @@ class Solver:
         self._check(rc, "mpm_time_substeps")
         return total.value, list(stages)
 
+    def time_kernels(self, dt, nsteps):
+        """(total ms, per-stage ms, per-stage main-kernel ms) summed over nsteps eager substeps
+        (mpm_time_kernels): the G2P entry of the last list is k_g2p alone."""
+        total = ctypes.c_float()
+        stages = (ctypes.c_float * _lib.NUM_STAGES)()
+        kern = (ctypes.c_float * _lib.NUM_STAGES)()
+        rc = self.lib.mpm_time_kernels(self.h, float(dt), int(nsteps), ctypes.byref(total), stages, kern)
+        self._check(rc, "mpm_time_kernels")
+        return total.value, list(stages), list(kern)
+
 
 def write_obj(path, x):
     """`v x y z` per particle, as the reference's frame writer (HybridSolver.cpp:996-1005)."""

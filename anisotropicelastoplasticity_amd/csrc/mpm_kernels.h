@@ -263,7 +263,9 @@ void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32
 enum P2GPass : int { P2G_ROWS = 0, P2G_LAYERS = 1 };
 void launch_p2g(hipStream_t s, const P2GArgs& a, int order, int mode, int grid, int ranks, int pass);
 void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid);
-void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks);
+// With ev_start set, the launch records the kernel's own start and end into ev_start / ev_stop.
+void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks, hipEvent_t ev_start = nullptr,
+                hipEvent_t ev_stop = nullptr);
 void launch_volumes(hipStream_t s, const SceneTable& T, int order, int nblocks, float* st, int64_t cap, int n,
                     const float* gm);
 void launch_unpermute(hipStream_t s, const float* st, int64_t cap, int n, int p0, int np, float* out);

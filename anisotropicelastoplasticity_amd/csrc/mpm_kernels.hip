@@ -20,6 +20,7 @@
 // p2g(force, gathers grad v) -> grid(force update) replace the p2g -> grid pair. A cloth adds its
 // element and vertex forces before P2G, the fixed-vertex node stops after the grid update, and the
 // element update (directions, QR return map) after G2P.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "mpm_device.hpp"
@@ -2194,8 +2195,13 @@ void launch_grid(hipStream_t s, const GridArgs& a, int mode, int grid) {
   else hipLaunchKernelGGL(k_grid<GRID_FORCE_UPDATE>, dim3(grid), dim3(64), 0, s, a);
 }
 
-void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks) {
+void launch_g2p(hipStream_t s, const G2PArgs& a, int order, int nblocks, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (a.n <= 0) return;
+  if (ev_start) {  // timed launch: the events take the dispatch's own start and end times
+    if (order == 2) hipExtLaunchKernelGGL(k_g2p<2>, dim3(nblocks), dim3(256), 0, s, ev_start, ev_stop, 0, a);
+    else hipExtLaunchKernelGGL(k_g2p<3>, dim3(nblocks), dim3(256), 0, s, ev_start, ev_stop, 0, a);
+    return;
+  }
   if (order == 2) hipLaunchKernelGGL(k_g2p<2>, dim3(nblocks), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_g2p<3>, dim3(nblocks), dim3(256), 0, s, a);
 }

@@ -148,6 +148,8 @@ struct mpm_ctx {
   float gate_tol = 1e-5f;
   float* cand = nullptr;
   int* cand_slot = nullptr;
+  // set by mpm_time_kernels: the start and end of a stage's main kernel (taken by its launch)
+  hipEvent_t ev_main[2] = {nullptr, nullptr};
   int* apex_slot = nullptr;  // sorted slots of the deferred FP64 apex projections (n)
   float* apex_usv = nullptr;  // their trial SVD (CAND_PLANES x n)
 };
@@ -512,7 +514,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.apex_usv = ctx->apex_usv;
   a.apex_count = ctx->counters + CNT_APEX;
   a.clk = ctx->clk_active;
-  launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p);
+  launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p, ctx->ev_main[0], ctx->ev_main[1]);
   if (a.apex_slot) {
     ApexArgs x;
     x.T = ctx->T;
@@ -1596,7 +1598,7 @@ int mpm_synchronize(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
-int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms) {
+int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms) {
   int rc = ready(ctx);
   if (rc) return rc;
   if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
@@ -1606,7 +1608,18 @@ int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, f
   hipEvent_t t0, t1;
   CK(hipEventCreate(&t0));
   CK(hipEventCreate(&t1));
-  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0};
+  CK(hipEventCreate(&ctx->ev_main[0]));
+  CK(hipEventCreate(&ctx->ev_main[1]));
+  struct Clear {  // the stages record the main-kernel events only inside this call
+    mpm_ctx* c;
+    ~Clear() {
+      for (auto& e : c->ev_main) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+      }
+    }
+  } clear{ctx};
+  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0}, g2p_main = 0.0f;
   CK(hipEventRecord(t0, ctx->stream));
   for (int i = 0; i < nsteps; ++i) {
     CK(hipEventRecord(ev[0], ctx->stream));
@@ -1621,6 +1634,9 @@ int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, f
       CK(hipEventElapsedTime(&ms, ev[st], ev[st + 1]));
       acc[st] += ms;
     }
+    float ms = 0;
+    CK(hipEventElapsedTime(&ms, ctx->ev_main[0], ctx->ev_main[1]));
+    g2p_main += ms;
   }
   CK(hipEventRecord(t1, ctx->stream));
   CK(hipEventSynchronize(t1));
@@ -1629,11 +1645,19 @@ int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, f
   if (total_ms) *total_ms = tot;
   if (stage_ms)
     for (int st = 0; st < MPM_NUM_STAGES; ++st) stage_ms[st] = acc[st];
+  if (kernel_ms) {
+    for (int st = 0; st < MPM_NUM_STAGES; ++st) kernel_ms[st] = acc[st];
+    kernel_ms[MPM_STAGE_G2P] = g2p_main;
+  }
   for (auto& e : ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   CK(hipGetLastError());
   return MPM_OK;
+}
+
+int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms) {
+  return mpm_time_kernels(ctx, dt, nsteps, total_ms, stage_ms, nullptr);
 }
 
 int mpm_bench_copy(mpm_ctx* ctx, int64_t nbytes, int32_t reps, float* ms) {

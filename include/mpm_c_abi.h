@@ -301,6 +301,11 @@ int mpm_synchronize(mpm_ctx* ctx);
  * around every stage; stage_ms[MPM_NUM_STAGES] receives the summed stage times and *total_ms the
  * whole-span time. */
 int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms);
+/* As mpm_time_substeps, and kernel_ms[MPM_NUM_STAGES] receives the summed time of each stage's main
+ * kernel: for G2P, the k_g2p dispatch alone (its start and end, as the launch records them into HIP
+ * events), without the deferred FP64 apex projections that follow it in the stage; for the other
+ * stages, the stage time. No counterpart in the reference. */
+int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms);
 
 /* ---- learned return-map surrogate (no counterpart in the reference; replaces the cone branch of
  * the Drucker-Prager projection, HybridSolver.cpp:667-673, behind a gate that falls back to it) ----

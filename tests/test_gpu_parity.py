@@ -326,3 +326,18 @@ def test_cpp_host_api_example(tmp_path):
         lines = (tmp_path / "particle" / f"particle_{f}.obj").read_text().splitlines()
         assert len(lines) == n and all(l.startswith("v ") and len(l.split()) == 4 for l in lines)
     assert not (tmp_path / "particle" / "particle_2.obj").exists()
+
+
+def test_time_kernels_split():
+    # mpm_time_kernels: the G2P main kernel is timed apart from the apex projections of its stage
+    sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB)
+    say("[timing] mpm_time_kernels")
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, 2)
+        total, stages, kern = s.time_kernels(sc.dt, 3)
+        st = s.stats()
+    assert st["status"] == 0
+    assert all(t > 0.0 for t in stages) and total >= sum(stages) * 0.99
+    assert 0.0 < kern[3] <= stages[3]
+    assert kern[:3] == stages[:3]
