@@ -753,6 +753,9 @@ __device__ __forceinline__ void p2g_load(const P2GArgs& a, const P2GScene& S, in
 template <int ORDER, int MODE>
 __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const P2GScene& S, const DevParams& P, int64_t noff,
                                           float dt, const StageIn& in, int g0, int g1, int g2, float4* dst) {
+  // No FMA contraction: the row and the layer pass inline this function into different kernels, and
+  // the payload must have the same bits in both (contraction is decided per kernel otherwise).
+#pragma clang fp contract(off)
   float pay[PAY];
 #pragma unroll
   for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;

@@ -194,8 +194,9 @@ def batch_scene(k, rng, n_range=(20000, 500000), res_choices=(64, 80, 96, 112, 1
     """Scene k of a batched run: snow or sand (50/50), a grid of 64^3..128^3 over the unit cube, a
     particle count log-uniform in n_range, E log-uniform in [1e5, 1e6], nu uniform in [0.2, 0.35],
     sand friction angle uniform in [25, 40] degrees. The particles form a cube of c^3 lattice cells
-    (8 jittered particles per cell, one lattice cell per grid cell) falling at 1 m/s inside box walls
-    [0.05, 0.95]^3 with Coulomb friction 0.4."""
+    (8 jittered particles per cell, one lattice cell per grid cell) resting 1 cm above the floor of the
+    box walls [0.05, 0.95]^3 (Coulomb friction 0.4) and moving into it at 1 m/s, so that the blocks
+    hit the floor within the first substeps and sand flows plastically."""
     sand = bool(rng.random() < 0.5)
     r = int(rng.choice(res_choices))
     n_want = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
@@ -204,7 +205,7 @@ def batch_scene(k, rng, n_range=(20000, 500000), res_choices=(64, 80, 96, 112, 1
     phi = float(rng.uniform(25.0, 40.0))
     c = max(2, int(round((n_want / 8.0) ** (1.0 / 3.0))))
     edge = c / r
-    lo = (0.5 - edge / 2, 0.1, 0.5 - edge / 2)
+    lo = (0.5 - edge / 2, 0.06, 0.5 - edge / 2)
     hi = (lo[0] + edge, lo[1] + edge, lo[2] + edge)
     x = generate_block(1000 + k, lo, hi, (c, c, c), 2)
     if sand:

@@ -169,7 +169,7 @@ def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=
                 done += 1 + max(k, 0)
             pairs = np.concatenate(pairs)
             s.surrogate_init(1)
-            s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch_size, lr=lr, seed=2)
+            losses = s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch_size, lr=lr, seed=2)
             weights = s.surrogate_get_weights()
 
     def run_batch(sur):
@@ -216,7 +216,9 @@ def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=
                             "speedup_vs_batched_exact": wall_b / wall_s,
                             "rel_pos_err": float(np.linalg.norm(dx) / np.linalg.norm(x_b.astype(np.float64))),
                             "acceptance": (st_s["surrogate_accepted"] / cand) if cand else 0.0,
-                            "pairs_trained": int(len(pairs)), "gate_tol": gate_tol, "status": st_s["status"]}
+                            "candidates": int(cand), "pairs_trained": int(len(pairs)), "train_epochs": epochs,
+                            "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+                            "gate_tol": gate_tol, "status": st_s["status"]}
     return out
 
 

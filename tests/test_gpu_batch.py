@@ -102,8 +102,15 @@ def test_batch_four_pass_sort_and_equality():
 
 def test_batch_surrogate_all_rejected_is_exact():
     batch = _small_batch(13, 4, (5000, 20000), (32, 48))
-    for sc in batch:  # drive the blocks into the floor so that sand reaches the cone branch
+    assert any(sc.params.material == _lib.SAND for sc in batch)
+    for sc in batch:
         sc.particles["v"][1, :] = -3.0
+        if sc.params.material == _lib.SAND:
+            # sheared and compressed elastic state (Hencky strains about (-0.030, 0.020, 0), beyond the
+            # apex radius): the first substeps project it onto the cone, i.e. hand it to the surrogate
+            F = np.zeros((9, sc.n), np.float32)
+            F[0], F[4], F[8] = 0.97, 1.02, 1.0
+            sc.particles["F"] = F
     w = np.zeros(mo.NPARAM, np.float32)
     w[mo.active_mask()] = np.nan
     nsteps = 40
@@ -111,6 +118,7 @@ def test_batch_surrogate_all_rejected_is_exact():
     sur = _run_batch(batch, nsteps, surrogate_weights=w)
     st = sur["stats"]
     say(f"[batch surrogate] candidates {st['surrogate_candidates']} accepted {st['surrogate_accepted']}")
+    assert st["surrogate_candidates"] > 0
     assert st["surrogate_accepted"] == 0
     _assert_bitwise(sur, ref, "batch")
 
