@@ -162,6 +162,40 @@ def config2_sand_column(seed=2, substeps=1000, cells=(32, 128, 32)):
                  5e-5, substeps, dict(ppc=n / (vol * 128 ** 3)))
 
 
+def particle_friction_angles(seed, n, lo=25.0, hi=40.0):
+    """Per-particle friction angles (degrees) uniform in [lo, hi): for particle id,
+    u = (splitmix64((seed * 0x9E3779B97F4A7C15) ^ id) >> 40) * 2^-24 (the counter hash of the position
+    generator with another key), angle = lo + (hi - lo) u."""
+    m64 = (1 << 64) - 1
+    z = (np.arange(n, dtype=np.uint64) ^ np.uint64((seed * 0x9E3779B97F4A7C15) & m64))
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(40)).astype(np.float64) * (1.0 / 16777216.0)
+    return (lo + (hi - lo) * u).astype(np.float32)
+
+
+def config4_sand_phi(seed=4, substeps=1000, cells=(64, 128, 64)):
+    """The surrogate's workload: 2^22 Drucker-Prager sand particles (config 2's material) with a
+    per-particle friction angle uniform in [25, 40) degrees, on a 256^3 grid: a 64x128x64 jittered
+    lattice of grid-cell-sized lattice cells (8 particles per grid cell) filling the column
+    [0.375,0.625]x[0.05,0.55]x[0.375,0.625]; box walls
+    [0.05,0.95]^3 plus a static sphere (r=0.1, centre (0.5,0.15,0.7)), Coulomb friction 0.4,
+    dt=5e-5 (CFL about 0.19 at dx=1/256)."""
+    lo, hi = (0.375, 0.05, 0.375), (0.625, 0.55, 0.625)
+    x = generate_block(seed, lo, hi, cells, 2)
+    vol = float(np.prod(np.subtract(hi, lo)))
+    n = x.shape[1]
+    parts = _particles(x, 2200.0, vol, (0.0, 0.0, 0.0), _lib.SAND, phi=particle_friction_angles(seed, n))
+    p = sand_params()
+    p.per_particle_phi = 1
+    prims = [levelset(_lib.LS_BOX, (0.05, 0.05, 0.05, 0.0), (0.95, 0.95, 0.95, 0.0)),
+             levelset(_lib.LS_SPHERE, (0.5, 0.15, 0.7, 0.1))]
+    return Scene("config4_sand_phi", p, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (256, 256, 256), prims, parts, 5e-5,
+                 substeps, dict(ppc=n / (vol * 256 ** 3)))
+
+
 def small_scene(material, order, predicted, friction_model, seed=7, res=(24, 24, 24), cells=(6, 5, 6), ppa=2,
                 lo=(0.3, 0.2, 0.35), hi=(0.55, 0.42, 0.6), v0=(0.3, -1.5, 0.2), z_up=False, prims=None,
                 dt=2e-4, E=None):

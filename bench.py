@@ -10,7 +10,9 @@ throughput and its position error against the exact run after the same substeps.
 object reports the batched many-scene workload (64 independent snow/sand scenes of 20k-500k
 particles on 64^3-128^3 grids, one context, one launch per kernel per substep, autotuned launch
 shapes) against the same scenes run one after another with the plain exact path, with and without
-the learned return map.
+the learned return map. The `config4` object reports the learned return map's own workload (2^22 sand
+particles with per-particle friction angles on a 256^3 grid): the exact path's throughput and the
+surrogate against it after 1020 substeps.
 
 Prints one JSON line. `roofline` is for the dominant kernel (its own algorithmic bytes over its
 HIP-event-timed duration); `substep_roofline` is the whole substep against 208 B/particle +
@@ -141,6 +143,30 @@ def surrogate_leg(sc, warmup, steps, x_exact, wall_exact, gate_tol=1e-5, pair_ch
     }, w
 
 
+def config4_leg(surrogate, steps, warmup):
+    """The surrogate's workload (2^22 sand particles with per-particle friction angles on 256^3): the
+    exact path's throughput, and the learned return map against it after warmup + steps substeps."""
+    from anisotropicelastoplasticity_amd import Solver, scenes
+    sc = scenes.config4_sand_phi()
+    with Solver(0) as s:
+        sc.load_into(s)
+        s.run(sc.dt, warmup)
+        s.synchronize()
+        _sync_torch()
+        t0 = time.perf_counter()
+        s.run(sc.dt, steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        x = s.get_particles()["x"]
+        st = s.stats()
+    out = {"workload": "config4_sand_phi", "particles": sc.n, "grid": list(sc.res), "dt": sc.dt,
+           "substeps": warmup + steps, "value": sc.n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+           "fp64_apex_count": st["fp64_apex_count"], "status": st["status"]}
+    if surrogate:
+        out["surrogate"], _ = surrogate_leg(sc, warmup, steps, x, wall)
+    return out
+
+
 def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=10, pairs_per_chunk=400000,
                 epochs=40, batch_size=4096, lr=1e-3):
     """64 scenes in one context against the same scenes one context at a time (exact path), and the
@@ -232,6 +258,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned return-map leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
+    ap.add_argument("--no-config4", action="store_true", help="skip the 2^22-particle per-particle-angle leg")
+    ap.add_argument("--config4-steps", type=int, default=1000, help="timed substeps of the config-4 leg (after 20)")
     ap.add_argument("--batch-steps", type=int, default=480, help="timed substeps of the batched leg (after 20 warm-up)")
     ap.add_argument("--seq-steps", type=int, default=40, help="timed substeps per scene of the one-at-a-time baseline")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -271,6 +299,7 @@ def main():
 
     sur, weights = (None, None) if args.no_surrogate else surrogate_leg(sc, args.warmup, args.steps, x_exact, wall)
     bat = None if args.no_batch else batched_leg(not args.no_surrogate, args.batch_steps, 20, args.seq_steps)
+    c4 = None if args.no_config4 else config4_leg(not args.no_surrogate, args.config4_steps, 20)
     value = n * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
     n_nodes = st["n_active_nodes"]
@@ -317,6 +346,8 @@ def main():
         out["surrogate"] = sur
     if bat is not None:
         out["batched"] = bat
+    if c4 is not None:
+        out["config4"] = c4
     if copy is not None:
         out["copy_calibration"] = copy
     if not args.no_cpu_baseline:

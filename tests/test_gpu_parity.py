@@ -126,8 +126,29 @@ def perturbed_scene(material, order, predicted, friction, z_up, ls, seed):
 @pytest.mark.parametrize("name,material,order,predicted,friction,z_up,ls", CASES)
 def test_substep_stages(name, material, order, predicted, friction, z_up, ls):
     sc = perturbed_scene(material, order, predicted, friction, z_up, ls, seed=21)
-    dt = sc.dt
     say(f"[stages] {name} n={sc.n} res={sc.res}")
+    _stage_parity(sc, predicted, friction)
+
+
+@pytest.mark.parametrize("angle", ["hardening", "per-particle"])
+def test_substep_stages_friction_angle(angle):
+    # the two other sources of the Drucker-Prager friction angle: the reference's hardening law
+    # phi(q) = h0 + (h1 q - h3) exp(-h2 q) with (35, 9, 0.2, 10) (HybridSolver.cpp:641-647), and a
+    # per-particle angle (uniform in [25, 40) degrees) in place of h0
+    sc = perturbed_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB, False, "box+sphere", seed=23)
+    if angle == "hardening":
+        for i, h in enumerate((35.0, 9.0, 0.2, 10.0)):
+            sc.params.dp_h[i] = h
+    else:
+        sc.params.per_particle_phi = 1
+        sc.particles["phi"] = scenes.particle_friction_angles(23, sc.n)
+    say(f"[stages] sand {angle} n={sc.n}")
+    q, qr = _stage_parity(sc, 0, _lib.FRICTION_COULOMB)
+    assert np.sum(qr["plastic"] > sc.particles["plastic"] + 1e-7) > sc.n // 10  # projections happened
+
+
+def _stage_parity(sc, predicted, friction):
+    dt = sc.dt
     o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
     o.load_scene(sc)
     with Solver() as s:
@@ -157,6 +178,7 @@ def test_substep_stages(name, material, order, predicted, friction, z_up, ls):
     assert np.max(np.abs(q["F"] - qr["F"])) < 2e-5
     assert np.max(np.abs(q["plastic"] - qr["plastic"]) / (1 + np.abs(qr["plastic"]))) < 1e-5
     assert st["status"] == 0, st
+    return q, qr
 
 
 def test_estimate_volumes_matches_oracle():
