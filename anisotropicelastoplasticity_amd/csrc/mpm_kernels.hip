@@ -1936,28 +1936,34 @@ __global__ __launch_bounds__(256) void k_cloth_elements(ClothArgs a) {
   if (e < a.ne) {
     const DevParams& P = a.P;
     const int id = P.mesh_e0 + e;
+    // the element's and its vertices' slots, then all of their data in one round of loads
     const int pe = clamp_index(a.pos[id], a.n);
     int pv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) pv[c] = clamp_index(a.pos[clamp_index(a.faces[3 * e + c], a.n)], a.n);
-    float xv[3][3], D[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) xv[c][d] = a.st[(PX + d) * a.cap + pv[c]];
+    float xv[3][3], vv[3][3], d3[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float vm = (a.st[(PV + d) * a.cap + pv[0]] + a.st[(PV + d) * a.cap + pv[1]] + a.st[(PV + d) * a.cap + pv[2]]) / 3.0f;
-      a.st[(PX + d) * a.cap + pe] = (xv[0][d] + xv[1][d] + xv[2][d]) / 3.0f;
-      a.st[(PV + d) * a.cap + pe] = vm;
+      d3[d] = a.st[(PF + 3 * d + 2) * a.cap + pe];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xv[c][d] = a.st[(PX + d) * a.cap + pv[c]];
+        vv[c][d] = a.st[(PV + d) * a.cap + pv[c]];
+      }
+    }
+    float D[9], xe[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xe[d] = (xv[0][d] + xv[1][d] + xv[2][d]) / 3.0f;
+      a.st[(PX + d) * a.cap + pe] = xe[d];
+      a.st[(PV + d) * a.cap + pe] = (vv[0][d] + vv[1][d] + vv[2][d]) / 3.0f;
       D[3 * d] = xv[1][d] - xv[0][d];
       D[3 * d + 1] = xv[2][d] - xv[0][d];
-      D[3 * d + 2] = a.st[(PF + 3 * d + 2) * a.cap + pe];
+      D[3 * d + 2] = d3[d];
     }
     cloth_return_map(P, D, fb);
 #pragma unroll
     for (int i = 0; i < 9; ++i) a.st[(PF + i) * a.cap + pe] = D[i];
-    const float xe[3] = {a.st[PX * a.cap + pe], a.st[(PX + 1) * a.cap + pe], a.st[(PX + 2) * a.cap + pe]};
     a.keys[id] = cell_key(P, a.thr, xe[0], xe[1], xe[2], oob);
   }
   if (fb) atomicAdd(a.fallback, (unsigned long long)fb);

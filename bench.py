@@ -10,7 +10,8 @@ throughput and its position error against the exact run after the same substeps.
 object reports the batched many-scene workload (64 independent snow/sand scenes of 20k-500k
 particles on 64^3-128^3 grids, one context, one launch per kernel per substep, autotuned launch
 shapes) against the same scenes run one after another with the plain exact path, with and without
-the learned return map. The `config4` object reports the learned return map's own workload (2^22 sand
+the learned return map. The `config3` object reports the cloth-sand coupling's throughput. The
+`config4` object reports the learned return map's own workload (2^22 sand
 particles with per-particle friction angles on a 256^3 grid): the exact path's throughput and the
 surrogate against it after 1020 substeps.
 
@@ -143,6 +144,33 @@ def surrogate_leg(sc, warmup, steps, x_exact, wall_exact, gate_tol=1e-5, pair_ch
     }, w
 
 
+def config3_leg(steps, warmup):
+    """Cloth-sand coupling: about 500k sand particles and a 256 x 256-vertex cloth (130,050 triangles)
+    on 128^3, dt = 2e-5: throughput of the exact path over all moving points (particles, cloth
+    vertices and elements)."""
+    from anisotropicelastoplasticity_amd import Solver, scenes
+    sc, mesh = scenes.config3_cloth_sand()
+    kw = {k: mesh[k] for k in ("youngs_modulus", "poisson_ratio", "shear_stiffness", "stiffness", "friction_angle",
+                               "density", "thickness")}
+    with Solver(0) as s:
+        sc.load_into(s)
+        s.set_mesh(mesh["x"], mesh["faces"], **kw)
+        s.run(sc.dt, warmup)
+        s.synchronize()
+        _sync_torch()
+        t0 = time.perf_counter()
+        s.run(sc.dt, steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        st = s.stats()
+    nv, nf = mesh["x"].shape[1], len(mesh["faces"])
+    npts = sc.n + nv + nf
+    return {"workload": "config3_cloth_sand", "sand_particles": sc.n, "cloth_vertices": nv, "cloth_elements": nf,
+            "grid": list(sc.res), "dt": sc.dt, "substeps": warmup + steps, "value": npts * steps / wall,
+            "unit": "point-substeps/s (particles + cloth vertices + elements)", "ms_per_step": 1e3 * wall / steps,
+            "status": st["status"]}
+
+
 def config4_leg(surrogate, steps, warmup):
     """The surrogate's workload (2^22 sand particles with per-particle friction angles on 256^3): the
     exact path's throughput, and the learned return map against it after warmup + steps substeps."""
@@ -258,17 +286,27 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned return-map leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
+    ap.add_argument("--no-config3", action="store_true", help="skip the cloth-sand leg")
+    ap.add_argument("--config3-steps", type=int, default=200, help="timed substeps of the cloth-sand leg (after 20)")
     ap.add_argument("--no-config4", action="store_true", help="skip the 2^22-particle per-particle-angle leg")
     ap.add_argument("--config4-steps", type=int, default=1000, help="timed substeps of the config-4 leg (after 20)")
     ap.add_argument("--batch-steps", type=int, default=480, help="timed substeps of the batched leg (after 20 warm-up)")
     ap.add_argument("--seq-steps", type=int, default=40, help="timed substeps per scene of the one-at-a-time baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--only", choices=["config3", "config4", "batched"],
+                    help="run this leg alone (exact path; for profiling) and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        leg = {"config3": lambda: config3_leg(args.config3_steps, 20),
+               "config4": lambda: config4_leg(False, args.config4_steps, 20),
+               "batched": lambda: batched_leg(False, args.batch_steps, 20, args.seq_steps)}[args.only]()
+        print(json.dumps(leg))
+        return
 
     from anisotropicelastoplasticity_amd import Solver, scenes
     sc = scenes.config2_sand_column()
@@ -299,6 +337,7 @@ def main():
 
     sur, weights = (None, None) if args.no_surrogate else surrogate_leg(sc, args.warmup, args.steps, x_exact, wall)
     bat = None if args.no_batch else batched_leg(not args.no_surrogate, args.batch_steps, 20, args.seq_steps)
+    c3 = None if args.no_config3 else config3_leg(args.config3_steps, 20)
     c4 = None if args.no_config4 else config4_leg(not args.no_surrogate, args.config4_steps, 20)
     value = n * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
@@ -346,6 +385,8 @@ def main():
         out["surrogate"] = sur
     if bat is not None:
         out["batched"] = bat
+    if c3 is not None:
+        out["config3"] = c3
     if c4 is not None:
         out["config4"] = c4
     if copy is not None:
