@@ -579,23 +579,30 @@ __global__ __launch_bounds__(256) void k_bounds(SceneTable T, const uint32_t* __
 }
 
 // Thread per tile (MODE 0) or node block (MODE 1): a flagged entry (-2) receives the next slot of
-// the active list (one atomic per wave). A claimed tile flags the 27 node blocks around it. Slot
+// the active list (one atomic per workgroup). A claimed tile flags the 27 node blocks around it. Slot
 // numbers depend on the order in which waves run; no result depends on them.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int* __restrict__ slot_of,
                                                int* __restrict__ active, int* __restrict__ counter,
                                                int* __restrict__ block_slot, const Clock* clk) {
   if (clock_skip(clk)) return;
+  __shared__ int s_cnt[4];
+  __shared__ int s_base;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool flagged = t < count && slot_of[t] == -2;
   const uint64_t mask = __ballot(flagged);
-  if (mask == 0ull) return;
-  const int lane = (int)lane_id();
-  const int first = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == first) base = atomicAdd(counter, __popcll(mask));
-  base = __shfl(base, first);
+  const int lane = (int)lane_id(), w = (int)threadIdx.x >> 6;
+  if (lane == 0) s_cnt[w] = __popcll(mask);
+  __syncthreads();
+  // one atomic per workgroup (a single counter takes many device-scope atomics slowly across XCDs)
+  if (threadIdx.x == 0) {
+    const int total = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = total ? atomicAdd(counter, total) : 0;
+  }
+  __syncthreads();
   if (!flagged) return;
+  int base = s_base;
+  for (int k = 0; k < w; ++k) base += s_cnt[k];
   const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
   slot_of[t] = slot;
   active[slot] = (int)t;
