@@ -578,46 +578,66 @@ __global__ __launch_bounds__(256) void k_bounds(SceneTable T, const uint32_t* __
   tile_slot[T.tile_off[b.s] + ((c2 >> 2) * P.tiles[1] + (c1 >> 2)) * P.tiles[0] + (c0 >> 2)] = -2;
 }
 
-// Thread per tile (MODE 0) or node block (MODE 1): a flagged entry (-2) receives the next slot of
-// the active list (one atomic per workgroup). A claimed tile flags the 27 node blocks around it. Slot
-// numbers depend on the order in which waves run; no result depends on them.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int* __restrict__ slot_of,
-                                               int* __restrict__ active, int* __restrict__ counter,
-                                               int* __restrict__ block_slot, const Clock* clk) {
+// Thread per tile t (tiles and node blocks share one index space). A flagged tile (-2) receives the
+// next slot of the active tile list. Node block t is active when any of its 27 neighbour tiles is
+// (the 7x7x7-node P2G footprint of those tiles covers it) and receives the next slot of the active
+// block list. A workgroup takes one atomic per list. While one thread reads a neighbour's entry,
+// another workgroup may be turning its -2 into a slot: both values mean "active", so the result does
+// not depend on the timing. Slot numbers depend on the order in which waves run; no result depends
+// on them.
+__global__ __launch_bounds__(256) void k_claim(SceneTable T, int64_t count, int* __restrict__ tile_slot,
+                                               int* __restrict__ active_tiles, int* __restrict__ block_slot,
+                                               int* __restrict__ active_blocks, int* __restrict__ counters,
+                                               const Clock* clk) {
   if (clock_skip(clk)) return;
-  __shared__ int s_cnt[4];
-  __shared__ int s_base;
+  __shared__ int s_cnt[2][4];
+  __shared__ int s_base[2];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool flagged = t < count && slot_of[t] == -2;
-  const uint64_t mask = __ballot(flagged);
-  const int lane = (int)lane_id(), w = (int)threadIdx.x >> 6;
-  if (lane == 0) s_cnt[w] = __popcll(mask);
-  __syncthreads();
-  // one atomic per workgroup (a single counter takes many device-scope atomics slowly across XCDs)
-  if (threadIdx.x == 0) {
-    const int total = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    s_base = total ? atomicAdd(counter, total) : 0;
-  }
-  __syncthreads();
-  if (!flagged) return;
-  int base = s_base;
-  for (int k = 0; k < w; ++k) base += s_cnt[k];
-  const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-  slot_of[t] = slot;
-  active[slot] = (int)t;
-  if (MODE == 0) {
+  bool tile_on = false, block_on = false;
+  if (t < count) {
     const int sc = scene_of(T.tile_off, T.nscenes, (int)t);
     const int* tl = T.params[sc].tiles;
-    const int lt = (int)t - T.tile_off[sc];
+    const int off = T.tile_off[sc], lt = (int)t - off;
     const int t0 = lt % tl[0], t1 = (lt / tl[0]) % tl[1], t2 = lt / (tl[0] * tl[1]);
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int b0 = t0 + dx, b1 = t1 + dy, b2 = t2 + dz;
-          if (b0 < 0 || b1 < 0 || b2 < 0 || b0 >= tl[0] || b1 >= tl[1] || b2 >= tl[2]) continue;
-          block_slot[T.tile_off[sc] + (b2 * tl[1] + b1) * tl[0] + b0] = -2;
-        }
+    int v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {  // all 27 loads issued together
+      const int b0 = t0 + q % 3 - 1, b1 = t1 + (q / 3) % 3 - 1, b2 = t2 + q / 9 - 1;
+      const bool in = b0 >= 0 && b1 >= 0 && b2 >= 0 && b0 < tl[0] && b1 < tl[1] && b2 < tl[2];
+      v[q] = in ? tile_slot[off + (b2 * tl[1] + b1) * tl[0] + b0] : -1;
+    }
+    tile_on = v[13] == -2;  // only this thread writes tile t
+#pragma unroll
+    for (int q = 0; q < 27; ++q) block_on |= v[q] != -1;
+  }
+  const uint64_t mt = __ballot(tile_on), mb = __ballot(block_on);
+  const int lane = (int)lane_id(), w = (int)threadIdx.x >> 6;
+  if (lane == 0) {
+    s_cnt[0][w] = __popcll(mt);
+    s_cnt[1][w] = __popcll(mb);
+  }
+  __syncthreads();
+  // one atomic per list and workgroup (a single counter takes many device-scope atomics slowly across XCDs)
+  if (threadIdx.x < 2) {
+    const int k = (int)threadIdx.x;
+    const int total = s_cnt[k][0] + s_cnt[k][1] + s_cnt[k][2] + s_cnt[k][3];
+    s_base[k] = total ? atomicAdd(&counters[k == 0 ? CNT_TILES : CNT_BLOCKS], total) : 0;
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (tile_on) {
+    int base = s_base[0];
+    for (int k = 0; k < w; ++k) base += s_cnt[0][k];
+    const int slot = base + __popcll(mt & below);
+    tile_slot[t] = slot;
+    active_tiles[slot] = (int)t;
+  }
+  if (block_on) {
+    int base = s_base[1];
+    for (int k = 0; k < w; ++k) base += s_cnt[1][k];
+    const int slot = base + __popcll(mb & below);
+    block_slot[t] = slot;
+    active_blocks[slot] = (int)t;
   }
 }
 
@@ -2155,10 +2175,8 @@ void launch_bounds(hipStream_t s, const SceneTable& T, int nblocks, const uint32
   if (n <= 0) return;
   if (mode != SORT_COUNTING)  // the counting sort's rank kernel has done this pass
     hipLaunchKernelGGL(k_bounds, dim3(nblocks), dim3(256), 0, s, T, ks, n, cell_count, tile_slot, perm, pos, src, clk);
-  hipLaunchKernelGGL(k_claim<0>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, tile_slot, active_tiles,
-                     counters + CNT_TILES, block_slot, clk);
-  hipLaunchKernelGGL(k_claim<1>, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, block_slot, active_blocks,
-                     counters + CNT_BLOCKS, (int*)nullptr, clk);
+  hipLaunchKernelGGL(k_claim, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, T, ntiles, tile_slot, active_tiles,
+                     block_slot, active_blocks, counters, clk);
 }
 
 template <int RANKS>
