@@ -177,6 +177,7 @@ struct G2PArgs {
   int n;
   const int* perm;     // caller ids in sorted order
   const int* src;      // sorted position -> physical index in sin
+  const uint32_t* ks;  // cell key of each sorted position (the last sort's; used only to place node boxes)
   int* pos_out;        // caller id -> physical index in sout
   const float4* gvel;
   float dt;

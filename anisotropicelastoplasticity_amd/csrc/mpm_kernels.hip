@@ -1426,6 +1426,17 @@ __global__ void k_clock_end(Clock* c) {
 }
 
 // ================================================================== G2P + F update + return map
+constexpr int G2P_BX = 12;               // x extent of a wave's node box (nodes)
+constexpr int G2P_BOX = G2P_BX * 4 * 4;  // nodes of the box: G2P_BX x 4 (y) x 4 (z)
+
+// floor(a / b) for 0 <= a < 2^31 and b > 0, given inv = 1/b in double: the product is within 2^-20
+// of a/b, so truncation is exact except just below an integer quotient, which one step corrects.
+__device__ __forceinline__ int floor_div(int a, int b, double inv) {
+  int q = (int)((double)a * inv);
+  if (((int64_t)q + 1) * b <= (int64_t)a) q += 1;
+  return q;
+}
+
 // Thread per sorted position j (caller id pid = perm[j]): gathers the stencil of grid v
 // (post-collision) and v_BF (pre-friction), computes v_p, APIC B (HybridSolver.cpp:739-825),
 // x += dt sum w v_BF (:942-950), F^ = (I + dt sum v_BF grad w^T) F (:553-577) and the return map
@@ -1441,25 +1452,82 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   const float4* gvel = a.gvel + 2 * noff;
   const int j = pb.j;
   int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
-  if (j < pb.end) {
-    const int pid = clamp_index(a.perm[j], a.n);
-    const int pi = clamp_index(a.src[j], a.n);
-    const float* st = a.sin;
-    const int64_t cap = a.cap;
-    float xp[3], g[3];
+  const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int64_t res0 = P.res[0], res01 = (int64_t)P.res[0] * P.res[1];
+  // The wave's node box. When the wave's positions lie in one row of cells (same y and z; the
+  // sorted cell keys of its first and last position tell), the nodes c-1 .. c+2 around those cells
+  // (at most G2P_BX x 4 x 4) are copied into the wave's LDS area, in the grid's own record layout,
+  // and the stencil gathers read them there: one round trip to HBM, made together with the
+  // particle loads, instead of one per z plane. The keys only place the box: a wave uses it only
+  // if every lane's stencil lies inside it, and it is clamped to the grid whatever the keys hold.
+  __shared__ float4 s_box[4][2 * G2P_BOX];
+  const double ir0 = a.T.params[pb.s].inv_res_d[0], ir1 = a.T.params[pb.s].inv_res_d[1];
+  // The particle's state and its sorted key, loaded by every lane (lanes past the end read the
+  // last position), so that the box below is placed while they are in flight.
+  const int jc = j < pb.end ? j : pb.end - 1;
+  const uint32_t kj = a.ks[jc];
+  const int perm_j = a.perm[jc], src_j = a.src[jc];
+  asm volatile("" ::: "memory");  // the three coalesced loads are issued together, first
+  const int pid = clamp_index(perm_j, a.n);
+  const int pi = clamp_index(src_j, a.n);
+  const float* st = a.sin;
+  const int64_t cap = a.cap;
+  float xp[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      xp[d] = st[(PX + d) * cap + pi];
-      g[d] = (xp[d] - P.gmin[d]) * P.inv_h[d];
+  for (int d = 0; d < 3; ++d) xp[d] = st[(PX + d) * cap + pi];
+  float Fn[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
+  float plastic = st[PPL * cap + pi];
+  const float phi_p = st[PPHI * cap + pi];
+  const float vol = st[PVOL * cap + pi];
+  const float mass = st[PM * cap + pi];
+  int bx0 = 0, by0 = 0, bz0 = 0, bnx = 0, bny = 0, bnz = 0;  // uniform
+  const int jw = __builtin_amdgcn_readfirstlane(j - lane);
+  if (jw < pb.end) {
+    const int64_t ncell = (int64_t)P.res[0] * P.res[1] * P.res[2];
+    const int64_t lf = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)kj, 0) - noff;
+    const int64_t ll = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)kj, 63) - noff;
+    if (lf >= 0 && ll >= lf && ll < ncell && ll <= INT32_MAX) {
+      const int rf = floor_div((int)lf, P.res[0], ir0);
+      const int rl = floor_div((int)ll, P.res[0], ir0);
+      if (rf == rl) {
+        const int xf = (int)lf - rf * P.res[0], xl = (int)ll - rf * P.res[0];
+        const int zc = floor_div(rf, P.res[1], ir1), yc = rf - zc * P.res[1];
+        bx0 = max(xf - 1, 0);
+        by0 = max(yc - 1, 0);
+        bz0 = max(zc - 1, 0);
+        bnx = min(xl + 2, P.res[0] - 1) - bx0 + 1;
+        bny = min(yc + 2, P.res[1] - 1) - by0 + 1;
+        bnz = min(zc + 2, P.res[2] - 1) - bz0 + 1;
+        if (bnx > G2P_BX || bnx < 1 || bny < 1 || bnz < 1) bnx = 0;  // bnx = 0: no box
+      }
     }
-    // the rest of the particle's state is loaded now, so that it arrives during the gather
-    float Fn[9];
+  }
+  if (bnx > 0) {
+    float4 t[2 * G2P_BOX / 64];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fn[i] = st[(PF + i) * cap + pi];
-    float plastic = st[PPL * cap + pi];
-    const float phi_p = st[PPHI * cap + pi];
-    const float vol = st[PVOL * cap + pi];
-    const float mass = st[PM * cap + pi];
+    for (int it = 0; it < G2P_BOX / 64; ++it) {
+      const int i = it * 64 + lane;
+      const int ix = i % G2P_BX, row = i / G2P_BX, iy = row & 3, iz = row >> 2;
+      const bool in = ix < bnx && iy < bny && iz < bnz;
+      const int64_t node = (bz0 + (in ? iz : 0)) * res01 + (by0 + (in ? iy : 0)) * res0 + (bx0 + (in ? ix : 0));
+      t[2 * it] = gvel[2 * node];
+      t[2 * it + 1] = gvel[2 * node + 1];
+    }
+#pragma unroll
+    for (int it = 0; it < G2P_BOX / 64; ++it) {
+      const int i = it * 64 + lane;
+      s_box[wv][2 * i] = t[2 * it];
+      s_box[wv][2 * i + 1] = t[2 * it + 1];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (j < pb.end) {
+    float g[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[d] = (xp[d] - P.gmin[d]) * P.inv_h[d];
     int base[3];
     float w[3][4], dw[3][4];
 #pragma unroll
@@ -1497,7 +1565,13 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     float Bn[9], Gv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) { Bn[i] = 0.0f; Gv[i] = 0.0f; }
-    const int64_t res0 = P.res[0], res01 = (int64_t)P.res[0] * P.res[1];
+    // node addressing: the wave's box when every lane's stencil lies inside it, else the grid
+    const bool inside = nid[0][0] >= bx0 && nid[0][ORDER] < bx0 + bnx && nid[1][0] >= by0 &&
+                        nid[1][ORDER] < by0 + bny && nid[2][0] >= bz0 && nid[2][ORDER] < bz0 + bnz;
+    const bool use_box = bnx > 0 && __ballot(!inside) == 0;
+    const float4* nodes = use_box ? &s_box[wv][0] : gvel;
+    const int64_t sy = use_box ? G2P_BX : res0, sz = use_box ? 4 * G2P_BX : res01;
+    const int ox0 = use_box ? bx0 : 0, oy0 = use_box ? by0 : 0, oz0 = use_box ? bz0 : 0;
     // one z plane of rows per iteration (not unrolled): its loads are in flight together without
     // holding the whole stencil in registers
 #pragma unroll 1
@@ -1511,18 +1585,18 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
         nz = oz == o ? nid[2][o] : nz;
       }
       const float dz = ((float)(base[2] + oz) - g[2]) * P.h[2];
-      const float4* pp = gvel + 2 * (nz * res01);
+      const float4* pp = nodes + 2 * ((nz - oz0) * sz);
       // the plane's (ORDER+1)^2 nodes are loaded together and waited for once (v and v_BF of a
       // node are its two float4, of which the second holds v_BF y, z in x, y)
       float4 qa[ORDER + 1][ORDER + 1];
       float2 qb[ORDER + 1][ORDER + 1];
 #pragma unroll
       for (int oy = 0; oy < ORDER + 1; ++oy) {
-        const float4* rp = pp + 2 * (nid[1][oy] * res0);
+        const float4* rp = pp + 2 * ((nid[1][oy] - oy0) * sy);
 #pragma unroll
         for (int ox = 0; ox < ORDER + 1; ++ox) {
-          qa[oy][ox] = rp[2 * nid[0][ox]];
-          qb[oy][ox] = *reinterpret_cast<const float2*>(rp + 2 * nid[0][ox] + 1);
+          qa[oy][ox] = rp[2 * (nid[0][ox] - ox0)];
+          qb[oy][ox] = *reinterpret_cast<const float2*>(rp + 2 * (nid[0][ox] - ox0) + 1);
         }
       }
       // keep the plane's loads ahead of their uses: issue all of them, then the arithmetic

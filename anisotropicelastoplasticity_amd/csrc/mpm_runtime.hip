@@ -252,6 +252,8 @@ static DevParams devparams_of(const SceneHost& sc) {
     d.res[a] = sc.res[a];
     d.tiles[a] = sc.tiles[a];
   }
+  d.inv_res_d[0] = 1.0 / sc.res[0];
+  d.inv_res_d[1] = 1.0 / sc.res[1];
   d.nls = (int)sc.ls.size();
   for (int k = 0; k < d.nls; ++k) d.ls[k] = sc.ls[k];
   d.mesh_v0 = sc.mesh_v0;
@@ -493,6 +495,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.n = ctx->n;
   a.perm = ctx->vs_sorted;
   a.src = ctx->src;
+  a.ks = ctx->ks_sorted;
   a.pos_out = ctx->pos[1 - ctx->cur];
   a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
