@@ -518,32 +518,49 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const uint32_t* __restrict
 // within the cell (a cell of c particles costs c comparisons per particle). It also does the bin
 // pass's work (k_bounds) for its new position: src = pos of the caller id, and the cell's first
 // particle clears the cell's count and flags the cell's tile.
+// The workgroup first copies the ids of its 256 positions and RANK_HALO on either side into LDS (one
+// coalesced round trip): a cell of at most RANK_HALO particles around any of its positions lies in
+// that window, and its ids are compared there; a larger cell reads them from global memory.
+constexpr int RANK_HALO = 64;
 __global__ __launch_bounds__(256) void k_cell_rank(SceneTable T, const uint32_t* __restrict__ keys,
                                                    const int* __restrict__ tmp, int n, const int* __restrict__ cell_start,
                                                    uint32_t* __restrict__ ks, int* __restrict__ vs, int* __restrict__ cnt,
                                                    int* __restrict__ tile_slot, const int* __restrict__ pos,
                                                    int* __restrict__ src, const Clock* clk) {
   if (clock_skip(clk)) return;
+  constexpr int WIN = 256 + 2 * RANK_HALO;
+  __shared__ int s_ids[WIN];
+  const int wlo = (int)blockIdx.x * 256 - RANK_HALO;  // the window holds positions wlo .. wlo + WIN - 1
+  for (int q = (int)threadIdx.x; q < WIN; q += 256) {
+    const int p = wlo + q;
+    s_ids[q] = (p >= 0 && p < n) ? tmp[p] : 0;
+  }
+  __syncthreads();
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
-  const int i = clamp_index(tmp[j], n);
+  const int i = clamp_index(s_ids[j - wlo], n);
   const uint32_t k = keys[i];
+  const int pi = pos[i];
   const int s0 = cell_start[k], e0 = min(cell_start[k + 1], n);
-  // 8 ids per step, loaded together (past the cell's end the loads read its last entry, in
-  // bounds, and are not counted)
   int r = 0;
-  for (int m0 = s0; m0 < e0; m0 += 8) {
-    int t[8];
+  if (s0 >= wlo && e0 <= wlo + WIN) {
+    for (int m = s0 - wlo; m < e0 - wlo; ++m) r += s_ids[m] < i ? 1 : 0;
+  } else {
+    // 8 ids per step, loaded together (past the cell's end the loads read its last entry, in
+    // bounds, and are not counted)
+    for (int m0 = s0; m0 < e0; m0 += 8) {
+      int t[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t[u] = tmp[min(m0 + u, e0 - 1)];
+      for (int u = 0; u < 8; ++u) t[u] = tmp[min(m0 + u, e0 - 1)];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) r += (m0 + u < e0 && t[u] < i) ? 1 : 0;
+      for (int u = 0; u < 8; ++u) r += (m0 + u < e0 && t[u] < i) ? 1 : 0;
+    }
   }
   const int d = s0 + r;
   if ((unsigned)d >= (unsigned)n) return;  // never, for consistent counts
   vs[d] = i;
   ks[d] = k;
-  src[d] = clamp_index(pos[i], n);
+  src[d] = clamp_index(pi, n);
   if (r != 0) return;
   cnt[k] = 0;
   const int sc = scene_of(T.part_off, T.nscenes, i);
