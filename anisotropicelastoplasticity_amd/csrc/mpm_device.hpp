@@ -106,13 +106,19 @@ __device__ __forceinline__ T mat_det(const T* A) {
   return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) + A[2] * (A[3] * A[7] - A[4] * A[6]);
 }
 
-// Reciprocal and reciprocal square root for the rotation parameters of the Jacobi SVD: the hardware
-// approximations (1 ulp) in FP32, exact operations in FP64. Any rotation angle is valid for the
-// iteration; only c^2 + s^2 = 1 matters, which holds to 2 ulp with c = rsqrt(1 + t^2), s = c t.
+// Reciprocal, square root and reciprocal square root for the Jacobi SVD: the hardware approximations
+// (1 ulp) in FP32, exact operations in FP64. Any rotation angle is valid for the iteration; only
+// c^2 + s^2 = 1 matters, which holds to 2 ulp with c = rsqrt(1 + t^2), s = c t. The singular values
+// (column norms) and the normalised columns of U carry 1-2 ulp as well.
 template <class T>
 __device__ __forceinline__ T jac_rcp(T x) {
   if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
   else return T(1) / x;
+}
+template <class T>
+__device__ __forceinline__ T jac_sqrt(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_sqrtf(x);
+  else return sqrt(x);
 }
 template <class T>
 __device__ __forceinline__ T jac_rsqrt(T x) {
@@ -145,7 +151,7 @@ __device__ __forceinline__ bool svd3(const T* A, T* U, T* S, T* V, int max_sweep
         rotated = true;
         T zeta = (beta - alpha) * jac_rcp(T(2) * gamma);
         T az = zeta < T(0) ? -zeta : zeta;
-        T t = (zeta >= T(0) ? T(1) : T(-1)) * jac_rcp(az + sqrt(T(1) + zeta * zeta));
+        T t = (zeta >= T(0) ? T(1) : T(-1)) * jac_rcp(az + jac_sqrt(T(1) + zeta * zeta));
         T c = jac_rsqrt(T(1) + t * t);
         T s = c * t;
 #pragma unroll
@@ -163,7 +169,7 @@ __device__ __forceinline__ bool svd3(const T* A, T* U, T* S, T* V, int max_sweep
   }
   T s[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) s[c] = sqrt(w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c]);
+  for (int c = 0; c < 3; ++c) s[c] = jac_sqrt(w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c]);
   // sort columns by decreasing singular value
   int o0 = 0, o1 = 1, o2 = 2;
   if (s[o0] < s[o1]) { int t = o0; o0 = o1; o1 = t; }
@@ -175,7 +181,7 @@ __device__ __forceinline__ bool svd3(const T* A, T* U, T* S, T* V, int max_sweep
     int oc = ord[c];
     T sc = oc == 0 ? s[0] : (oc == 1 ? s[1] : s[2]);
     S[c] = sc;
-    T inv = sc > T(0) ? T(1) / sc : T(0);
+    T inv = sc > T(0) ? jac_rcp(sc) : T(0);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       T wv = oc == 0 ? w[3 * r] : (oc == 1 ? w[3 * r + 1] : w[3 * r + 2]);
