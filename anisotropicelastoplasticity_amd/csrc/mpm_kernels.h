@@ -35,7 +35,9 @@ constexpr int SCAN_TILE = 256 * SCAN_IPT;  // cells per cell-scan block
 constexpr int TILE = 4;                    // cells per tile edge
 constexpr int FOOT = TILE + 3;             // footprint nodes per edge (cell window c-1 .. c+2)
 constexpr int FOOT3 = FOOT * FOOT * FOOT;
-constexpr int PART_STRIDE = 7 * FOOT3;     // m, p(3), f(3) per tile slot
+// Per tile slot: m, p(3) as one float4 per footprint node, then f(3) as three planes; padded to a
+// multiple of 4 floats so that every slot's float4 part is 16-byte aligned.
+constexpr int PART_STRIDE = 7 * FOOT3 + 3;
 
 enum Counter : int { CNT_TILES = 0, CNT_BLOCKS = 1, CNT_CAND = 2, CNT_APEX = 3, CNT_NUM = 4 };
 enum P2GMode : int { P2G_MOM_STRESS = 0, P2G_MOM = 1, P2G_FORCE = 2 };

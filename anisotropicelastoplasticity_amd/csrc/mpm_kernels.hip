@@ -1126,8 +1126,12 @@ __global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
           s_slab[wv][q][ln] = 0.0f;
         }
       }
+      if constexpr (NQ == 4) {
+        reinterpret_cast<float4*>(part)[i] = make_float4(s[0], s[1], s[2], s[3]);
+      } else {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = s[q];
+        for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = s[q];
+      }
     }
     __syncthreads();
   }
@@ -1288,8 +1292,12 @@ __global__ __launch_bounds__(256) void k_p2g_layer(P2GArgs a) {
           s_slab[wv][q][ln] = 0.0f;
         }
       }
+      if constexpr (NQ == 4) {
+        reinterpret_cast<float4*>(part)[i] = make_float4(sum[0], sum[1], sum[2], sum[3]);
+      } else {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = sum[q];
+        for (int q = 0; q < NQ; ++q) part[q * FOOT3 + i] = sum[q];
+      }
     }
     __syncthreads();
   }
@@ -1353,10 +1361,19 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
       const int sl = __shfl(slot27, q27);  // q27 is in [0, 27): lanes 0..26 hold the slots
       use[k] = in_grid && t0 <= thi[0] && t1 <= thi[1] && t2 <= thi[2] && q27 >= 0 && q27 < 27 && sl >= 0;
       const int ln = (n0 - (4 * t0 - 1)) + FOOT * (n1 - (4 * t1 - 1)) + FOOT * FOOT * (n2 - (4 * t2 - 1));
-      const float* part = a.partials + (size_t)(use[k] ? sl : 0) * PART_STRIDE + qoff * FOOT3 + (use[k] ? ln : 0);
+      const float* part = a.partials + (size_t)(use[k] ? sl : 0) * PART_STRIDE + qoff * FOOT3;
+      const int pn = use[k] ? ln : 0;
       // unconditional loads (an unused tile reads slot 0, node 0: in bounds) keep them all in flight
+      if constexpr (NQ == 4) {  // m, p of a node: one float4
+        const float4 p4 = reinterpret_cast<const float4*>(part)[pn];
+        val[k][0] = p4.x;
+        val[k][1] = p4.y;
+        val[k][2] = p4.z;
+        val[k][3] = p4.w;
+      } else {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) val[k][q] = part[q * FOOT3];
+        for (int q = 0; q < NQ; ++q) val[k][q] = part[q * FOOT3 + pn];
+      }
     }
     float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
