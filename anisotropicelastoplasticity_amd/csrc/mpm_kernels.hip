@@ -1603,9 +1603,13 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     const bool inside = nid[0][0] >= bx0 && nid[0][ORDER] < bx0 + bnx && nid[1][0] >= by0 &&
                         nid[1][ORDER] < by0 + bny && nid[2][0] >= bz0 && nid[2][ORDER] < bz0 + bnz;
     const bool use_box = bnx > 0 && __ballot(!inside) == 0;
-    const float4* nodes = use_box ? &s_box[wv][0] : gvel;
-    const int64_t sy = use_box ? G2P_BX : res0, sz = use_box ? 4 * G2P_BX : res01;
-    const int ox0 = use_box ? bx0 : 0, oy0 = use_box ? by0 : 0, oz0 = use_box ? bz0 : 0;
+    // LDS offsets (in float4) of the stencil's x and y nodes inside the box
+    int bxo[ORDER + 1], byo[ORDER + 1];
+#pragma unroll
+    for (int o = 0; o < ORDER + 1; ++o) {
+      bxo[o] = 2 * (nid[0][o] - bx0);
+      byo[o] = 2 * G2P_BX * (nid[1][o] - by0);
+    }
     // one z plane of rows per iteration (not unrolled): its loads are in flight together without
     // holding the whole stencil in registers
 #pragma unroll 1
@@ -1619,23 +1623,32 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
         nz = oz == o ? nid[2][o] : nz;
       }
       const float dz = ((float)(base[2] + oz) - g[2]) * P.h[2];
-      const float4* pp = nodes + 2 * ((nz - oz0) * sz);
       // the plane's (ORDER+1)^2 nodes are loaded together and waited for once (v and v_BF of a
-      // node are its two float4, of which the second holds v_BF y, z in x, y)
+      // node are its two float4, of which the second holds v_BF y, z in x, y), from the wave's box
+      // in LDS or from the grid (the branch is uniform)
       float4 qa[ORDER + 1][ORDER + 1];
       float2 qb[ORDER + 1][ORDER + 1];
+      if (use_box) {
+        const float4* pp = &s_box[wv][0] + 2 * (4 * G2P_BX) * (nz - bz0);
 #pragma unroll
-      for (int oy = 0; oy < ORDER + 1; ++oy) {
-        const float4* rp = pp + 2 * ((nid[1][oy] - oy0) * sy);
+        for (int oy = 0; oy < ORDER + 1; ++oy)
 #pragma unroll
-        for (int ox = 0; ox < ORDER + 1; ++ox) {
-          qa[oy][ox] = rp[2 * (nid[0][ox] - ox0)];
-          qb[oy][ox] = *reinterpret_cast<const float2*>(rp + 2 * (nid[0][ox] - ox0) + 1);
+          for (int ox = 0; ox < ORDER + 1; ++ox) {
+            qa[oy][ox] = pp[byo[oy] + bxo[ox]];
+            qb[oy][ox] = *reinterpret_cast<const float2*>(pp + byo[oy] + bxo[ox] + 1);
+          }
+      } else {
+        const float4* pp = gvel + 2 * (nz * res01);
+#pragma unroll
+        for (int oy = 0; oy < ORDER + 1; ++oy) {
+          const float4* rp = pp + 2 * (nid[1][oy] * res0);
+#pragma unroll
+          for (int ox = 0; ox < ORDER + 1; ++ox) {
+            qa[oy][ox] = rp[2 * nid[0][ox]];
+            qb[oy][ox] = *reinterpret_cast<const float2*>(rp + 2 * nid[0][ox] + 1);
+          }
         }
       }
-      // keep the plane's loads ahead of their uses: issue all of them, then the arithmetic
-      __builtin_amdgcn_sched_group_barrier(0x0020, 2 * (ORDER + 1) * (ORDER + 1), 0);
-      __builtin_amdgcn_sched_group_barrier(0x0002, 1000, 0);
 #pragma unroll
       for (int oy = 0; oy < ORDER + 1; ++oy) {
         const float dy = ((float)(base[1] + oy) - g[1]) * P.h[1];
