@@ -1460,8 +1460,31 @@ __global__ void k_clock_end(Clock* c) {
 }
 
 // ================================================================== G2P + F update + return map
-constexpr int G2P_BX = 12;               // x extent of a wave's node box (nodes)
+constexpr int G2P_BX = 8;                // x extent of a wave's node box (nodes)
 constexpr int G2P_BOX = G2P_BX * 4 * 4;  // nodes of the box: G2P_BX x 4 (y) x 4 (z)
+
+// A node box's nodes, G2P_BOX / 64 per lane (slot i = it * 64 + lane: x = i % G2P_BX, then y, then
+// z), in the grid's record layout; slots outside the box read the box's corner (in bounds).
+__device__ __forceinline__ void g2p_box_load(const float4* gvel, int lane, int x0, int y0, int z0, int nx, int ny,
+                                             int nz, int64_t res0, int64_t res01, float4 (&t)[2 * G2P_BOX / 64]) {
+#pragma unroll
+  for (int it = 0; it < G2P_BOX / 64; ++it) {
+    const int i = it * 64 + lane;
+    const int ix = i % G2P_BX, row = i / G2P_BX, iy = row & 3, iz = row >> 2;
+    const bool in = ix < nx && iy < ny && iz < nz;
+    const int64_t node = (z0 + (in ? iz : 0)) * res01 + (y0 + (in ? iy : 0)) * res0 + (x0 + (in ? ix : 0));
+    t[2 * it] = gvel[2 * node];
+    t[2 * it + 1] = gvel[2 * node + 1];
+  }
+}
+__device__ __forceinline__ void g2p_box_store(float4* box, int lane, const float4 (&t)[2 * G2P_BOX / 64]) {
+#pragma unroll
+  for (int it = 0; it < G2P_BOX / 64; ++it) {
+    const int i = it * 64 + lane;
+    box[2 * i] = t[2 * it];
+    box[2 * i + 1] = t[2 * it + 1];
+  }
+}
 
 // floor(a / b) for 0 <= a < 2^31 and b > 0, given inv = 1/b in double: the product is within 2^-20
 // of a/b, so truncation is exact except just below an integer quotient, which one step corrects.
@@ -1488,13 +1511,14 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
   const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
   const int64_t res0 = P.res[0], res01 = (int64_t)P.res[0] * P.res[1];
-  // The wave's node box. When the wave's positions lie in one row of cells (same y and z; the
-  // sorted cell keys of its first and last position tell), the nodes c-1 .. c+2 around those cells
-  // (at most G2P_BX x 4 x 4) are copied into the wave's LDS area, in the grid's own record layout,
-  // and the stencil gathers read them there: one round trip to HBM, made together with the
-  // particle loads, instead of one per z plane. The keys only place the box: a wave uses it only
-  // if every lane's stencil lies inside it, and it is clamped to the grid whatever the keys hold.
-  __shared__ float4 s_box[4][2 * G2P_BOX];
+  // The wave's node boxes. When the wave's positions lie in one row of cells, or in two consecutive
+  // rows of one z plane (the sorted cell keys of its positions tell), the nodes c-1 .. c+2 around
+  // each row's cells (at most G2P_BX x 4 x 4 per row) are copied into the wave's LDS area, in the
+  // grid's own record layout, and the stencil gathers read them there: one round trip to HBM, made
+  // together with the particle loads, instead of one per z plane. The keys only place the boxes: a
+  // wave uses them only if every lane's stencil lies inside its row's box, and they are clamped to
+  // the grid whatever the keys hold.
+  __shared__ float4 s_box[4][2][2 * G2P_BOX];
   const double ir0 = a.T.params[pb.s].inv_res_d[0], ir1 = a.T.params[pb.s].inv_res_d[1];
   // The particle's state and its sorted key, loaded by every lane (lanes past the end read the
   // last position), so that the box below is placed while they are in flight.
@@ -1516,7 +1540,11 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   const float phi_p = st[PPHI * cap + pi];
   const float vol = st[PVOL * cap + pi];
   const float mass = st[PM * cap + pi];
-  int bx0 = 0, by0 = 0, bz0 = 0, bnx = 0, bny = 0, bnz = 0;  // uniform
+  // Box q covers x bx0q .. +bnxq, y by0q .. +bnyq, z bz0 .. +bnz (uniform). One box when the wave's
+  // cells are in one row; two when they are in two consecutive rows of one z plane (lanes below
+  // `split` in the first). nbox = 0: the wave gathers from the grid.
+  int bx00 = 0, by00 = 0, bnx0 = 0, bny0 = 0, bx01 = 0, by01 = 0, bnx1 = 0, bny1 = 0;
+  int bz0 = 0, bnz = 0, nbox = 0, split = 64;
   const int jw = __builtin_amdgcn_readfirstlane(j - lane);
   if (jw < pb.end) {
     const int64_t ncell = (int64_t)P.res[0] * P.res[1] * P.res[2];
@@ -1525,36 +1553,46 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     if (lf >= 0 && ll >= lf && ll < ncell && ll <= INT32_MAX) {
       const int rf = floor_div((int)lf, P.res[0], ir0);
       const int rl = floor_div((int)ll, P.res[0], ir0);
+      const int zc = floor_div(rf, P.res[1], ir1), yc = rf - zc * P.res[1];
+      const int xf = (int)lf - rf * P.res[0], xl = (int)ll - rl * P.res[0];
+      int xa = xl, xb = xf;  // last x of the first row, first x of the second
       if (rf == rl) {
-        const int xf = (int)lf - rf * P.res[0], xl = (int)ll - rf * P.res[0];
-        const int zc = floor_div(rf, P.res[1], ir1), yc = rf - zc * P.res[1];
-        bx0 = max(xf - 1, 0);
-        by0 = max(yc - 1, 0);
-        bz0 = max(zc - 1, 0);
-        bnx = min(xl + 2, P.res[0] - 1) - bx0 + 1;
-        bny = min(yc + 2, P.res[1] - 1) - by0 + 1;
-        bnz = min(zc + 2, P.res[2] - 1) - bz0 + 1;
-        if (bnx > G2P_BX || bnx < 1 || bny < 1 || bnz < 1) bnx = 0;  // bnx = 0: no box
+        nbox = 1;
+      } else if (rl == rf + 1 && yc + 1 < P.res[1]) {
+        // keys are sorted: the lanes whose cell precedes row rl's first cell are the first row's
+        const int64_t r0 = P.res[0], row_end = (int64_t)rl * r0;
+        split = __popcll(__ballot((int64_t)(uint32_t)kj - noff < row_end));
+        if (split >= 1 && split <= 63) {
+          // (clamped to the row: only keys that are not sorted could fall outside it)
+          const int64_t ka = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)kj, split - 1) - noff - (int64_t)rf * r0;
+          const int64_t kb = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)kj, split) - noff - row_end;
+          xa = (int)(ka < 0 ? 0 : (ka >= r0 ? r0 - 1 : ka));
+          xb = (int)(kb < 0 ? 0 : (kb >= r0 ? r0 - 1 : kb));
+          nbox = 2;
+        }
       }
+      if (nbox == 1) split = 64;
+      bz0 = max(zc - 1, 0);
+      bnz = min(zc + 2, P.res[2] - 1) - bz0 + 1;
+      bx00 = max(xf - 1, 0);
+      bnx0 = min((nbox == 2 ? xa : xl) + 2, P.res[0] - 1) - bx00 + 1;
+      by00 = max(yc - 1, 0);
+      bny0 = min(yc + 2, P.res[1] - 1) - by00 + 1;
+      bx01 = max(xb - 1, 0);
+      bnx1 = min(xl + 2, P.res[0] - 1) - bx01 + 1;
+      by01 = max(yc, 0);
+      bny1 = min(yc + 3, P.res[1] - 1) - by01 + 1;
+      bool ok = bnz >= 1 && bnx0 >= 1 && bnx0 <= G2P_BX && bny0 >= 1;
+      if (nbox == 2) ok = ok && bnx1 >= 1 && bnx1 <= G2P_BX && bny1 >= 1;
+      if (!ok) nbox = 0;
     }
   }
-  if (bnx > 0) {
-    float4 t[2 * G2P_BOX / 64];
-#pragma unroll
-    for (int it = 0; it < G2P_BOX / 64; ++it) {
-      const int i = it * 64 + lane;
-      const int ix = i % G2P_BX, row = i / G2P_BX, iy = row & 3, iz = row >> 2;
-      const bool in = ix < bnx && iy < bny && iz < bnz;
-      const int64_t node = (bz0 + (in ? iz : 0)) * res01 + (by0 + (in ? iy : 0)) * res0 + (bx0 + (in ? ix : 0));
-      t[2 * it] = gvel[2 * node];
-      t[2 * it + 1] = gvel[2 * node + 1];
-    }
-#pragma unroll
-    for (int it = 0; it < G2P_BOX / 64; ++it) {
-      const int i = it * 64 + lane;
-      s_box[wv][2 * i] = t[2 * it];
-      s_box[wv][2 * i + 1] = t[2 * it + 1];
-    }
+  if (nbox > 0) {
+    float4 t0[2 * G2P_BOX / 64], t1[2 * G2P_BOX / 64];
+    g2p_box_load(gvel, lane, bx00, by00, bz0, bnx0, bny0, bnz, res0, res01, t0);
+    if (nbox == 2) g2p_box_load(gvel, lane, bx01, by01, bz0, bnx1, bny1, bnz, res0, res01, t1);
+    g2p_box_store(&s_box[wv][0][0], lane, t0);
+    if (nbox == 2) g2p_box_store(&s_box[wv][1][0], lane, t1);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
@@ -1600,15 +1638,17 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) { Bn[i] = 0.0f; Gv[i] = 0.0f; }
     // node addressing: the wave's box when every lane's stencil lies inside it, else the grid
-    const bool inside = nid[0][0] >= bx0 && nid[0][ORDER] < bx0 + bnx && nid[1][0] >= by0 &&
-                        nid[1][ORDER] < by0 + bny && nid[2][0] >= bz0 && nid[2][ORDER] < bz0 + bnz;
-    const bool use_box = bnx > 0 && __ballot(!inside) == 0;
-    // LDS offsets (in float4) of the stencil's x and y nodes inside the box
+    const int mb = lane >= split ? 1 : 0;  // this lane's box
+    const int lx0 = mb ? bx01 : bx00, ly0 = mb ? by01 : by00, lnx = mb ? bnx1 : bnx0, lny = mb ? bny1 : bny0;
+    const bool inside = nid[0][0] >= lx0 && nid[0][ORDER] < lx0 + lnx && nid[1][0] >= ly0 &&
+                        nid[1][ORDER] < ly0 + lny && nid[2][0] >= bz0 && nid[2][ORDER] < bz0 + bnz;
+    const bool use_box = nbox > 0 && __ballot(!inside) == 0;
+    // LDS offsets (in float4, from the wave's first box) of the stencil's x and y nodes in its box
     int bxo[ORDER + 1], byo[ORDER + 1];
 #pragma unroll
     for (int o = 0; o < ORDER + 1; ++o) {
-      bxo[o] = 2 * (nid[0][o] - bx0);
-      byo[o] = 2 * G2P_BX * (nid[1][o] - by0);
+      bxo[o] = 2 * (nid[0][o] - lx0);
+      byo[o] = 2 * G2P_BOX * mb + 2 * G2P_BX * (nid[1][o] - ly0);
     }
     // one z plane of rows per iteration (not unrolled): its loads are in flight together without
     // holding the whole stencil in registers
@@ -1629,7 +1669,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
       float4 qa[ORDER + 1][ORDER + 1];
       float2 qb[ORDER + 1][ORDER + 1];
       if (use_box) {
-        const float4* pp = &s_box[wv][0] + 2 * (4 * G2P_BX) * (nz - bz0);
+        const float4* pp = &s_box[wv][0][0] + 2 * (4 * G2P_BX) * (nz - bz0);
 #pragma unroll
         for (int oy = 0; oy < ORDER + 1; ++oy)
 #pragma unroll
