@@ -1460,7 +1460,7 @@ __global__ void k_clock_end(Clock* c) {
 }
 
 // ================================================================== G2P + F update + return map
-constexpr int G2P_BX = 8;                // x extent of a wave's node box (nodes)
+constexpr int G2P_BX = 12;               // x extent of a wave's node box (nodes)
 constexpr int G2P_BOX = G2P_BX * 4 * 4;  // nodes of the box: G2P_BX x 4 (y) x 4 (z)
 
 // A node box's nodes, G2P_BOX / 64 per lane (slot i = it * 64 + lane: x = i % G2P_BX, then y, then
@@ -1588,11 +1588,18 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     }
   }
   if (nbox > 0) {
-    float4 t0[2 * G2P_BOX / 64], t1[2 * G2P_BOX / 64];
-    g2p_box_load(gvel, lane, bx00, by00, bz0, bnx0, bny0, bnz, res0, res01, t0);
-    if (nbox == 2) g2p_box_load(gvel, lane, bx01, by01, bz0, bnx1, bny1, bnz, res0, res01, t1);
-    g2p_box_store(&s_box[wv][0][0], lane, t0);
-    if (nbox == 2) g2p_box_store(&s_box[wv][1][0], lane, t1);
+    // the second row's box after the first (one more round trip for those waves, but no more live
+    // registers than one box needs)
+    {
+      float4 t[2 * G2P_BOX / 64];
+      g2p_box_load(gvel, lane, bx00, by00, bz0, bnx0, bny0, bnz, res0, res01, t);
+      g2p_box_store(&s_box[wv][0][0], lane, t);
+    }
+    if (nbox == 2) {
+      float4 t[2 * G2P_BOX / 64];
+      g2p_box_load(gvel, lane, bx01, by01, bz0, bnx1, bny1, bnz, res0, res01, t);
+      g2p_box_store(&s_box[wv][1][0], lane, t);
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
