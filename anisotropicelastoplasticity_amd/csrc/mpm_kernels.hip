@@ -373,6 +373,10 @@ __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ rank, ResetArgs r, int reset,
                                                     const Clock* clk) {
   if (clock_skip(clk)) return;
+  // the key is loaded first, so that its round trip overlaps the reset's
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < n;
+  const uint32_t key = valid ? keys[i] : 0u;
   if (reset) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -381,9 +385,6 @@ __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__
     for (int64_t q = t0; q < nt; q += stride) r.tile_slot[r.active_tiles[q]] = -1;
     for (int64_t q = t0; q < nb; q += stride) r.block_slot[r.active_blocks[q]] = -1;
   }
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i < n;
-  const uint32_t key = valid ? keys[i] : 0u;
   const int lane = (int)lane_id();
   const uint64_t lt = (1ull << lane) - 1ull;
   uint64_t todo = __ballot(valid);
