@@ -363,3 +363,51 @@ def test_time_kernels_split():
     assert all(t > 0.0 for t in stages) and total >= sum(stages) * 0.99
     assert 0.0 < kern[3] <= stages[3]
     assert kern[:3] == stages[:3]
+
+
+def _g2p_box_cases(keys_sorted, res, bx=12):
+    """Which stencil source each 64-position wave of k_g2p uses, restated from the kernel's placement
+    rule: 1 = one node box (the wave's cells lie in one x row, at most bx nodes wide), 2 = two boxes
+    (two consecutive rows of one z plane), 0 = gathers from the grid."""
+    r0, r1 = res[0], res[1]
+    k = np.asarray(keys_sorted, np.int64)
+    nw = (len(k) + 63) // 64
+    kp = np.concatenate([k, np.full(nw * 64 - len(k), k[-1])]).reshape(nw, 64)
+    cases = []
+    for w in kp:
+        rf, rl = w[0] // r0, w[-1] // r0
+        yc = rf % r1
+        if rf == rl:
+            spans = [(w[0] % r0, w[-1] % r0)]
+        elif rl == rf + 1 and yc + 1 < r1:
+            split = int(np.sum(w < rl * r0))
+            spans = [(w[0] % r0, w[split - 1] % r0), (w[split] % r0, w[-1] % r0)]
+        else:
+            cases.append(0)
+            continue
+        fits = all(min(hi + 2, r0 - 1) - max(lo - 1, 0) + 1 <= bx for lo, hi in spans)
+        cases.append(len(spans) if fits else 0)
+    return np.array(cases)
+
+
+def test_g2p_node_boxes_all_cases():
+    # a dense sand block (27 particles per cell: waves in one row of cells or across two), plus a
+    # sparse cloud (one particle per far-apart cell: waves that span many rows and gather from the
+    # grid); the substep matches the oracle whichever source a wave's stencil came from
+    sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB, seed=31, ppa=3, z_up=True)
+    rng = np.random.default_rng(31)
+    p = sc.particles
+    sparse = rng.choice(sc.n, size=sc.n // 8, replace=False)
+    p["x"][:, sparse] = rng.uniform(0.25, 0.75, size=(3, sparse.size)).astype(np.float32)
+    p["v"] = (p["v"] + 0.3 * rng.normal(size=(3, sc.n))).astype(np.float32)
+    p["F"] = (np.eye(3).reshape(9, 1) + 0.02 * rng.normal(size=(9, sc.n))).astype(np.float32)
+    # the isolated particles stay unstrained: a lone particle's corner nodes have masses down to
+    # ~1e-9 of its own, and a stress impulse divided by such a mass would magnify FP32 rounding far
+    # beyond the grid tolerance (with no stress, v = p/m of a single term)
+    p["F"][:, sparse] = np.eye(3, dtype=np.float32).reshape(9, 1)
+    h = [(b - a) / r for a, b, r in zip(sc.gmin, sc.gmax, sc.res)]
+    cases = _g2p_box_cases(np.sort(ob.cell_keys(p["x"], sc.gmin, h, sc.res).astype(np.int64)), sc.res)
+    counts = [int(np.sum(cases == c)) for c in (0, 1, 2)]
+    say(f"[node boxes] n={sc.n} waves: grid {counts[0]}, one box {counts[1]}, two boxes {counts[2]}")
+    assert min(counts) > 0, counts
+    _stage_parity(sc, 0, _lib.FRICTION_COULOMB)
