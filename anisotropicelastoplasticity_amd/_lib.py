@@ -16,7 +16,7 @@ FRICTION_REFERENCE, FRICTION_COULOMB, FRICTION_STICKY = 0, 1, 2
 LS_PLANE, LS_WALL2GROUND, LS_BOX, LS_SPHERE = 0, 1, 2, 3
 STAGE_SORT, STAGE_P2G, STAGE_GRID, STAGE_G2P = 0, 1, 2, 3
 NUM_STAGES = 4
-STATUS_NONFINITE, STATUS_OUT_OF_DOMAIN, STATUS_CFL, STATUS_TILE_OVERFLOW = 1, 2, 4, 8
+STATUS_NONFINITE, STATUS_OUT_OF_DOMAIN, STATUS_CFL = 1, 2, 4
 
 
 class Params(ctypes.Structure):

@@ -116,6 +116,7 @@ class Solver:
         self._check(self.lib.mpm_set_scenes(self.h, len(scenes), descs), "mpm_set_scenes")
         self.res = None
         self.scene_counts = [int(sc.n) for sc in scenes]
+        self.scene_nodes = [int(np.prod(sc.res)) for sc in scenes]
 
     def load_batch(self, scenes):
         """set_scenes + set_particles with every scene's particles concatenated in order."""
@@ -288,14 +289,14 @@ class Solver:
         self._check(rc, "mpm_get_sorted")
         return keys, ids, b
 
-    def get_grid(self):
-        ng = int(np.prod(self.res))
-        m = np.empty(ng, np.float32)
-        mom = np.empty((3, ng), np.float32)
-        v = np.empty((3, ng), np.float32)
-        vbf = np.empty((3, ng), np.float32)
-        self._check(self.lib.mpm_get_grid(self.h, _fp(m), _fp(mom), _fp(v), _fp(vbf)), "mpm_get_grid")
-        return dict(m=m, mom=mom, v=v, vbf=vbf)
+    def get_grid(self, m=True, mom=True, v=True, vbf=True):
+        """Dense grid arrays (mpm_get_grid); a batched context returns its scenes' nodes concatenated
+        scene by scene. Planes not asked for are not copied (None)."""
+        ng = int(np.prod(self.res)) if self.res is not None else int(sum(self.scene_nodes))
+        out = dict(m=np.empty(ng, np.float32) if m else None, mom=np.empty((3, ng), np.float32) if mom else None,
+                   v=np.empty((3, ng), np.float32) if v else None, vbf=np.empty((3, ng), np.float32) if vbf else None)
+        self._check(self.lib.mpm_get_grid(self.h, *[_fp(out[k]) for k in ("m", "mom", "v", "vbf")]), "mpm_get_grid")
+        return out
 
     def stats(self):
         s = Stats()

@@ -2085,8 +2085,12 @@ __global__ __launch_bounds__(256) void k_cloth_vertex_forces(ClothArgs a) {
   for (int r = 0; r < 3; ++r) a.vert_force[3 * (size_t)v + r] = f[r];
 }
 
-// Fixed vertices (HybridSolver.cpp:513-550): every node within one node (per axis) of a node with
-// positive weight for a fixed vertex is stopped (v = v_BF = 0). Out-of-grid neighbours are skipped.
+// Fixed vertices (HybridSolver.cpp:513-550): for every in-grid node with positive weight for a fixed
+// vertex, the 27 nodes (i, j, k) around it are stopped (v = v_BF = 0) through their linear index
+// k r0 r1 + j r0 + i, which is checked against [0, gridNumber) only (:536-539). The union of those
+// neighbourhoods is the stencil box widened by one node per axis; a neighbour one step outside the
+// grid in x or y therefore stops the node its linear index names, e.g. (-1, j, k) stops
+// (r0 - 1, j - 1, k), as in the reference.
 template <int ORDER>
 __global__ __launch_bounds__(64) void k_cloth_fixed(ClothArgs a) {
   if (clock_skip(a.clk)) return;
@@ -2109,14 +2113,18 @@ __global__ __launch_bounds__(64) void k_cloth_fixed(ClothArgs a) {
         hi[d] = max(hi[d], base + o);
       }
     }
-    lo[d] = max(lo[d] - 1, 0);
-    hi[d] = min(hi[d] + 1, P.res[d] - 1);
+    // the weight matrix holds in-grid nodes only (HybridSolver.cpp:44-47)
+    lo[d] = max(lo[d], 0);
+    hi[d] = min(hi[d], P.res[d] - 1);
   }
+  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
+  const int64_t ng = (int64_t)P.res[0] * P.res[1] * P.res[2];
   const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  for (int n2 = lo[2]; n2 <= hi[2]; ++n2)
-    for (int n1 = lo[1]; n1 <= hi[1]; ++n1)
-      for (int n0 = lo[0]; n0 <= hi[0]; ++n0) {
+  for (int n2 = lo[2] - 1; n2 <= hi[2] + 1; ++n2)
+    for (int n1 = lo[1] - 1; n1 <= hi[1] + 1; ++n1)
+      for (int n0 = lo[0] - 1; n0 <= hi[0] + 1; ++n0) {
         const int64_t node = ((int64_t)n2 * P.res[1] + n1) * P.res[0] + n0;
+        if (node < 0 || node >= ng) continue;
         a.gvel[2 * node] = z;
         a.gvel[2 * node + 1] = z;
       }

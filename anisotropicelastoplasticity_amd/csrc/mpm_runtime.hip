@@ -728,6 +728,9 @@ static int set_geometry(mpm_ctx* ctx, SceneHost& sc, const double min_bound[3], 
     if (!(max_bound[a] > min_bound[a])) return fail(ctx, MPM_ERR_INVALID, "maxBound must be bigger than minBound");
     if (res[a] < 1) return fail(ctx, MPM_ERR_INVALID, "resolution must be positive");
   }
+  // the kernels form a scene's cell and node indices in 32-bit signed arithmetic
+  if ((int64_t)res[0] * res[1] * res[2] > (int64_t)INT32_MAX)
+    return fail(ctx, MPM_ERR_UNSUPPORTED, "a scene has more than 2^31 - 1 cells");
   for (int a = 0; a < 3; ++a) {
     sc.gmin[a] = min_bound[a];
     sc.gmax[a] = max_bound[a];
@@ -1263,13 +1266,20 @@ int mpm_run(mpm_ctx* ctx, float dt, int32_t nsteps) {
       }
       hipGraph_t g = nullptr;
       const int64_t steps0 = ctx->substeps;
+      const int sort0 = ctx->sort_buf;
       CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int r1 = one_substep(ctx, dt);
       int r2 = r1 ? r1 : one_substep(ctx, dt);
       hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+      // capture executed nothing: the state is still in the buffers it was in
       ctx->substeps = steps0;
-      if (r2) return r2;
-      if (ec != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+      ctx->cur = c0;
+      ctx->sort_buf = sort0;
+      if (r2 || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (r2) return r2;
+        return fail(ctx, MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+      }
       hipError_t ei = hipGraphInstantiate(&ctx->graph[c0], g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (ei != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
@@ -1340,14 +1350,19 @@ int mpm_solve(mpm_ctx* ctx, double cfl, double maxt, mpm_frame_callback cb, void
       if (!ctx->graph_adapt[c0]) {
         hipGraph_t gr = nullptr;
         const int64_t steps0 = ctx->substeps;
+        const int sort0 = ctx->sort_buf;
         CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int r1 = one_substep(ctx, 0.0f);
         int r2 = r1 ? r1 : one_substep(ctx, 0.0f);
         hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
         ctx->substeps = steps0;
         ctx->cur = c0;  // capture did not execute anything
-        if (r2) return r2;
-        if (ec != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        ctx->sort_buf = sort0;
+        if (r2 || ec != hipSuccess) {
+          if (gr) (void)hipGraphDestroy(gr);
+          if (r2) return r2;
+          return fail(ctx, MPM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        }
         hipError_t ei = hipGraphInstantiate(&ctx->graph_adapt[c0], gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         if (ei != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));

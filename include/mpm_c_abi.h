@@ -89,7 +89,6 @@ extern "C" {
 #define MPM_STATUS_NONFINITE 1      /* a particle state value became NaN/Inf */
 #define MPM_STATUS_OUT_OF_DOMAIN 2  /* a particle cell was clamped to the grid */
 #define MPM_STATUS_CFL 4            /* a particle moved more than one cell in a substep */
-#define MPM_STATUS_TILE_OVERFLOW 8  /* a tile held more particles than the P2G staging limit */
 
 /* Simulation parameters. Defaults (mpm_params_default) reproduce the reference's hard-coded
  * values; every deviation the configurations need is one switch here, not a second code path. */

@@ -11,17 +11,18 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libmpm_oracle.so")
 
 _d = ctypes.POINTER(ctypes.c_double)
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def load(path=None):
+    """The oracle library (the default build, or another build of the oracle source at `path`)."""
+    path = path or LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if path == LIB_PATH and not os.path.exists(LIB_PATH):
         from . import build as _b
         _b.build()
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     vp = ctypes.c_void_p
     sig = {
         "orc_create": (vp, [ctypes.c_int, vp, _d, _d, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]),
@@ -43,6 +44,8 @@ def load():
         "orc_solve": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d,
                                      ctypes.POINTER(ctypes.c_int), _d]),
         "orc_estimate_volumes": (ctypes.c_int, [vp]),
+        "orc_forces_at": (ctypes.c_int, [vp, _d, ctypes.c_double, _d]),
+        "orc_fixed_nodes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint8)]),
         "orc_get_grid": (ctypes.c_int, [vp, _d, _d, _d, _d, _d]),
         "orc_cell_keys": (None, [ctypes.c_int, _d, _d, _d, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint32)]),
         "orc_stable_sort": (None, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32),
@@ -58,7 +61,7 @@ def load():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -76,8 +79,8 @@ def _f64(a, shape=None):
 class Oracle:
     """A CPU restatement of the substep on one scene (precision 64 or 32)."""
 
-    def __init__(self, params, gmin, gmax, res, prims=(), precision=64, reference_forms=False):
-        self.lib = load()
+    def __init__(self, params, gmin, gmax, res, prims=(), precision=64, reference_forms=False, lib_path=None):
+        self.lib = load(lib_path)
         mn = (ctypes.c_double * 3)(*gmin)
         mx = (ctypes.c_double * 3)(*gmax)
         rs = (ctypes.c_int32 * 3)(*res)
@@ -146,6 +149,21 @@ class Oracle:
 
     def estimate_volumes(self):
         self.lib.orc_estimate_volumes(self.h)
+
+    def forces_at(self, gv, dt):
+        """Grid forces (3, ng) of the particle force pass on the grid velocities gv (3, ng)."""
+        ng = int(np.prod(self.res))
+        gv = _f64(gv, (3, ng))
+        f = np.empty((3, ng))
+        self.lib.orc_forces_at(self.h, _dp(gv), float(dt), _dp(f))
+        return f
+
+    def fixed_nodes(self):
+        """Mask (ng,) of the nodes the fixed cloth vertices stop."""
+        ng = int(np.prod(self.res))
+        mask = np.zeros(ng, np.uint8)
+        self.lib.orc_fixed_nodes(self.h, mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return mask.astype(bool)
 
     def get_particles(self):
         n = self.n

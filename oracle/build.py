@@ -16,19 +16,22 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libmpm_oracle.so")
 
 
-def build(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared",
-           os.path.join(HERE, "mpm_oracle.cpp"), "-o", tmp]
+def build(verbose=False, source=None, out=None, opt="-O3"):
+    """Compile the oracle (or another copy of its source, `source`, into `out`)."""
+    lib = out or LIB
+    os.makedirs(os.path.dirname(lib), exist_ok=True)
+    tmp = lib + ".tmp"
+    cmd = ["g++", opt, "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared",
+           "-I", os.path.join(os.path.dirname(HERE), "include"),
+           source or os.path.join(HERE, "mpm_oracle.cpp"), "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("g++ failed building the oracle")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

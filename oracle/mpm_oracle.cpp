@@ -24,7 +24,7 @@
 // non-negative and sorted in decreasing order (Eigen JacobiSVD's convention; every use in the
 // reference is an isotropic function of Sigma recombined with the same U and V, so the sign
 // and ordering conventions of U, V are not observable).
-#include "../include/mpm_c_abi.h"
+#include "mpm_c_abi.h"
 
 #include <algorithm>
 #include <cmath>
@@ -32,6 +32,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace orc {
@@ -643,9 +644,12 @@ template <class T> struct Sim {
     }
     fix_vertices();
   }
-  // Fixed cloth vertices (HybridSolver.cpp:513-550): every node within one node (per axis) of a node
-  // that a fixed vertex touches is stopped (v = v_BF = 0). Out-of-grid neighbours are skipped per
-  // axis.
+  // Fixed cloth vertices (HybridSolver.cpp:513-550): for every node (ri, rj, rk) with positive weight
+  // for a fixed vertex (in-grid nodes only, as the weight matrix holds them), each neighbour
+  // (i, j, k) in ri-1..ri+1 x rj-1..rj+1 x rk-1..rk+1 (the test `<= 12` at :532-535 holds for all 27)
+  // is stopped (v = v_BF = 0) through its linear index toIndex(i, j, k) = k r0 r1 + j r0 + i, which
+  // is only checked against [0, gridNumber) (:536-539). A neighbour one step outside the grid in x
+  // or y therefore stops the node its linear index names: (-1, j, k) stops (r0 - 1, j - 1, k).
   void fix_vertices() {
     std::vector<NodeW> nw;
     for (int k = 0; k < nv; ++k) {
@@ -657,8 +661,8 @@ template <class T> struct Sim {
         for (int i = ri - 1; i <= ri + 1; ++i)
           for (int j = rj - 1; j <= rj + 1; ++j)
             for (int kk = rk - 1; kk <= rk + 1; ++kk) {
-              if (i < 0 || j < 0 || kk < 0 || i >= res[0] || j >= res[1] || kk >= res[2]) continue;
-              const int id = to_index(i, j, kk);
+              const long long id = (long long)kk * res[0] * res[1] + (long long)j * res[0] + i;
+              if (id < 0 || id >= (long long)ng) continue;
               for (int a = 0; a < 3; ++a) gv[(size_t)a * ng + id] = gvbf[(size_t)a * ng + id] = T(0);
             }
       }
@@ -1089,6 +1093,38 @@ int orc_solve(void* hp, double cfl, double maxt, int max_substeps, double* dts, 
 }
 int orc_run(void* hp, double dt, int nsteps) {
   for (int i = 0; i < nsteps; ++i) orc_substep(hp, dt);
+  return 0;
+}
+
+// The particle force pass alone, on grid velocities chosen by the caller: gv (3 x ng planar) replaces
+// the grid velocity that the P2G would leave (the predicted F^ is built from it), then forces(dt)
+// runs and f (3 x ng planar) receives the grid forces without gravity.
+int orc_forces_at(void* hp, const double* gv, double dt, double* f) {
+  Handle* h = static_cast<Handle*>(hp);
+  auto go = [&](auto& s) {
+    using T = typename std::decay_t<decltype(s.gv)>::value_type;
+    for (size_t i = 0; i < s.gv.size(); ++i) s.gv[i] = T(gv[i]);
+    s.forces(T(dt));
+    store(s.gf, f);
+  };
+  if (h->prec == 64) go(h->d);
+  else go(h->f);
+  return 0;
+}
+
+// The nodes stopped by the fixed cloth vertices: every node's v and v_BF are set to 1, the
+// fixed-vertex pass runs, and mask[node] = 1 where it zeroed them.
+int orc_fixed_nodes(void* hp, uint8_t* mask) {
+  Handle* h = static_cast<Handle*>(hp);
+  auto go = [&](auto& s) {
+    using T = typename std::decay_t<decltype(s.gv)>::value_type;
+    std::fill(s.gv.begin(), s.gv.end(), T(1));
+    std::fill(s.gvbf.begin(), s.gvbf.end(), T(1));
+    s.fix_vertices();
+    for (int c = 0; c < s.ng; ++c) mask[c] = (s.gv[c] == T(0) && s.gvbf[c] == T(0)) ? 1 : 0;
+  };
+  if (h->prec == 64) go(h->d);
+  else go(h->f);
   return 0;
 }
 

@@ -149,3 +149,51 @@ def test_launch_shapes_give_identical_bits_and_autotune():
         assert s.tuning()["p2g_ranks"] == t["p2g_ranks"]
         s.run(dt, nsteps)
         _assert_bitwise(s.get_particles(), outs[0], "after autotune")
+
+
+def test_config5_full_batch_50_substeps():
+    """Config 5 at full size: 64 scenes (about 8.1M particles, snow and sand, 64^3..128^3 grids) in
+    one context for 50 substeps. Status clear and state finite; each scene's grid mass equals its
+    particle mass (relative 1e-5, FP32 sums); four scenes (the smallest, the largest, the first snow
+    and the first sand scene) equal their single-scene runs bit for bit; the smallest scene follows
+    the FP64 oracle (relative L2 of x <= 1e-5, v <= 1e-3, F <= 1e-4)."""
+    batch = scenes.config5_batch()
+    nsteps = 50
+    ntot = sum(sc.n for sc in batch)
+    say(f"[config5] {len(batch)} scenes, {ntot} particles, {nsteps} substeps")
+    with Solver() as s:
+        s.load_batch(batch)
+        s.run(batch[0].dt, nsteps)
+        out = s.get_particles()
+        st = s.stats()
+        g = s.get_grid(m=True, mom=False, v=False, vbf=False)
+    assert st["status"] == 0, st
+    for k in ("x", "v", "F", "plastic"):
+        assert np.all(np.isfinite(out[k])), k
+    parts = _split(out, batch)
+    node_off = np.cumsum([0] + [int(np.prod(sc.res)) for sc in batch])
+    for i, sc in enumerate(batch):
+        gm = float(g["m"][node_off[i]:node_off[i + 1]].astype(np.float64).sum())
+        pm = float(sc.particles["mass"].astype(np.float64).sum())
+        assert abs(gm / pm - 1.0) < 1e-5, (i, gm, pm)
+    sizes = [sc.n for sc in batch]
+    chosen = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)),
+                     next(i for i, sc in enumerate(batch) if sc.params.material == _lib.SNOW),
+                     next(i for i, sc in enumerate(batch) if sc.params.material == _lib.SAND)})
+    for i in chosen:
+        say(f"  scene {i}: n {batch[i].n}, grid {batch[i].res[0]}^3, {'sand' if batch[i].meta['sand'] else 'snow'}")
+        _assert_bitwise(parts[i], _run_single(batch[i], nsteps), f"scene {i}")
+    i = int(np.argmin(sizes))
+    sc = batch[i]
+    o = ob.Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
+    o.load_scene(sc)
+    o.run(sc.dt, nsteps)
+    r = o.get_particles()
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+    ex, ev, eF = rel(parts[i]["x"], r["x"]), rel(parts[i]["v"], r["v"]), rel(parts[i]["F"], r["F"])
+    say(f"  scene {i} vs FP64 oracle: rel x {ex:.3e}, v {ev:.3e}, F {eF:.3e}")
+    assert ex < 1e-5 and ev < 1e-3 and eF < 1e-4

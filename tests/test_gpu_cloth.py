@@ -233,3 +233,105 @@ def test_cpp_cloth_example(tmp_path):
         assert x.shape == (3, nv) and faces.shape == (nf, 3) and np.all(np.isfinite(x))
     assert not (tmp_path / "mesh" / "mesh_2.obj").exists()
     assert not (tmp_path / "particle" / "particle_0.obj").exists()
+
+
+def _plane_cloth(plane, n_side=16, edge=1.0 / 40.0, seed=9):
+    """A cloth in the plane y = 0.6 or z = 0.6 with the stretching, shearing and bending velocity
+    field of cloth_scene, alone in a 32^3 grid with box walls."""
+    x, faces = scenes.grid_mesh(n_side, n_side, edge, (0.5 - edge * (n_side - 1) / 2, 0.5 - edge * (n_side - 1) / 2), 0.6)
+    if plane == "z":
+        x = x[[0, 2, 1]].copy()  # (x, y, z) <- (x, z, y): the sheet lies in z = 0.6
+    v = np.zeros_like(x)
+    v[0] = (x[0] - 0.5) * 2.0
+    inplane = 2 if plane == "y" else 1
+    normal = 1 if plane == "y" else 2
+    v[inplane] = -(x[inplane] - 0.5) * 1.5
+    v[normal] = -0.5 + 0.3 * np.sin(25.0 * x[0]) * np.cos(18.0 * x[inplane])
+    prims = [levelset(_lib.LS_BOX, (0.05, 0.05, 0.05, 0.0), (0.95, 0.95, 0.95, 0.0))]
+    sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB, seed=seed, res=(32, 32, 32), prims=prims, dt=1e-4)
+    return sc, x, v.astype(np.float32), faces
+
+
+@pytest.mark.parametrize("plane", ["z", "y"])
+def test_cloth_reference_normal_row(plane):
+    """mpm_mesh_params.reference_normal_row = 1: the normal-force direction vector is
+    (D_rest^T).col(2) as in HybridSolver.cpp:425-427,432. It equals D_rest^-T e3 for a sheet in a
+    z = const plane and differs for one in y = const; the device follows the FP64 oracle in both,
+    with the drape test's bar (mesh |dx| <= 1e-4 h, D relative L2 <= 1e-3) after 40 substeps."""
+    sc, x, v, faces = _plane_cloth(plane)
+    n = 40
+    say(f"[cloth reference normal row] plane {plane}, vertices {x.shape[1]}, substeps {n}")
+    res = {}
+    for ref_row in (True, False):
+        o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
+        o.set_mesh(x, faces, v=v, reference_normal_row=ref_row)
+        o.run(sc.dt, n)
+        res[ref_row] = o.get_mesh()
+    with Solver() as s:
+        s.set_grid(sc.gmin, sc.gmax, sc.res)
+        s.set_params(sc.params)
+        s.set_levelset(sc.prims)
+        s.set_mesh(x, faces, v=v, reference_normal_row=True)
+        s.run(sc.dt, n)
+        m = s.get_mesh()
+        st = s.stats()
+    h = 1.0 / sc.res[0]
+    mr = res[True]
+    dxm = np.max(np.abs(m["vx"] - mr["vx"]))
+    d_forms = rel(res[True]["D"], res[False]["D"])
+    say(f"  mesh |dx|/h {dxm / h:.3e}, D rel {rel(m['D'], mr['D']):.3e}, oracle forms differ by {d_forms:.3e}")
+    if plane == "z":
+        assert d_forms < 1e-9  # the two forms agree for a z = const sheet
+    else:
+        assert d_forms > 1e-4  # and differ for a y = const sheet
+    assert dxm <= 1e-4 * h
+    assert rel(m["D"], mr["D"]) < 1e-3
+    assert st["status"] == 0, st
+
+
+def test_cloth_fixed_vertices_at_grid_faces():
+    """Fixed vertices one step from the x faces stop the nodes their neighbours' linear indices name
+    (HybridSolver.cpp:536-539): the neighbour (-1, j, k) of the low vertex is node (31, j - 1, k) and
+    the neighbour (32, j, k) of the high vertex is node (0, j + 1, k). A cloth at rest spans the
+    grid in x, so those nodes carry mass; after one P2G and grid update the device stops the same
+    active nodes as the FP64 oracle and leaves v = dt g on the others."""
+    n_x, n_z = 40, 6
+    edge = 0.975 / (n_x - 1)
+    x, faces = scenes.grid_mesh(n_x, n_z, edge, (0.01, 0.4), 0.6)
+    fixed = np.zeros(x.shape[1], np.uint8)
+    lo_v, hi_v = 0 * n_z + 5, (n_x - 1) * n_z + 0  # (x 0.01, z 0.525) and (x 0.985, z 0.4)
+    fixed[[lo_v, hi_v]] = 1
+    p = scenes.sand_params()
+    res = (32, 32, 32)
+    dt = 1e-4
+    say(f"[cloth fixed at faces] vertices {x.shape[1]}, fixed {np.flatnonzero(fixed).tolist()}")
+    o = Oracle(p, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), res)
+    o.set_mesh(x, faces, fixed=fixed)
+    o.stage(_lib.STAGE_P2G, dt)
+    o.stage(_lib.STAGE_GRID, dt)
+    g64 = o.get_grid()
+    with Solver() as s:
+        s.set_grid((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), res)
+        s.set_params(p)
+        s.set_levelset([])
+        s.set_mesh(x, faces, fixed=fixed)
+        for st in (_lib.STAGE_SORT, _lib.STAGE_P2G, _lib.STAGE_GRID):
+            s.stage(st, dt)
+        g = s.get_grid()
+        stats = s.stats()
+    on = g64["m"] > 0
+    assert np.array_equal(g["m"] > 0, on), "active node sets differ"
+    stop64 = on & np.all(g64["v"] == 0.0, axis=0)
+    stop = on & np.all(g["v"] == 0.0, axis=0) & np.all(g["vbf"] == 0.0, axis=0)
+
+    def node(i, j, k):
+        return (k * res[1] + j) * res[0] + i
+
+    # nodes stopped only through the wrap, and active: (31, 18, 17) from the low vertex (stencil
+    # j 18..20, k 16..18) and (0, 19, 12) from the high vertex (stencil j 18..20, k 12..14)
+    for i, j, k in [(31, 18, 17), (0, 19, 12)]:
+        assert on[node(i, j, k)] and stop64[node(i, j, k)], (i, j, k)
+    assert np.array_equal(stop, stop64)
+    free = on & ~stop64
+    assert np.allclose(g["v"][1, free], -9.8 * dt, rtol=1e-5)
+    assert stats["status"] == 0, stats

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from anisotropicelastoplasticity_amd import Solver, _lib, levelset, scenes
+from anisotropicelastoplasticity_amd import api, Solver, _lib, levelset, scenes
 from oracle import bind as ob
 from oracle.bind import Oracle
 
@@ -248,20 +248,72 @@ def test_config2_one_substep():
             if st == _lib.STAGE_GRID:
                 g = s.get_grid()
         q = s.get_particles()
+        # the next substep's P2G: the stress impulses of the now deformed particles sum to zero over
+        # the grid (sum_i grad w_ip = 0 for particles whose stencil lies inside the grid), so the
+        # grid momentum equals the particle momentum
+        s.stage(_lib.STAGE_SORT, sc.dt)
+        s.stage(_lib.STAGE_P2G, sc.dt)
+        g2 = s.get_grid()
     for st in (1, 2, 3):
         o.stage(st, sc.dt)
     r, qr = o.get_grid(), o.get_particles()
     on = r["m"] > 0
+    say(f"[config2] rel m {rel(g['m'], r['m']):.2e} v {rel(g['v'][:, on], r['v'][:, on]):.2e} particle v "
+        f"{rel(q['v'], qr['v']):.2e} F {rel(q['F'], qr['F']):.2e} B {rel(q['B'], qr['B']):.2e} "
+        f"max|dx|*128 {np.max(np.abs(q['x'] - qr['x'])) * 128:.2e}")
     assert rel(g["m"], r["m"]) < 2e-5
     assert rel(g["v"][:, on], r["v"][:, on]) < 2e-5
     assert rel(q["v"], qr["v"]) < 2e-5
+    assert rel(q["B"], qr["B"]) < 2e-5
+    assert np.max(np.abs(q["F"] - qr["F"])) < 2e-5
+    assert np.array_equal(q["plastic"] > 0, qr["plastic"] > 0)
+    assert np.max(np.abs(q["plastic"] - qr["plastic"])) < 2e-5
     assert np.max(np.abs(q["x"] - qr["x"])) < 1e-5 / 128
-    # size-independent properties at full size: mass conservation, stress forces sum to zero
+    # size-independent properties at full size: mass conservation; momentum with the stress impulse
     m = sc.particles["mass"].astype(np.float64)
     assert abs(g["m"].astype(np.float64).sum() / m.sum() - 1) < 1e-5
+    pm = (q["v"].astype(np.float64) * m).sum(axis=1)
+    gm = g2["mom"].astype(np.float64).sum(axis=1)
+    scale = np.abs(q["v"].astype(np.float64) * m).sum()
+    assert np.max(np.abs(gm - pm)) < 1e-5 * scale
 
 
-# ------------------------------------------------------------------ determinism, graphs, flags
+def test_dp_sand_column_200_substeps():
+    """Drucker-Prager sand (config 2's material: E 3.5e5, nu 0.3, rho 2200, 30 deg, box walls, sphere,
+    Coulomb 0.4, dt 5e-5) in a reduced column of 2^15 particles (8 x 64 x 8 grid cells, 8 per cell,
+    moving down at 0.5 m/s onto the floor) on the 128^3 grid, 200 substeps against the FP64 oracle.
+    Bars: relative L2 of x <= 1e-5, of v <= 1e-3, of F <= 1e-4; the cone and the FP64 apex
+    projections both ran."""
+    h = 1.0 / 128
+    lo, hi = (0.5 - 4 * h, 0.05, 0.5 - 4 * h), (0.5 + 4 * h, 0.05 + 64 * h, 0.5 + 4 * h)
+    x = api.generate_block(21, lo, hi, (8, 64, 8), 2)
+    vol = float(np.prod(np.subtract(hi, lo)))
+    parts = scenes._particles(x, 2200.0, vol, (0.0, -0.5, 0.0), _lib.SAND)
+    full = scenes.config2_sand_column(cells=(2, 2, 2))
+    sc = scenes.Scene("dp_column", full.params, full.gmin, full.gmax, full.res, full.prims, parts, 5e-5, 200)
+    say(f"[dp column] n={sc.n} substeps={sc.substeps}")
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, sc.substeps)
+        q = s.get_particles()
+        st = s.stats()
+    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
+    o.load_scene(sc)
+    o.run(sc.dt, sc.substeps)
+    qr = o.get_particles()
+    ex, ev, eF = rel(q["x"], qr["x"]), rel(q["v"], qr["v"]), rel(q["F"], qr["F"])
+    moved = np.max(np.abs(qr["x"] - x.astype(np.float64))) / h
+    plastic = float(np.mean(qr["plastic"] > 0))
+    say(f"[dp column] rel x {ex:.3e}, v {ev:.3e}, F {eF:.3e}, q {rel(q['plastic'], qr['plastic']):.3e}; "
+        f"max move {moved:.2f} h, plastic fraction {plastic:.3f}, fp64 apex {st['fp64_apex_count']}")
+    assert st["status"] == 0, st
+    assert plastic > 0.1
+    assert st["fp64_apex_count"] > 0
+    assert ex < 1e-5
+    assert ev < 1e-3
+    assert eF < 1e-4
+
+
 def test_bitwise_determinism_and_graph_replay():
     sc = scenes.config2_sand_column(cells=(16, 48, 16))
     say(f"[determinism] n={sc.n}")
@@ -309,6 +361,20 @@ def test_status_flags():
         q = s.get_particles()
     assert st & _lib.STATUS_OUT_OF_DOMAIN
     assert np.all(np.isfinite(q["x"][:, 10:]))
+
+
+def test_grid_beyond_int32_cells_is_refused():
+    # the kernels index a scene's cells in 32-bit signed arithmetic: 1291^3 > 2^31 - 1 cells is refused
+    # before anything is allocated, and the context keeps working with its previous grid
+    sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB)
+    say("[grid size] 1291^3 refused")
+    with Solver() as s:
+        sc.load_into(s)
+        with pytest.raises(api.MPMError):
+            s.set_grid(sc.gmin, sc.gmax, (1291, 1291, 1291))
+        s.substep(sc.dt)
+        assert s.stats()["status"] == 0
+        assert np.all(np.isfinite(s.get_particles()["x"]))
 
 
 def test_fp64_svd_fallback():
