@@ -17,6 +17,8 @@ LS_PLANE, LS_WALL2GROUND, LS_BOX, LS_SPHERE = 0, 1, 2, 3
 STAGE_SORT, STAGE_P2G, STAGE_GRID, STAGE_G2P = 0, 1, 2, 3
 NUM_STAGES = 4
 STATUS_NONFINITE, STATUS_OUT_OF_DOMAIN, STATUS_CFL = 1, 2, 4
+# return codes of every mpm_* call (include/mpm_c_abi.h MPM_OK, MPM_ERR_*)
+OK, ERR_INVALID, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 
 
 class Params(ctypes.Structure):
@@ -135,6 +137,8 @@ SIGNATURES = [
     ("mpm_synchronize", ctypes.c_int, [_ctx]),
     ("mpm_time_substeps", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _f, _f]),
     ("mpm_time_kernels", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _f, _f, _f]),
+    ("mpm_time_surrogate", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _P(ctypes.c_float), _P(ctypes.c_float),
+                                          _P(ctypes.c_int64)]),
     ("mpm_surrogate_nparam", ctypes.c_int32, []),
     ("mpm_surrogate_init", ctypes.c_int, [_ctx, ctypes.c_uint64]),
     ("mpm_surrogate_set_weights", ctypes.c_int, [_ctx, _f, ctypes.c_int32]),
@@ -145,6 +149,8 @@ SIGNATURES = [
      [_ctx, _f, _f, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _f]),
     ("mpm_set_pair_logging", ctypes.c_int, [_ctx, ctypes.c_int64]),
     ("mpm_get_pairs", ctypes.c_int, [_ctx, _f, ctypes.c_int64, _P(ctypes.c_int64)]),
+    ("mpm_set_surrogate_log", ctypes.c_int, [_ctx, ctypes.c_int64]),
+    ("mpm_get_surrogate_log", ctypes.c_int, [_ctx, _f, ctypes.c_int64, _P(ctypes.c_int64)]),
     ("mpm_set_surrogate", ctypes.c_int, [_ctx, ctypes.c_int32, ctypes.c_float]),
     ("mpm_bench_copy", ctypes.c_int, [_ctx, ctypes.c_int64, ctypes.c_int32, _f]),
     ("mpm_generate_block", ctypes.c_int,

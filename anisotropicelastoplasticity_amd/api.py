@@ -357,6 +357,23 @@ class Solver:
         self._check(self.lib.mpm_get_pairs(self.h, _fp(out), int(max_pairs), ctypes.byref(cnt)), "mpm_get_pairs")
         return out[:cnt.value]
 
+    def set_surrogate_log(self, capacity):
+        """Record the gate decision of up to `capacity` surrogate candidates per substep."""
+        self._check(self.lib.mpm_set_surrogate_log(self.h, int(capacity)), "mpm_set_surrogate_log")
+
+    def get_surrogate_log(self, max_records=1 << 24):
+        """The last surrogate substep's gate records: dict of slot (int32), eps (n, 3), phi, kmu,
+        H (n, 3) and accepted (bool)."""
+        cnt = ctypes.c_int64(0)
+        self._check(self.lib.mpm_get_surrogate_log(self.h, None, int(max_records), ctypes.byref(cnt)),
+                    "mpm_get_surrogate_log")
+        m = int(cnt.value)
+        rec = np.empty((max(m, 1), 10), np.float32)
+        self._check(self.lib.mpm_get_surrogate_log(self.h, _fp(rec), m, ctypes.byref(cnt)), "mpm_get_surrogate_log")
+        rec = rec[:m]
+        return dict(slot=rec[:, 0].copy().view(np.int32), eps=rec[:, 1:4], phi=rec[:, 4], kmu=rec[:, 5], H=rec[:, 6:9],
+                    accepted=rec[:, 9] > 0.5)
+
     def set_surrogate(self, enable, gate_tol=1e-5):
         self._check(self.lib.mpm_set_surrogate(self.h, int(bool(enable)), float(gate_tol)), "mpm_set_surrogate")
 
@@ -381,6 +398,14 @@ class Solver:
         rc = self.lib.mpm_time_kernels(self.h, float(dt), int(nsteps), ctypes.byref(total), stages, kern)
         self._check(rc, "mpm_time_kernels")
         return total.value, list(stages), list(kern)
+
+    def time_surrogate(self, dt, nsteps):
+        """(network kernel ms, gate kernel ms, candidates) summed over nsteps eager substeps with the
+        surrogate enabled (mpm_time_surrogate)."""
+        f, a, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_int64()
+        rc = self.lib.mpm_time_surrogate(self.h, float(dt), int(nsteps), ctypes.byref(f), ctypes.byref(a), ctypes.byref(c))
+        self._check(rc, "mpm_time_surrogate")
+        return f.value, a.value, c.value
 
 
 def write_obj(path, x):

@@ -224,7 +224,9 @@ struct ApexArgs {
 void launch_apex_finish(hipStream_t s, const ApexArgs& a, int grid);
 
 // Candidate planes of the surrogate return map.
-constexpr int CAND_SIGMA = 0, CAND_U = 3, CAND_V = 12, CAND_PLANES = 21;
+constexpr int CAND_SIGMA = 0, CAND_U = 3, CAND_V = 12, CAND_SVD_PLANES = 21;
+// surrogate candidates also keep the network's normalised output y(3)
+constexpr int CAND_Y = 21, CAND_PLANES = 24;
 
 struct SurrogateArgs {
   SceneTable T;
@@ -233,13 +235,15 @@ struct SurrogateArgs {
   float* st;             // particle state written by k_g2p (sorted slots)
   int64_t cap;
   int n;
-  const float* cand;
+  float* cand;  // candidate SVDs (written by k_g2p) and network outputs
   const int* cand_slot;
   const int* cand_count;
   float gate_tol;
   int write_stress;
   unsigned long long* accepted;
   unsigned long long* candidates;  // cumulative candidate count (block 0 adds this substep's)
+  float* dec_log;                  // optional per-candidate gate record (10 floats per row)
+  int64_t dec_cap;
   const Clock* clk;
 };
 
@@ -313,6 +317,8 @@ void launch_cloth_forces(hipStream_t s, const ClothArgs& a);   // element forces
 void launch_cloth_fixed(hipStream_t s, const ClothArgs& a);    // after the grid update
 void launch_cloth_elements(hipStream_t s, const ClothArgs& a); // after G2P: positions, velocities, D, return map
 void surrogate_set_smem_limit();
-void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid);
+// k_surrogate_forward (MFMA network over the candidates) then k_surrogate_apply (gate and finish)
+// (ev: optional start/stop event pairs of the two dispatches)
+void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev);
 
 }  // namespace mpm

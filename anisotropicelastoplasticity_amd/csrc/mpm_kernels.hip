@@ -1916,58 +1916,85 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
 }
 
 // ================================================================== surrogate return map
-// Cone-branch particles deferred by k_g2p (sur_mode 2) are finished here. Each wave evaluates the MLP
-// for 16 candidates with BF16 MFMA (weights staged once per workgroup in LDS), then lanes 0..15
-// gate their prediction H of the projected Hencky strains:
+// Cone-branch particles deferred by k_g2p (sur_mode 2) are finished by two kernels.
+// k_surrogate_forward evaluates the MLP for 64 candidates per wave, one per lane, with BF16 MFMA
+// (weights staged once per workgroup in LDS, activations in registers) and keeps the outputs y
+// with the candidate. k_surrogate_apply then gates each prediction H = eps + 1e-3 y of the
+// projected Hencky strains:
 //   yield equality   | |dev H| + c alpha tr H | <= tol       (c = (3 lambda + 2 mu) / (2 mu))
 //   volume           | tr H - tr eps | <= tol                (the cone return is deviatoric)
 //   flow direction   the deviatoric correction eps - H is parallel to dev eps (orthogonal part <= tol)
+//                    and points along it
 // An accepted prediction gives Sigma' = exp(H), dq = |eps - H|; a rejected one runs the exact
 // projection (HybridSolver.cpp:646-673) through the same code as k_g2p, so rejecting every
-// prediction reproduces the exact path bit for bit.
-__global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
-#pragma clang fp contract(off)
+// prediction reproduces the exact path bit for bit. With a decision log, candidate row r < dec_cap
+// records (slot bits, eps, phi, K/mu, H, accepted) at dec_log[10 r ..].
+__global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate_forward(SurrogateArgs a) {
   if (clock_skip(a.clk)) return;
   extern __shared__ __align__(16) unsigned char smem[];
   MLPShared& sh = *reinterpret_cast<MLPShared*>(smem);
-  uint16_t* acts = reinterpret_cast<uint16_t*>(smem + sizeof(MLPShared));
   // the counter is cleared by the sort stage; a G2P stage repeated without it cannot list more than n
   const int count = min(*a.cand_count, a.n);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
-  if (blockIdx.x * 64 >= count) return;  // uniform per workgroup: nothing to do, skip the weight staging
+  const int nw = blockDim.x >> 6;
+  // uniform per workgroup: nothing to do, skip the weight staging
+  if ((int64_t)blockIdx.x * nw * 64 >= count) return;
   stage_weights(sh, a.wbf, a.master);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t* act = acts + w * 16 * LDA;
   const int64_t cap = a.cap, n = a.n;
-  for (int blk = blockIdx.x * 4 + w; blk * 16 < count; blk += gridDim.x * 4) {
-    const int row = blk * 16 + (lane & 15);
-    const bool valid = lane < 16 && row < count;
+  for (int g = blockIdx.x * nw + w; (int64_t)g * 64 < count; g += gridDim.x * nw) {
+    const int row = g * 64 + lane;
+    float eps[3] = {0.0f, 0.0f, 0.0f}, phi = 0.0f, kmu = 0.0f;
+    if (row < count) {
+      const int j = clamp_index(a.cand_slot[row], (int)n);
+      const DevParams& P = a.T.params[scene_of(a.T.part_off, a.T.nscenes, j)];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) eps[k] = logf(a.cand[(CAND_SIGMA + k) * n + row]);
+      phi = dp_phi(P, a.st[PPHI * cap + j], a.st[PPL * cap + j]);
+      kmu = bulk_over_shear(P);
+    }
+    const float x[5] = {eps[0], eps[1], eps[2], phi, kmu};
+    float y[3];
+    wave_forward64(sh, x, y, nullptr, 0, 0);
+    if (row < count) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.cand[(CAND_Y + k) * n + row] = y[k];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
+#pragma clang fp contract(off)
+  if (clock_skip(a.clk)) return;
+  const int count = min(*a.cand_count, a.n);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
+  const int64_t cap = a.cap, n = a.n;
+  const int lane = threadIdx.x & 63;
+  // whole waves stride over the rows, so the ballot below sees every lane
+  for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) - lane; base < count;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(base + lane);
+    const bool valid = row < count;
     int j = 0;
-    // the candidate's scene: its sorted slot lies in the scene's particle range
-    const int sc = valid ? scene_of(a.T.part_off, a.T.nscenes, clamp_index(a.cand_slot[row], (int)n)) : 0;
-    const DevParams& P = a.T.params[sc];
-    const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
-    float S[3] = {1.0f, 1.0f, 1.0f}, eps[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f, phi_p = 0.0f, phi = 0.0f;
+    float S[3] = {1.0f, 1.0f, 1.0f}, eps[3] = {0.0f, 0.0f, 0.0f}, y[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f,
+          phi_p = 0.0f, phi = 0.0f;
+    const DevParams* Pp = &a.T.params[0];
     if (valid) {
       j = clamp_index(a.cand_slot[row], (int)n);
+      Pp = &a.T.params[scene_of(a.T.part_off, a.T.nscenes, j)];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         S[k] = a.cand[(CAND_SIGMA + k) * n + row];
         eps[k] = logf(S[k]);
+        y[k] = a.cand[(CAND_Y + k) * n + row];
       }
       q = a.st[PPL * cap + j];
       phi_p = a.st[PPHI * cap + j];
-      phi = dp_phi(P, phi_p, q);
+      phi = dp_phi(*Pp, phi_p, q);
     }
-    if (lane < 16) {
-      const float x[5] = {eps[0], eps[1], eps[2], phi, bulk_over_shear(P)};
-      mlp_input_row(act + lane * LDA, x);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    float y[3] = {0.0f, 0.0f, 0.0f};
-    wave_forward16(sh, act, y, nullptr, 16);
+    const DevParams& P = *Pp;
+    const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
+    const float kmu = bulk_over_shear(P);
     bool accepted = false;
     if (valid) {
       float H[3];
@@ -1997,6 +2024,19 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
       const float tol = a.gate_tol;
       accepted = fabsf(nH + c_yield * alpha * trH) <= tol && fabsf(trH - trE) <= tol && along > 0.0f &&
                  sqrtf(ortho2) <= tol;
+      if (a.dec_log && row < a.dec_cap) {
+        float* rec = a.dec_log + (int64_t)10 * row;
+        rec[0] = __int_as_float(j);
+        rec[1] = eps[0];
+        rec[2] = eps[1];
+        rec[3] = eps[2];
+        rec[4] = phi;
+        rec[5] = kmu;
+        rec[6] = H[0];
+        rec[7] = H[1];
+        rec[8] = H[2];
+        rec[9] = accepted ? 1.0f : 0.0f;
+      }
       float U[9], V[9], Sn[3], lnSn[3], dq;
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
@@ -2452,15 +2492,22 @@ void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4) {
   if (n4 > 0) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, src, dst, n4);
 }
 
-static size_t surrogate_smem() { return sizeof(MLPShared) + 4 * 16 * LDA * sizeof(uint16_t); }
+static size_t surrogate_smem() { return sizeof(MLPShared); }
 
 void surrogate_set_smem_limit() {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_surrogate_apply),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_surrogate_forward),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)surrogate_smem());
 }
 
-void launch_surrogate_apply(hipStream_t s, const SurrogateArgs& a, int grid) {
-  hipLaunchKernelGGL(k_surrogate_apply, dim3(grid), dim3(256), surrogate_smem(), s, a);
+void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev) {
+  if (ev && ev[0]) {  // timed launches: each dispatch records its own start and end
+    hipExtLaunchKernelGGL(k_surrogate_forward, dim3(n_cu), dim3(MLP_FWD_THREADS), surrogate_smem(), s, ev[0], ev[1], 0,
+                          a);
+    hipExtLaunchKernelGGL(k_surrogate_apply, dim3(4 * n_cu), dim3(256), 0, s, ev[2], ev[3], 0, a);
+    return;
+  }
+  hipLaunchKernelGGL(k_surrogate_forward, dim3(n_cu), dim3(MLP_FWD_THREADS), surrogate_smem(), s, a);
+  hipLaunchKernelGGL(k_surrogate_apply, dim3(4 * n_cu), dim3(256), 0, s, a);
 }
 
 void launch_apex_finish(hipStream_t s, const ApexArgs& a, int grid) {

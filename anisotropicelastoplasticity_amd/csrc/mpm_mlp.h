@@ -12,6 +12,8 @@ constexpr float MLP_EPS_SCALE = 0.02f;
 constexpr float MLP_PHI_MID = 32.5f, MLP_PHI_SPAN = 7.5f;
 constexpr float MLP_KMU_MID = 2.5f, MLP_KMU_SPAN = 1.0f;
 constexpr float MLP_CORR_SCALE = 1e-3f;
+// forward kernels: 8 waves of 64 rows share one LDS copy of the weights (one workgroup per CU)
+constexpr int MLP_FWD_THREADS = 512;
 
 // Master (FP32) layout: for l = 0..4, W_l [N_l][K_l] then b_l [N_l];
 // N = (128,128,128,128,16), K = (32,128,128,128,128).

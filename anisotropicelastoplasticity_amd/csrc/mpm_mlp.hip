@@ -24,33 +24,25 @@ namespace mpm {
 
 // ------------------------------------------------------------------ batch forward (inference / training)
 // in: n x 5 raw inputs (row-major), out: n x 3 normalised outputs; zsave (optional): per 16-row block b,
-// 4 x 16 x 128 FP32 pre-activations at zsave + b * 4 * 16 * 128.
-__global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* __restrict__ wbf, const float* __restrict__ master,
-                                                     const float* __restrict__ in, float* __restrict__ out, int n,
-                                                     float* __restrict__ zsave) {
+// 4 x 16 x 128 FP32 pre-activations at zsave + b * 4 * 16 * 128. Each wave evaluates 64 rows.
+__global__ __launch_bounds__(MLP_FWD_THREADS) void k_mlp_forward(const uint16_t* __restrict__ wbf,
+                                                                 const float* __restrict__ master,
+                                                                 const float* __restrict__ in, float* __restrict__ out,
+                                                                 int n, float* __restrict__ zsave) {
   extern __shared__ __align__(16) unsigned char smem[];
   MLPShared& s = *reinterpret_cast<MLPShared*>(smem);
-  uint16_t* acts = reinterpret_cast<uint16_t*>(smem + sizeof(MLPShared));
   stage_weights(s, wbf, master);
   __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t* act = acts + w * 16 * LDA;
-  const int nblk = (n + 15) / 16;
-  for (int blk = blockIdx.x * 4 + w; blk < nblk; blk += gridDim.x * 4) {
-    const int row0 = blk * 16;
-    if (lane < 16) {
-      float x[5] = {0, 0, 0, 0, 0};
-      const int r = row0 + lane;
-      if (r < n)
-        for (int c = 0; c < 5; ++c) x[c] = in[(size_t)r * 5 + c];
-      mlp_input_row(act + lane * LDA, x);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int g = blockIdx.x * nw + w; (int64_t)g * 64 < n; g += gridDim.x * nw) {
+    const int64_t r = (int64_t)g * 64 + lane;
+    float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (r < n)
+      for (int c = 0; c < 5; ++c) x[c] = in[r * 5 + c];
     float y[3];
-    wave_forward16(s, act, y, zsave ? zsave + (size_t)blk * 4 * 16 * MLP_HIDDEN : nullptr, 16);
-    if (lane < 16 && row0 + lane < n)
-      for (int c = 0; c < 3; ++c) out[(size_t)(row0 + lane) * 3 + c] = y[c];
+    wave_forward64(s, x, y, zsave, (int64_t)g * 64, n);
+    if (r < n)
+      for (int c = 0; c < 3; ++c) out[r * 3 + c] = y[c];
   }
 }
 
@@ -250,7 +242,7 @@ __global__ __launch_bounds__(256) void k_mlp_to_bf16(const float* __restrict__ m
   mlp_store_bf16(i, mlp_param_active(i) ? master[i] : 0.0f, wbf, wbft);
 }
 
-size_t mlp_forward_smem() { return sizeof(MLPShared) + 4 * 16 * LDA * sizeof(uint16_t); }
+size_t mlp_forward_smem() { return sizeof(MLPShared); }
 size_t mlp_backward_smem() { return sizeof(BackShared); }
 
 void mlp_set_smem_limits() {
@@ -263,7 +255,8 @@ void mlp_set_smem_limits() {
 void launch_mlp_forward(hipStream_t s, const uint16_t* wbf, const float* master, const float* in, float* out, int n,
                         float* zsave, int grid) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(256), mlp_forward_smem(), s, wbf, master, in, out, n, zsave);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(MLP_FWD_THREADS), mlp_forward_smem(), s, wbf, master, in, out, n,
+                     zsave);
 }
 
 void launch_mlp_backward(hipStream_t s, const uint16_t* wbft, const float* in, const float* y, const float* target,

@@ -18,17 +18,28 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware exp2 and reciprocal (about 1 ulp each); the activations
+// are rounded to BF16 right after, so the last FP32 bits do not matter.
+__device__ __forceinline__ float sigmoid_fast(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid_fast(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
 // LDS row strides (in bf16 elements) padded against bank conflicts
 constexpr int LDW_IN = MLP_IN_PAD + 8;   // 40
 constexpr int LDW_H = MLP_HIDDEN + 8;    // 136
-constexpr int LDA = MLP_HIDDEN + 8;      // activation rows
+constexpr int LDA = MLP_HIDDEN + 8;      // activation rows of the backward pass
 
+// Staged weights. Every row is stored in the K order of the forward MFMA operands: slot
+// 32 s + 8 q + j (K step s, lane group q = lane / 16, j = 0..7) holds input feature
+// 32 s + 4 q + j for j < 4 and 32 s + 16 + 4 q + (j - 4) for j >= 4. Those are the features that
+// lane group q holds, four from each of the output tiles 2 s and 2 s + 1, after a 16x16 MFMA tile
+// product (lane i: rows 4 (i / 16) .. +4 of column i % 16), so one layer's outputs are the next
+// layer's operand without any data movement.
 struct MLPShared {
   uint16_t w1[MLP_HIDDEN * LDW_IN];
   uint16_t wh[3][MLP_HIDDEN * LDW_H];
@@ -36,21 +47,33 @@ struct MLPShared {
   float b[4 * MLP_HIDDEN + MLP_OUT_PAD];
 };
 
+// Slot of the first of four consecutive input features k0 .. k0 + 3 (k0 a multiple of 4).
+__device__ __forceinline__ int mlp_slot4(int k0) {
+  const int r = k0 & 31;
+  return (k0 & ~31) + (r < 16 ? 2 * r : 2 * (r - 16) + 4);
+}
+
+static __device__ void stage_rows(uint16_t* dst, int ldw, const uint16_t* __restrict__ src, int rows, int K) {
+  // 4 features (8 bytes) per copy: they stay consecutive in the staged order
+  const int chunks = rows * K / 4;
+  for (int i = threadIdx.x; i < chunks; i += blockDim.x) {
+    const int o = i / (K / 4), k0 = 4 * (i % (K / 4));
+    *reinterpret_cast<uint2*>(dst + o * ldw + mlp_slot4(k0)) = reinterpret_cast<const uint2*>(src)[i];
+  }
+}
+
 static __device__ void stage_weights(MLPShared& s, const uint16_t* __restrict__ wbf, const float* __restrict__ master) {
-  // forward BF16 copy [N][K] per layer -> padded LDS rows
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < MLP_HIDDEN * MLP_IN_PAD; i += nt) s.w1[(i / MLP_IN_PAD) * LDW_IN + i % MLP_IN_PAD] = wbf[MLP_BF_W1 + i];
+  stage_rows(s.w1, LDW_IN, wbf + MLP_BF_W1, MLP_HIDDEN, MLP_IN_PAD);
   for (int l = 0; l < 3; ++l)
-    for (int i = tid; i < MLP_HIDDEN * MLP_HIDDEN; i += nt)
-      s.wh[l][(i / MLP_HIDDEN) * LDW_H + i % MLP_HIDDEN] = wbf[MLP_BF_WH + l * MLP_HIDDEN * MLP_HIDDEN + i];
-  for (int i = tid; i < MLP_OUT_PAD * MLP_HIDDEN; i += nt) s.w5[(i / MLP_HIDDEN) * LDW_H + i % MLP_HIDDEN] = wbf[MLP_BF_W5 + i];
+    stage_rows(s.wh[l], LDW_H, wbf + MLP_BF_WH + l * MLP_HIDDEN * MLP_HIDDEN, MLP_HIDDEN, MLP_HIDDEN);
+  stage_rows(s.w5, LDW_H, wbf + MLP_BF_W5, MLP_OUT_PAD, MLP_HIDDEN);
   for (int l = 0; l < 4; ++l)
-    for (int i = tid; i < MLP_HIDDEN; i += nt) s.b[l * MLP_HIDDEN + i] = master[mlp_bias_offset(l) + i];
-  for (int i = tid; i < MLP_OUT_PAD; i += nt) s.b[4 * MLP_HIDDEN + i] = master[mlp_bias_offset(4) + i];
+    for (int i = threadIdx.x; i < MLP_HIDDEN; i += blockDim.x) s.b[l * MLP_HIDDEN + i] = master[mlp_bias_offset(l) + i];
+  for (int i = threadIdx.x; i < MLP_OUT_PAD; i += blockDim.x) s.b[4 * MLP_HIDDEN + i] = master[mlp_bias_offset(4) + i];
 }
 
 // acc[t] (16 rows x 16 cols of output tile t) += A (16 x 32*KS, row stride lda) * W^T where W is
-// [16*NT][32*KS] with row stride ldw: every operand is 8 contiguous bf16 per lane.
+// [16*NT][32*KS] with row stride ldw: every operand is 8 contiguous bf16 per lane (backward pass).
 template <int NT, int KS>
 __device__ __forceinline__ void wave_gemm_nt(const uint16_t* A, int lda, const uint16_t* W, int ldw, f32x4* acc) {
   const int lane = threadIdx.x & 63;
@@ -66,70 +89,121 @@ __device__ __forceinline__ void wave_gemm_nt(const uint16_t* A, int lda, const u
   }
 }
 
-// Normalised input row of one sample into the wave's activation rows (row = lane & 15 is written
-// by lanes 0..15; the rest of the padded row is zero).
-__device__ __forceinline__ void mlp_input_row(uint16_t* Arow, const float* in5) {
-  Arow[0] = f2bf(in5[0] * (1.0f / MLP_EPS_SCALE));
-  Arow[1] = f2bf(in5[1] * (1.0f / MLP_EPS_SCALE));
-  Arow[2] = f2bf(in5[2] * (1.0f / MLP_EPS_SCALE));
-  Arow[3] = f2bf((in5[3] - MLP_PHI_MID) * (1.0f / MLP_PHI_SPAN));
-  Arow[4] = f2bf((in5[4] - MLP_KMU_MID) * (1.0f / MLP_KMU_SPAN));
-  for (int c = 5; c < MLP_IN_PAD; ++c) Arow[c] = 0;
-}
-
-// Forward pass of 16 rows held by one wave. `act` is the wave's [16][LDA] bf16 scratch whose first
-// MLP_IN_PAD columns hold the input rows on entry. Returns the 3 outputs of row (lane & 15) in
-// lanes 0..15 through out3 (others undefined). When zsave is not null, the pre-activations of the 4
-// hidden layers are stored to zsave[layer][row][128] (FP32).
-static __device__ void wave_forward16(const MLPShared& s, uint16_t* act, float* out3, float* zsave, int zstride_rows) {
-  const int lane = threadIdx.x & 63;
-  const int col = lane & 15, rq = (lane >> 4) * 4;
-  f32x4 acc[8];
-  // layer 1
-  for (int l = 0; l < 4; ++l) {
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const float bb = s.b[l * MLP_HIDDEN + 16 * t + col];
-      acc[t] = f32x4{bb, bb, bb, bb};
-    }
-    if (l == 0) wave_gemm_nt<8, 1>(act, LDA, s.w1, LDW_IN, acc);
-    else wave_gemm_nt<8, 4>(act, LDA, s.wh[l - 1], LDW_H, acc);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float z = acc[t][j];
-        const int row = rq + j, c = 16 * t + col;
-        if (zsave) zsave[(size_t)(l * zstride_rows + row) * MLP_HIDDEN + c] = z;
-        act[row * LDA + c] = f2bf(silu(z));
-      }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
-  f32x4 o[1];
+// Forward pass of 64 samples held by one wave, one per lane: lane L passes the raw inputs x of
+// sample L (eps(3), phi, K/mu) and receives its 3 normalised outputs in y. Each layer computes
+// Z^T = W A^T on MFMA with the staged weights as the A operand (16 output features x 32 inputs)
+// and the activations as the B operand (32 inputs x 16 samples), for the wave's four 16-sample
+// tiles T: lane i of tile T holds output features 16 t + 4 (i / 16) + 0..3 of sample 16 T + i % 16,
+// which SiLU and BF16 rounding turn into the next layer's operand in place. When zsave is not null,
+// the FP32 pre-activations of sample r = row0 + L < nrows go to zsave[(r / 16) * 4 + layer][r % 16][128].
+static __device__ void wave_forward64(const MLPShared& s, const float x[5], float y[3], float* __restrict__ zsave,
+                                      int64_t row0, int64_t nrows) {
+  const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
+  bf16x8 b[4][4];
   {
-    const float bb = s.b[4 * MLP_HIDDEN + col];
-    o[0] = f32x4{bb, bb, bb, bb};
-  }
-  wave_gemm_nt<1, 4>(act, LDA, s.w5, LDW_H, o);
-  // redistribute: output (row, col) lives in lane col + 16*(row/4), element row%4
-  __builtin_amdgcn_wave_barrier();
-  float* scratch = reinterpret_cast<float*>(act);  // 16 x 16 floats fit in the activation rows
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const float xn[5] = {x[0] * (1.0f / MLP_EPS_SCALE), x[1] * (1.0f / MLP_EPS_SCALE), x[2] * (1.0f / MLP_EPS_SCALE),
+                         (x[3] - MLP_PHI_MID) * (1.0f / MLP_PHI_SPAN), (x[4] - MLP_KMU_MID) * (1.0f / MLP_KMU_SPAN)};
 #pragma unroll
-  for (int j = 0; j < 4; ++j) scratch[(rq + j) * 16 + col] = o[0][j];
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  if (lane < 16) {
-    out3[0] = scratch[lane * 16 + 0];
-    out3[1] = scratch[lane * 16 + 1];
-    out3[2] = scratch[lane * 16 + 2];
+    for (int T = 0; T < 4; ++T) {
+      float v[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v[k] = __shfl(xn[k], 16 * T + c);
+      // layer-1 slots: group 0 holds inputs 0..3 (slots 0..3), group 1 input 4 (slot 0)
+      bf16x8 o;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) o[j] = (__bf16)0.0f;
+      if (q == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = (__bf16)v[j];
+      } else if (q == 1) {
+        o[0] = (__bf16)v[4];
+      }
+      b[T][0] = o;
+#pragma unroll
+      for (int ss = 1; ss < 4; ++ss) b[T][ss] = o;  // not read by layer 1
+    }
   }
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const uint16_t* W = l == 0 ? s.w1 : s.wh[l - 1];
+    const int ldw = l == 0 ? LDW_IN : LDW_H;
+    const int KS = l == 0 ? 1 : 4;
+    bf16x8 nb[4][4];
+    // two halves of 4 output tiles each (64 accumulator registers live at a time): half h gives
+    // the next layer's K steps 2h and 2h + 1
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      f32x4 acc[4][4];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) {
+        const f32x4 bb = *reinterpret_cast<const f32x4*>(&s.b[l * MLP_HIDDEN + 16 * (4 * h + tt) + 4 * q]);
+#pragma unroll
+        for (int T = 0; T < 4; ++T) acc[T][tt] = bb;
+      }
+#pragma unroll
+      for (int ss = 0; ss < KS; ++ss) {
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * (4 * h + tt) + c) * ldw + 32 * ss + 8 * q);
+#pragma unroll
+          for (int T = 0; T < 4; ++T)
+            acc[T][tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[T][ss], acc[T][tt], 0, 0, 0);
+        }
+        // keep the scheduler from hoisting the weight loads of later K steps (registers)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (zsave) {
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+          const int64_t r = row0 + 16 * T + c;
+          if (r < nrows) {
+            float* zr = zsave + (((r >> 4) * 4 + l) * 16 + (r & 15)) * MLP_HIDDEN + 64 * h + 4 * q;
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) *reinterpret_cast<f32x4*>(zr + 16 * tt) = acc[T][tt];
+          }
+        }
+      }
+#pragma unroll
+      for (int T = 0; T < 4; ++T)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          bf16x8 o;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            o[j] = (__bf16)silu(acc[T][2 * k][j]);
+            o[4 + j] = (__bf16)silu(acc[T][2 * k + 1][j]);
+          }
+          nb[T][2 * h + k] = o;
+        }
+    }
+#pragma unroll
+    for (int T = 0; T < 4; ++T)
+#pragma unroll
+      for (int ss = 0; ss < 4; ++ss) b[T][ss] = nb[T][ss];
+  }
+  f32x4 out[4];
+  {
+    const f32x4 bb = *reinterpret_cast<const f32x4*>(&s.b[4 * MLP_HIDDEN + 4 * q]);
+#pragma unroll
+    for (int T = 0; T < 4; ++T) out[T] = bb;
+  }
+#pragma unroll
+  for (int ss = 0; ss < 4; ++ss) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(s.w5 + c * LDW_H + 32 * ss + 8 * q);
+#pragma unroll
+    for (int T = 0; T < 4; ++T) out[T] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[T][ss], out[T], 0, 0, 0);
+  }
+  // outputs 0..2 of sample 16 T + c sit in lane c (group 0) of tile T
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float v = 0.0f;
+#pragma unroll
+    for (int T = 0; T < 4; ++T) {
+      const float u = __shfl(out[T][k], c);
+      if (q == T) v = u;
+    }
+    y[k] = v;
+  }
 }
-
 
 }  // namespace mpm

@@ -143,6 +143,8 @@ struct mpm_ctx {
   bool mlp_ready = false;
   float* pair_log = nullptr;
   int64_t pair_cap = 0;
+  float* dec_log = nullptr;  // per-candidate gate records of the last surrogate substep
+  int64_t dec_cap = 0;
   unsigned long long* sur_counts = nullptr;  // [0] candidates, [1] accepted, [2] pairs logged
   int sur_enabled = 0;
   float gate_tol = 1e-5f;
@@ -150,8 +152,9 @@ struct mpm_ctx {
   int* cand_slot = nullptr;
   // set by mpm_time_kernels: the start and end of a stage's main kernel (taken by its launch)
   hipEvent_t ev_main[2] = {nullptr, nullptr};
+  hipEvent_t ev_sur[4] = {nullptr, nullptr, nullptr, nullptr};  // surrogate forward, apply: start, stop
   int* apex_slot = nullptr;  // sorted slots of the deferred FP64 apex projections (n)
-  float* apex_usv = nullptr;  // their trial SVD (CAND_PLANES x n)
+  float* apex_usv = nullptr;  // their trial SVD (CAND_SVD_PLANES x n)
 };
 
 static void drop_mesh(mpm_ctx* ctx);
@@ -552,8 +555,10 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     sa.write_stress = a.write_stress;
     sa.accepted = ctx->sur_counts + 1;
     sa.candidates = ctx->sur_counts;
+    sa.dec_log = ctx->dec_log;
+    sa.dec_cap = ctx->dec_cap;
     sa.clk = ctx->clk_active;
-    launch_surrogate_apply(ctx->stream, sa, ctx->n_cu);
+    launch_surrogate(ctx->stream, sa, ctx->n_cu, ctx->ev_sur);
   }
   // cloth elements follow their vertices (positions, velocities, d1 d2) and are return-mapped
   if (ctx->mesh.ne > 0) launch_cloth_elements(ctx->stream, cloth_args(ctx, 1 - ctx->cur));
@@ -684,7 +689,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   dfree(ctx->clock);
   dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
-  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot);
+  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->dec_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPM_OK;
@@ -877,7 +882,7 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
     if (!rc) rc = dalloc(ctx, ctx->v2, n);
     if (!rc) rc = dalloc(ctx, ctx->src, n);
     if (!rc) rc = dalloc(ctx, ctx->apex_slot, n);
-    if (!rc) rc = dalloc(ctx, ctx->apex_usv, (size_t)CAND_PLANES * n);
+    if (!rc) rc = dalloc(ctx, ctx->apex_usv, (size_t)CAND_SVD_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
     if (rc) return rc;
@@ -1616,7 +1621,10 @@ int mpm_synchronize(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
-int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms) {
+// Eagerly launched substeps with HIP events around every stage, around the G2P main kernel and,
+// when sur_ms is given and the surrogate is on, around its two kernels (sur_ms[0] forward, [1] apply).
+static int time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms,
+                        float* sur_ms) {
   int rc = ready(ctx);
   if (rc) return rc;
   if (ctx->next_stage != MPM_STAGE_SORT) return fail(ctx, MPM_ERR_STATE, "a staged substep is in progress");
@@ -1628,6 +1636,9 @@ int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, fl
   CK(hipEventCreate(&t1));
   CK(hipEventCreate(&ctx->ev_main[0]));
   CK(hipEventCreate(&ctx->ev_main[1]));
+  const bool sur = sur_ms && ctx->sur_enabled;
+  if (sur)
+    for (auto& e : ctx->ev_sur) CK(hipEventCreate(&e));
   struct Clear {  // the stages record the main-kernel events only inside this call
     mpm_ctx* c;
     ~Clear() {
@@ -1635,9 +1646,13 @@ int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, fl
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
       }
+      for (auto& e : c->ev_sur) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+      }
     }
   } clear{ctx};
-  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0}, g2p_main = 0.0f;
+  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0}, g2p_main = 0.0f, sacc[2] = {0.0f, 0.0f};
   CK(hipEventRecord(t0, ctx->stream));
   for (int i = 0; i < nsteps; ++i) {
     CK(hipEventRecord(ev[0], ctx->stream));
@@ -1655,6 +1670,11 @@ int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, fl
     float ms = 0;
     CK(hipEventElapsedTime(&ms, ctx->ev_main[0], ctx->ev_main[1]));
     g2p_main += ms;
+    if (sur)
+      for (int k = 0; k < 2; ++k) {
+        CK(hipEventElapsedTime(&ms, ctx->ev_sur[2 * k], ctx->ev_sur[2 * k + 1]));
+        sacc[k] += ms;
+      }
   }
   CK(hipEventRecord(t1, ctx->stream));
   CK(hipEventSynchronize(t1));
@@ -1667,10 +1687,36 @@ int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, fl
     for (int st = 0; st < MPM_NUM_STAGES; ++st) kernel_ms[st] = acc[st];
     kernel_ms[MPM_STAGE_G2P] = g2p_main;
   }
+  if (sur_ms) {
+    sur_ms[0] = sacc[0];
+    sur_ms[1] = sacc[1];
+  }
   for (auto& e : ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   CK(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms) {
+  return time_kernels(ctx, dt, nsteps, total_ms, stage_ms, kernel_ms, nullptr);
+}
+
+int mpm_time_surrogate(mpm_ctx* ctx, float dt, int32_t nsteps, float* forward_ms, float* apply_ms,
+                       int64_t* candidates) {
+  if (!ctx || !forward_ms || !apply_ms) return MPM_ERR_INVALID;
+  if (!ctx->sur_enabled) return fail(ctx, MPM_ERR_STATE, "the surrogate is not enabled");
+  unsigned long long c0 = 0, c1 = 0;
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(&c0, ctx->sur_counts, sizeof(c0), hipMemcpyDeviceToHost));
+  float ms[2] = {0.0f, 0.0f};
+  int rc = time_kernels(ctx, dt, nsteps, nullptr, nullptr, nullptr, ms);
+  if (rc) return rc;
+  CK(hipMemcpy(&c1, ctx->sur_counts, sizeof(c1), hipMemcpyDeviceToHost));
+  *forward_ms = ms[0];
+  *apply_ms = ms[1];
+  if (candidates) *candidates = (int64_t)(c1 - c0);
   return MPM_OK;
 }
 
@@ -1781,6 +1827,12 @@ static void targets_of(const float* in, const float* H, int64_t n, std::vector<f
     for (int c = 0; c < 3; ++c) y[3 * r + c] = (H[3 * r + c] - in[5 * r + c]) / MLP_CORR_SCALE;
 }
 
+// workgroups of the forward kernels: one per CU at most (each stages the weights once and its 8
+// waves take 64 rows at a time)
+static int forward_grid(const mpm_ctx* ctx, int64_t rows) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(ctx->n_cu, (rows + MLP_FWD_THREADS - 1) / MLP_FWD_THREADS));
+}
+
 int mpm_surrogate_predict(mpm_ctx* ctx, const float* in, float* H, int32_t n) {
   if (!ctx || !in || !H || n < 0) return MPM_ERR_INVALID;
   if (!ctx->mlp_ready) return fail(ctx, MPM_ERR_STATE, "surrogate weights not set");
@@ -1792,8 +1844,7 @@ int mpm_surrogate_predict(mpm_ctx* ctx, const float* in, float* H, int32_t n) {
   if (rc) { dfree(din); dfree(dout); return rc; }
   hipError_t e = hipMemcpyAsync(din, in, sizeof(float) * 5 * n, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
-    launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, din, dout, n, nullptr,
-                       std::min(4 * ctx->n_cu, (n + 63) / 64));
+    launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, din, dout, n, nullptr, forward_grid(ctx, n));
     e = hipMemcpyAsync(H, dout, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1837,8 +1888,7 @@ static int alloc_train(mpm_ctx* ctx, TrainBufs& b, int64_t n, int batch, int nac
 
 static void train_batch(mpm_ctx* ctx, TrainBufs& b, int64_t r0, int nb, float* lacc) {
   const int groups = backward_grid(nb);
-  launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, b.in + 5 * r0, b.y, nb, b.z,
-                     std::min((nb + 63) / 64, 4 * ctx->n_cu));
+  launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, b.in + 5 * r0, b.y, nb, b.z, forward_grid(ctx, nb));
   launch_mlp_backward(ctx->stream, ctx->mlp_wbft, b.in + 5 * r0, b.y, b.tgt + 3 * r0, b.z, nb, b.gpart, b.lpart, groups);
   launch_mlp_reduce(ctx->stream, b.gpart, groups, ctx->mlp_grad, b.lpart, lacc);
 }
@@ -1941,6 +1991,30 @@ int mpm_get_pairs(mpm_ctx* ctx, float* pairs, int64_t max_pairs, int64_t* count)
   m = std::min<int64_t>(m, max_pairs);
   if (count) *count = m;
   if (pairs && m > 0) CK(hipMemcpy(pairs, ctx->pair_log, sizeof(float) * 8 * m, hipMemcpyDeviceToHost));
+  return MPM_OK;
+}
+
+int mpm_set_surrogate_log(mpm_ctx* ctx, int64_t capacity) {
+  if (!ctx || capacity < 0) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  int rc = dalloc(ctx, ctx->dec_log, (size_t)10 * capacity);
+  if (rc) return rc;
+  ctx->dec_cap = capacity;
+  return MPM_OK;
+}
+
+int mpm_get_surrogate_log(mpm_ctx* ctx, float* records, int64_t max_records, int64_t* count) {
+  if (!ctx || max_records < 0) return MPM_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(hipStreamSynchronize(ctx->stream));
+  int c = 0;
+  if (ctx->counters) CK(hipMemcpy(&c, ctx->counters + CNT_CAND, sizeof(c), hipMemcpyDeviceToHost));
+  int64_t m = ctx->sur_enabled ? std::min<int64_t>(std::max(0, std::min(c, ctx->n)), ctx->dec_cap) : 0;
+  m = std::min<int64_t>(m, max_records);
+  if (count) *count = m;
+  if (records && m > 0) CK(hipMemcpy(records, ctx->dec_log, sizeof(float) * 10 * m, hipMemcpyDeviceToHost));
   return MPM_OK;
 }
 

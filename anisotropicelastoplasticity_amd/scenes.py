@@ -196,6 +196,17 @@ def config4_sand_phi(seed=4, substeps=1000, cells=(64, 128, 64)):
                  substeps, dict(ppc=n / (vol * 256 ** 3)))
 
 
+def surrogate_training_column(seed=41, substeps=1000, cells=(32, 128, 32)):
+    """The run whose exact return map supplies the surrogate's training pairs: config 2's sand column
+    (2^20 particles, 128^3, dt 5e-5, same walls and sphere) with config 4's per-particle friction
+    angles uniform in [25, 40) degrees, from its own seed."""
+    sc = config2_sand_column(seed=seed, substeps=substeps, cells=cells)
+    sc.params.per_particle_phi = 1
+    sc.particles["phi"] = particle_friction_angles(seed, sc.n)
+    sc.name = "surrogate_training_column"
+    return sc
+
+
 def small_scene(material, order, predicted, friction_model, seed=7, res=(24, 24, 24), cells=(6, 5, 6), ppa=2,
                 lo=(0.3, 0.2, 0.35), hi=(0.55, 0.42, 0.6), v0=(0.3, -1.5, 0.2), z_up=False, prims=None,
                 dt=2e-4, E=None):

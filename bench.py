@@ -1,24 +1,27 @@
-"""Benchmark: particle-substeps/s of the explicit MPM substep on one MI355X.
+"""Benchmark: particle-substeps/s of the explicit MPM substep on one MI355X, surrogate against exact
+return map.
 
-Workload: Drucker-Prager sand column collapse, 1,048,576 particles on a 128^3 grid, dt = 5e-5,
-quadratic B-spline APIC, stress at F^n, box walls + sphere with Coulomb friction, exact return
-mapping. Inputs are synthetic (seeded jittered lattice), resident in HBM before timing starts.
+Headline workload (`value`): the learned return-map surrogate's own configuration, 2^22 Drucker-Prager
+sand particles with per-particle friction angles uniform in [25, 40) degrees on a 256^3 grid,
+dt = 5e-5, quadratic B-spline APIC, stress at F^n, box walls + sphere with Coulomb friction 0.4.
+`value` is the throughput of the surrogate path (the MLP evaluated on BF16 MFMA for every cone-branch
+particle, its prediction gated by the yield residual, the exact projection as fallback) over the
+last `--steps` substeps of a run of max(1020, warmup + steps) substeps. The same run with the exact
+return map gives `exact`; the relative position error of the surrogate run against it after those
+substeps is `rel_pos_err`, and `error_bar_met` says whether it is within 1e-3. The network is
+trained on the GPU beforehand, on (trial, projected) pairs logged by the exact path in other runs:
+a sand column of config 2's shape with per-particle angles and a batch of 64 mixed scenes, each
+from its own seed. Inputs are synthetic (seeded jittered lattices), resident in HBM before timing.
 
-`value` is the exact-return-map path; the `surrogate` object reports the same workload with the
-learned return map (trained on the GPU from pairs the exact path logs during the run), its
-throughput and its position error against the exact run after the same substeps. The `batched`
-object reports the batched many-scene workload (64 independent snow/sand scenes of 20k-500k
-particles on 64^3-128^3 grids, one context, one launch per kernel per substep, autotuned launch
-shapes) against the same scenes run one after another with the plain exact path, with and without
-the learned return map. The `config3` object reports the cloth-sand coupling's throughput. The
-`config4` object reports the learned return map's own workload (2^22 sand
-particles with per-particle friction angles on a 256^3 grid): the exact path's throughput and the
-surrogate against it after 1020 substeps.
-
-Prints one JSON line. `roofline` is for the dominant kernel (its own algorithmic bytes over its
-HIP-event-timed duration); `substep_roofline` is the whole substep against 208 B/particle +
-56 B/active node. `cpu_baseline` is the CPU oracle (a single-threaded FP64 restatement of the
-reference substep) timed on this host on a bounded sample of the same workload.
+Other objects on the same line: `config2` (the 2^20-particle sand column, exact path, with its
+dominant kernel's roofline), `batched` (64 independent snow/sand scenes in one context against the
+same scenes one context at a time over the same substep window, exact and with the surrogate),
+`config3` (cloth-sand coupling). `roofline` is the headline workload's dominant kernel: its own
+algorithmic bytes (HBM-bound kernels) or flops (the MFMA network kernel) over its HIP-event-timed
+duration. `cpu_baseline` is the CPU oracle (a C++ FP64 restatement of the reference substep) timed on
+this host on a bounded sample of the headline workload: one thread (the reference is single-threaded)
+and, when built, all cores with OpenMP; the reference's own mode (cubic B-splines, stress at the
+predicted F) is timed beside it.
 """
 import argparse
 import json
@@ -32,13 +35,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 METRIC = "particle-substeps/sec, surrogate vs exact return map at ≤1e-3 rel pos err"
+COMPARE_SUBSTEPS = 1020
 
 # Algorithmic bytes of each stage (see the kernel descriptions in mpm_kernels.hip):
 #   P2G  reads x, v, B, F, m, V (26 floats) per particle, writes m and momentum (16 B) per active node
 #   GRID reads m, momentum (16 B), writes v and v_BF (24 B) per active node
 #   G2P  reads x, F, plastic (13 floats), writes x, v, B, F, plastic (25 floats) per particle,
-#        reads v and v_BF (24 B) per active node
+#        reads v and v_BF (24 B) per active node; a per-particle friction angle adds 4 B per particle
 #   SORT (counting sort) reads the key twice and the rank once, writes rank, scattered id, sorted key,
 #        id and slot index: 36 B per particle, plus the cell counts scanned into the bins
 STAGES = ["sort", "p2g", "grid", "g2p"]
@@ -46,12 +51,15 @@ ALG_PER_PARTICLE = {"sort": 36, "p2g": 104, "grid": 0, "g2p": 152}
 ALG_PER_NODE = {"sort": 0, "p2g": 16, "grid": 40, "g2p": 24}
 SUBSTEP_PER_PARTICLE, SUBSTEP_PER_NODE = 208, 56
 KERNEL_NAMES = {"p2g": "k_p2g<2, P2G_MOM_STRESS>", "g2p": "k_g2p<2>", "grid": "k_grid<GRID_FUSED>",
-                "sort": "k_cell_count + k_cell_scan + k_cell_scatter + k_cell_rank + k_claim x2"}
+                "sort": "k_cell_count + k_cell_scan + k_cell_scatter + k_cell_rank + k_claim",
+                "surrogate": "k_surrogate_forward"}
+# flops of one network evaluation: 2 x (5x128 + 3 x 128x128 + 128x3) multiply-adds
+MLP_FLOPS = 2 * (5 * 128 + 3 * 128 * 128 + 128 * 3)
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, workload):
     """HBM traffic per launch from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc_config2.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc_{workload}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -63,85 +71,210 @@ def load_traffic(kernel):
     return None
 
 
-def cpu_baseline(sample_steps=3):
-    """Time the FP64 oracle on this host: config 2 at full size, 1 warm-up + `sample_steps` substeps."""
-    from anisotropicelastoplasticity_amd import scenes
-    from oracle.bind import Oracle
-    sc = scenes.config2_sand_column()
-    o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
-    o.load_scene(sc)
-    o.substep(sc.dt)
-    t0 = time.perf_counter()
-    o.run(sc.dt, sample_steps)
-    t = time.perf_counter() - t0
-    return dict(value=sc.n * sample_steps / t, unit="particle-substeps/s", cores=1, kind="port",
-                sample=f"config2 full size ({sc.n} particles, 128^3), 1 warm-up + {sample_steps} timed substeps, "
-                       "FP64 oracle (C++ restatement of the reference substep, -O3, one thread)")
-
-
-def _sync_torch():
+def host_cpu():
+    model = None
     try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
         pass
+    return os.cpu_count(), model
 
 
-def surrogate_leg(sc, warmup, steps, x_exact, wall_exact, gate_tol=1e-5, pair_chunks=10, pairs_per_chunk=200000,
-                  epochs=8, batch=4096, lr=1e-3):
-    """The learned return map against the exact one on the same workload: (trial, projected) pairs are
-    logged by the exact path at `pair_chunks` substeps spread over the run, the network is trained on
-    the GPU, then the run is repeated from the same initial state with the surrogate (gate `gate_tol`,
-    exact fallback) and timed; positions are compared after warmup + steps substeps."""
-    from anisotropicelastoplasticity_amd import Solver
-    total = warmup + steps
-    pairs = []
-    with Solver(0) as s:
-        sc.load_into(s)
-        every = max(1, total // pair_chunks)
-        done = 0
-        while done < total:
-            s.set_pair_logging(pairs_per_chunk)
-            s.run(sc.dt, 1)
-            pairs.append(s.get_pairs())
-            s.set_pair_logging(0)
-            k = min(every - 1, total - done - 1)
-            if k > 0:
-                s.run(sc.dt, k)
-            done += 1 + max(k, 0)
-        pairs = np.concatenate(pairs) if pairs else np.zeros((0, 8), np.float32)
+def cpu_baseline(sc):
+    """The FP64 oracle on this host, on the headline scene at full size: 1 warm-up substep, then 1
+    timed substep single-threaded; the same with the OpenMP build on all cores; and one substep in
+    the reference's own mode (cubic B-splines, stress at the predicted F)."""
+    import copy
+    import ctypes
+    from oracle.bind import Oracle, load
+    from oracle.build import LIB_OMP
+    nproc, model = host_cpu()
+
+    def timed(params, lib_path=None):
+        o = Oracle(params, sc.gmin, sc.gmax, sc.res, prims=sc.prims, lib_path=lib_path)
+        o.load_scene(sc)
+        o.substep(sc.dt)
         t0 = time.perf_counter()
+        o.substep(sc.dt)
+        t = time.perf_counter() - t0
+        o.close()
+        return sc.n / t
+
+    out = dict(value=timed(sc.params), unit="particle-substeps/s", cores=1, kind="port",
+               sample=f"{sc.name} full size ({sc.n} particles, {sc.res[0]}^3), 1 warm-up + 1 timed substep, FP64 "
+                      "oracle (C++ restatement of the reference substep, -O3), one thread",
+               host_nproc=nproc, host_cpu=model)
+    if os.path.exists(LIB_OMP):
+        # the host's CPU share for this process (OMP_NUM_THREADS where the machine sets it), else nproc
+        threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+        load(LIB_OMP).orc_set_threads(ctypes.c_int(threads))
+        out["all_cores"] = dict(value=timed(sc.params, LIB_OMP), cores=threads, kind="port",
+                                sample="same, the oracle's OpenMP build (particle and node loops in parallel, "
+                                       f"atomic grid scatters) on {threads} threads")
+    p = copy.copy(sc.params)
+    p.kernel_order = 3
+    p.stress_at_predicted_F = 1
+    out["reference_mode"] = dict(value=timed(p), cores=1, kind="port",
+                                 sample="same scene in the reference's own mode: cubic B-splines, stress at the "
+                                        "predicted F, one thread")
+    return out
+
+
+def log_pairs(s, dt, total, chunks, per_chunk):
+    """Run `total` substeps, logging the exact path's (trial, projected) pairs in `chunks` substeps
+    spread over the run (at most per_chunk pairs each)."""
+    pairs = []
+    every = max(1, total // chunks)
+    done = 0
+    while done < total:
+        s.set_pair_logging(per_chunk)
+        s.run(dt, 1)
+        pairs.append(s.get_pairs())
+        s.set_pair_logging(0)
+        k = min(every - 1, total - done - 1)
+        if k > 0:
+            s.run(dt, k)
+        done += 1 + max(k, 0)
+    return np.concatenate(pairs) if pairs else np.zeros((0, 8), np.float32)
+
+
+def train_surrogate(target_pairs=10_000_000, epochs=4, batch=4096, lr=1e-3):
+    """Pairs logged by the exact path in a config-2-shaped sand column with per-particle angles and in
+    a batch of 64 mixed scenes (other seeds than the measured workloads), then Adam on the GPU."""
+    from anisotropicelastoplasticity_amd import Solver, scenes
+    t0 = time.perf_counter()
+    col = scenes.surrogate_training_column()
+    with Solver(0) as s:
+        col.load_into(s)
+        p_col = log_pairs(s, col.dt, 1000, 20, target_pairs // 40)
+    bat = scenes.config5_batch(seed=55)
+    with Solver(0) as s:
+        s.load_batch(bat)
+        p_bat = log_pairs(s, bat[0].dt, 500, 20, target_pairs // 40)
+    pairs = np.concatenate([p_col, p_bat])
+    log_s = time.perf_counter() - t0
+    with Solver(0) as s:
+        t1 = time.perf_counter()
         s.surrogate_init(1)
         losses = s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch, lr=lr, seed=2)
         s.synchronize()
-        train_s = time.perf_counter() - t0
+        train_s = time.perf_counter() - t1
         w = s.surrogate_get_weights()
+    info = {"pairs": int(len(pairs)), "pairs_column": int(len(p_col)), "pairs_batch": int(len(p_bat)),
+            "epochs": epochs, "batch": batch, "loss_first_last": [float(losses[0]), float(losses[-1])],
+            "log_s": log_s, "train_s": train_s,
+            "sources": "surrogate_training_column (seed 41, 1000 substeps) + config5_batch(seed=55) (500 substeps)"}
+    return w, info
+
+
+def timed_run(sc, total, steps, weights=None, gate_tol=1e-5):
+    """Run `total` substeps of sc; time the last `steps` of them. Returns wall, positions, stats and
+    a per-kernel breakdown measured afterwards on eagerly launched substeps."""
+    from anisotropicelastoplasticity_amd import Solver
     with Solver(0) as s:
         sc.load_into(s)
-        s.surrogate_set_weights(w)
-        s.set_surrogate(True, gate_tol)
-        s.run(sc.dt, warmup)
+        if weights is not None:
+            s.surrogate_set_weights(weights)
+            s.set_surrogate(True, gate_tol)
+        s.run(sc.dt, total - steps)
         s.synchronize()
-        _sync_torch()
         t0 = time.perf_counter()
         s.run(sc.dt, steps)
         s.synchronize()
         wall = time.perf_counter() - t0
         x = s.get_particles()["x"]
         st = s.stats()
-    dx = x.astype(np.float64) - x_exact.astype(np.float64)
+        kt = 10
+        _, stage_ms, kernel_ms = s.time_kernels(sc.dt, kt)
+        kern = {k: kernel_ms[i] / kt for i, k in enumerate(STAGES)}
+        stages = {k: stage_ms[i] / kt for i, k in enumerate(STAGES)}
+        sur = None
+        if weights is not None:
+            f_ms, a_ms, cand = s.time_surrogate(sc.dt, kt)
+            sur = {"forward_ms": f_ms / kt, "apply_ms": a_ms / kt, "candidates": cand / kt}
+        st_after = s.stats()
+    return dict(wall=wall, x=x, stats=st, kernel_ms=kern, stage_ms=stages, sur=sur,
+                active_nodes=st_after["n_active_nodes"])
+
+
+def roofline_of(kernel_ms, n, nodes, sur, per_particle_phi, workload):
+    """The dominant kernel's roofline: HBM-bound stencil kernels by algorithmic bytes, the network
+    kernel by algorithmic flops."""
+    cands = {k: kernel_ms[k] for k in ("p2g", "grid", "g2p")}
+    if sur is not None:
+        cands["surrogate"] = sur["forward_ms"]
+    dom = max(cands, key=cands.get)
+    ms = cands[dom]
+    if dom == "surrogate":
+        flops = MLP_FLOPS * sur["candidates"]
+        ach = flops / (ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": KERNEL_NAMES[dom], "achieved": ach, "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": load_traffic("surrogate", workload),
+                "alg_flops_per_launch": flops, "avg_launch_ms": ms}
+    alg = (ALG_PER_PARTICLE[dom] + (4 if dom == "g2p" and per_particle_phi else 0)) * n + ALG_PER_NODE[dom] * nodes
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": load_traffic(dom, workload), "alg_bytes_per_launch": alg,
+            "avg_launch_ms": ms}
+
+
+def headline(args, weights, train_info):
+    """Config 4 with the surrogate (value) and with the exact return map, over the same window."""
+    from anisotropicelastoplasticity_amd import scenes
+    sc = scenes.config4_sand_phi()
+    total = max(COMPARE_SUBSTEPS, args.warmup + args.steps)
+    ex = timed_run(sc, total, args.steps)
+    su = timed_run(sc, total, args.steps, weights=weights)
+    dx = su["x"].astype(np.float64) - ex["x"].astype(np.float64)
     h = (sc.gmax[0] - sc.gmin[0]) / sc.res[0]
+    rel_err = float(np.linalg.norm(dx) / np.linalg.norm(ex["x"].astype(np.float64)))
+    st = su["stats"]
     cand = st["surrogate_candidates"]
-    return {
-        "value": sc.n * steps / wall, "ms_per_step": 1e3 * wall / steps, "speedup_vs_exact": wall_exact / wall,
-        "rel_pos_err": float(np.linalg.norm(dx) / np.linalg.norm(x_exact.astype(np.float64))),
-        "max_dx_over_h": float(np.max(np.abs(dx)) / h), "substeps_compared": total,
-        "gate_tol": gate_tol, "candidates_per_substep": cand / total,
-        "acceptance": (st["surrogate_accepted"] / cand) if cand else 0.0,
-        "pairs_trained": int(len(pairs)), "train_epochs": epochs, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
-        "train_s": train_s, "status": st["status"],
-    }, w
+    value = sc.n * args.steps / su["wall"]
+    exact_value = sc.n * args.steps / ex["wall"]
+    sur = su["sur"]
+    sur_flops = MLP_FLOPS * sur["candidates"]
+    return sc, {
+        "value": value, "ms_per_step": 1e3 * su["wall"] / args.steps,
+        "exact": {"value": exact_value, "ms_per_step": 1e3 * ex["wall"] / args.steps, "kernel_ms": ex["kernel_ms"],
+                  "stage_ms": ex["stage_ms"],
+                  "roofline": roofline_of(ex["kernel_ms"], sc.n, ex["active_nodes"], None, True, "config4"),
+                  "fp64_apex_count": ex["stats"]["fp64_apex_count"], "status": ex["stats"]["status"]},
+        "speedup_vs_exact": value / exact_value,
+        "rel_pos_err": rel_err, "max_dx_over_h": float(np.max(np.abs(dx)) / h), "error_bar_met": rel_err <= 1e-3,
+        "substeps_compared": total, "timed_window": [total - args.steps, total],
+        "candidates_per_substep": cand / total, "acceptance": (st["surrogate_accepted"] / cand) if cand else 0.0,
+        "fallback_rate": 1.0 - ((st["surrogate_accepted"] / cand) if cand else 0.0),
+        "kernel_ms": su["kernel_ms"], "stage_ms": su["stage_ms"],
+        "surrogate_kernels": {"forward_ms": sur["forward_ms"], "apply_ms": sur["apply_ms"],
+                              "candidates_per_launch": sur["candidates"],
+                              "bf16_tflops": sur_flops / (sur["forward_ms"] * 1e-3) / 1e12 if sur["forward_ms"] else None,
+                              "bf16_frac": (sur_flops / (sur["forward_ms"] * 1e-3) / 1e12) / BF16_PEAK_TFLOPS
+                              if sur["forward_ms"] else None},
+        "roofline": roofline_of(su["kernel_ms"], sc.n, su["active_nodes"], sur, True, "config4"),
+        "active_nodes": su["active_nodes"], "status": st["status"], "train": train_info,
+    }
+
+
+def config2_leg(steps, warmup):
+    """The 2^20-particle sand column, exact path, with the dominant kernel's roofline."""
+    from anisotropicelastoplasticity_amd import scenes
+    sc = scenes.config2_sand_column()
+    r = timed_run(sc, warmup + steps, steps)
+    n, nodes = sc.n, r["active_nodes"]
+    b_alg = SUBSTEP_PER_PARTICLE * n + SUBSTEP_PER_NODE * nodes
+    ms = 1e3 * r["wall"] / steps
+    sub_ach = b_alg / (ms * 1e-3) / 1e9
+    return {"workload": "config2_sand_column", "particles": n, "grid": list(sc.res), "dt": sc.dt,
+            "substeps": warmup + steps, "value": n * steps / r["wall"], "ms_per_step": ms,
+            "kernel_ms": r["kernel_ms"], "stage_ms": r["stage_ms"], "active_nodes": nodes,
+            "roofline": roofline_of(r["kernel_ms"], n, nodes, None, False, "config2"),
+            "substep_roofline": {"achieved": sub_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": sub_ach / HBM_PEAK_GBS, "alg_bytes_per_substep": b_alg},
+            "fp64_apex_count": r["stats"]["fp64_apex_count"], "status": r["stats"]["status"]}
 
 
 def config3_leg(steps, warmup):
@@ -157,7 +290,6 @@ def config3_leg(steps, warmup):
         s.set_mesh(mesh["x"], mesh["faces"], **kw)
         s.run(sc.dt, warmup)
         s.synchronize()
-        _sync_torch()
         t0 = time.perf_counter()
         s.run(sc.dt, steps)
         s.synchronize()
@@ -171,65 +303,21 @@ def config3_leg(steps, warmup):
             "status": st["status"]}
 
 
-def config4_leg(surrogate, steps, warmup):
-    """The surrogate's workload (2^22 sand particles with per-particle friction angles on 256^3): the
-    exact path's throughput, and the learned return map against it after warmup + steps substeps."""
-    from anisotropicelastoplasticity_amd import Solver, scenes
-    sc = scenes.config4_sand_phi()
-    with Solver(0) as s:
-        sc.load_into(s)
-        s.run(sc.dt, warmup)
-        s.synchronize()
-        _sync_torch()
-        t0 = time.perf_counter()
-        s.run(sc.dt, steps)
-        s.synchronize()
-        wall = time.perf_counter() - t0
-        x = s.get_particles()["x"]
-        st = s.stats()
-    out = {"workload": "config4_sand_phi", "particles": sc.n, "grid": list(sc.res), "dt": sc.dt,
-           "substeps": warmup + steps, "value": sc.n * steps / wall, "ms_per_step": 1e3 * wall / steps,
-           "fp64_apex_count": st["fp64_apex_count"], "status": st["status"]}
-    if surrogate:
-        out["surrogate"], _ = surrogate_leg(sc, warmup, steps, x, wall)
-    return out
-
-
-def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=10, pairs_per_chunk=400000,
-                epochs=40, batch_size=4096, lr=1e-3):
-    """64 scenes in one context against the same scenes one context at a time (exact path), and the
-    batch with the learned return map (trained on the GPU from pairs the exact batch logs, exact
-    fallback) against the exact batch after the same substeps."""
+def batched_leg(weights, steps, warmup, gate_tol=1e-5):
+    """64 scenes in one context (exact, and with the surrogate) against the same scenes one context at
+    a time with the exact path; every leg times the same substep window [warmup, warmup + steps)."""
     from anisotropicelastoplasticity_amd import Solver, scenes
     batch = scenes.config5_batch()
     dt = batch[0].dt
     ntot = sum(sc.n for sc in batch)
-    weights = None
-    if surrogate:
-        total = warmup + steps
-        pairs = []
-        with Solver(0) as s:
-            s.load_batch(batch)
-            every = max(1, total // pair_chunks)
-            done = 0
-            while done < total:
-                s.set_pair_logging(pairs_per_chunk)
-                s.run(dt, 1)
-                pairs.append(s.get_pairs())
-                s.set_pair_logging(0)
-                k = min(every - 1, total - done - 1)
-                if k > 0:
-                    s.run(dt, k)
-                done += 1 + max(k, 0)
-            pairs = np.concatenate(pairs)
-            s.surrogate_init(1)
-            losses = s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch_size, lr=lr, seed=2)
-            weights = s.surrogate_get_weights()
 
     def run_batch(sur):
         with Solver(0) as s:
             s.load_batch(batch)
             tuning = s.autotune(dt, reps=3)
+            s.load_batch(batch)  # the autotuner advanced the state: start again from the inputs
+            s.set_tuning(tuning["p2g_ranks"], tuning["p2g_blocks"], tuning["grid_blocks"], tuning["sort_mode"],
+                         tuning["p2g_pass"])
             if sur:
                 s.surrogate_set_weights(weights)
                 s.set_surrogate(True, gate_tol)
@@ -245,20 +333,20 @@ def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=
 
     wall_b, x_b, st_b, tuning = run_batch(False)
     seq_wall = 0.0
-    for sc in batch:  # the plain path: one context per scene, default launch shapes
+    for sc in batch:  # the plain path: one context per scene, default launch shapes, same window
         with Solver(0) as s:
             sc.load_into(s)
             s.run(dt, warmup)
             s.synchronize()
             t0 = time.perf_counter()
-            s.run(dt, seq_steps)
+            s.run(dt, steps)
             s.synchronize()
             seq_wall += time.perf_counter() - t0
-    seq_value = ntot * seq_steps / seq_wall
+    seq_value = ntot * steps / seq_wall
     out = {"workload": "config5 batched scenes", "scenes": len(batch), "particles": ntot, "dt": dt,
-           "substeps": warmup + steps, "value": ntot * steps / wall_b, "ms_per_step": 1e3 * wall_b / steps,
-           "sequential_value": seq_value, "sequential_substeps_per_scene": seq_steps,
-           "speedup_vs_sequential": (ntot * steps / wall_b) / seq_value,
+           "substeps": warmup + steps, "timed_window": [warmup, warmup + steps],
+           "value": ntot * steps / wall_b, "ms_per_step": 1e3 * wall_b / steps,
+           "sequential_value": seq_value, "speedup_vs_sequential": (ntot * steps / wall_b) / seq_value,
            "tuning": {k: tuning[k] for k in ("sort_mode", "p2g_ranks", "p2g_blocks", "grid_blocks")},
            "status": st_b["status"]}
     if weights is not None:
@@ -270,129 +358,109 @@ def batched_leg(surrogate, steps, warmup, seq_steps, gate_tol=1e-5, pair_chunks=
                             "speedup_vs_batched_exact": wall_b / wall_s,
                             "rel_pos_err": float(np.linalg.norm(dx) / np.linalg.norm(x_b.astype(np.float64))),
                             "acceptance": (st_s["surrogate_accepted"] / cand) if cand else 0.0,
-                            "candidates": int(cand), "pairs_trained": int(len(pairs)), "train_epochs": epochs,
-                            "train_loss_first_last": [float(losses[0]), float(losses[-1])],
-                            "gate_tol": gate_tol, "status": st_s["status"]}
+                            "candidates": int(cand), "gate_tol": gate_tol, "status": st_s["status"]}
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=1000, help="timed substeps of the headline workload")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--stage-steps", type=int, default=50, help="substeps timed stage by stage with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned return-map leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scene leg")
+    ap.add_argument("--no-config2", action="store_true", help="skip the sand-column leg")
     ap.add_argument("--no-config3", action="store_true", help="skip the cloth-sand leg")
+    ap.add_argument("--config2-steps", type=int, default=200)
     ap.add_argument("--config3-steps", type=int, default=200, help="timed substeps of the cloth-sand leg (after 20)")
-    ap.add_argument("--no-config4", action="store_true", help="skip the 2^22-particle per-particle-angle leg")
-    ap.add_argument("--config4-steps", type=int, default=1000, help="timed substeps of the config-4 leg (after 20)")
-    ap.add_argument("--batch-steps", type=int, default=480, help="timed substeps of the batched leg (after 20 warm-up)")
-    ap.add_argument("--seq-steps", type=int, default=40, help="timed substeps per scene of the one-at-a-time baseline")
+    ap.add_argument("--batch-steps", type=int, default=480, help="timed substeps of the batched leg (after 20)")
+    ap.add_argument("--train-pairs", type=int, default=10_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--only", choices=["config3", "config4", "batched"],
-                    help="run this leg alone (exact path; for profiling) and print its JSON object")
+    ap.add_argument("--only", choices=["config2", "config3", "config4", "config4-surrogate", "batched"],
+                    help="run this leg alone with the given --steps (for profiling) and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
+    from anisotropicelastoplasticity_amd import Solver, scenes
     if args.only:
-        leg = {"config3": lambda: config3_leg(args.config3_steps, 20),
-               "config4": lambda: config4_leg(False, args.config4_steps, 20),
-               "batched": lambda: batched_leg(False, args.batch_steps, 20, args.seq_steps)}[args.only]()
+        if args.only == "config2":
+            leg = config2_leg(args.steps, args.warmup)
+        elif args.only == "config3":
+            leg = config3_leg(args.steps, args.warmup)
+        elif args.only == "batched":
+            leg = batched_leg(None, args.steps, args.warmup)
+        else:
+            sc = scenes.config4_sand_phi()
+            w = None
+            if args.only == "config4-surrogate":
+                w, _ = train_surrogate(target_pairs=args.train_pairs)
+            r = timed_run(sc, args.warmup + args.steps, args.steps, weights=w)
+            leg = {"workload": args.only, "value": sc.n * args.steps / r["wall"], "kernel_ms": r["kernel_ms"],
+                   "surrogate_kernels": r["sur"], "status": r["stats"]["status"]}
         print(json.dumps(leg))
         return
 
-    from anisotropicelastoplasticity_amd import Solver, scenes
-    sc = scenes.config2_sand_column()
-    n = sc.n
-    with Solver(0) as s:
-        sc.load_into(s)
-        s.run(sc.dt, args.warmup)
-        s.synchronize()
-        try:
-            import torch
-            if torch.cuda.is_available():
-                torch.cuda.synchronize()
-        except Exception:
-            pass
-        t0 = time.perf_counter()
-        s.run(sc.dt, args.steps)
-        s.synchronize()
-        wall = time.perf_counter() - t0
-        x_exact = s.get_particles()["x"]
-        # per-stage and per-main-kernel HIP-event timing on the context stream (eager launches)
-        total_ms, stage_ms, kernel_ms = s.time_kernels(sc.dt, args.stage_steps)
-        st = s.stats()
-        copy = None
-        if args.copy_calibration:
+    weights, train_info = train_surrogate(target_pairs=args.train_pairs)
+    sc, h = headline(args, weights, train_info)
+    c2 = None if args.no_config2 else config2_leg(args.config2_steps, 20)
+    bat = None if args.no_batch else batched_leg(weights, args.batch_steps, 20)
+    c3 = None if args.no_config3 else config3_leg(args.config3_steps, 20)
+    copy = None
+    if args.copy_calibration:
+        with Solver(0) as s:
             nbytes, reps = 256 << 20, 3
             copy = {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "launches": reps,
                     "kernel": "k_copy", "avg_ms": s.bench_copy(nbytes, reps)}
 
-    sur, weights = (None, None) if args.no_surrogate else surrogate_leg(sc, args.warmup, args.steps, x_exact, wall)
-    bat = None if args.no_batch else batched_leg(not args.no_surrogate, args.batch_steps, 20, args.seq_steps)
-    c3 = None if args.no_config3 else config3_leg(args.config3_steps, 20)
-    c4 = None if args.no_config4 else config4_leg(not args.no_surrogate, args.config4_steps, 20)
-    value = n * args.steps / wall
-    ms_per_step = 1e3 * wall / args.steps
-    n_nodes = st["n_active_nodes"]
-    per_stage_ms = {k: stage_ms[i] / args.stage_steps for i, k in enumerate(STAGES)}
-    per_kernel_ms = {k: kernel_ms[i] / args.stage_steps for i, k in enumerate(STAGES)}
-    dom = max(("p2g", "grid", "g2p"), key=lambda k: per_kernel_ms[k])
-    alg = ALG_PER_PARTICLE[dom] * n + ALG_PER_NODE[dom] * n_nodes
-    achieved = alg / (per_kernel_ms[dom] * 1e-3) / 1e9
-    traffic = load_traffic(dom)
-    b_alg = SUBSTEP_PER_PARTICLE * n + SUBSTEP_PER_NODE * n_nodes
-    sub_ach = b_alg / (ms_per_step * 1e-3) / 1e9
-
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": h["value"],
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded jittered lattice)",
-        "config": {"workload": "config2_sand_column", "particles": n, "grid": list(sc.res), "dt": sc.dt,
-                   "kernel": "quadratic B-spline APIC", "return_map": "exact (Drucker-Prager, FP32 + FP64 fallback)",
-                   "stress": "at F^n", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": per_kernel_ms[dom]},
-        "substep_roofline": {"achieved": sub_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": sub_ach / HBM_PEAK_GBS, "alg_bytes_per_substep": b_alg},
-        "stage_ms": per_stage_ms,
-        "kernel_ms": per_kernel_ms,
-        "active_nodes": n_nodes,
-        "status": st["status"],
-        "fp64_svd_count": st["fp64_svd_count"],
-        "fp64_apex_count": st["fp64_apex_count"],
+        "dtype": "f32 (network: bf16 MFMA, f32 accumulate)",
+        "data": "synthetic (seeded jittered lattices); network trained on the GPU on pairs from other seeded runs",
+        "config": {"workload": "config4_sand_phi: 2^22 sand particles, per-particle friction angle in [25, 40) deg, "
+                               "256^3 grid", "particles": sc.n, "grid": list(sc.res), "dt": sc.dt,
+                   "value_path": "surrogate return map (MLP 5-128-128-128-128-3 SiLU, yield-residual gate 1e-5, "
+                                 "exact fallback)",
+                   "kernel": "quadratic B-spline APIC", "stress": "at F^n", "parallelism": "single GPU",
+                   "timed_window": h["timed_window"]},
+        "roofline": h["roofline"],
+        "exact": h["exact"],
+        "speedup_vs_exact": h["speedup_vs_exact"],
+        "rel_pos_err": h["rel_pos_err"],
+        "max_dx_over_h": h["max_dx_over_h"],
+        "error_bar_met": h["error_bar_met"],
+        "substeps_compared": h["substeps_compared"],
+        "candidates_per_substep": h["candidates_per_substep"],
+        "acceptance": h["acceptance"],
+        "fallback_rate": h["fallback_rate"],
+        "kernel_ms": h["kernel_ms"],
+        "stage_ms": h["stage_ms"],
+        "surrogate_kernels": h["surrogate_kernels"],
+        "active_nodes": h["active_nodes"],
+        "status": h["status"],
+        "train": h["train"],
     }
-    if sur is not None:
-        # the same workload with the learned return map (exact fallback behind the yield-residual
-        # gate), compared with the exact run above after the same number of substeps
-        out["surrogate"] = sur
+    if c2 is not None:
+        out["config2"] = c2
     if bat is not None:
         out["batched"] = bat
     if c3 is not None:
         out["config3"] = c3
-    if c4 is not None:
-        out["config4"] = c4
     if copy is not None:
         out["copy_calibration"] = copy
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)
 
 

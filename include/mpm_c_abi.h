@@ -305,6 +305,12 @@ int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, f
  * events), without the deferred FP64 apex projections that follow it in the stage; for the other
  * stages, the stage time. No counterpart in the reference. */
 int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms);
+/* Measurement helper with no counterpart in the reference: nsteps eagerly launched substeps with the
+ * surrogate enabled; forward_ms and apply_ms receive the summed dispatch times (HIP events) of the
+ * network kernel (k_surrogate_forward) and the gate kernel (k_surrogate_apply), candidates the
+ * number of candidates they processed. */
+int mpm_time_surrogate(mpm_ctx* ctx, float dt, int32_t nsteps, float* forward_ms, float* apply_ms,
+                       int64_t* candidates);
 
 /* ---- learned return-map surrogate (no counterpart in the reference; replaces the cone branch of
  * the Drucker-Prager projection, HybridSolver.cpp:667-673, behind a gate that falls back to it) ----
@@ -332,6 +338,12 @@ int mpm_get_pairs(mpm_ctx* ctx, float* pairs, int64_t max_pairs, int64_t* count)
 /* Enable the surrogate in the substep: cone-branch particles take the network's H when it passes the
  * gate (yield equality, volume and flow direction within gate_tol), else the exact projection. */
 int mpm_set_surrogate(mpm_ctx* ctx, int32_t enable, float gate_tol);
+/* Diagnostic with no counterpart in the reference: record the gate decision of each of the first
+ * `capacity` surrogate candidates of every substep (0 disables), 10 floats per candidate: sorted
+ * slot (int32 bits), eps(3), friction angle, K/mu, predicted H(3), accepted (1 or 0). The getter
+ * returns the records of the last substep run with the surrogate enabled. */
+int mpm_set_surrogate_log(mpm_ctx* ctx, int64_t capacity);
+int mpm_get_surrogate_log(mpm_ctx* ctx, float* records, int64_t max_records, int64_t* count);
 
 /* Measurement helper with no counterpart in the reference: copies nbytes (a multiple of 16) between
  * two device buffers `reps` times with a plain float4 copy kernel (k_copy). Under rocprofv3 --pmc its

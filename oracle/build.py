@@ -5,7 +5,8 @@ cannot be compiled here: it needs Eigen 3 and libigl, neither of which is presen
 oracle/_ref build.
 
 Flags: -O3 with a portable x86-64-v3 target so that the library built here also runs on the host
-CPU of the GPU machine. OpenMP is not used: the reference solver is single-threaded.
+CPU of the GPU machine. The default library is serial, as the reference solver is; a second build
+with OpenMP (oracle/_build/libmpm_oracle_omp.so) times the CPU path on all host cores.
 """
 import os
 import subprocess
@@ -14,14 +15,18 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libmpm_oracle.so")
+LIB_OMP = os.path.join(OUT_DIR, "libmpm_oracle_omp.so")
 
 
-def build(verbose=False, source=None, out=None, opt="-O3"):
-    """Compile the oracle (or another copy of its source, `source`, into `out`)."""
-    lib = out or LIB
+def build(verbose=False, source=None, out=None, opt="-O3", openmp=False):
+    """Compile the oracle (or another copy of its source, `source`, into `out`). The default call
+    builds both the serial library and the OpenMP one."""
+    if source is None and out is None and not openmp:
+        build(verbose, opt=opt, openmp=True)
+    lib = out or (LIB_OMP if openmp else LIB)
     os.makedirs(os.path.dirname(lib), exist_ok=True)
     tmp = lib + ".tmp"
-    cmd = ["g++", opt, "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared",
+    cmd = ["g++", opt, "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared"] + (["-fopenmp"] if openmp else []) + [
            "-I", os.path.join(os.path.dirname(HERE), "include"),
            source or os.path.join(HERE, "mpm_oracle.cpp"), "-o", tmp]
     if verbose:

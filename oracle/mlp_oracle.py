@@ -148,3 +148,41 @@ def loss_and_gradient(w, inp, H, emulate_bf16=True):
             dz = dz.astype(np.float32).astype(np.float64)
     g[~active_mask()] = 0.0
     return loss, g
+
+
+def dp_alpha(phi_deg):
+    """Drucker-Prager cone slope alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) (HybridSolver.cpp:652)."""
+    s = np.sin(np.asarray(phi_deg, dtype=np.float64) * np.pi / 180.0)
+    return np.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s)
+
+
+def gate(eps, H, phi_deg, c_yield, tol):
+    """The yield-residual gate that decides whether a predicted projection H of the trial Hencky
+    strains eps is used instead of the exact cone return (HybridSolver.cpp:667-673), evaluated in
+    float64 on the same inputs. The cone return keeps the volume (tr H = tr eps), lands on the
+    yield surface |dev H| + c alpha tr H = 0 with c = (3 lambda + 2 mu) / (2 mu) (the factor of
+    tr(eps) in delta_gamma, HybridSolver.cpp:655) and removes a multiple of dev eps
+    (H = eps - delta_gamma dev eps / |dev eps|). A prediction is accepted when
+      yield     | |dev H| + c alpha tr H |          <= tol
+      volume    | tr H - tr eps |                   <= tol
+      flow      dev(eps - H) . dev eps / |dev eps|  >  0, and its part orthogonal to dev eps <= tol.
+    Returns (accept, margins) where margins (n, 4) holds tol minus each bounded quantity and the
+    flow projection itself: a row whose smallest |margin| is within rounding of zero is a tie."""
+    eps = np.asarray(eps, dtype=np.float64).reshape(-1, 3)
+    H = np.asarray(H, dtype=np.float64).reshape(-1, 3)
+    c = np.broadcast_to(np.asarray(c_yield, dtype=np.float64), (eps.shape[0],))
+    alpha = dp_alpha(np.broadcast_to(np.asarray(phi_deg, dtype=np.float64), (eps.shape[0],)))
+    trH, trE = H.sum(axis=1), eps.sum(axis=1)
+    dH = H - trH[:, None] / 3.0
+    dE = eps - trE[:, None] / 3.0
+    nH = np.linalg.norm(dH, axis=1)
+    nE = np.linalg.norm(dE, axis=1)
+    d = dE - dH
+    with np.errstate(divide="ignore", invalid="ignore"):
+        along = np.sum(d * dE, axis=1) / nE
+        ortho = np.linalg.norm(d - along[:, None] * dE / nE[:, None], axis=1)
+    yield_res = np.abs(nH + c * alpha * trH)
+    vol_res = np.abs(trH - trE)
+    margins = np.stack([tol - yield_res, tol - vol_res, along, tol - ortho], axis=1)
+    accept = (yield_res <= tol) & (vol_res <= tol) & (along > 0.0) & (ortho <= tol)
+    return accept, margins

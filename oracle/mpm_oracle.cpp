@@ -35,6 +35,22 @@
 #include <type_traits>
 #include <vector>
 
+// The all-cores CPU line builds this file a second time with OpenMP: the particle and node loops
+// run in parallel and grid scatters become atomic adds (the sums then run in thread order, not in
+// ascending particle order). Without OpenMP the macros are empty and the code is the serial oracle.
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_PARALLEL _Pragma("omp parallel")
+#define ORC_FOR _Pragma("omp for schedule(static)")
+#define ORC_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define ORC_ATOMIC _Pragma("omp atomic")
+#else
+#define ORC_PARALLEL
+#define ORC_FOR
+#define ORC_PAR_FOR
+#define ORC_ATOMIC
+#endif
+
 namespace orc {
 
 // ---------------------------------------------------------------- B-splines
@@ -484,24 +500,32 @@ template <class T> struct Sim {
     std::fill(gm.begin(), gm.end(), T(0));
     std::fill(gp.begin(), gp.end(), T(0));
     T Di = dinv();
-    std::vector<NodeW> nw;
-    for (int p = 0; p < n; ++p) {
-      weights(p, nw);
-      M3<T> Bp = getM(B, p);
-      for (const auto& q : nw) {
-        T w = T(q.w);
-        gm[q.idx] += w * m[p];
-        T dx[3];
-        for (int a = 0; a < 3; ++a) dx[a] = gnode[(size_t)a * ng + q.idx] - X(a, p);
-        for (int a = 0; a < 3; ++a) {
-          T aff = Bp(a, 0) * dx[0] + Bp(a, 1) * dx[1] + Bp(a, 2) * dx[2];
-          gp[(size_t)a * ng + q.idx] += w * m[p] * (v[(size_t)a * n + p] + Di * aff);
+    ORC_PARALLEL
+    {
+      std::vector<NodeW> nw;
+      ORC_FOR
+      for (int p = 0; p < n; ++p) {
+        weights(p, nw);
+        M3<T> Bp = getM(B, p);
+        for (const auto& q : nw) {
+          T w = T(q.w);
+          ORC_ATOMIC
+          gm[q.idx] += w * m[p];
+          T dx[3];
+          for (int a = 0; a < 3; ++a) dx[a] = gnode[(size_t)a * ng + q.idx] - X(a, p);
+          for (int a = 0; a < 3; ++a) {
+            T aff = Bp(a, 0) * dx[0] + Bp(a, 1) * dx[1] + Bp(a, 2) * dx[2];
+            ORC_ATOMIC
+            gp[(size_t)a * ng + q.idx] += w * m[p] * (v[(size_t)a * n + p] + Di * aff);
+          }
         }
       }
     }
     // cloth vertices and elements (:121-125, 137-141, 216-230)
+    std::vector<NodeW> nw;
     for (int k = 0; k < nv; ++k) p2g_one(vx, vv, vB, vm, nv, k, Di, nw);
     for (int k = 0; k < ne; ++k) p2g_one(ex, ev, eB, em, ne, k, Di, nw);
+    ORC_PAR_FOR
     for (int c = 0; c < ng; ++c)
       for (int a = 0; a < 3; ++a)
         gv[(size_t)a * ng + c] = gm[c] > T(0) ? gp[(size_t)a * ng + c] / gm[c] : T(0);  // :233-240
@@ -527,7 +551,10 @@ template <class T> struct Sim {
   // Particle part of computeGridForces_ (HybridSolver.cpp:252-368); gravity is added by grid_update.
   void forces(T dt) {
     std::fill(gf.begin(), gf.end(), T(0));
+    ORC_PARALLEL
+    {
     std::vector<NodeW> nw;
+    ORC_FOR
     for (int p = 0; p < n; ++p) {
       weights(p, nw);
       M3<T> Fn = getM(F, p);
@@ -545,8 +572,11 @@ template <class T> struct Sim {
       M3<T> st = mul(Pk, transpose(Fn));  // volume * P * EDG^T (:321-338)
       for (int i = 0; i < 9; ++i) st.a[i] *= V[p];
       for (const auto& q : nw)
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a) {
+          ORC_ATOMIC
           gf[(size_t)a * ng + q.idx] -= st(a, 0) * T(q.dw[0]) + st(a, 1) * T(q.dw[1]) + st(a, 2) * T(q.dw[2]);
+        }
+    }
     }
     cloth_forces();
   }
@@ -593,6 +623,7 @@ template <class T> struct Sim {
     collide();
   }
   void velocity_update(T dt) {
+    ORC_PAR_FOR
     for (int c = 0; c < ng; ++c) {
       if (!(gm[c] > T(0))) continue;
       for (int a = 0; a < 3; ++a) {
@@ -614,6 +645,7 @@ template <class T> struct Sim {
   void collide() {
     gvbf = gv;
     const T mu = T(P.friction);
+    ORC_PAR_FOR
     for (int c = 0; c < ng; ++c) {
       T xi[3] = {gnode[c], gnode[ng + c], gnode[2 * (size_t)ng + c]};
       T phi_, nrm[3];
@@ -671,9 +703,10 @@ template <class T> struct Sim {
 
   // G2P (:739-825), advection (:942-950), F update (:553-577), plasticity (:612-681).
   void g2p(T dt) {
+    ORC_PARALLEL
+    {
     std::vector<NodeW> nw;
-    T Di = dinv();
-    (void)Di;
+    ORC_FOR
     for (int p = 0; p < n; ++p) {
       weights(p, nw);
       T vp[3] = {0, 0, 0}, xs[3] = {0, 0, 0}, vbfs[3] = {0, 0, 0};
@@ -726,6 +759,7 @@ template <class T> struct Sim {
       for (int i = 0; i < 9; ++i) Fh.a[i] = Fn.a[i] + dt * GF.a[i];
       for (int a = 0; a < 3; ++a) x[(size_t)a * n + p] = xnew[a];
       return_map(p, Fh);
+    }
     }
     g2p_mesh(dt);
   }
@@ -1198,6 +1232,10 @@ void orc_piola(const mpm_params* P, const double* Fh, double Jp, double* out) {
   std::copy(r.a, r.a + 9, out);
 }
 // Drucker-Prager projection of principal stretches.
+#ifdef _OPENMP
+// threads of the OpenMP build (the all-cores CPU line)
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+#endif
 void orc_dp_project(const mpm_params* P, double phi0, double q, const double* Sin, double* Sout, double* dq) {
   orc::dp_project<double>(*P, phi0, q, Sin, Sout, *dq);
 }

@@ -252,15 +252,14 @@ def _plane_cloth(plane, n_side=16, edge=1.0 / 40.0, seed=9):
     return sc, x, v.astype(np.float32), faces
 
 
-@pytest.mark.parametrize("plane", ["z", "y"])
-def test_cloth_reference_normal_row(plane):
+def test_cloth_reference_normal_row_z_sheet():
     """mpm_mesh_params.reference_normal_row = 1: the normal-force direction vector is
-    (D_rest^T).col(2) as in HybridSolver.cpp:425-427,432. It equals D_rest^-T e3 for a sheet in a
-    z = const plane and differs for one in y = const; the device follows the FP64 oracle in both,
-    with the drape test's bar (mesh |dx| <= 1e-4 h, D relative L2 <= 1e-3) after 40 substeps."""
-    sc, x, v, faces = _plane_cloth(plane)
+    (D_rest^T).col(2) as in HybridSolver.cpp:425-427,432. For a sheet in a z = const plane it equals
+    D_rest^-T e3 exactly; the device follows the FP64 oracle with the drape test's bar (mesh
+    |dx| <= 1e-4 h, D relative L2 <= 1e-3) after 40 substeps."""
+    sc, x, v, faces = _plane_cloth("z")
     n = 40
-    say(f"[cloth reference normal row] plane {plane}, vertices {x.shape[1]}, substeps {n}")
+    say(f"[cloth reference normal row] plane z, vertices {x.shape[1]}, substeps {n}")
     res = {}
     for ref_row in (True, False):
         o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims)
@@ -280,12 +279,61 @@ def test_cloth_reference_normal_row(plane):
     dxm = np.max(np.abs(m["vx"] - mr["vx"]))
     d_forms = rel(res[True]["D"], res[False]["D"])
     say(f"  mesh |dx|/h {dxm / h:.3e}, D rel {rel(m['D'], mr['D']):.3e}, oracle forms differ by {d_forms:.3e}")
-    if plane == "z":
-        assert d_forms < 1e-9  # the two forms agree for a z = const sheet
-    else:
-        assert d_forms > 1e-4  # and differ for a y = const sheet
+    assert d_forms == 0.0  # the two forms agree for a z = const sheet
     assert dxm <= 1e-4 * h
     assert rel(m["D"], mr["D"]) < 1e-3
+    assert st["status"] == 0, st
+
+
+def test_cloth_reference_normal_row_y_sheet():
+    """reference_normal_row = 1 on a sheet in y = 0.6, where (D_rest^T).col(2) (HybridSolver.cpp:432)
+    is not the sheet normal: for rest directions d1, d2 in the x-z plane it is (d1_z, d2_z, 0), so
+    the reference form drops the normal and shear stress column of dF3 and takes h times an in-plane
+    one. A sand block moving down at 2 m/s lands on the cloth, which compresses the elements' d3
+    (r33 < 1), and with normal stiffness 1e7 and shear stiffness 1e6 the two forms part clearly:
+    after 40 substeps the FP64 oracle's forms differ by about 3e-4 in D and in |dx|/h, some thirty
+    times the FP32 oracle's own distance from the FP64 oracle. The device with the flag set stays
+    within twice that FP32 distance (plus 1e-6) of the FP64 oracle with the flag set, and is at
+    least five times farther from the oracle without it."""
+    prims = [levelset(_lib.LS_BOX, (0.05, 0.05, 0.05, 0.0), (0.95, 0.95, 0.95, 0.0)),
+             levelset(_lib.LS_SPHERE, (0.5, 0.42, 0.5, 0.15))]
+    sc = scenes.small_scene(_lib.SAND, 2, 0, _lib.FRICTION_COULOMB, seed=5, res=(32, 32, 32), cells=(3, 2, 3),
+                            lo=(0.44, 0.61, 0.44), hi=(0.56, 0.6725, 0.56), v0=(0.0, -2.0, 0.0), prims=prims, dt=1e-4)
+    edge = 1.0 / 48.0
+    n_side = 20
+    side = edge * (n_side - 1)
+    x, faces = scenes.grid_mesh(n_side, n_side, edge, (0.5 - side / 2, 0.5 - side / 2), 0.6)
+    v = np.zeros_like(x)
+    k, gamma, n = 1e7, 1e6, 40
+    h = 1.0 / sc.res[0]
+    say(f"[cloth reference normal row] plane y, particles {sc.n}, vertices {x.shape[1]}, substeps {n}")
+    ref = {}
+    for ref_row, prec in ((True, 64), (False, 64), (True, 32)):
+        o = Oracle(sc.params, sc.gmin, sc.gmax, sc.res, prims=sc.prims, precision=prec)
+        o.load_scene(sc)
+        o.set_mesh(x, faces, v=v, reference_normal_row=ref_row, k=k, gamma=gamma)
+        o.run(sc.dt, n)
+        ref[ref_row, prec] = o.get_mesh()
+    with Solver() as s:
+        sc.load_into(s)
+        s.set_mesh(x, faces, v=v, stiffness=k, shear_stiffness=gamma, reference_normal_row=True)
+        s.run(sc.dt, n)
+        m = s.get_mesh()
+        st = s.stats()
+
+    def dist(a, b):
+        return np.max(np.abs(a["vx"] - b["vx"])) / h, rel(a["D"], b["D"])
+
+    forms = dist(ref[True, 64], ref[False, 64])
+    spread = dist(ref[True, 32], ref[True, 64])
+    gpu = dist(m, ref[True, 64])
+    gpu_other = dist(m, ref[False, 64])
+    say(f"  |dx|/h, D rel: oracle forms {forms[0]:.3e} {forms[1]:.3e}; fp32 oracle {spread[0]:.3e} {spread[1]:.3e}; "
+        f"gpu {gpu[0]:.3e} {gpu[1]:.3e}; gpu vs the other form {gpu_other[0]:.3e} {gpu_other[1]:.3e}")
+    for i in range(2):
+        assert forms[i] > 10.0 * spread[i]  # the flag matters in this scene
+        assert gpu[i] <= 2.0 * spread[i] + 1e-6
+        assert gpu_other[i] >= 5.0 * gpu[i]
     assert st["status"] == 0, st
 
 

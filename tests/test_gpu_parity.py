@@ -250,9 +250,11 @@ def test_config2_one_substep():
         q = s.get_particles()
         # the next substep's P2G: the stress impulses of the now deformed particles sum to zero over
         # the grid (sum_i grad w_ip = 0 for particles whose stencil lies inside the grid), so the
-        # grid momentum equals the particle momentum
+        # grid momentum with the stress impulse (the dense grid holds it after the grid stage, before
+        # gravity) equals the particle momentum
         s.stage(_lib.STAGE_SORT, sc.dt)
         s.stage(_lib.STAGE_P2G, sc.dt)
+        s.stage(_lib.STAGE_GRID, sc.dt)
         g2 = s.get_grid()
     for st in (1, 2, 3):
         o.stage(st, sc.dt)
@@ -266,7 +268,13 @@ def test_config2_one_substep():
     assert rel(q["v"], qr["v"]) < 2e-5
     assert rel(q["B"], qr["B"]) < 2e-5
     assert np.max(np.abs(q["F"] - qr["F"])) < 2e-5
-    assert np.array_equal(q["plastic"] > 0, qr["plastic"] > 0)
+    # one substep from rest: the plastic increments are below 1e-7, where the sign of the yield
+    # function is FP32 rounding (an FP32 run of the oracle flips 287 of the 289 FP64 plastic flags),
+    # so flags may differ only where both amounts are negligible
+    flip = (q["plastic"] > 0) != (qr["plastic"] > 0)
+    say(f"[config2] plastic flags: oracle {int((qr['plastic'] > 0).sum())}, device {int((q['plastic'] > 0).sum())}, "
+        f"differing {int(flip.sum())}")
+    assert np.all(np.maximum(q["plastic"][flip], qr["plastic"][flip]) < 1e-6)
     assert np.max(np.abs(q["plastic"] - qr["plastic"])) < 2e-5
     assert np.max(np.abs(q["x"] - qr["x"])) < 1e-5 / 128
     # size-independent properties at full size: mass conservation; momentum with the stress impulse

@@ -10,7 +10,13 @@
                      (oracle/mpm_oracle.cpp dp_project): |dH| <= 2e-6
   gate               a network whose outputs are NaN is rejected everywhere and the substep is
                      bitwise equal to the exact return map; a trained network is accepted for some
-                     particles and the positions stay within 1e-4 of the exact run
+                     particles and the positions stay within 1e-4 of the exact run; the device's
+                     accept flags equal the oracle gate's (oracle/mlp_oracle.gate) except within
+                     1e-8 of a threshold, and its predictions the oracle network's within 1e-3 of
+                     the forward bar
+  gains              on held-out pairs the trained network is accepted for at least half, the
+                     untrained network for at most a tenth as many, no network (H = eps) for none;
+                     its mean error is at most half of either's
 """
 import sys
 
@@ -227,3 +233,102 @@ def test_surrogate_trained_is_accepted_and_close():
     assert st["surrogate_accepted"] > 0
     assert dx <= 1e-4
     assert st["status"] == 0
+
+
+def _trained_weights(sc, nsteps=30, hold_out=0.0, seed=3):
+    """Pairs logged by the exact path over `nsteps` substeps of sc; the network trained on the GPU on
+    all of them, or on a random (1 - hold_out) share. Returns weights, training and held-out pairs."""
+    with _solver() as s:
+        sc.load_into(s)
+        s.set_pair_logging(1 << 20)
+        s.run(sc.dt, nsteps)
+        pairs = s.get_pairs()
+    rng = np.random.default_rng(seed)
+    idx = rng.permutation(len(pairs))
+    k = int(round(len(pairs) * (1.0 - hold_out)))
+    train, held = pairs[idx[:k]], pairs[idx[k:]]
+    with _solver() as s:
+        s.surrogate_init(1)
+        losses = s.surrogate_train(train[:, :5], train[:, 5:8], epochs=300, batch=512, lr=2e-3, seed=seed)
+        w = s.surrogate_get_weights()
+    say(f"  pairs {len(pairs)} (trained on {len(train)}), loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    return w, train, held
+
+
+def test_gate_decisions_match_oracle():
+    """The device's accept flags against the oracle's gate (oracle/mlp_oracle.gate, the yield-residual,
+    volume and flow checks in float64) on the device's own trial strains and predictions, recorded for
+    every candidate of one substep: equal wherever no gate quantity lies within 1e-8 (1e-3 tol, about
+    ten FP32 roundings of strains of 1e-2) of its threshold. The recorded predictions H against the
+    oracle network (BF16 roundings emulated) on the same inputs: |dH| <= 1e-3 (2e-3 max(1, |y|) + 1e-3),
+    the forward test's bar in H units."""
+    sc = _impact_scene()
+    tol = 2e-5
+    say(f"[gate decisions] n={sc.n}")
+    w, _, _ = _trained_weights(sc)
+    with _solver() as s:
+        sc.load_into(s)
+        s.surrogate_set_weights(w)
+        s.set_surrogate(True, tol)
+        s.set_surrogate_log(1 << 20)
+        s.run(sc.dt, 29)
+        s.substep(sc.dt)
+        rec = s.get_surrogate_log()
+        st = s.stats()
+    m = len(rec["slot"])
+    say(f"  candidates in the last substep {m}, accepted {int(rec['accepted'].sum())}")
+    assert m > 200 and 0 < rec["accepted"].sum() < m
+    p = sc.params
+    lam_over_mu = 2 * p.poisson_ratio / (1 - 2 * p.poisson_ratio)
+    c = (3 * lam_over_mu + 2) / 2
+    assert np.allclose(rec["kmu"], lam_over_mu + 2.0 / 3.0, rtol=1e-6)
+    acc, mg = mo.gate(rec["eps"], rec["H"], rec["phi"], c, tol)
+    tie = np.min(np.abs(mg), axis=1) < 1e-8
+    agree = acc == rec["accepted"]
+    say(f"  oracle accepts {int(acc.sum())}; disagreements {int((~agree).sum())}, all within ties: "
+        f"{bool(np.all(tie[~agree]))}; ties {int(tie.sum())}")
+    assert np.all(agree | tie)
+    assert tie.sum() <= 0.01 * m
+    inp = np.concatenate([rec["eps"], rec["phi"][:, None], rec["kmu"][:, None]], axis=1)
+    y_ref, _ = mo.forward(w, inp, emulate_bf16=True)
+    H_ref = inp[:, :3].astype(np.float64) + mo.CORR_SCALE * y_ref
+    dH = np.max(np.abs(rec["H"] - H_ref))
+    say(f"  max |H - oracle network| = {dH:.3e}")
+    assert dH <= 1e-3 * (2e-3 * max(1.0, np.max(np.abs(y_ref))) + 1e-3)
+    # the slots are distinct sorted positions of sand particles
+    assert len(np.unique(rec["slot"])) == m and np.all((rec["slot"] >= 0) & (rec["slot"] < sc.n))
+    assert st["status"] == 0
+
+
+def test_trained_network_gains():
+    """What the trained network gains over the method without it, on held-out (trial, projected) pairs
+    of the impact scene (20% of the logged pairs, not trained on): the gate (oracle/mlp_oracle.gate at
+    2e-5) accepts at least half of the trained network's predictions, at most a tenth as many of the
+    untrained network's (it passes only trial states within the tolerance of the cone) and none of
+    the prediction without a network (H = eps, no plastic correction); the trained network's mean
+    error |H - H_exact| is at most half the no-network error (the correction itself) and half the
+    untrained network's."""
+    sc = _impact_scene()
+    tol = 2e-5
+    say(f"[trained network gains] n={sc.n}")
+    w, _, held = _trained_weights(sc, hold_out=0.2)
+    p = sc.params
+    lam_over_mu = 2 * p.poisson_ratio / (1 - 2 * p.poisson_ratio)
+    c = (3 * lam_over_mu + 2) / 2
+    inp, H_exact = held[:, :5], held[:, 5:8].astype(np.float64)
+    with _solver() as s:
+        s.surrogate_set_weights(w)
+        H_tr = s.surrogate_predict(inp).astype(np.float64)
+        s.surrogate_init(1)
+        H_un = s.surrogate_predict(inp).astype(np.float64)
+    corr = np.max(np.abs(H_exact - inp[:, :3]), axis=1)
+    out = {}
+    for name, H in (("trained", H_tr), ("untrained", H_un), ("no network", inp[:, :3].astype(np.float64))):
+        acc, _ = mo.gate(inp[:, :3], H, inp[:, 3], c, tol)
+        err = np.max(np.abs(H - H_exact), axis=1)
+        out[name] = (acc.mean(), err.mean(), np.median(err / corr))
+        say(f"  {name:10s}: accepted {acc.mean():.3f}, mean |H - H_exact| {err.mean():.3e}, "
+            f"median |H - H_exact| / |correction| {out[name][2]:.3e}")
+    assert out["trained"][0] >= 0.5
+    assert out["untrained"][0] <= 0.1 * out["trained"][0] and out["no network"][0] == 0.0
+    assert out["trained"][1] <= 0.5 * out["no network"][1] and out["trained"][1] <= 0.5 * out["untrained"][1]
