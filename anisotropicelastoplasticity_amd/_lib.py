@@ -145,6 +145,8 @@ SIGNATURES = [
     ("mpm_surrogate_get_weights", ctypes.c_int, [_ctx, _f, ctypes.c_int32]),
     ("mpm_surrogate_predict", ctypes.c_int, [_ctx, _f, _f, ctypes.c_int32]),
     ("mpm_surrogate_gradient", ctypes.c_int, [_ctx, _f, _f, ctypes.c_int32, _f, _f]),
+    ("mpm_surrogate_gradient_noalloc", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                      ctypes.c_void_p, ctypes.c_void_p]),
     ("mpm_surrogate_train", ctypes.c_int,
      [_ctx, _f, _f, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _f]),
     ("mpm_set_pair_logging", ctypes.c_int, [_ctx, ctypes.c_int64]),

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpm_hip.so")
-SOURCES = ["mpm_kernels.hip", "mpm_mlp.hip", "mpm_runtime.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_mlp.hip", "mpm_noalloc.hip", "mpm_runtime.hip"]
 # programs over the C++ host API (include/mpm/*.h), linked against the library
 EXAMPLES = ["sand_column_example", "cloth_example"]
 

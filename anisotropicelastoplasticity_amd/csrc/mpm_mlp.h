@@ -63,5 +63,7 @@ void launch_mlp_reduce(hipStream_t s, const float* part, int nparts, float* grad
 void launch_mlp_adam(hipStream_t s, float* master, float* m1, float* m2, const float* grad, uint16_t* wbf,
                      uint16_t* wbft, float lr, int step);
 void launch_mlp_to_bf16(hipStream_t s, const float* master, uint16_t* wbf, uint16_t* wbft);
+void launch_mlp_targets(hipStream_t s, const float* in, const float* H, int n, float* tgt);
+void launch_mlp_loss_mean(hipStream_t s, const float* loss_part, int nparts, int n, float* loss);
 
 }  // namespace mpm

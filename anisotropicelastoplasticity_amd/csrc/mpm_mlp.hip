@@ -242,6 +242,23 @@ __global__ __launch_bounds__(256) void k_mlp_to_bf16(const float* __restrict__ m
   mlp_store_bf16(i, mlp_param_active(i) ? master[i] : 0.0f, wbf, wbft);
 }
 
+// Normalised targets t = (H - eps) / CORR_SCALE of n rows (in: n x 5, H: n x 3).
+__global__ __launch_bounds__(256) void k_mlp_targets(const float* __restrict__ in, const float* __restrict__ H, int n,
+                                                     float* __restrict__ tgt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * n) return;
+  const int r = i / 3, c = i % 3;
+  tgt[i] = (H[i] - in[(size_t)r * 5 + c]) / MLP_CORR_SCALE;
+}
+
+// Mean loss: the per-workgroup losses summed in workgroup order, over n.
+__global__ void k_mlp_loss_mean(const float* __restrict__ loss_part, int nparts, int n, float* __restrict__ loss) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  float l = 0.0f;
+  for (int p = 0; p < nparts; ++p) l += loss_part[p];
+  *loss = l / (float)n;
+}
+
 size_t mlp_forward_smem() { return sizeof(MLPShared); }
 size_t mlp_backward_smem() { return sizeof(BackShared); }
 
@@ -269,6 +286,14 @@ void launch_mlp_reduce(hipStream_t s, const float* part, int nparts, float* grad
                        float* loss_acc) {
   hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s, part, nparts, grad, loss_part,
                      loss_acc);
+}
+
+void launch_mlp_targets(hipStream_t s, const float* in, const float* H, int n, float* tgt) {
+  hipLaunchKernelGGL(k_mlp_targets, dim3((3 * n + 255) / 256), dim3(256), 0, s, in, H, n, tgt);
+}
+
+void launch_mlp_loss_mean(hipStream_t s, const float* loss_part, int nparts, int n, float* loss) {
+  hipLaunchKernelGGL(k_mlp_loss_mean, dim3(1), dim3(64), 0, s, loss_part, nparts, n, loss);
 }
 
 void launch_mlp_adam(hipStream_t s, float* master, float* m1, float* m2, const float* grad, uint16_t* wbf,

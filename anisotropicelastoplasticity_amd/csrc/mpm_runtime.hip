@@ -14,6 +14,7 @@
 #include "../../include/mpm_c_abi.h"
 #include "mpm_kernels.h"
 #include "mpm_mlp.h"
+#include "mpm_noalloc.h"
 
 using namespace mpm;
 
@@ -137,6 +138,7 @@ struct mpm_ctx {
   float* mlp_m1 = nullptr;
   float* mlp_m2 = nullptr;
   float* mlp_grad = nullptr;
+  float* scratch = nullptr;  // the fixed scratch of the allocation-free entry point (this device's address)
   uint16_t* mlp_wbf = nullptr;
   uint16_t* mlp_wbft = nullptr;
   int mlp_step = 0;
@@ -647,6 +649,13 @@ int mpm_create(mpm_ctx** out, int32_t device) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_create_error = std::string("stream creation failed: ") + hipGetErrorString(e);
+    delete ctx;
+    return MPM_ERR_HIP;
+  }
+  e = noalloc_scratch(device, &ctx->scratch);
+  if (e != hipSuccess) {
+    g_create_error = std::string("scratch address: ") + hipGetErrorString(e);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MPM_ERR_HIP;
   }
@@ -1916,6 +1925,29 @@ int mpm_surrogate_gradient(mpm_ctx* ctx, const float* in, const float* H, int32_
   free_train(b);
   if (e != hipSuccess) return fail(ctx, MPM_ERR_HIP, std::string("mpm_surrogate_gradient: ") + hipGetErrorString(e));
   if (loss) *loss = lsum / (float)n;
+  return MPM_OK;
+}
+
+int mpm_surrogate_gradient_noalloc(mpm_ctx* ctx, const float* in, const float* H, int32_t n, float* grad, float* loss) {
+  if (!ctx) return MPM_ERR_INVALID;
+  if (!in || !H || !grad || !loss) return fail(ctx, MPM_ERR_INVALID, "mpm_surrogate_gradient_noalloc: null pointer");
+  if (n < 1 || n > NOALLOC_MAX_ROWS)
+    return fail(ctx, MPM_ERR_INVALID, "mpm_surrogate_gradient_noalloc: n must be in [1, " +
+                                          std::to_string(NOALLOC_MAX_ROWS) + "], got " + std::to_string(n));
+  if (!ctx->mlp_ready) return fail(ctx, MPM_ERR_STATE, "surrogate weights not set");
+  if (!ctx->scratch) return fail(ctx, MPM_ERR_STATE, "no scratch address for this device");
+  CK(hipSetDevice(ctx->device));
+  // the same kernels and launch shapes as mpm_surrogate_gradient; every scratch region a kernel
+  // reads (rows < n, the first `groups` partials) is written earlier in the same call
+  float* s = ctx->scratch;
+  const int groups = backward_grid(n);
+  launch_mlp_targets(ctx->stream, in, H, n, s + NOALLOC_TGT);
+  launch_mlp_forward(ctx->stream, ctx->mlp_wbf, ctx->mlp_master, in, s + NOALLOC_Y, n, s + NOALLOC_Z, forward_grid(ctx, n));
+  launch_mlp_backward(ctx->stream, ctx->mlp_wbft, in, s + NOALLOC_Y, s + NOALLOC_TGT, s + NOALLOC_Z, n, s + NOALLOC_GPART,
+                      s + NOALLOC_LPART, groups);
+  launch_mlp_reduce(ctx->stream, s + NOALLOC_GPART, groups, grad, s + NOALLOC_LPART, nullptr);
+  launch_mlp_loss_mean(ctx->stream, s + NOALLOC_LPART, groups, n, loss);
+  CK(hipGetLastError());
   return MPM_OK;
 }
 

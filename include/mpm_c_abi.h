@@ -327,6 +327,16 @@ int mpm_surrogate_predict(mpm_ctx* ctx, const float* in, float* H, int32_t n);
 /* Gradient of L = sum (y - t)^2 / n over normalised outputs y = (H - eps)/1e-3, with the same kernels
  * the trainer uses; *loss receives L. */
 int mpm_surrogate_gradient(mpm_ctx* ctx, const float* in, const float* H, int32_t n, float* grad, float* loss);
+/* Allocation-free variant of mpm_surrogate_gradient. It has no counterpart in the reference; it is
+ * for callers that cannot allocate device memory: code under HIP graph capture, tight serving or
+ * online-training loops, harnesses that forbid hipMalloc. in (n x 5), H (n x 3), grad
+ * (mpm_surrogate_nparam() floats) and loss (1 float, the mean loss) are DEVICE pointers. The call
+ * only enqueues kernels on the context's stream (results are ready after mpm_synchronize) and
+ * allocates nothing: its intermediates live in one fixed library-owned device array of about
+ * 9.3 MB per GPU, which keeps the last call's contents. n must be in [1, 2048] (MPM_ERR_INVALID
+ * otherwise). Calls that overlap on one GPU (from different contexts or threads) race on that
+ * array. Results equal mpm_surrogate_gradient's bit for bit. */
+int mpm_surrogate_gradient_noalloc(mpm_ctx* ctx, const float* in, const float* H, int32_t n, float* grad, float* loss);
 /* Adam on the GPU over `epochs` passes of minibatches of `batch` rows in a seeded shuffled order;
  * epoch_loss[epochs] receives the mean loss of each pass. */
 int mpm_surrogate_train(mpm_ctx* ctx, const float* in, const float* H, int32_t n, int32_t epochs, int32_t batch,
