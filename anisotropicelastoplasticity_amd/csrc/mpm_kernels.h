@@ -194,12 +194,11 @@ struct G2PArgs {
   float* pair_log;
   int64_t pair_cap;
   unsigned long long* pair_count;
-  float* cand;      // candidate records, 21 planes (sigma 3, U 9, V 9) of n floats
-  int* cand_slot;   // sorted slot of each candidate
-  int* cand_count;
-  int* apex_slot;   // sorted slots of the particles whose FP64 apex projection is deferred
-  float* apex_usv;  // their trial SVD, 21 planes (sigma 3, U 9, V 9) of n floats
-  int* apex_count;
+  float* cand;      // candidate records at their sorted slot: planes sigma 3, U 9, V 9 (of n floats)
+  uint64_t* cand_mask;  // per wave (4 blockIdx.x + wave): which lanes are candidates
+  int* wave_pos;        // per wave: the sorted position of its lane 0
+  uint64_t* apex_mask;  // per wave: which lanes deferred their FP64 apex projection (null: none defer)
+  float* apex_usv;      // their trial SVD at their sorted slot: planes sigma 3, U 9, V 9 (of n floats)
   const Clock* clk;
 };
 
@@ -209,8 +208,8 @@ struct ApexArgs {
   float* st;   // the G2P output buffer
   int64_t cap;
   int n;
-  const int* apex_slot;
-  const float* apex_usv;
+  const int* apex_slot;   // the deferred particles' sorted slots, ascending
+  const float* apex_usv;  // their trial SVD at the sorted slot
   const int* apex_count;
   int write_stress;
   int log_pairs;
@@ -235,7 +234,7 @@ struct SurrogateArgs {
   float* st;             // particle state written by k_g2p (sorted slots)
   int64_t cap;
   int n;
-  float* cand;  // candidate SVDs (written by k_g2p) and network outputs
+  float* cand;  // candidate SVDs (written by k_g2p) and network outputs, at the sorted slot
   const int* cand_slot;
   const int* cand_count;
   float gate_tol;
@@ -317,6 +316,10 @@ void launch_cloth_forces(hipStream_t s, const ClothArgs& a);   // element forces
 void launch_cloth_fixed(hipStream_t s, const ClothArgs& a);    // after the grid update
 void launch_cloth_elements(hipStream_t s, const ClothArgs& a); // after G2P: positions, velocities, D, return map
 void surrogate_set_smem_limit();
+// The sorted positions G2P flagged (mask[w] of wave w, whose lane 0 sits at wpos[w]), as a list in
+// ascending order built without atomics; the length goes to *count. gsum: (nwaves + 255) / 256 ints.
+void launch_position_list(hipStream_t s, const uint64_t* mask, const int* wpos, int nwaves, int* gsum, int* list,
+                          int* count);
 // k_surrogate_forward (MFMA network over the candidates) then k_surrogate_apply (gate and finish)
 // (ev: optional start/stop event pairs of the two dispatches)
 void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev);

@@ -1510,6 +1510,7 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   const float4* gvel = a.gvel + 2 * noff;
   const int j = pb.j;
   int oob = 0, bad = 0, cfl = 0, fb_svd = 0, fb_apex = 0;
+  bool cand = false, apex = false;
   const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
   const int64_t res0 = P.res[0], res01 = (int64_t)P.res[0] * P.res[1];
   // The wave's node boxes. When the wave's positions lie in one row of cells, or in two consecutive
@@ -1761,55 +1762,34 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     io.pair_cap = a.pair_cap;
     io.pair_count = a.pair_count;
     io.deferred = false;
-    io.defer_apex = a.apex_slot != nullptr;
+    io.defer_apex = a.apex_mask != nullptr;
     io.apex_deferred = false;
     // particles: return mapping. A cloth element's F planes hold D, of which F^ updates d3
     // (HybridSolver.cpp:599-603); its d1, d2 and return mapping follow in k_cloth_elements.
     if (kind == KIND_PARTICLE) return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
-    if (a.sur_mode == 2) {
-      // compact the deferred particles of this wave into the candidate list (one atomic per wave)
-      const uint64_t dm = __ballot(io.deferred);
-      if (dm) {
-        const int lane = (int)lane_id();
-        const int first = __ffsll((long long)dm) - 1;
-        int base = 0;
-        if (lane == first) base = atomicAdd(a.cand_count, __popcll(dm));
-        base = __shfl(base, first);
-        const int c = base + __popcll(dm & ((1ull << lane) - 1ull));
-        const int64_t n = a.n;
-        if (io.deferred && c < n) {
-          a.cand_slot[c] = j;
+    if (io.deferred) {
+      // a candidate of the surrogate: its trial SVD at its sorted slot; the wave's mask below lists it
+      cand = true;
+      const int64_t n = a.n;
 #pragma unroll
-          for (int k = 0; k < 3; ++k) a.cand[(CAND_SIGMA + k) * n + c] = io.sigma[k];
+      for (int k = 0; k < 3; ++k) a.cand[(CAND_SIGMA + k) * n + j] = io.sigma[k];
 #pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            a.cand[(CAND_U + k) * n + c] = io.U[k];
-            a.cand[(CAND_V + k) * n + c] = io.V[k];
-          }
-        }
+      for (int k = 0; k < 9; ++k) {
+        a.cand[(CAND_U + k) * n + j] = io.U[k];
+        a.cand[(CAND_V + k) * n + j] = io.V[k];
       }
     }
-    {
-      // compact the deferred FP64 apex projections of this wave into their list (one atomic per wave)
-      const uint64_t am = __ballot(io.apex_deferred);
-      if (am) {
-        const int lane = (int)lane_id();
-        const int first = __ffsll((long long)am) - 1;
-        int base = 0;
-        if (lane == first) base = atomicAdd(a.apex_count, __popcll(am));
-        base = __shfl(base, first);
-        const int c = base + __popcll(am & ((1ull << lane) - 1ull));
-        if (io.apex_deferred && c < a.n) {
-          a.apex_slot[c] = j;
-          const int64_t n = a.n;
+    if (io.apex_deferred) {
+      // an FP64 apex projection for k_apex_finish: the trial SVD at the sorted slot; the wave's mask
+      // below lists it
+      apex = true;
+      const int64_t n = a.n;
 #pragma unroll
-          for (int k = 0; k < 3; ++k) a.apex_usv[(CAND_SIGMA + k) * n + c] = io.sigma[k];
+      for (int k = 0; k < 3; ++k) a.apex_usv[(CAND_SIGMA + k) * n + j] = io.sigma[k];
 #pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            a.apex_usv[(CAND_U + k) * n + c] = io.U[k];
-            a.apex_usv[(CAND_V + k) * n + c] = io.V[k];
-          }
-        }
+      for (int k = 0; k < 9; ++k) {
+        a.apex_usv[(CAND_U + k) * n + j] = io.U[k];
+        a.apex_usv[(CAND_V + k) * n + j] = io.V[k];
       }
     }
     float xn[3];
@@ -1842,6 +1822,15 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     a.keys[pid] = (uint32_t)noff + cell_key(P, a.thr + a.T.thr_off[pb.s], xn[0], xn[1], xn[2], oob);
     bad = !(isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]) && isfinite(vp[0]) && isfinite(vp[1]) &&
             isfinite(vp[2]) && isfinite(Fh[0]) && isfinite(Fh[4]) && isfinite(Fh[8]) && isfinite(plastic));
+  }
+  // every wave records its deferred lanes (none past the end of the scene) for the position lists
+  if (a.apex_mask || a.sur_mode == 2) {
+    const uint64_t am = __ballot(apex), dm = __ballot(cand);
+    if (lane == 0) {
+      a.wave_pos[4 * blockIdx.x + wv] = j;
+      if (a.apex_mask) a.apex_mask[4 * blockIdx.x + wv] = am;
+      if (a.sur_mode == 2) a.cand_mask[4 * blockIdx.x + wv] = dm;
+    }
   }
   const int flags = (oob ? MPM_STATUS_OUT_OF_DOMAIN : 0) | (bad ? MPM_STATUS_NONFINITE : 0) | (cfl ? MPM_STATUS_CFL : 0);
   if (__ballot(flags != 0)) {
@@ -1880,11 +1869,11 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
       const int64_t cap = a.cap, n = a.n;
       float S[3], U[9], V[9];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) S[k] = a.apex_usv[(CAND_SIGMA + k) * n + c];
+      for (int k = 0; k < 3; ++k) S[k] = a.apex_usv[(CAND_SIGMA + k) * n + j];
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        U[k] = a.apex_usv[(CAND_U + k) * n + c];
-        V[k] = a.apex_usv[(CAND_V + k) * n + c];
+        U[k] = a.apex_usv[(CAND_U + k) * n + j];
+        V[k] = a.apex_usv[(CAND_V + k) * n + j];
       }
       float plastic = st[PPL * cap + j];
       const float phi_p = st[PPHI * cap + j];
@@ -1912,7 +1901,81 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
       }
     }
   }
-  if (fb_apex) atomicAdd(a.fallback_apex, (unsigned long long)fb_apex);
+  // one atomic per wave
+  unsigned long long c = (unsigned long long)fb_apex;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(a.fallback_apex, c);
+}
+
+// ================================================================== position lists
+// The sorted positions that G2P flagged (candidates of the surrogate, deferred FP64 apex
+// projections), as a list in ascending order and without atomics. Every G2P wave w stores the
+// ballot of its flagged lanes (mask[w]) and the position of its lane 0 (wpos[w]). k_list_sums
+// totals each group of 256 waves, k_list_scan turns the group totals into offsets (one workgroup)
+// and writes the list length, and k_list_write places each wave's positions after its group's
+// offset and the waves before it in the group.
+__global__ __launch_bounds__(256) void k_list_sums(const uint64_t* __restrict__ mask, int nwaves, int* __restrict__ gsum) {
+  const int w = blockIdx.x * 256 + threadIdx.x;
+  int c = w < nwaves ? __popcll(mask[w]) : 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  __shared__ int s_c[4];
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) gsum[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+__global__ __launch_bounds__(1024) void k_list_scan(int* __restrict__ gsum, int ngroups, int* __restrict__ count) {
+  __shared__ int s_sum[1024];
+  const int t = threadIdx.x;
+  const int per = (ngroups + 1023) / 1024;
+  const int b0 = min(t * per, ngroups), b1 = min(b0 + per, ngroups);
+  int sum = 0;
+  for (int i = b0; i < b1; ++i) sum += gsum[i];
+  s_sum[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? s_sum[t - off] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int run = s_sum[t] - sum;  // exclusive offset of this thread's groups
+  for (int i = b0; i < b1; ++i) {
+    const int v = gsum[i];
+    gsum[i] = run;
+    run += v;
+  }
+  if (t == 1023) *count = s_sum[1023];
+}
+
+__global__ __launch_bounds__(256) void k_list_write(const uint64_t* __restrict__ mask, const int* __restrict__ wpos,
+                                                    int nwaves, const int* __restrict__ goff, int* __restrict__ list) {
+  __shared__ uint64_t s_m[256];
+  __shared__ int s_off[256], s_p[256], s_wsum[4];
+  const int t = threadIdx.x, w = blockIdx.x * 256 + t, lane = t & 63, wv = t >> 6;
+  const uint64_t m = w < nwaves ? mask[w] : 0ull;
+  const int c = __popcll(m);
+  int x = c;  // inclusive scan over the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_wsum[wv] = x;
+  __syncthreads();
+  int base = goff[blockIdx.x];
+  for (int k = 0; k < wv; ++k) base += s_wsum[k];
+  s_m[t] = m;
+  s_off[t] = base + x - c;
+  s_p[t] = (w < nwaves && m) ? wpos[w] : 0;
+  __syncthreads();
+  // wave wv writes the positions of the block's masks 64 wv .. 64 wv + 63, one mask at a time
+  for (int i = 64 * wv; i < 64 * wv + 64; ++i) {
+    const uint64_t mi = s_m[i];
+    if ((mi >> lane) & 1ull) list[s_off[i] + __popcll(mi & ((1ull << lane) - 1ull))] = s_p[i] + lane;
+  }
 }
 
 // ================================================================== surrogate return map
@@ -1945,11 +2008,12 @@ __global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate_forward(Surrogate
   for (int g = blockIdx.x * nw + w; (int64_t)g * 64 < count; g += gridDim.x * nw) {
     const int row = g * 64 + lane;
     float eps[3] = {0.0f, 0.0f, 0.0f}, phi = 0.0f, kmu = 0.0f;
+    int j = 0;
     if (row < count) {
-      const int j = clamp_index(a.cand_slot[row], (int)n);
+      j = clamp_index(a.cand_slot[row], (int)n);
       const DevParams& P = a.T.params[scene_of(a.T.part_off, a.T.nscenes, j)];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) eps[k] = logf(a.cand[(CAND_SIGMA + k) * n + row]);
+      for (int k = 0; k < 3; ++k) eps[k] = logf(a.cand[(CAND_SIGMA + k) * n + j]);
       phi = dp_phi(P, a.st[PPHI * cap + j], a.st[PPL * cap + j]);
       kmu = bulk_over_shear(P);
     }
@@ -1958,7 +2022,7 @@ __global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate_forward(Surrogate
     wave_forward64(sh, x, y, nullptr, 0, 0);
     if (row < count) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) a.cand[(CAND_Y + k) * n + row] = y[k];
+      for (int k = 0; k < 3; ++k) a.cand[(CAND_Y + k) * n + j] = y[k];
     }
   }
 }
@@ -1970,6 +2034,7 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
   const int64_t cap = a.cap, n = a.n;
   const int lane = threadIdx.x & 63;
+  unsigned n_acc = 0;
   // whole waves stride over the rows, so the ballot below sees every lane
   for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) - lane; base < count;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -1984,9 +2049,9 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
       Pp = &a.T.params[scene_of(a.T.part_off, a.T.nscenes, j)];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        S[k] = a.cand[(CAND_SIGMA + k) * n + row];
+        S[k] = a.cand[(CAND_SIGMA + k) * n + j];
         eps[k] = logf(S[k]);
-        y[k] = a.cand[(CAND_Y + k) * n + row];
+        y[k] = a.cand[(CAND_Y + k) * n + j];
       }
       q = a.st[PPL * cap + j];
       phi_p = a.st[PPHI * cap + j];
@@ -2040,8 +2105,8 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
       float U[9], V[9], Sn[3], lnSn[3], dq;
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        U[k] = a.cand[(CAND_U + k) * n + row];
-        V[k] = a.cand[(CAND_V + k) * n + row];
+        U[k] = a.cand[(CAND_U + k) * n + j];
+        V[k] = a.cand[(CAND_V + k) * n + j];
       }
       if (accepted) {
 #pragma unroll
@@ -2066,9 +2131,9 @@ __global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
         for (int e = 0; e < 6; ++e) a.st[(PK + e) * cap + j] = K[e];
       }
     }
-    const uint64_t am = __ballot(accepted);
-    if (lane == 0 && am) atomicAdd(a.accepted, (unsigned long long)__popcll(am));
+    n_acc += (unsigned)__popcll(__ballot(accepted));
   }
+  if (lane == 0 && n_acc) atomicAdd(a.accepted, (unsigned long long)n_acc);  // one atomic per wave
 }
 
 // ================================================================== cloth
@@ -2497,6 +2562,15 @@ static size_t surrogate_smem() { return sizeof(MLPShared); }
 void surrogate_set_smem_limit() {
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_surrogate_forward),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)surrogate_smem());
+}
+
+void launch_position_list(hipStream_t s, const uint64_t* mask, const int* wpos, int nwaves, int* gsum, int* list,
+                          int* count) {
+  if (nwaves <= 0) return;
+  const int ng = (nwaves + 255) / 256;
+  hipLaunchKernelGGL(k_list_sums, dim3(ng), dim3(256), 0, s, mask, nwaves, gsum);
+  hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, gsum, ng, count);
+  hipLaunchKernelGGL(k_list_write, dim3(ng), dim3(256), 0, s, mask, wpos, nwaves, gsum, list);
 }
 
 void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev) {

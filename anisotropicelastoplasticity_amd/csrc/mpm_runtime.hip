@@ -145,6 +145,13 @@ struct mpm_ctx {
   bool mlp_ready = false;
   float* pair_log = nullptr;
   int64_t pair_cap = 0;
+  // per G2P wave (4 per particle block): sorted position of lane 0, and the lanes that deferred to
+  // the surrogate or to the FP64 apex pass; the lists are built from them without atomics
+  int* wave_pos = nullptr;
+  uint64_t* cand_mask = nullptr;
+  uint64_t* apex_mask = nullptr;
+  int* list_gsum = nullptr;  // per group of 256 waves: count, then offset
+  int64_t wave_cap = 0;
   float* dec_log = nullptr;  // per-candidate gate records of the last surrogate substep
   int64_t dec_cap = 0;
   unsigned long long* sur_counts = nullptr;  // [0] candidates, [1] accepted, [2] pairs logged
@@ -273,6 +280,8 @@ static DevParams devparams_of(const SceneHost& sc) {
 }
 
 // Scene table on the device: per-scene DevParams and the offset tables of SceneTable.
+static int alloc_wave_tables(mpm_ctx* ctx);
+
 static int upload_table(mpm_ctx* ctx) {
   const int S = (int)ctx->scenes.size();
   if (S == 0) return MPM_OK;
@@ -325,6 +334,7 @@ static int upload_table(mpm_ctx* ctx) {
   T.thr_off = reinterpret_cast<const int*>(ctx->table_buf + o_thr);
   ctx->stress_valid = false;
   drop_graphs(ctx);
+  if (ctx->wave_cap < 4 * (int64_t)ctx->nblocks_p) return alloc_wave_tables(ctx);
   return MPM_OK;
 }
 
@@ -511,19 +521,25 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.fallback_svd = ctx->fallbacks;
   a.fallback_apex = ctx->fallbacks + 1;
   const bool surrogate = ctx->sur_enabled && ctx->mlp_ready && ctx->cand && ctx->any_sand;
+  if (ctx->wave_cap < 4 * (int64_t)ctx->nblocks_p)
+    return fail(ctx, MPM_ERR_STATE, "per-wave tables smaller than the particle blocks");
   a.sur_mode = surrogate ? 2 : (ctx->pair_cap > 0 ? 1 : 0);
   a.pair_log = ctx->pair_log;
   a.pair_cap = ctx->pair_cap;
   a.pair_count = ctx->sur_counts ? ctx->sur_counts + 2 : nullptr;
   a.cand = ctx->cand;
-  a.cand_slot = ctx->cand_slot;
-  a.cand_count = ctx->counters + CNT_CAND;
-  a.apex_slot = ctx->any_sand ? ctx->apex_slot : nullptr;
+  a.cand_mask = ctx->cand_mask;
+  a.wave_pos = ctx->wave_pos;
+  a.apex_mask = ctx->any_sand ? ctx->apex_mask : nullptr;
   a.apex_usv = ctx->apex_usv;
-  a.apex_count = ctx->counters + CNT_APEX;
   a.clk = ctx->clk_active;
   launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p, ctx->ev_main[0], ctx->ev_main[1]);
-  if (a.apex_slot) {
+  if (surrogate)
+    launch_position_list(ctx->stream, ctx->cand_mask, ctx->wave_pos, 4 * ctx->nblocks_p, ctx->list_gsum, ctx->cand_slot,
+                         ctx->counters + CNT_CAND);
+  if (a.apex_mask) {
+    launch_position_list(ctx->stream, ctx->apex_mask, ctx->wave_pos, 4 * ctx->nblocks_p, ctx->list_gsum, ctx->apex_slot,
+                         ctx->counters + CNT_APEX);
     ApexArgs x;
     x.T = ctx->T;
     x.st = ctx->state[1 - ctx->cur];
@@ -698,7 +714,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   dfree(ctx->clock);
   dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
-  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->dec_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot);
+  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->dec_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot); dfree(ctx->cand_mask); dfree(ctx->wave_pos); dfree(ctx->apex_mask); dfree(ctx->list_gsum);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPM_OK;
@@ -874,6 +890,18 @@ int mpm_set_scenes(mpm_ctx* ctx, int32_t nscenes, const mpm_scene_desc* descs) {
   return rc;
 }
 
+// Per-wave tables of G2P's deferred lanes (positions, masks, group sums) for the current particle
+// blocks.
+static int alloc_wave_tables(mpm_ctx* ctx) {
+  const int64_t nw = 4 * (int64_t)ctx->nblocks_p;
+  int rc = dalloc(ctx, ctx->wave_pos, (size_t)nw);
+  if (!rc) rc = dalloc(ctx, ctx->cand_mask, (size_t)nw);
+  if (!rc) rc = dalloc(ctx, ctx->apex_mask, (size_t)nw);
+  if (!rc) rc = dalloc(ctx, ctx->list_gsum, (size_t)((nw + 255) / 256));
+  ctx->wave_cap = rc ? 0 : nw;
+  return rc;
+}
+
 // Install n particles given as host planes (NPLANES x n, caller order) as the context's state.
 static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
   CK(hipSetDevice(ctx->device));
@@ -894,6 +922,7 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
     if (!rc) rc = dalloc(ctx, ctx->apex_usv, (size_t)CAND_SVD_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
+    if (!rc && ctx->wave_cap < 4 * (int64_t)ctx->nblocks_p) rc = alloc_wave_tables(ctx);
     if (rc) return rc;
     CK(hipMemsetAsync(ctx->state[1], 0, sizeof(float) * NPLANES * (size_t)n, ctx->stream));
     CK(hipMemsetAsync(ctx->keys, 0, sizeof(uint32_t) * n, ctx->stream));
@@ -2059,6 +2088,10 @@ int mpm_set_surrogate(mpm_ctx* ctx, int32_t enable, float gate_tol) {
   if (enable && ctx->n > 0 && !ctx->cand) {
     int rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * ctx->n);
     if (!rc) rc = dalloc(ctx, ctx->cand_slot, (size_t)ctx->n);
+    if (rc) return rc;
+  }
+  if (ctx->n > 0 && ctx->wave_cap < 4 * (int64_t)ctx->nblocks_p) {
+    int rc = alloc_wave_tables(ctx);
     if (rc) return rc;
   }
   ctx->sur_enabled = enable ? 1 : 0;

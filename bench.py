@@ -123,6 +123,18 @@ def cpu_baseline(sc):
     return out
 
 
+EAGER = False  # --eager: every substep launched eagerly (mpm_substep), no graph replay
+
+
+def advance(s, dt, k):
+    """k substeps: graph replay (mpm_run), or eager launches under --eager (for profilers)."""
+    if EAGER:
+        for _ in range(k):
+            s.substep(dt)
+    else:
+        s.run(dt, k)
+
+
 def log_pairs(s, dt, total, chunks, per_chunk):
     """Run `total` substeps, logging the exact path's (trial, projected) pairs in `chunks` substeps
     spread over the run (at most per_chunk pairs each)."""
@@ -131,12 +143,12 @@ def log_pairs(s, dt, total, chunks, per_chunk):
     done = 0
     while done < total:
         s.set_pair_logging(per_chunk)
-        s.run(dt, 1)
+        advance(s, dt, 1)
         pairs.append(s.get_pairs())
         s.set_pair_logging(0)
         k = min(every - 1, total - done - 1)
         if k > 0:
-            s.run(dt, k)
+            advance(s, dt, k)
         done += 1 + max(k, 0)
     return np.concatenate(pairs) if pairs else np.zeros((0, 8), np.float32)
 
@@ -179,10 +191,10 @@ def timed_run(sc, total, steps, weights=None, gate_tol=1e-5):
         if weights is not None:
             s.surrogate_set_weights(weights)
             s.set_surrogate(True, gate_tol)
-        s.run(sc.dt, total - steps)
+        advance(s, sc.dt, total - steps)
         s.synchronize()
         t0 = time.perf_counter()
-        s.run(sc.dt, steps)
+        advance(s, sc.dt, steps)
         s.synchronize()
         wall = time.perf_counter() - t0
         x = s.get_particles()["x"]
@@ -288,10 +300,10 @@ def config3_leg(steps, warmup):
     with Solver(0) as s:
         sc.load_into(s)
         s.set_mesh(mesh["x"], mesh["faces"], **kw)
-        s.run(sc.dt, warmup)
+        advance(s, sc.dt, warmup)
         s.synchronize()
         t0 = time.perf_counter()
-        s.run(sc.dt, steps)
+        advance(s, sc.dt, steps)
         s.synchronize()
         wall = time.perf_counter() - t0
         st = s.stats()
@@ -321,10 +333,10 @@ def batched_leg(weights, steps, warmup, gate_tol=1e-5):
             if sur:
                 s.surrogate_set_weights(weights)
                 s.set_surrogate(True, gate_tol)
-            s.run(dt, warmup)
+            advance(s, dt, warmup)
             s.synchronize()
             t0 = time.perf_counter()
-            s.run(dt, steps)
+            advance(s, dt, steps)
             s.synchronize()
             wall = time.perf_counter() - t0
             x = s.get_particles()["x"]
@@ -336,10 +348,10 @@ def batched_leg(weights, steps, warmup, gate_tol=1e-5):
     for sc in batch:  # the plain path: one context per scene, default launch shapes, same window
         with Solver(0) as s:
             sc.load_into(s)
-            s.run(dt, warmup)
+            advance(s, dt, warmup)
             s.synchronize()
             t0 = time.perf_counter()
-            s.run(dt, steps)
+            advance(s, dt, steps)
             s.synchronize()
             seq_wall += time.perf_counter() - t0
     seq_value = ntot * steps / seq_wall
@@ -379,7 +391,13 @@ def main():
                     help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--only", choices=["config2", "config3", "config4", "config4-surrogate", "batched"],
                     help="run this leg alone with the given --steps (for profiling) and print its JSON object")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every substep eagerly (no graph replay), e.g. under a profiler")
+    ap.add_argument("--weights", help="with --only config4-surrogate: network weights (.npy) instead of training")
+    ap.add_argument("--save-weights", help="with --only config4-surrogate: save the trained weights (.npy)")
     args = ap.parse_args()
+    global EAGER
+    EAGER = args.eager
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
@@ -396,7 +414,12 @@ def main():
             sc = scenes.config4_sand_phi()
             w = None
             if args.only == "config4-surrogate":
-                w, _ = train_surrogate(target_pairs=args.train_pairs)
+                if args.weights:
+                    w = np.load(args.weights)
+                else:
+                    w, _ = train_surrogate(target_pairs=args.train_pairs)
+                if args.save_weights:
+                    np.save(args.save_weights, w)
             r = timed_run(sc, args.warmup + args.steps, args.steps, weights=w)
             leg = {"workload": args.only, "value": sc.n * args.steps / r["wall"], "kernel_ms": r["kernel_ms"],
                    "surrogate_kernels": r["sur"], "status": r["stats"]["status"]}
