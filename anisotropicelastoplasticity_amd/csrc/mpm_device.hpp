@@ -34,7 +34,8 @@ struct DevParams {
   float gmin[3], h[3], inv_h[3];
   int res[3];
   int tiles[3];          // ceil(res/4)
-  double inv_res_d[2];   // 1/res[0], 1/res[1] (floor division of cell keys, floor_div)
+  uint32_t div_m[2];     // multiplier and shift of an exact integer division by res[0], res[1]
+  int div_s[2];          // (floor_div: cell keys to rows and planes)
   int nls;
   mpm_levelset ls[16];
   // cloth: caller ids [mesh_v0, mesh_e0) are mesh vertices, [mesh_e0, mesh_e1) mesh elements

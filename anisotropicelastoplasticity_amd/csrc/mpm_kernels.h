@@ -126,6 +126,9 @@ struct SortBufs {
   uint32_t* scan_agg;   // [scan blocks] particles counted in each scan block's cells
   int* cell_count;    // [cells] zero between substeps (the bin pass clears the occupied cells)
   int* cell_start;    // [cells + 1] exclusive prefix of the counts; cell c holds [start[c], start[c+1])
+                      // (written for the scan blocks with particles, see occ)
+  uint32_t* occ;        // [scan blocks] 1 where a cell of the block holds a particle (set by the count)
+  uint32_t* occ_clear;  // [scan blocks] the other substep's flags, cleared by the count
   int64_t ncells;
   SceneTable T;       // the counting sort also writes src and flags the occupied tiles
   int* tile_slot;
@@ -145,6 +148,7 @@ struct P2GArgs {
   const int* src;   // sorted position -> physical index in `state`
   const int* cell_start;
   const int* cell_end;
+  const uint32_t* occ;  // per scan block of cells: 1 if it holds particles (else its cells are empty)
   const int* counters;
   const int* active_tiles;
   float* partials;
@@ -252,6 +256,8 @@ void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* s
 void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* st, int64_t cap, int n,
                         unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
+// cell_start of the scan blocks without particles (only the getter of the bins needs them).
+void launch_fill_bins(hipStream_t s, const uint32_t* occ, const uint32_t* agg, int64_t ncells, int* cell_start);
 // Stable sort of the cell keys by caller id. Both modes count the particles of every cell and scan
 // the counts into cell_start. SORT_COUNTING then places each caller id at its stable rank inside
 // its cell; SORT_RADIX runs `passes` 8-bit LSD digit passes (1..4). Both give the same order. The
@@ -316,10 +322,17 @@ void launch_cloth_forces(hipStream_t s, const ClothArgs& a);   // element forces
 void launch_cloth_fixed(hipStream_t s, const ClothArgs& a);    // after the grid update
 void launch_cloth_elements(hipStream_t s, const ClothArgs& a); // after G2P: positions, velocities, D, return map
 void surrogate_set_smem_limit();
-// The sorted positions G2P flagged (mask[w] of wave w, whose lane 0 sits at wpos[w]), as a list in
-// ascending order built without atomics; the length goes to *count. gsum: (nwaves + 255) / 256 ints.
-void launch_position_list(hipStream_t s, const uint64_t* mask, const int* wpos, int nwaves, int* gsum, int* list,
-                          int* count);
+// Up to two lists of the sorted positions G2P flagged (mask[l][w] of wave w, whose lane 0 sits at
+// wpos[w]), in ascending order and built without atomics; list l's length goes to *count[l].
+struct ListArgs {
+  const uint64_t* mask[2];
+  int* list[2];
+  int* count[2];
+  int* gsum;  // [2][(nwaves + 63) / 64]
+  const int* wpos;
+  int nwaves, nlists;
+};
+void launch_position_lists(hipStream_t s, const ListArgs& a);
 // k_surrogate_forward (MFMA network over the candidates) then k_surrogate_apply (gate and finish)
 // (ev: optional start/stop event pairs of the two dispatches)
 void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev);

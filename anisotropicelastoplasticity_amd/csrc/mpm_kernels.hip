@@ -371,12 +371,15 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 // lists are read with are zeroed by k_cell_scan.
 __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__ keys, int n, int* __restrict__ cnt,
                                                     uint32_t* __restrict__ rank, ResetArgs r, int reset,
-                                                    const Clock* clk) {
+                                                    uint32_t* __restrict__ occ, uint32_t* __restrict__ occ_clear,
+                                                    int nocc, const Clock* clk) {
   if (clock_skip(clk)) return;
   // the key is loaded first, so that its round trip overlaps the reset's
   const int i = blockIdx.x * 256 + threadIdx.x;
   const bool valid = i < n;
   const uint32_t key = valid ? keys[i] : 0u;
+  // the previous substep's block flags (its P2G has run) start clean for the next one
+  for (int q = i; q < nocc; q += gridDim.x * 256) occ_clear[q] = 0u;
   if (reset) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,17 +404,24 @@ __global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__
     todo &= ~peers;
   }
   int base = 0;
-  if (valid && leader == lane) base = atomicAdd(&cnt[key], group);
+  if (valid && leader == lane) {
+    base = atomicAdd(&cnt[key], group);
+    occ[key / SCAN_TILE] = 1u;
+  }
   base = __shfl(base, leader);
   if (valid) rank[i] = (uint32_t)(base + within);
 }
 
 // Total of each scan block's SCAN_TILE cell counts (the blocks are independent).
 __global__ __launch_bounds__(256) void k_scan_sums(const int* __restrict__ cnt, int64_t ncells, uint32_t* __restrict__ agg,
-                                                   const Clock* clk) {
+                                                   const uint32_t* __restrict__ occ, const Clock* clk) {
   if (clock_skip(clk)) return;
   __shared__ uint32_t s_wsum[4];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  if (!occ[blockIdx.x]) {  // no particle in the block's cells: nothing to read
+    if (tid == 0) agg[blockIdx.x] = 0u;
+    return;
+  }
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)tid * SCAN_IPT;
   uint32_t t = 0;
   if (base + SCAN_IPT <= ncells) {
@@ -434,18 +444,24 @@ __global__ __launch_bounds__(256) void k_scan_sums(const int* __restrict__ cnt, 
 
 // Exclusive scan of the cell counts into cell_start[0 .. ncells]: block b starts at the sum of the
 // totals of the blocks before it (read and summed by the block itself) and scans its SCAN_TILE
-// cells; the blocks are independent. The last block writes cell_start[ncells].
+// cells; the blocks are independent. A block without particles writes nothing: its cells are
+// empty, which every reader learns from occ (the getter of the bins fills them, k_fill_bins). A
+// block with particles also writes the start of the next block's first cell (its own end), which
+// the next block, if it has particles, writes with the same value. The last block writes
+// cell_start[ncells].
 // Block 0 also zeroes the substep's counters (active tiles and node blocks, candidates, apex
 // projections), which k_cell_count has read.
 __global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, int64_t ncells, int* __restrict__ cell_start,
-                                                   const uint32_t* __restrict__ agg, int* __restrict__ counters,
-                                                   const Clock* clk) {
+                                                   const uint32_t* __restrict__ agg, const uint32_t* __restrict__ occ,
+                                                   int* __restrict__ counters, const Clock* clk) {
   if (clock_skip(clk)) return;
   if (blockIdx.x == 0 && threadIdx.x < CNT_NUM) counters[threadIdx.x] = 0;
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_boff[4];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int b = blockIdx.x;
+  const bool last = (int64_t)(b + 1) * SCAN_TILE >= ncells;
+  if (!occ[b] && !last) return;
   {
     uint32_t p = 0;
     for (int q = tid; q < b; q += 256) p += agg[q];
@@ -503,6 +519,24 @@ __global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, 
     }
   }
   if (base < ncells && base + SCAN_IPT >= ncells) cell_start[ncells] = (int)run;  // the total
+  if (tid == 255 && !last) cell_start[(int64_t)(b + 1) * SCAN_TILE] = (int)run;  // the next block's first cell
+}
+
+// cell_start of the blocks without particles: every cell starts (and ends) at the block's offset.
+__global__ __launch_bounds__(256) void k_fill_bins(const uint32_t* __restrict__ occ, const uint32_t* __restrict__ agg,
+                                                   int64_t ncells, int* __restrict__ cell_start) {
+  const int b = blockIdx.x;
+  if (occ[b]) return;
+  __shared__ uint32_t s_boff[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  uint32_t p = 0;
+  for (int q = tid; q < b; q += 256) p += agg[q];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
+  if (lane == 0) s_boff[w] = p;
+  __syncthreads();
+  const int boff = (int)(s_boff[0] + s_boff[1] + s_boff[2] + s_boff[3]);
+  for (int64_t c = (int64_t)b * SCAN_TILE + tid; c < min((int64_t)(b + 1) * SCAN_TILE, ncells); c += 256) cell_start[c] = boff;
 }
 
 __global__ __launch_bounds__(256) void k_cell_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rank,
@@ -981,13 +1015,17 @@ __global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
       if (g0 < S.res[0] && g2 < S.res[2]) {
         if (g1a < S.res[1]) {
           const int64_t key = noff + (g2 * S.res[1] + g1a) * S.res[0] + g0;
-          rstart0 = a.cell_start[key];
-          rcnt0 = a.cell_end[key] - rstart0;
+          if (a.occ[key / SCAN_TILE]) {
+            rstart0 = a.cell_start[key];
+            rcnt0 = a.cell_end[key] - rstart0;
+          }
         }
         if (g1b < S.res[1]) {
           const int64_t key = noff + (g2 * S.res[1] + g1b) * S.res[0] + g0;
-          rstart1 = a.cell_start[key];
-          rcnt1 = a.cell_end[key] - rstart1;
+          if (a.occ[key / SCAN_TILE]) {
+            rstart1 = a.cell_start[key];
+            rcnt1 = a.cell_end[key] - rstart1;
+          }
         }
       }
     }
@@ -1176,8 +1214,10 @@ __global__ __launch_bounds__(256) void k_p2g_layer(P2GArgs a) {
     int start = 0, cnt = 0;
     if (g0 < S.res[0] && g1 < S.res[1] && g2 < S.res[2]) {
       const int64_t key = noff + (g2 * S.res[1] + g1) * S.res[0] + g0;
-      start = a.cell_start[key];
-      cnt = a.cell_end[key] - start;
+      if (a.occ[key / SCAN_TILE]) {
+        start = a.cell_start[key];
+        cnt = a.cell_end[key] - start;
+      }
     }
     int cmax = cnt;
 #pragma unroll
@@ -1487,12 +1527,12 @@ __device__ __forceinline__ void g2p_box_store(float4* box, int lane, const float
   }
 }
 
-// floor(a / b) for 0 <= a < 2^31 and b > 0, given inv = 1/b in double: the product is within 2^-20
-// of a/b, so truncation is exact except just below an integer quotient, which one step corrects.
-__device__ __forceinline__ int floor_div(int a, int b, double inv) {
-  int q = (int)((double)a * inv);
-  if (((int64_t)q + 1) * b <= (int64_t)a) q += 1;
-  return q;
+// floor(a / d) for 0 <= a < 2^31 with the multiplier m and shift l of d (integer arithmetic only;
+// l = 0 means d = 1): t = floor(a m / 2^32), then (t + ((a - t) >> 1)) >> (l - 1).
+__device__ __forceinline__ int floor_div(int a, uint32_t m, int l) {
+  if (l == 0) return a;
+  const uint32_t n = (uint32_t)a, t = __umulhi(n, m);
+  return (int)((t + ((n - t) >> 1)) >> (l - 1));
 }
 
 // Thread per sorted position j (caller id pid = perm[j]): gathers the stencil of grid v
@@ -1521,7 +1561,8 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   // wave uses them only if every lane's stencil lies inside its row's box, and they are clamped to
   // the grid whatever the keys hold.
   __shared__ float4 s_box[4][2][2 * G2P_BOX];
-  const double ir0 = a.T.params[pb.s].inv_res_d[0], ir1 = a.T.params[pb.s].inv_res_d[1];
+  const uint32_t dm0 = a.T.params[pb.s].div_m[0], dm1 = a.T.params[pb.s].div_m[1];
+  const int ds0 = a.T.params[pb.s].div_s[0], ds1 = a.T.params[pb.s].div_s[1];
   // The particle's state and its sorted key, loaded by every lane (lanes past the end read the
   // last position), so that the box below is placed while they are in flight.
   const int jc = j < pb.end ? j : pb.end - 1;
@@ -1553,9 +1594,9 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     const int64_t lf = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)kj, 0) - noff;
     const int64_t ll = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)kj, 63) - noff;
     if (lf >= 0 && ll >= lf && ll < ncell && ll <= INT32_MAX) {
-      const int rf = floor_div((int)lf, P.res[0], ir0);
-      const int rl = floor_div((int)ll, P.res[0], ir0);
-      const int zc = floor_div(rf, P.res[1], ir1), yc = rf - zc * P.res[1];
+      const int rf = floor_div((int)lf, dm0, ds0);
+      const int rl = floor_div((int)ll, dm0, ds0);
+      const int zc = floor_div(rf, dm1, ds1), yc = rf - zc * P.res[1];
       const int xf = (int)lf - rf * P.res[0], xl = (int)ll - rl * P.res[0];
       int xa = xl, xb = xf;  // last x of the first row, first x of the second
       if (rf == rl) {
@@ -1909,73 +1950,81 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
 }
 
 // ================================================================== position lists
-// The sorted positions that G2P flagged (candidates of the surrogate, deferred FP64 apex
-// projections), as a list in ascending order and without atomics. Every G2P wave w stores the
-// ballot of its flagged lanes (mask[w]) and the position of its lane 0 (wpos[w]). k_list_sums
-// totals each group of 256 waves, k_list_scan turns the group totals into offsets (one workgroup)
-// and writes the list length, and k_list_write places each wave's positions after its group's
-// offset and the waves before it in the group.
-__global__ __launch_bounds__(256) void k_list_sums(const uint64_t* __restrict__ mask, int nwaves, int* __restrict__ gsum) {
-  const int w = blockIdx.x * 256 + threadIdx.x;
-  int c = w < nwaves ? __popcll(mask[w]) : 0;
+// Lists of the sorted positions that G2P flagged, in ascending order and without atomics: the
+// deferred FP64 apex projections and the surrogate's candidates (two lists, one set of launches).
+// Every G2P wave w stores the ballots of its flagged lanes (mask[l][w]) and the position of its
+// lane 0 (wpos[w]). Waves form groups of 64: k_list_sums totals each group, k_list_scan turns the
+// group totals into offsets (one workgroup) and writes the list lengths, and k_list_write places
+// each wave's positions after its group's offset and the waves before it in the group.
+__global__ __launch_bounds__(256) void k_list_sums(ListArgs a) {
+  const int ng = (a.nwaves + 63) / 64;
+  const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (g >= ng) return;
+  const int w = g * 64 + lane;
+  for (int l = 0; l < a.nlists; ++l) {
+    int c = w < a.nwaves ? __popcll(a.mask[l][w]) : 0;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  __shared__ int s_c[4];
-  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) gsum[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) a.gsum[l * ng + g] = c;
+  }
 }
 
-__global__ __launch_bounds__(1024) void k_list_scan(int* __restrict__ gsum, int ngroups, int* __restrict__ count) {
+__global__ __launch_bounds__(1024) void k_list_scan(ListArgs a) {
   __shared__ int s_sum[1024];
+  const int ng = (a.nwaves + 63) / 64;
   const int t = threadIdx.x;
-  const int per = (ngroups + 1023) / 1024;
-  const int b0 = min(t * per, ngroups), b1 = min(b0 + per, ngroups);
-  int sum = 0;
-  for (int i = b0; i < b1; ++i) sum += gsum[i];
-  s_sum[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? s_sum[t - off] : 0;
+  const int per = (ng + 1023) / 1024;
+  const int b0 = min(t * per, ng), b1 = min(b0 + per, ng);
+  for (int l = 0; l < a.nlists; ++l) {
+    int* gs = a.gsum + l * ng;
+    int sum = 0;
+    for (int i = b0; i < b1; ++i) sum += gs[i];
+    s_sum[t] = sum;
     __syncthreads();
-    s_sum[t] += v;
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int v = t >= off ? s_sum[t - off] : 0;
+      __syncthreads();
+      s_sum[t] += v;
+      __syncthreads();
+    }
+    int run = s_sum[t] - sum;  // exclusive offset of this thread's groups
+    for (int i = b0; i < b1; ++i) {
+      const int v = gs[i];
+      gs[i] = run;
+      run += v;
+    }
+    if (t == 1023) *a.count[l] = s_sum[1023];
     __syncthreads();
   }
-  int run = s_sum[t] - sum;  // exclusive offset of this thread's groups
-  for (int i = b0; i < b1; ++i) {
-    const int v = gsum[i];
-    gsum[i] = run;
-    run += v;
-  }
-  if (t == 1023) *count = s_sum[1023];
 }
 
-__global__ __launch_bounds__(256) void k_list_write(const uint64_t* __restrict__ mask, const int* __restrict__ wpos,
-                                                    int nwaves, const int* __restrict__ goff, int* __restrict__ list) {
-  __shared__ uint64_t s_m[256];
-  __shared__ int s_off[256], s_p[256], s_wsum[4];
-  const int t = threadIdx.x, w = blockIdx.x * 256 + t, lane = t & 63, wv = t >> 6;
-  const uint64_t m = w < nwaves ? mask[w] : 0ull;
-  const int c = __popcll(m);
-  int x = c;  // inclusive scan over the wave
+__global__ __launch_bounds__(256) void k_list_write(ListArgs a) {
+  __shared__ uint64_t s_m[2][64];
+  __shared__ int s_off[2][64], s_p[64];
+  const int ng = (a.nwaves + 63) / 64;
+  const int g = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int w = g * 64 + lane;
+  if (wv < a.nlists) {  // wave l: the offsets of list l's 64 masks of the group
+    const uint64_t m = w < a.nwaves ? a.mask[wv][w] : 0ull;
+    const int c = __popcll(m);
+    int x = c;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(x, off);
-    if (lane >= off) x += y;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    s_m[wv][lane] = m;
+    s_off[wv][lane] = a.gsum[wv * ng + g] + x - c;
+  } else if (wv == 3) {
+    s_p[lane] = w < a.nwaves ? a.wpos[w] : 0;
   }
-  if (lane == 63) s_wsum[wv] = x;
   __syncthreads();
-  int base = goff[blockIdx.x];
-  for (int k = 0; k < wv; ++k) base += s_wsum[k];
-  s_m[t] = m;
-  s_off[t] = base + x - c;
-  s_p[t] = (w < nwaves && m) ? wpos[w] : 0;
-  __syncthreads();
-  // wave wv writes the positions of the block's masks 64 wv .. 64 wv + 63, one mask at a time
-  for (int i = 64 * wv; i < 64 * wv + 64; ++i) {
-    const uint64_t mi = s_m[i];
-    if ((mi >> lane) & 1ull) list[s_off[i] + __popcll(mi & ((1ull << lane) - 1ull))] = s_p[i] + lane;
-  }
+  // wave wv writes the positions of masks 16 wv .. 16 wv + 15 of each list, one mask at a time
+  for (int l = 0; l < a.nlists; ++l)
+    for (int i = 16 * wv; i < 16 * wv + 16; ++i) {
+      const uint64_t mi = s_m[l][i];
+      if ((mi >> lane) & 1ull) a.list[l][s_off[l][i] + __popcll(mi & ((1ull << lane) - 1ull))] = s_p[i] + lane;
+    }
 }
 
 // ================================================================== surrogate return map
@@ -2411,10 +2460,10 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
     uint32_t* rank = (passes & 1) ? b.k2 : b.k1;
     const int nscan = blocks_for(b.ncells, SCAN_TILE);
     hipLaunchKernelGGL(k_cell_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, n, b.cell_count, rank,
-                       b.reset, mode == SORT_COUNTING ? 1 : 0, b.clk);
-    hipLaunchKernelGGL(k_scan_sums, dim3(nscan), dim3(256), 0, s, b.cell_count, b.ncells, b.scan_agg, b.clk);
+                       b.reset, mode == SORT_COUNTING ? 1 : 0, b.occ, b.occ_clear, nscan, b.clk);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nscan), dim3(256), 0, s, b.cell_count, b.ncells, b.scan_agg, b.occ, b.clk);
     hipLaunchKernelGGL(k_cell_scan, dim3(nscan), dim3(256), 0, s, b.cell_count, b.ncells, b.cell_start, b.scan_agg,
-                       b.counters, b.clk);
+                       b.occ, b.counters, b.clk);
   }
   if (mode == SORT_COUNTING) {
     uint32_t* kf = (passes & 1) ? b.k1 : b.k2;  // the result goes where the digit passes put it
@@ -2439,6 +2488,11 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
     kin = ko;
     vin = vo;
   }
+}
+
+void launch_fill_bins(hipStream_t s, const uint32_t* occ, const uint32_t* agg, int64_t ncells, int* cell_start) {
+  if (ncells <= 0) return;
+  hipLaunchKernelGGL(k_fill_bins, dim3(blocks_for(ncells, SCAN_TILE)), dim3(256), 0, s, occ, agg, ncells, cell_start);
 }
 
 void launch_reset(hipStream_t s, const ResetArgs& r) {
@@ -2564,13 +2618,12 @@ void surrogate_set_smem_limit() {
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)surrogate_smem());
 }
 
-void launch_position_list(hipStream_t s, const uint64_t* mask, const int* wpos, int nwaves, int* gsum, int* list,
-                          int* count) {
-  if (nwaves <= 0) return;
-  const int ng = (nwaves + 255) / 256;
-  hipLaunchKernelGGL(k_list_sums, dim3(ng), dim3(256), 0, s, mask, nwaves, gsum);
-  hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, gsum, ng, count);
-  hipLaunchKernelGGL(k_list_write, dim3(ng), dim3(256), 0, s, mask, wpos, nwaves, gsum, list);
+void launch_position_lists(hipStream_t s, const ListArgs& a) {
+  if (a.nwaves <= 0 || a.nlists <= 0) return;
+  const int ng = (a.nwaves + 63) / 64;
+  hipLaunchKernelGGL(k_list_sums, dim3((ng + 3) / 4), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(k_list_write, dim3(ng), dim3(256), 0, s, a);
 }
 
 void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev) {

@@ -84,6 +84,11 @@ struct mpm_ctx {
   int* cell_start = nullptr;
   int* cell_count = nullptr;  // per-cell particle counts of the counting sort (zero between substeps)
   int* cell_end = nullptr;  // cell_start + 1 (not a separate allocation)
+  // per scan block of cells: 1 where a cell holds a particle; two sets, used by alternate substeps
+  // (each substep's count clears the other set, whose P2G has run)
+  uint32_t* occ[2] = {nullptr, nullptr};
+  int occ_par = 0;
+  const uint32_t* occ_last = nullptr;  // the set of the last sort
   int* tile_slot = nullptr;
   int* active_tiles = nullptr;
   int* block_slot = nullptr;
@@ -150,7 +155,7 @@ struct mpm_ctx {
   int* wave_pos = nullptr;
   uint64_t* cand_mask = nullptr;
   uint64_t* apex_mask = nullptr;
-  int* list_gsum = nullptr;  // per group of 256 waves: count, then offset
+  int* list_gsum = nullptr;  // per list and group of 64 waves: count, then offset
   int64_t wave_cap = 0;
   float* dec_log = nullptr;  // per-candidate gate records of the last surrogate substep
   int64_t dec_cap = 0;
@@ -224,6 +229,16 @@ static SceneHost& scene0(mpm_ctx* ctx) {
 }
 
 // ------------------------------------------------------------------ parameters
+// Multiplier m and shift l of the exact unsigned division by d >= 1 (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, floor(n / d) = (t + ((n - t) >> 1)) >> (l - 1) for every 32-bit
+// n, where t = floor(n m / 2^32). d = 1 is marked by l = 0.
+static void div_magic(uint32_t d, uint32_t& m, int& l) {
+  l = 0;
+  while ((1ull << l) < d) ++l;
+  m = l == 0 ? 0u : (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+}
+
 static DevParams devparams_of(const SceneHost& sc) {
   DevParams d{};
   const mpm_params& p = sc.params;
@@ -264,8 +279,7 @@ static DevParams devparams_of(const SceneHost& sc) {
     d.res[a] = sc.res[a];
     d.tiles[a] = sc.tiles[a];
   }
-  d.inv_res_d[0] = 1.0 / sc.res[0];
-  d.inv_res_d[1] = 1.0 / sc.res[1];
+  for (int a = 0; a < 2; ++a) div_magic((uint32_t)sc.res[a], d.div_m[a], d.div_s[a]);
   d.nls = (int)sc.ls.size();
   for (int k = 0; k < d.nls; ++k) d.ls[k] = sc.ls[k];
   d.mesh_v0 = sc.mesh_v0;
@@ -396,6 +410,10 @@ static int stage_sort(mpm_ctx* ctx) {
   b.src = ctx->src;
   b.counters = ctx->counters;
   b.clk = ctx->clk_active;
+  b.occ = ctx->occ[ctx->occ_par];
+  b.occ_clear = ctx->occ[1 - ctx->occ_par];
+  ctx->occ_last = b.occ;
+  ctx->occ_par = 1 - ctx->occ_par;
   launch_sort(s, b, ctx->keys, ctx->n, ctx->sort_passes, ctx->sort_mode);
   launch_bounds(s, ctx->T, ctx->nblocks_p, ctx->ks_sorted, ctx->n, ctx->ntiles, ctx->cell_count,
                 ctx->tile_slot, ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters, ctx->vs_sorted,
@@ -442,6 +460,7 @@ static P2GArgs p2g_args(mpm_ctx* ctx, float dt) {
   a.src = ctx->src;
   a.cell_start = ctx->cell_start;
   a.cell_end = ctx->cell_end;
+  a.occ = ctx->occ_last;
   a.counters = ctx->counters;
   a.active_tiles = ctx->active_tiles;
   a.partials = ctx->partials;
@@ -534,12 +553,24 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.apex_usv = ctx->apex_usv;
   a.clk = ctx->clk_active;
   launch_g2p(ctx->stream, a, ctx->order, ctx->nblocks_p, ctx->ev_main[0], ctx->ev_main[1]);
-  if (surrogate)
-    launch_position_list(ctx->stream, ctx->cand_mask, ctx->wave_pos, 4 * ctx->nblocks_p, ctx->list_gsum, ctx->cand_slot,
-                         ctx->counters + CNT_CAND);
+  {
+    ListArgs la{};
+    la.wpos = ctx->wave_pos;
+    la.gsum = ctx->list_gsum;
+    la.nwaves = 4 * ctx->nblocks_p;
+    if (a.apex_mask) {
+      la.mask[la.nlists] = ctx->apex_mask;
+      la.list[la.nlists] = ctx->apex_slot;
+      la.count[la.nlists++] = ctx->counters + CNT_APEX;
+    }
+    if (surrogate) {
+      la.mask[la.nlists] = ctx->cand_mask;
+      la.list[la.nlists] = ctx->cand_slot;
+      la.count[la.nlists++] = ctx->counters + CNT_CAND;
+    }
+    launch_position_lists(ctx->stream, la);
+  }
   if (a.apex_mask) {
-    launch_position_list(ctx->stream, ctx->apex_mask, ctx->wave_pos, 4 * ctx->nblocks_p, ctx->list_gsum, ctx->apex_slot,
-                         ctx->counters + CNT_APEX);
     ApexArgs x;
     x.T = ctx->T;
     x.st = ctx->state[1 - ctx->cur];
@@ -705,7 +736,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   drop_graphs(ctx);
   dfree(ctx->table_buf);
   drop_mesh(ctx);
-  dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_count);
+  dfree(ctx->thr); dfree(ctx->grid); dfree(ctx->cell_start); dfree(ctx->cell_count); dfree(ctx->occ[0]); dfree(ctx->occ[1]);
   ctx->cell_end = nullptr;
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
@@ -808,6 +839,9 @@ static int layout(mpm_ctx* ctx) {
   ctx->sort_passes = std::max(1, (bits + 7) / 8);
   int rc = dalloc(ctx, ctx->grid, (size_t)GRID_FLOATS * ng);
   if (!rc) rc = dalloc(ctx, ctx->cell_start, ng + 1);
+  const int64_t nscan = (ng + SCAN_TILE - 1) / SCAN_TILE;
+  if (!rc) rc = dalloc(ctx, ctx->occ[0], (size_t)nscan);
+  if (!rc) rc = dalloc(ctx, ctx->occ[1], (size_t)nscan);
   if (!rc) rc = dalloc(ctx, ctx->cell_count, ng);
   if (!rc) rc = dalloc(ctx, ctx->tile_slot, nt);
   if (!rc) rc = dalloc(ctx, ctx->active_tiles, nt);
@@ -818,6 +852,9 @@ static int layout(mpm_ctx* ctx) {
   CK(hipMemsetAsync(ctx->grid, 0, sizeof(float) * GRID_FLOATS * ng, ctx->stream));
   ctx->cell_end = ctx->cell_start + 1;  // cell c holds sorted positions [cell_start[c], cell_start[c + 1])
   CK(hipMemsetAsync(ctx->cell_start, 0, sizeof(int) * (ng + 1), ctx->stream));
+  CK(hipMemsetAsync(ctx->occ[0], 0, sizeof(uint32_t) * nscan, ctx->stream));
+  CK(hipMemsetAsync(ctx->occ[1], 0, sizeof(uint32_t) * nscan, ctx->stream));
+  ctx->occ_last = ctx->occ[0];
   CK(hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * ng, ctx->stream));
   launch_fill_int(ctx->stream, ctx->tile_slot, nt, -1);
   launch_fill_int(ctx->stream, ctx->block_slot, nt, -1);
@@ -897,7 +934,7 @@ static int alloc_wave_tables(mpm_ctx* ctx) {
   int rc = dalloc(ctx, ctx->wave_pos, (size_t)nw);
   if (!rc) rc = dalloc(ctx, ctx->cand_mask, (size_t)nw);
   if (!rc) rc = dalloc(ctx, ctx->apex_mask, (size_t)nw);
-  if (!rc) rc = dalloc(ctx, ctx->list_gsum, (size_t)((nw + 255) / 256));
+  if (!rc) rc = dalloc(ctx, ctx->list_gsum, (size_t)(2 * ((nw + 63) / 64)));
   ctx->wave_cap = rc ? 0 : nw;
   return rc;
 }
@@ -1585,7 +1622,12 @@ int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins) {
   CK(hipStreamSynchronize(ctx->stream));
   if (keys) CK(hipMemcpy(keys, ctx->ks_sorted, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
   if (ids) CK(hipMemcpy(ids, ctx->vs_sorted, sizeof(int) * n, hipMemcpyDeviceToHost));
-  if (bins) CK(hipMemcpy(bins, ctx->cell_start, sizeof(int) * (ctx->ng + 1), hipMemcpyDeviceToHost));
+  if (bins) {
+    // the scan skipped the blocks without particles: their cells start at the block's offset
+    launch_fill_bins(ctx->stream, ctx->occ_last, ctx->sortws + SORTWS_SCAN, ctx->ng, ctx->cell_start);
+    CK(hipStreamSynchronize(ctx->stream));
+    CK(hipMemcpy(bins, ctx->cell_start, sizeof(int) * (ctx->ng + 1), hipMemcpyDeviceToHost));
+  }
   return MPM_OK;
 }
 

@@ -182,12 +182,15 @@ def train_surrogate(target_pairs=10_000_000, epochs=4, batch=4096, lr=1e-3):
     return w, info
 
 
-def timed_run(sc, total, steps, weights=None, gate_tol=1e-5):
+def timed_run(sc, total, steps, weights=None, gate_tol=1e-5, load_checkpoint=None, save_checkpoint=None):
     """Run `total` substeps of sc; time the last `steps` of them. Returns wall, positions, stats and
-    a per-kernel breakdown measured afterwards on eagerly launched substeps."""
+    a per-kernel breakdown measured afterwards on eagerly launched substeps. A checkpoint can replace
+    the scene's initial state (load) or keep the state after the run (save)."""
     from anisotropicelastoplasticity_amd import Solver
     with Solver(0) as s:
         sc.load_into(s)
+        if load_checkpoint:
+            s.load_checkpoint(load_checkpoint)
         if weights is not None:
             s.surrogate_set_weights(weights)
             s.set_surrogate(True, gate_tol)
@@ -199,6 +202,8 @@ def timed_run(sc, total, steps, weights=None, gate_tol=1e-5):
         wall = time.perf_counter() - t0
         x = s.get_particles()["x"]
         st = s.stats()
+        if save_checkpoint:
+            s.save_checkpoint(save_checkpoint)
         kt = 10
         _, stage_ms, kernel_ms = s.time_kernels(sc.dt, kt)
         kern = {k: kernel_ms[i] / kt for i, k in enumerate(STAGES)}
@@ -374,6 +379,15 @@ def batched_leg(weights, steps, warmup, gate_tol=1e-5):
     return out
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """A plain device copy of known size: the reference point that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to HBM bytes."""
+    from anisotropicelastoplasticity_amd import Solver
+    with Solver(0) as s:
+        return {"workload": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                "launches": reps, "kernel": "k_copy", "avg_ms": s.bench_copy(nbytes, reps)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,12 +403,14 @@ def main():
     ap.add_argument("--train-pairs", type=int, default=10_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--only", choices=["config2", "config3", "config4", "config4-surrogate", "batched"],
+    ap.add_argument("--only", choices=["config2", "config3", "config4", "config4-surrogate", "batched", "copy"],
                     help="run this leg alone with the given --steps (for profiling) and print its JSON object")
     ap.add_argument("--eager", action="store_true",
                     help="launch every substep eagerly (no graph replay), e.g. under a profiler")
     ap.add_argument("--weights", help="with --only config4-surrogate: network weights (.npy) instead of training")
     ap.add_argument("--save-weights", help="with --only config4-surrogate: save the trained weights (.npy)")
+    ap.add_argument("--load-checkpoint", help="with --only config4/config4-surrogate: start from this state (.npz)")
+    ap.add_argument("--save-checkpoint", help="with --only config4/config4-surrogate: save the state after the run")
     args = ap.parse_args()
     global EAGER
     EAGER = args.eager
@@ -410,6 +426,8 @@ def main():
             leg = config3_leg(args.steps, args.warmup)
         elif args.only == "batched":
             leg = batched_leg(None, args.steps, args.warmup)
+        elif args.only == "copy":
+            leg = copy_calibration()
         else:
             sc = scenes.config4_sand_phi()
             w = None
@@ -420,7 +438,8 @@ def main():
                     w, _ = train_surrogate(target_pairs=args.train_pairs)
                 if args.save_weights:
                     np.save(args.save_weights, w)
-            r = timed_run(sc, args.warmup + args.steps, args.steps, weights=w)
+            r = timed_run(sc, args.warmup + args.steps, args.steps, weights=w, load_checkpoint=args.load_checkpoint,
+                          save_checkpoint=args.save_checkpoint)
             leg = {"workload": args.only, "value": sc.n * args.steps / r["wall"], "kernel_ms": r["kernel_ms"],
                    "surrogate_kernels": r["sur"], "status": r["stats"]["status"]}
         print(json.dumps(leg))
@@ -431,12 +450,7 @@ def main():
     c2 = None if args.no_config2 else config2_leg(args.config2_steps, 20)
     bat = None if args.no_batch else batched_leg(weights, args.batch_steps, 20)
     c3 = None if args.no_config3 else config3_leg(args.config3_steps, 20)
-    copy = None
-    if args.copy_calibration:
-        with Solver(0) as s:
-            nbytes, reps = 256 << 20, 3
-            copy = {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "launches": reps,
-                    "kernel": "k_copy", "avg_ms": s.bench_copy(nbytes, reps)}
+    copy = copy_calibration() if args.copy_calibration else None
 
     out = {
         "metric": METRIC,
