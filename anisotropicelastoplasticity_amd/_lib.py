@@ -137,8 +137,7 @@ SIGNATURES = [
     ("mpm_synchronize", ctypes.c_int, [_ctx]),
     ("mpm_time_substeps", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _f, _f]),
     ("mpm_time_kernels", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _f, _f, _f]),
-    ("mpm_time_surrogate", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _P(ctypes.c_float), _P(ctypes.c_float),
-                                          _P(ctypes.c_int64)]),
+    ("mpm_time_surrogate", ctypes.c_int, [_ctx, ctypes.c_float, ctypes.c_int32, _P(ctypes.c_float), _P(ctypes.c_int64)]),
     ("mpm_surrogate_nparam", ctypes.c_int32, []),
     ("mpm_surrogate_init", ctypes.c_int, [_ctx, ctypes.c_uint64]),
     ("mpm_surrogate_set_weights", ctypes.c_int, [_ctx, _f, ctypes.c_int32]),

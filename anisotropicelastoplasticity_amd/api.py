@@ -400,12 +400,12 @@ class Solver:
         return total.value, list(stages), list(kern)
 
     def time_surrogate(self, dt, nsteps):
-        """(network kernel ms, gate kernel ms, candidates) summed over nsteps eager substeps with the
-        surrogate enabled (mpm_time_surrogate)."""
-        f, a, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_int64()
-        rc = self.lib.mpm_time_surrogate(self.h, float(dt), int(nsteps), ctypes.byref(f), ctypes.byref(a), ctypes.byref(c))
+        """(surrogate kernel ms, candidates) summed over nsteps eager substeps with the surrogate
+        enabled (mpm_time_surrogate); the kernel evaluates the network, gates and updates the state."""
+        f, c = ctypes.c_float(), ctypes.c_int64()
+        rc = self.lib.mpm_time_surrogate(self.h, float(dt), int(nsteps), ctypes.byref(f), ctypes.byref(c))
         self._check(rc, "mpm_time_surrogate")
-        return f.value, a.value, c.value
+        return f.value, c.value
 
 
 def write_obj(path, x):

@@ -788,7 +788,7 @@ __device__ __forceinline__ P2GScene p2g_scene(const DevParams& P) {
 // chunk are in flight while the wave accumulates the current one). The force pass of the
 // predicted-stress mode loads inside p2g_stage instead.
 struct StageIn {
-  int pi, kind, mid;
+  int pi, pid;  // pid: the particle's id when the scene has a cloth mesh (else -1)
   float x[3], m, v[3], B[9], K[6];
 };
 
@@ -802,13 +802,10 @@ __device__ __forceinline__ void p2g_load_at(const P2GArgs& a, const P2GScene& S,
   const float* st = a.state;
   const int64_t cap = a.cap;
   in.pi = pi;
-  in.kind = KIND_PARTICLE;
-  in.mid = 0;
-  if (S.mesh_e1 > S.mesh_v0) {
-    const int pid = reinterpret_cast<const int*>(st + (size_t)PID * cap)[pi];
-    in.kind = pid < S.mesh_v0 ? KIND_PARTICLE : (pid < S.mesh_e0 ? KIND_VERTEX : KIND_ELEMENT);
-    in.mid = pid - (in.kind == KIND_VERTEX ? S.mesh_v0 : S.mesh_e0);
-  }
+  // a scene without a mesh loads word 0 of the id plane in every lane (one cache line) instead of a
+  // branch around the load: a branch here makes the wave wait for this chunk's loads before the
+  // accumulation of the previous one
+  in.pid = reinterpret_cast<const int*>(st + (size_t)PID * cap)[S.mesh_e1 > S.mesh_v0 ? pi : 0];
 #pragma unroll
   for (int d = 0; d < 3; ++d) in.x[d] = st[(PX + d) * cap + pi];
   if constexpr (MODE != P2G_FORCE) {
@@ -838,7 +835,11 @@ __device__ __forceinline__ void p2g_stage(const P2GArgs& a, const P2GScene& S, c
   float pay[PAY];
 #pragma unroll
   for (int i = 0; i < PAY; ++i) pay[i] = 0.0f;
-  const int pi = in.pi, kind = in.kind, mid = in.mid;
+  // kind of the point from its id (loaded with the rest of its data, used only here)
+  const int pi = in.pi;
+  const int kind = (S.mesh_e1 <= S.mesh_v0 || in.pid < S.mesh_v0) ? KIND_PARTICLE
+                                                                    : (in.pid < S.mesh_e0 ? KIND_VERTEX : KIND_ELEMENT);
+  const int mid = in.pid - (kind == KIND_VERTEX ? S.mesh_v0 : S.mesh_e0);
   const float* st = a.state;
   const int64_t cap = a.cap;
   const int cc[3] = {g0, g1, g2};
@@ -994,10 +995,12 @@ __global__ __launch_bounds__(64 * P2G_WAVES) void k_p2g(P2GArgs a) {
     nt0_1 = __builtin_amdgcn_readfirstlane(a.T.params[0].tiles[0]);
     nt1_1 = __builtin_amdgcn_readfirstlane(a.T.params[0].tiles[1]);
   }
-  int gtile_next = (int)blockIdx.x < ntile ? a.active_tiles[blockIdx.x] : 0;
-  for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
+  // the tiles of one XCD are a contiguous range of the (ascending) active tiles
+  const XcdRange tr = xcd_range(ntile);
+  int gtile_next = tr.first < tr.end ? a.active_tiles[tr.first] : 0;
+  for (int slot = tr.first; slot < tr.end; slot += tr.stride) {
     const int gtile = __builtin_amdgcn_readfirstlane(gtile_next);
-    if (slot + (int)gridDim.x < ntile) gtile_next = a.active_tiles[slot + gridDim.x];
+    if (slot + tr.stride < tr.end) gtile_next = a.active_tiles[slot + tr.stride];
     const int sc = one_scene ? 0 : __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
     const DevParams& P = a.T.params[sc];
     const P2GScene S = one_scene ? S1 : p2g_scene(P);
@@ -1201,7 +1204,8 @@ __global__ __launch_bounds__(256) void k_p2g_layer(P2GArgs a) {
   for (int i = tid; i < 8 * 4 * SLAB; i += 256) (&s_slab[0][0][0])[i] = 0.0f;
   __syncthreads();
   float* slab = &s_slab[w + 4 * (cy >> 1)][0][0];
-  for (int slot = blockIdx.x; slot < ntile; slot += gridDim.x) {
+  const XcdRange tr = xcd_range(ntile);
+  for (int slot = tr.first; slot < tr.end; slot += tr.stride) {
     const int gtile = a.active_tiles[slot];
     const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gtile));
     const DevParams& P = a.T.params[sc];
@@ -1355,7 +1359,8 @@ __global__ __launch_bounds__(64) void k_grid(GridArgs a) {
   const int nblk = a.counters[CNT_BLOCKS];
   float vmax = 0.0f;  // max |v| over this lane's nodes after the velocity update
   const int lx = threadIdx.x & 3, ly = (threadIdx.x >> 2) & 3, lz = threadIdx.x >> 4;
-  for (int s = blockIdx.x; s < nblk; s += gridDim.x) {
+  const XcdRange br = xcd_range(nblk);
+  for (int s = br.first; s < br.end; s += br.stride) {
     const int gblk = a.active_blocks[s];
     const int sc = __builtin_amdgcn_readfirstlane(scene_of(a.T.tile_off, a.T.nscenes, gblk));
     const DevParams& P = a.T.params[sc];
@@ -1544,7 +1549,8 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   if (clock_skip(a.clk)) return;
   const float dt = clock_dt(a.clk, a.dt);
-  const PBlock pb = particle_block(a.T);
+  const int pblk = (int)blockIdx.x;
+  const PBlock pb = particle_block(a.T, pblk);
   const DevParams P = uniform_params(a.T.params[pb.s]);
   const int64_t noff = a.T.node_off[pb.s];
   const float4* gvel = a.gvel + 2 * noff;
@@ -1808,29 +1814,31 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     // particles: return mapping. A cloth element's F planes hold D, of which F^ updates d3
     // (HybridSolver.cpp:599-603); its d1, d2 and return mapping follow in k_cloth_elements.
     if (kind == KIND_PARTICLE) return_map(P, Fh, plastic, phi_p, vol, K, fb_svd, fb_apex, io);
-    if (io.deferred) {
-      // a candidate of the surrogate: its trial SVD at its sorted slot; the wave's mask below lists it
+    // A deferred particle's trial SVD goes to its record: the wave's deferred lanes write next to
+    // each other from the position of lane 0 (coalesced even where few lanes defer); the wave's mask
+    // below lists the particle, and the list carries the record index.
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t dmask = __ballot(io.deferred), amask = __ballot(io.apex_deferred);
+    if (io.deferred) {  // a candidate of the surrogate
       cand = true;
-      const int64_t n = a.n;
+      const int64_t n = a.n, r = (j - lane) + __popcll(dmask & below);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) a.cand[(CAND_SIGMA + k) * n + j] = io.sigma[k];
+      for (int k = 0; k < 3; ++k) a.cand[(CAND_SIGMA + k) * n + r] = io.sigma[k];
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        a.cand[(CAND_U + k) * n + j] = io.U[k];
-        a.cand[(CAND_V + k) * n + j] = io.V[k];
+        a.cand[(CAND_U + k) * n + r] = io.U[k];
+        a.cand[(CAND_V + k) * n + r] = io.V[k];
       }
     }
-    if (io.apex_deferred) {
-      // an FP64 apex projection for k_apex_finish: the trial SVD at the sorted slot; the wave's mask
-      // below lists it
+    if (io.apex_deferred) {  // an FP64 apex projection for k_apex_finish
       apex = true;
-      const int64_t n = a.n;
+      const int64_t n = a.n, r = (j - lane) + __popcll(amask & below);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) a.apex_usv[(CAND_SIGMA + k) * n + j] = io.sigma[k];
+      for (int k = 0; k < 3; ++k) a.apex_usv[(CAND_SIGMA + k) * n + r] = io.sigma[k];
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        a.apex_usv[(CAND_U + k) * n + j] = io.U[k];
-        a.apex_usv[(CAND_V + k) * n + j] = io.V[k];
+        a.apex_usv[(CAND_U + k) * n + r] = io.U[k];
+        a.apex_usv[(CAND_V + k) * n + r] = io.V[k];
       }
     }
     float xn[3];
@@ -1868,9 +1876,9 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
   if (a.apex_mask || a.sur_mode == 2) {
     const uint64_t am = __ballot(apex), dm = __ballot(cand);
     if (lane == 0) {
-      a.wave_pos[4 * blockIdx.x + wv] = j;
-      if (a.apex_mask) a.apex_mask[4 * blockIdx.x + wv] = am;
-      if (a.sur_mode == 2) a.cand_mask[4 * blockIdx.x + wv] = dm;
+      a.wave_pos[4 * pblk + wv] = j;
+      if (a.apex_mask) a.apex_mask[4 * pblk + wv] = am;
+      if (a.sur_mode == 2) a.cand_mask[4 * pblk + wv] = dm;
     }
   }
   const int flags = (oob ? MPM_STATUS_OUT_OF_DOMAIN : 0) | (bad ? MPM_STATUS_NONFINITE : 0) | (cfl ? MPM_STATUS_CFL : 0);
@@ -1904,17 +1912,18 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
     const int c = c0 + (int)threadIdx.x;
     if (c < count) {
       const int j = clamp_index(a.apex_slot[c], a.n);
+      const int r = clamp_index(a.apex_idx[c], a.n);
       const int sc = scene_of(a.T.part_off, a.T.nscenes, j);
       const DevParams& P = a.T.params[sc];
       float* st = a.st;
       const int64_t cap = a.cap, n = a.n;
       float S[3], U[9], V[9];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) S[k] = a.apex_usv[(CAND_SIGMA + k) * n + j];
+      for (int k = 0; k < 3; ++k) S[k] = a.apex_usv[(CAND_SIGMA + k) * n + r];
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        U[k] = a.apex_usv[(CAND_U + k) * n + j];
-        V[k] = a.apex_usv[(CAND_V + k) * n + j];
+        U[k] = a.apex_usv[(CAND_U + k) * n + r];
+        V[k] = a.apex_usv[(CAND_V + k) * n + r];
       }
       float plastic = st[PPL * cap + j];
       const float phi_p = st[PPHI * cap + j];
@@ -2023,16 +2032,19 @@ __global__ __launch_bounds__(256) void k_list_write(ListArgs a) {
   for (int l = 0; l < a.nlists; ++l)
     for (int i = 16 * wv; i < 16 * wv + 16; ++i) {
       const uint64_t mi = s_m[l][i];
-      if ((mi >> lane) & 1ull) a.list[l][s_off[l][i] + __popcll(mi & ((1ull << lane) - 1ull))] = s_p[i] + lane;
+      if ((mi >> lane) & 1ull) {
+        const int rank = __popcll(mi & ((1ull << lane) - 1ull));
+        a.list[l][s_off[l][i] + rank] = s_p[i] + lane;
+        a.idx[l][s_off[l][i] + rank] = s_p[i] + rank;
+      }
     }
 }
 
 // ================================================================== surrogate return map
-// Cone-branch particles deferred by k_g2p (sur_mode 2) are finished by two kernels.
-// k_surrogate_forward evaluates the MLP for 64 candidates per wave, one per lane, with BF16 MFMA
-// (weights staged once per workgroup in LDS, activations in registers) and keeps the outputs y
-// with the candidate. k_surrogate_apply then gates each prediction H = eps + 1e-3 y of the
-// projected Hencky strains:
+// Cone-branch particles deferred by k_g2p (sur_mode 2) are finished by k_surrogate. Each wave takes
+// 64 listed candidates, one per lane, evaluates the MLP for them with BF16 MFMA (weights staged once
+// per workgroup in LDS, activations in registers) and then gates each prediction H = eps + 1e-3 y
+// of the projected Hencky strains:
 //   yield equality   | |dev H| + c alpha tr H | <= tol       (c = (3 lambda + 2 mu) / (2 mu))
 //   volume           | tr H - tr eps | <= tol                (the cone return is deviatoric)
 //   flow direction   the deviatoric correction eps - H is parallel to dev eps (orthogonal part <= tol)
@@ -2041,12 +2053,91 @@ __global__ __launch_bounds__(256) void k_list_write(ListArgs a) {
 // projection (HybridSolver.cpp:646-673) through the same code as k_g2p, so rejecting every
 // prediction reproduces the exact path bit for bit. With a decision log, candidate row r < dec_cap
 // records (slot bits, eps, phi, K/mu, H, accepted) at dec_log[10 r ..].
-__global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate_forward(SurrogateArgs a) {
+// The gate and the update of a wave's candidates follow its network evaluation in the same kernel:
+// their loads and stores (the trial SVD, the new F, q and stress) overlap the other waves' network
+// arithmetic, which is bound by the SiLU's transcendentals, instead of taking a pass of their own.
+
+// Gate and state update of candidate `row` at sorted slot j (S, U, V: its trial SVD, eps = log S,
+// y: the network's outputs, vol: its volume). Returns whether the prediction was accepted.
+__device__ __forceinline__ bool surrogate_update(const SurrogateArgs& a, int row, int j, const DevParams& P,
+                                                 const float S[3], const float eps[3], const float y[3], float q,
+                                                 float phi_p, float phi, float kmu, const float U[9], const float V[9],
+                                                 float vol) {
+#pragma clang fp contract(off)
+  const int64_t cap = a.cap;
+  const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
+  float H[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) H[k] = eps[k] + MLP_CORR_SCALE * y[k];
+  const float fa = phi * 3.14159265358979323846f / 180.0f;
+  const float sf = sinf(fa);
+  const float alpha = sqrtf(2.0f / 3.0f) * 2.0f * sf / (3.0f - sf);
+  const float trH = H[0] + H[1] + H[2], trE = eps[0] + eps[1] + eps[2];
+  float dH[3], dE[3], d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dH[k] = H[k] - trH / 3.0f;
+    dE[k] = eps[k] - trE / 3.0f;
+  }
+  const float nH = sqrtf(dH[0] * dH[0] + dH[1] * dH[1] + dH[2] * dH[2]);
+  const float nE = sqrtf(dE[0] * dE[0] + dE[1] * dE[1] + dE[2] * dE[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = dE[k] - dH[k];  // deviatoric part of eps - H
+  const float along = (d[0] * dE[0] + d[1] * dE[1] + d[2] * dE[2]) / nE;
+  float ortho2 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float o = d[k] - along * dE[k] / nE;
+    ortho2 += o * o;
+  }
+  const float tol = a.gate_tol;
+  const bool accepted = fabsf(nH + c_yield * alpha * trH) <= tol && fabsf(trH - trE) <= tol && along > 0.0f &&
+                        sqrtf(ortho2) <= tol;
+  if (a.dec_log && row < a.dec_cap) {
+    float* rec = a.dec_log + (int64_t)10 * row;
+    rec[0] = __int_as_float(j);
+    rec[1] = eps[0];
+    rec[2] = eps[1];
+    rec[3] = eps[2];
+    rec[4] = phi;
+    rec[5] = kmu;
+    rec[6] = H[0];
+    rec[7] = H[1];
+    rec[8] = H[2];
+    rec[9] = accepted ? 1.0f : 0.0f;
+  }
+  float Sn[3], lnSn[3], dq;
+  if (accepted) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Sn[k] = expf(H[k]);
+      lnSn[k] = H[k];
+    }
+    dq = sqrtf((eps[0] - H[0]) * (eps[0] - H[0]) + (eps[1] - H[1]) * (eps[1] - H[1]) + (eps[2] - H[2]) * (eps[2] - H[2]));
+  } else {
+    const float phi0 = P.per_particle_phi ? phi_p : P.h0;
+    float lnS[3];
+    sand_project(P, phi0, q, S, Sn, lnS, lnSn, dq);
+  }
+  float Fe[9], K[6];
+  sand_finish(P, U, V, Sn, lnSn, dq, q, vol, Fe, K);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) a.st[(PF + k) * cap + j] = Fe[k];
+  a.st[PPL * cap + j] = q;
+  if (a.write_stress) {
+#pragma unroll
+    for (int e = 0; e < 6; ++e) a.st[(PK + e) * cap + j] = K[e];
+  }
+  return accepted;
+}
+
+__global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate(SurrogateArgs a) {
   if (clock_skip(a.clk)) return;
   extern __shared__ __align__(16) unsigned char smem[];
   MLPShared& sh = *reinterpret_cast<MLPShared*>(smem);
   // the counter is cleared by the sort stage; a G2P stage repeated without it cannot list more than n
   const int count = min(*a.cand_count, a.n);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
   const int nw = blockDim.x >> 6;
   // uniform per workgroup: nothing to do, skip the weight staging
   if ((int64_t)blockIdx.x * nw * 64 >= count) return;
@@ -2054,135 +2145,49 @@ __global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate_forward(Surrogate
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t cap = a.cap, n = a.n;
+  unsigned n_acc = 0;
   for (int g = blockIdx.x * nw + w; (int64_t)g * 64 < count; g += gridDim.x * nw) {
     const int row = g * 64 + lane;
-    float eps[3] = {0.0f, 0.0f, 0.0f}, phi = 0.0f, kmu = 0.0f;
-    int j = 0;
+    float S[3] = {1.0f, 1.0f, 1.0f}, eps[3] = {0.0f, 0.0f, 0.0f}, phi = 0.0f, kmu = 0.0f, q = 0.0f, phi_p = 0.0f;
+    int j = 0, sc = 0;
+    float U[9], V[9], vol = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) U[k] = V[k] = 0.0f;
     if (row < count) {
       j = clamp_index(a.cand_slot[row], (int)n);
-      const DevParams& P = a.T.params[scene_of(a.T.part_off, a.T.nscenes, j)];
+      const int r = clamp_index(a.cand_idx[row], (int)n);
+      sc = scene_of(a.T.part_off, a.T.nscenes, j);
+      const DevParams& P = a.T.params[sc];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) eps[k] = logf(a.cand[(CAND_SIGMA + k) * n + j]);
-      phi = dp_phi(P, a.st[PPHI * cap + j], a.st[PPL * cap + j]);
+      for (int k = 0; k < 3; ++k) {
+        S[k] = a.cand[(CAND_SIGMA + k) * n + r];
+        eps[k] = logf(S[k]);
+      }
+      q = a.st[PPL * cap + j];
+      phi_p = a.st[PPHI * cap + j];
+      phi = dp_phi(P, phi_p, q);
       kmu = bulk_over_shear(P);
+      // what the update needs after the network: these loads complete under its arithmetic
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        U[k] = a.cand[(CAND_U + k) * n + r];
+        V[k] = a.cand[(CAND_V + k) * n + r];
+      }
+      vol = a.st[PVOL * cap + j];
     }
     const float x[5] = {eps[0], eps[1], eps[2], phi, kmu};
     float y[3];
     wave_forward64(sh, x, y, nullptr, 0, 0);
-    if (row < count) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) a.cand[(CAND_Y + k) * n + j] = y[k];
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_surrogate_apply(SurrogateArgs a) {
-#pragma clang fp contract(off)
-  if (clock_skip(a.clk)) return;
-  const int count = min(*a.cand_count, a.n);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) atomicAdd(a.candidates, (unsigned long long)count);
-  const int64_t cap = a.cap, n = a.n;
-  const int lane = threadIdx.x & 63;
-  unsigned n_acc = 0;
-  // whole waves stride over the rows, so the ballot below sees every lane
-  for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) - lane; base < count;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int row = (int)(base + lane);
-    const bool valid = row < count;
-    int j = 0;
-    float S[3] = {1.0f, 1.0f, 1.0f}, eps[3] = {0.0f, 0.0f, 0.0f}, y[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f,
-          phi_p = 0.0f, phi = 0.0f;
-    const DevParams* Pp = &a.T.params[0];
-    if (valid) {
-      j = clamp_index(a.cand_slot[row], (int)n);
-      Pp = &a.T.params[scene_of(a.T.part_off, a.T.nscenes, j)];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        S[k] = a.cand[(CAND_SIGMA + k) * n + j];
-        eps[k] = logf(S[k]);
-        y[k] = a.cand[(CAND_Y + k) * n + j];
-      }
-      q = a.st[PPL * cap + j];
-      phi_p = a.st[PPHI * cap + j];
-      phi = dp_phi(*Pp, phi_p, q);
-    }
-    const DevParams& P = *Pp;
-    const float c_yield = (3.0f * P.lambda0 + 2.0f * P.mu0) / (2.0f * P.mu0);
-    const float kmu = bulk_over_shear(P);
+    // the network's registers are free from here on: keep the update's loads below this point
+    __builtin_amdgcn_sched_barrier(0);
     bool accepted = false;
-    if (valid) {
-      float H[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) H[k] = eps[k] + MLP_CORR_SCALE * y[k];
-      const float fa = phi * 3.14159265358979323846f / 180.0f;
-      const float sf = sinf(fa);
-      const float alpha = sqrtf(2.0f / 3.0f) * 2.0f * sf / (3.0f - sf);
-      const float trH = H[0] + H[1] + H[2], trE = eps[0] + eps[1] + eps[2];
-      float dH[3], dE[3], d[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        dH[k] = H[k] - trH / 3.0f;
-        dE[k] = eps[k] - trE / 3.0f;
-      }
-      const float nH = sqrtf(dH[0] * dH[0] + dH[1] * dH[1] + dH[2] * dH[2]);
-      const float nE = sqrtf(dE[0] * dE[0] + dE[1] * dE[1] + dE[2] * dE[2]);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) d[k] = dE[k] - dH[k];  // deviatoric part of eps - H
-      const float along = (d[0] * dE[0] + d[1] * dE[1] + d[2] * dE[2]) / nE;
-      float ortho2 = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float o = d[k] - along * dE[k] / nE;
-        ortho2 += o * o;
-      }
-      const float tol = a.gate_tol;
-      accepted = fabsf(nH + c_yield * alpha * trH) <= tol && fabsf(trH - trE) <= tol && along > 0.0f &&
-                 sqrtf(ortho2) <= tol;
-      if (a.dec_log && row < a.dec_cap) {
-        float* rec = a.dec_log + (int64_t)10 * row;
-        rec[0] = __int_as_float(j);
-        rec[1] = eps[0];
-        rec[2] = eps[1];
-        rec[3] = eps[2];
-        rec[4] = phi;
-        rec[5] = kmu;
-        rec[6] = H[0];
-        rec[7] = H[1];
-        rec[8] = H[2];
-        rec[9] = accepted ? 1.0f : 0.0f;
-      }
-      float U[9], V[9], Sn[3], lnSn[3], dq;
-#pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        U[k] = a.cand[(CAND_U + k) * n + j];
-        V[k] = a.cand[(CAND_V + k) * n + j];
-      }
-      if (accepted) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          Sn[k] = expf(H[k]);
-          lnSn[k] = H[k];
-        }
-        dq = sqrtf((eps[0] - H[0]) * (eps[0] - H[0]) + (eps[1] - H[1]) * (eps[1] - H[1]) + (eps[2] - H[2]) * (eps[2] - H[2]));
-      } else {
-        const float phi0 = P.per_particle_phi ? phi_p : P.h0;
-        float lnS[3];
-        sand_project(P, phi0, q, S, Sn, lnS, lnSn, dq);
-      }
-      float Fe[9], K[6];
-      const float vol = a.st[PVOL * cap + j];
-      sand_finish(P, U, V, Sn, lnSn, dq, q, vol, Fe, K);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) a.st[(PF + k) * cap + j] = Fe[k];
-      a.st[PPL * cap + j] = q;
-      if (a.write_stress) {
-#pragma unroll
-        for (int e = 0; e < 6; ++e) a.st[(PK + e) * cap + j] = K[e];
-      }
-    }
+    if (row < count)
+      accepted = surrogate_update(a, row, j, a.T.params[sc], S, eps, y, q, phi_p, phi, kmu, U, V, vol);
     n_acc += (unsigned)__popcll(__ballot(accepted));
+    __builtin_amdgcn_sched_barrier(0);
   }
-  if (lane == 0 && n_acc) atomicAdd(a.accepted, (unsigned long long)n_acc);  // one atomic per wave
+  // one atomic per wave
+  if (lane == 0 && n_acc) atomicAdd(a.accepted, (unsigned long long)n_acc);
 }
 
 // ================================================================== cloth
@@ -2614,7 +2619,7 @@ void launch_copy(hipStream_t s, const float4* src, float4* dst, int64_t n4) {
 static size_t surrogate_smem() { return sizeof(MLPShared); }
 
 void surrogate_set_smem_limit() {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_surrogate_forward),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_surrogate),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)surrogate_smem());
 }
 
@@ -2627,14 +2632,11 @@ void launch_position_lists(hipStream_t s, const ListArgs& a) {
 }
 
 void launch_surrogate(hipStream_t s, const SurrogateArgs& a, int n_cu, hipEvent_t* ev) {
-  if (ev && ev[0]) {  // timed launches: each dispatch records its own start and end
-    hipExtLaunchKernelGGL(k_surrogate_forward, dim3(n_cu), dim3(MLP_FWD_THREADS), surrogate_smem(), s, ev[0], ev[1], 0,
-                          a);
-    hipExtLaunchKernelGGL(k_surrogate_apply, dim3(4 * n_cu), dim3(256), 0, s, ev[2], ev[3], 0, a);
+  if (ev && ev[0]) {  // a timed launch: the dispatch records its own start and end
+    hipExtLaunchKernelGGL(k_surrogate, dim3(n_cu), dim3(MLP_FWD_THREADS), surrogate_smem(), s, ev[0], ev[1], 0, a);
     return;
   }
-  hipLaunchKernelGGL(k_surrogate_forward, dim3(n_cu), dim3(MLP_FWD_THREADS), surrogate_smem(), s, a);
-  hipLaunchKernelGGL(k_surrogate_apply, dim3(4 * n_cu), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_surrogate, dim3(n_cu), dim3(MLP_FWD_THREADS), surrogate_smem(), s, a);
 }
 
 void launch_apex_finish(hipStream_t s, const ApexArgs& a, int grid) {

@@ -164,10 +164,12 @@ struct mpm_ctx {
   float gate_tol = 1e-5f;
   float* cand = nullptr;
   int* cand_slot = nullptr;
+  int* cand_idx = nullptr;  // the listed candidates' record indices in cand
   // set by mpm_time_kernels: the start and end of a stage's main kernel (taken by its launch)
   hipEvent_t ev_main[2] = {nullptr, nullptr};
-  hipEvent_t ev_sur[4] = {nullptr, nullptr, nullptr, nullptr};  // surrogate forward, apply: start, stop
+  hipEvent_t ev_sur[2] = {nullptr, nullptr};  // the surrogate kernel's start and stop (timing only)
   int* apex_slot = nullptr;  // sorted slots of the deferred FP64 apex projections (n)
+  int* apex_idx = nullptr;    // their record indices in apex_usv (n)
   float* apex_usv = nullptr;  // their trial SVD (CAND_SVD_PLANES x n)
 };
 
@@ -561,11 +563,13 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     if (a.apex_mask) {
       la.mask[la.nlists] = ctx->apex_mask;
       la.list[la.nlists] = ctx->apex_slot;
+      la.idx[la.nlists] = ctx->apex_idx;
       la.count[la.nlists++] = ctx->counters + CNT_APEX;
     }
     if (surrogate) {
       la.mask[la.nlists] = ctx->cand_mask;
       la.list[la.nlists] = ctx->cand_slot;
+      la.idx[la.nlists] = ctx->cand_idx;
       la.count[la.nlists++] = ctx->counters + CNT_CAND;
     }
     launch_position_lists(ctx->stream, la);
@@ -577,6 +581,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     x.cap = ctx->cap;
     x.n = ctx->n;
     x.apex_slot = ctx->apex_slot;
+    x.apex_idx = ctx->apex_idx;
     x.apex_usv = ctx->apex_usv;
     x.apex_count = ctx->counters + CNT_APEX;
     x.write_stress = a.write_stress;
@@ -599,6 +604,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
     sa.n = ctx->n;
     sa.cand = ctx->cand;
     sa.cand_slot = ctx->cand_slot;
+    sa.cand_idx = ctx->cand_idx;
     sa.cand_count = ctx->counters + CNT_CAND;
     sa.gate_tol = ctx->gate_tol;
     sa.write_stress = a.write_stress;
@@ -741,11 +747,11 @@ int mpm_destroy(mpm_ctx* ctx) {
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
   dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
-  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws); dfree(ctx->apex_slot); dfree(ctx->apex_usv);
+  dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws); dfree(ctx->apex_slot); dfree(ctx->apex_idx); dfree(ctx->apex_usv);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   dfree(ctx->clock);
   dfree(ctx->mlp_master); dfree(ctx->mlp_m1); dfree(ctx->mlp_m2); dfree(ctx->mlp_grad); dfree(ctx->mlp_wbf);
-  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->dec_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot); dfree(ctx->cand_mask); dfree(ctx->wave_pos); dfree(ctx->apex_mask); dfree(ctx->list_gsum);
+  dfree(ctx->mlp_wbft); dfree(ctx->pair_log); dfree(ctx->dec_log); dfree(ctx->sur_counts); dfree(ctx->cand); dfree(ctx->cand_slot); dfree(ctx->cand_idx); dfree(ctx->cand_mask); dfree(ctx->wave_pos); dfree(ctx->apex_mask); dfree(ctx->list_gsum);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPM_OK;
@@ -956,9 +962,11 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
     if (!rc) rc = dalloc(ctx, ctx->v2, n);
     if (!rc) rc = dalloc(ctx, ctx->src, n);
     if (!rc) rc = dalloc(ctx, ctx->apex_slot, n);
+    if (!rc) rc = dalloc(ctx, ctx->apex_idx, n);
     if (!rc) rc = dalloc(ctx, ctx->apex_usv, (size_t)CAND_SVD_PLANES * n);
-    if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * n);
+    if (!rc && (ctx->sur_enabled || ctx->cand)) rc = dalloc(ctx, ctx->cand, (size_t)CAND_SVD_PLANES * n);
     if (!rc && (ctx->sur_enabled || ctx->cand_slot)) rc = dalloc(ctx, ctx->cand_slot, (size_t)n);
+    if (!rc && (ctx->sur_enabled || ctx->cand_idx)) rc = dalloc(ctx, ctx->cand_idx, (size_t)n);
     if (!rc && ctx->wave_cap < 4 * (int64_t)ctx->nblocks_p) rc = alloc_wave_tables(ctx);
     if (rc) return rc;
     CK(hipMemsetAsync(ctx->state[1], 0, sizeof(float) * NPLANES * (size_t)n, ctx->stream));
@@ -1702,7 +1710,7 @@ int mpm_synchronize(mpm_ctx* ctx) {
 }
 
 // Eagerly launched substeps with HIP events around every stage, around the G2P main kernel and,
-// when sur_ms is given and the surrogate is on, around its two kernels (sur_ms[0] forward, [1] apply).
+// when sur_ms is given and the surrogate is on, around its kernel (*sur_ms).
 static int time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms,
                         float* sur_ms) {
   int rc = ready(ctx);
@@ -1732,7 +1740,7 @@ static int time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms,
       }
     }
   } clear{ctx};
-  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0}, g2p_main = 0.0f, sacc[2] = {0.0f, 0.0f};
+  float acc[MPM_NUM_STAGES] = {0, 0, 0, 0}, g2p_main = 0.0f, sacc = 0.0f;
   CK(hipEventRecord(t0, ctx->stream));
   for (int i = 0; i < nsteps; ++i) {
     CK(hipEventRecord(ev[0], ctx->stream));
@@ -1750,11 +1758,10 @@ static int time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms,
     float ms = 0;
     CK(hipEventElapsedTime(&ms, ctx->ev_main[0], ctx->ev_main[1]));
     g2p_main += ms;
-    if (sur)
-      for (int k = 0; k < 2; ++k) {
-        CK(hipEventElapsedTime(&ms, ctx->ev_sur[2 * k], ctx->ev_sur[2 * k + 1]));
-        sacc[k] += ms;
-      }
+    if (sur) {
+      CK(hipEventElapsedTime(&ms, ctx->ev_sur[0], ctx->ev_sur[1]));
+      sacc += ms;
+    }
   }
   CK(hipEventRecord(t1, ctx->stream));
   CK(hipEventSynchronize(t1));
@@ -1767,10 +1774,7 @@ static int time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms,
     for (int st = 0; st < MPM_NUM_STAGES; ++st) kernel_ms[st] = acc[st];
     kernel_ms[MPM_STAGE_G2P] = g2p_main;
   }
-  if (sur_ms) {
-    sur_ms[0] = sacc[0];
-    sur_ms[1] = sacc[1];
-  }
+  if (sur_ms) *sur_ms = sacc;
   for (auto& e : ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
@@ -1782,20 +1786,18 @@ int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, fl
   return time_kernels(ctx, dt, nsteps, total_ms, stage_ms, kernel_ms, nullptr);
 }
 
-int mpm_time_surrogate(mpm_ctx* ctx, float dt, int32_t nsteps, float* forward_ms, float* apply_ms,
-                       int64_t* candidates) {
-  if (!ctx || !forward_ms || !apply_ms) return MPM_ERR_INVALID;
+int mpm_time_surrogate(mpm_ctx* ctx, float dt, int32_t nsteps, float* kernel_ms, int64_t* candidates) {
+  if (!ctx || !kernel_ms) return MPM_ERR_INVALID;
   if (!ctx->sur_enabled) return fail(ctx, MPM_ERR_STATE, "the surrogate is not enabled");
   unsigned long long c0 = 0, c1 = 0;
   CK(hipSetDevice(ctx->device));
   CK(hipStreamSynchronize(ctx->stream));
   CK(hipMemcpy(&c0, ctx->sur_counts, sizeof(c0), hipMemcpyDeviceToHost));
-  float ms[2] = {0.0f, 0.0f};
-  int rc = time_kernels(ctx, dt, nsteps, nullptr, nullptr, nullptr, ms);
+  float ms = 0.0f;
+  int rc = time_kernels(ctx, dt, nsteps, nullptr, nullptr, nullptr, &ms);
   if (rc) return rc;
   CK(hipMemcpy(&c1, ctx->sur_counts, sizeof(c1), hipMemcpyDeviceToHost));
-  *forward_ms = ms[0];
-  *apply_ms = ms[1];
+  *kernel_ms = ms;
   if (candidates) *candidates = (int64_t)(c1 - c0);
   return MPM_OK;
 }
@@ -2128,8 +2130,9 @@ int mpm_set_surrogate(mpm_ctx* ctx, int32_t enable, float gate_tol) {
   CK(hipStreamSynchronize(ctx->stream));
   drop_graphs(ctx);
   if (enable && ctx->n > 0 && !ctx->cand) {
-    int rc = dalloc(ctx, ctx->cand, (size_t)CAND_PLANES * ctx->n);
+    int rc = dalloc(ctx, ctx->cand, (size_t)CAND_SVD_PLANES * ctx->n);
     if (!rc) rc = dalloc(ctx, ctx->cand_slot, (size_t)ctx->n);
+    if (!rc) rc = dalloc(ctx, ctx->cand_idx, (size_t)ctx->n);
     if (rc) return rc;
   }
   if (ctx->n > 0 && ctx->wave_cap < 4 * (int64_t)ctx->nblocks_p) {

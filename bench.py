@@ -52,7 +52,7 @@ ALG_PER_NODE = {"sort": 0, "p2g": 16, "grid": 40, "g2p": 24}
 SUBSTEP_PER_PARTICLE, SUBSTEP_PER_NODE = 208, 56
 KERNEL_NAMES = {"p2g": "k_p2g<2, P2G_MOM_STRESS>", "g2p": "k_g2p<2>", "grid": "k_grid<GRID_FUSED>",
                 "sort": "k_cell_count + k_cell_scan + k_cell_scatter + k_cell_rank + k_claim",
-                "surrogate": "k_surrogate_forward"}
+                "surrogate": "k_surrogate"}
 # flops of one network evaluation: 2 x (5x128 + 3 x 128x128 + 128x3) multiply-adds
 MLP_FLOPS = 2 * (5 * 128 + 3 * 128 * 128 + 128 * 3)
 
@@ -153,9 +153,15 @@ def log_pairs(s, dt, total, chunks, per_chunk):
     return np.concatenate(pairs) if pairs else np.zeros((0, 8), np.float32)
 
 
-def train_surrogate(target_pairs=10_000_000, epochs=4, batch=4096, lr=1e-3):
+# Adam learning rate by stage: 4 epochs each at 1e-3, 3e-4 and 1e-4. Against 4 epochs at 1e-3 alone
+# this raised the gate's acceptance at config 4 (after 700 substeps) from 64% to 88%.
+LR_SCHEDULE = ((4, 1e-3), (4, 3e-4), (4, 1e-4))
+
+
+def train_surrogate(target_pairs=10_000_000, schedule=LR_SCHEDULE, batch=4096):
     """Pairs logged by the exact path in a config-2-shaped sand column with per-particle angles and in
-    a batch of 64 mixed scenes (other seeds than the measured workloads), then Adam on the GPU."""
+    a batch of 64 mixed scenes (other seeds than the measured workloads), then Adam on the GPU with a
+    stepped learning rate."""
     from anisotropicelastoplasticity_amd import Solver, scenes
     t0 = time.perf_counter()
     col = scenes.surrogate_training_column()
@@ -171,12 +177,16 @@ def train_surrogate(target_pairs=10_000_000, epochs=4, batch=4096, lr=1e-3):
     with Solver(0) as s:
         t1 = time.perf_counter()
         s.surrogate_init(1)
-        losses = s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch, lr=lr, seed=2)
+        losses = []
+        for epochs, lr in schedule:
+            losses += [float(x) for x in s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=epochs, batch=batch,
+                                                           lr=lr, seed=2)]
         s.synchronize()
         train_s = time.perf_counter() - t1
         w = s.surrogate_get_weights()
     info = {"pairs": int(len(pairs)), "pairs_column": int(len(p_col)), "pairs_batch": int(len(p_bat)),
-            "epochs": epochs, "batch": batch, "loss_first_last": [float(losses[0]), float(losses[-1])],
+            "epochs": sum(e for e, _ in schedule), "lr_schedule": [list(x) for x in schedule], "batch": batch,
+            "loss_first_last": [losses[0], losses[-1]],
             "log_s": log_s, "train_s": train_s,
             "sources": "surrogate_training_column (seed 41, 1000 substeps) + config5_batch(seed=55) (500 substeps)"}
     return w, info
@@ -210,8 +220,8 @@ def timed_run(sc, total, steps, weights=None, gate_tol=1e-5, load_checkpoint=Non
         stages = {k: stage_ms[i] / kt for i, k in enumerate(STAGES)}
         sur = None
         if weights is not None:
-            f_ms, a_ms, cand = s.time_surrogate(sc.dt, kt)
-            sur = {"forward_ms": f_ms / kt, "apply_ms": a_ms / kt, "candidates": cand / kt}
+            k_ms, cand = s.time_surrogate(sc.dt, kt)
+            sur = {"kernel_ms": k_ms / kt, "candidates": cand / kt}
         st_after = s.stats()
     return dict(wall=wall, x=x, stats=st, kernel_ms=kern, stage_ms=stages, sur=sur,
                 active_nodes=st_after["n_active_nodes"])
@@ -222,7 +232,7 @@ def roofline_of(kernel_ms, n, nodes, sur, per_particle_phi, workload):
     kernel by algorithmic flops."""
     cands = {k: kernel_ms[k] for k in ("p2g", "grid", "g2p")}
     if sur is not None:
-        cands["surrogate"] = sur["forward_ms"]
+        cands["surrogate"] = sur["kernel_ms"]
     dom = max(cands, key=cands.get)
     ms = cands[dom]
     if dom == "surrogate":
@@ -266,11 +276,12 @@ def headline(args, weights, train_info):
         "candidates_per_substep": cand / total, "acceptance": (st["surrogate_accepted"] / cand) if cand else 0.0,
         "fallback_rate": 1.0 - ((st["surrogate_accepted"] / cand) if cand else 0.0),
         "kernel_ms": su["kernel_ms"], "stage_ms": su["stage_ms"],
-        "surrogate_kernels": {"forward_ms": sur["forward_ms"], "apply_ms": sur["apply_ms"],
-                              "candidates_per_launch": sur["candidates"],
-                              "bf16_tflops": sur_flops / (sur["forward_ms"] * 1e-3) / 1e12 if sur["forward_ms"] else None,
-                              "bf16_frac": (sur_flops / (sur["forward_ms"] * 1e-3) / 1e12) / BF16_PEAK_TFLOPS
-                              if sur["forward_ms"] else None},
+        "surrogate_kernel": {"kernel": KERNEL_NAMES["surrogate"], "avg_launch_ms": sur["kernel_ms"],
+                             "candidates_per_launch": sur["candidates"],
+                             "bf16_tflops": sur_flops / (sur["kernel_ms"] * 1e-3) / 1e12 if sur["kernel_ms"] else None,
+                             "bf16_frac": (sur_flops / (sur["kernel_ms"] * 1e-3) / 1e12) / BF16_PEAK_TFLOPS
+                             if sur["kernel_ms"] else None,
+                             "traffic": load_traffic("surrogate", "config4")},
         "roofline": roofline_of(su["kernel_ms"], sc.n, su["active_nodes"], sur, True, "config4"),
         "active_nodes": su["active_nodes"], "status": st["status"], "train": train_info,
     }
@@ -441,7 +452,7 @@ def main():
             r = timed_run(sc, args.warmup + args.steps, args.steps, weights=w, load_checkpoint=args.load_checkpoint,
                           save_checkpoint=args.save_checkpoint)
             leg = {"workload": args.only, "value": sc.n * args.steps / r["wall"], "kernel_ms": r["kernel_ms"],
-                   "surrogate_kernels": r["sur"], "status": r["stats"]["status"]}
+                   "surrogate_kernel": r["sur"], "status": r["stats"]["status"]}
         print(json.dumps(leg))
         return
 
@@ -483,7 +494,7 @@ def main():
         "fallback_rate": h["fallback_rate"],
         "kernel_ms": h["kernel_ms"],
         "stage_ms": h["stage_ms"],
-        "surrogate_kernels": h["surrogate_kernels"],
+        "surrogate_kernel": h["surrogate_kernel"],
         "active_nodes": h["active_nodes"],
         "status": h["status"],
         "train": h["train"],

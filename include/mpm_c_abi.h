@@ -306,11 +306,9 @@ int mpm_time_substeps(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, f
  * stages, the stage time. No counterpart in the reference. */
 int mpm_time_kernels(mpm_ctx* ctx, float dt, int32_t nsteps, float* total_ms, float* stage_ms, float* kernel_ms);
 /* Measurement helper with no counterpart in the reference: nsteps eagerly launched substeps with the
- * surrogate enabled; forward_ms and apply_ms receive the summed dispatch times (HIP events) of the
- * network kernel (k_surrogate_forward) and the gate kernel (k_surrogate_apply), candidates the
- * number of candidates they processed. */
-int mpm_time_surrogate(mpm_ctx* ctx, float dt, int32_t nsteps, float* forward_ms, float* apply_ms,
-                       int64_t* candidates);
+ * surrogate enabled; kernel_ms receives the summed dispatch times (HIP events) of the surrogate kernel
+ * (k_surrogate: network, gate and state update), candidates the number of candidates it processed. */
+int mpm_time_surrogate(mpm_ctx* ctx, float dt, int32_t nsteps, float* kernel_ms, int64_t* candidates);
 
 /* ---- learned return-map surrogate (no counterpart in the reference; replaces the cone branch of
  * the Drucker-Prager projection, HybridSolver.cpp:667-673, behind a gate that falls back to it) ----
