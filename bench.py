@@ -123,6 +123,12 @@ def cpu_baseline(sc):
     return out
 
 
+def progress(msg):
+    """One line per finished leg on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg} ({time.perf_counter() - T_START:.0f} s)", file=sys.stderr, flush=True)
+
+
+T_START = time.perf_counter()
 EAGER = False  # --eager: every substep launched eagerly (mpm_substep), no graph replay
 
 
@@ -457,10 +463,15 @@ def main():
         return
 
     weights, train_info = train_surrogate(target_pairs=args.train_pairs)
+    progress("network trained")
     sc, h = headline(args, weights, train_info)
+    progress("headline (config 4, surrogate and exact) done")
     c2 = None if args.no_config2 else config2_leg(args.config2_steps, 20)
+    progress("config 2 done")
     bat = None if args.no_batch else batched_leg(weights, args.batch_steps, 20)
+    progress("batched scenes done")
     c3 = None if args.no_config3 else config3_leg(args.config3_steps, 20)
+    progress("config 3 done")
     copy = copy_calibration() if args.copy_calibration else None
 
     out = {
@@ -509,6 +520,7 @@ def main():
         out["copy_calibration"] = copy
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc)
+        progress("CPU baseline done")
     print(json.dumps(out), flush=True)
 
 
