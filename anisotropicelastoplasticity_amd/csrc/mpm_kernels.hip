@@ -1962,9 +1962,10 @@ __global__ __launch_bounds__(256) void k_apex_finish(ApexArgs a) {
 // Lists of the sorted positions that G2P flagged, in ascending order and without atomics: the
 // deferred FP64 apex projections and the surrogate's candidates (two lists, one set of launches).
 // Every G2P wave w stores the ballots of its flagged lanes (mask[l][w]) and the position of its
-// lane 0 (wpos[w]). Waves form groups of 64: k_list_sums totals each group, k_list_scan turns the
-// group totals into offsets (one workgroup) and writes the list lengths, and k_list_write places
-// each wave's positions after its group's offset and the waves before it in the group.
+// lane 0 (wpos[w]). Waves form groups of 64: k_list_sums totals each group; k_list_write (one
+// workgroup per group) adds up the totals of the groups before its own, places each wave's
+// positions after that offset and the waves before it in the group, and the last group's
+// workgroup writes the list lengths.
 __global__ __launch_bounds__(256) void k_list_sums(ListArgs a) {
   const int ng = (a.nwaves + 63) / 64;
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1978,42 +1979,24 @@ __global__ __launch_bounds__(256) void k_list_sums(ListArgs a) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_list_scan(ListArgs a) {
-  __shared__ int s_sum[1024];
-  const int ng = (a.nwaves + 63) / 64;
-  const int t = threadIdx.x;
-  const int per = (ng + 1023) / 1024;
-  const int b0 = min(t * per, ng), b1 = min(b0 + per, ng);
-  for (int l = 0; l < a.nlists; ++l) {
-    int* gs = a.gsum + l * ng;
-    int sum = 0;
-    for (int i = b0; i < b1; ++i) sum += gs[i];
-    s_sum[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int v = t >= off ? s_sum[t - off] : 0;
-      __syncthreads();
-      s_sum[t] += v;
-      __syncthreads();
-    }
-    int run = s_sum[t] - sum;  // exclusive offset of this thread's groups
-    for (int i = b0; i < b1; ++i) {
-      const int v = gs[i];
-      gs[i] = run;
-      run += v;
-    }
-    if (t == 1023) *a.count[l] = s_sum[1023];
-    __syncthreads();
-  }
-}
-
 __global__ __launch_bounds__(256) void k_list_write(ListArgs a) {
   __shared__ uint64_t s_m[2][64];
-  __shared__ int s_off[2][64], s_p[64];
+  __shared__ int s_off[2][64], s_p[64], s_red[2][4];
   const int ng = (a.nwaves + 63) / 64;
   const int g = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int w = g * 64 + lane;
+  // each list's offset of this group: the totals of the groups before it, summed by the workgroup
+  // itself (integer sums, so the order does not matter)
+  for (int l = 0; l < a.nlists; ++l) {
+    int sum = 0;
+    for (int i = t; i < g; i += 256) sum += a.gsum[l * ng + i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) s_red[l][wv] = sum;
+  }
+  __syncthreads();
   if (wv < a.nlists) {  // wave l: the offsets of list l's 64 masks of the group
+    const int pre = s_red[wv][0] + s_red[wv][1] + s_red[wv][2] + s_red[wv][3];
     const uint64_t m = w < a.nwaves ? a.mask[wv][w] : 0ull;
     const int c = __popcll(m);
     int x = c;
@@ -2023,7 +2006,9 @@ __global__ __launch_bounds__(256) void k_list_write(ListArgs a) {
       if (lane >= off) x += y;
     }
     s_m[wv][lane] = m;
-    s_off[wv][lane] = a.gsum[wv * ng + g] + x - c;
+    s_off[wv][lane] = pre + x - c;
+    // the last group's workgroup writes the list's length
+    if (g == ng - 1 && lane == 63) *a.count[wv] = pre + x;
   } else if (wv == 3) {
     s_p[lane] = w < a.nwaves ? a.wpos[w] : 0;
   }
@@ -2627,7 +2612,6 @@ void launch_position_lists(hipStream_t s, const ListArgs& a) {
   if (a.nwaves <= 0 || a.nlists <= 0) return;
   const int ng = (a.nwaves + 63) / 64;
   hipLaunchKernelGGL(k_list_sums, dim3((ng + 3) / 4), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, a);
   hipLaunchKernelGGL(k_list_write, dim3(ng), dim3(256), 0, s, a);
 }
 
