@@ -14,6 +14,13 @@ constexpr float MLP_KMU_MID = 2.5f, MLP_KMU_SPAN = 1.0f;
 constexpr float MLP_CORR_SCALE = 1e-3f;
 // forward kernels: 8 waves of 64 rows share one LDS copy of the weights (one workgroup per CU)
 constexpr int MLP_FWD_THREADS = 512;
+// The forward pass works on pre-activations scaled by log2(e): the forward copy of the first layer's
+// weights and the staged biases of layers 1-4 are multiplied by log2(e), so the MFMA yields
+// z' = log2(e) z and SiLU is z' / (1 + 2^-z') = log2(e) silu(z) with no multiply before the exp2;
+// the next layer's unscaled weights then give log2(e) times its pre-activation again, and the
+// forward copy of the output layer's weights carries the ln 2 that undoes the factor. The backward
+// pass works unscaled (its transposed weights are not scaled; the saved pre-activations are z).
+constexpr float MLP_LOG2E = 1.44269504088896341f, MLP_LN2 = 0.693147180559945309f;
 
 // Master (FP32) layout: for l = 0..4, W_l [N_l][K_l] then b_l [N_l];
 // N = (128,128,128,128,16), K = (32,128,128,128,128).

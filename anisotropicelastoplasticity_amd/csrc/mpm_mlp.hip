@@ -207,9 +207,9 @@ __device__ __forceinline__ void mlp_store_bf16(int i, float wv, uint16_t* wbf, u
     if (i >= w0 && i < b0) {
       const int K = mlp_K(l), o = (i - w0) / K, k = (i - w0) % K;
       const int fo = l == 0 ? MLP_BF_W1 : (l == 4 ? MLP_BF_W5 : MLP_BF_WH + (l - 1) * MLP_HIDDEN * MLP_HIDDEN);
-      const uint16_t h = f2bf(wv);
-      wbf[fo + o * K + k] = h;
-      if (l >= 1) wbft[mlp_t_offset(l) + k * (l == 4 ? 32 : MLP_HIDDEN) + o] = h;
+      // forward copy in the log2(e)-scaled convention (MLP_LOG2E); the backward's transposed copy unscaled
+      wbf[fo + o * K + k] = f2bf(l == 0 ? wv * MLP_LOG2E : (l == 4 ? wv * MLP_LN2 : wv));
+      if (l >= 1) wbft[mlp_t_offset(l) + k * (l == 4 ? 32 : MLP_HIDDEN) + o] = f2bf(wv);
       return;
     }
   }

@@ -24,6 +24,8 @@ __device__ __forceinline__ float sigmoid_fast(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
 }
 __device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
+// log2(e) silu(z) from z' = log2(e) z (the forward pass's scaled pre-activations)
+__device__ __forceinline__ float silu_log2(float zs) { return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-zs)); }
 __device__ __forceinline__ float dsilu(float z) {
   const float s = sigmoid_fast(z);
   return s * (1.0f + z * (1.0f - s));
@@ -68,7 +70,8 @@ static __device__ void stage_weights(MLPShared& s, const uint16_t* __restrict__ 
     stage_rows(s.wh[l], LDW_H, wbf + MLP_BF_WH + l * MLP_HIDDEN * MLP_HIDDEN, MLP_HIDDEN, MLP_HIDDEN);
   stage_rows(s.w5, LDW_H, wbf + MLP_BF_W5, MLP_OUT_PAD, MLP_HIDDEN);
   for (int l = 0; l < 4; ++l)
-    for (int i = threadIdx.x; i < MLP_HIDDEN; i += blockDim.x) s.b[l * MLP_HIDDEN + i] = master[mlp_bias_offset(l) + i];
+    for (int i = threadIdx.x; i < MLP_HIDDEN; i += blockDim.x)
+      s.b[l * MLP_HIDDEN + i] = master[mlp_bias_offset(l) + i] * MLP_LOG2E;
   for (int i = threadIdx.x; i < MLP_OUT_PAD; i += blockDim.x) s.b[4 * MLP_HIDDEN + i] = master[mlp_bias_offset(4) + i];
 }
 
@@ -94,8 +97,9 @@ __device__ __forceinline__ void wave_gemm_nt(const uint16_t* A, int lda, const u
 // Z^T = W A^T on MFMA with the staged weights as the A operand (16 output features x 32 inputs)
 // and the activations as the B operand (32 inputs x 16 samples), for the wave's four 16-sample
 // tiles T: lane i of tile T holds output features 16 t + 4 (i / 16) + 0..3 of sample 16 T + i % 16,
-// which SiLU and BF16 rounding turn into the next layer's operand in place. When zsave is not null,
-// the FP32 pre-activations of sample r = row0 + L < nrows go to zsave[(r / 16) * 4 + layer][r % 16][128].
+// which SiLU and BF16 rounding turn into the next layer's operand in place (pre-activations and
+// activations scaled by log2(e), see MLP_LOG2E). When zsave is not null, the FP32 pre-activations z
+// (unscaled) of sample r = row0 + L < nrows go to zsave[(r / 16) * 4 + layer][r % 16][128].
 static __device__ void wave_forward64(const MLPShared& s, const float x[5], float y[3], float* __restrict__ zsave,
                                       int64_t row0, int64_t nrows) {
   const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
@@ -159,7 +163,7 @@ static __device__ void wave_forward64(const MLPShared& s, const float x[5], floa
           if (r < nrows) {
             float* zr = zsave + (((r >> 4) * 4 + l) * 16 + (r & 15)) * MLP_HIDDEN + 64 * h + 4 * q;
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) *reinterpret_cast<f32x4*>(zr + 16 * tt) = acc[T][tt];
+            for (int tt = 0; tt < 4; ++tt) *reinterpret_cast<f32x4*>(zr + 16 * tt) = acc[T][tt] * MLP_LN2;
           }
         }
       }
@@ -170,8 +174,8 @@ static __device__ void wave_forward64(const MLPShared& s, const float x[5], floa
           bf16x8 o;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            o[j] = (__bf16)silu(acc[T][2 * k][j]);
-            o[4 + j] = (__bf16)silu(acc[T][2 * k + 1][j]);
+            o[j] = (__bf16)silu_log2(acc[T][2 * k][j]);
+            o[4 + j] = (__bf16)silu_log2(acc[T][2 * k + 1][j]);
           }
           nb[T][2 * h + k] = o;
         }

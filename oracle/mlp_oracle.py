@@ -13,6 +13,11 @@ normalised plastic correction, H = eps + 1e-3 * y.
 backward pass, dZ and A_prev operands rounded to BF16 with round-to-nearest-even; accumulation and
 pre-activations kept wide), so the kernels can be compared with a tight tolerance; `emulate_bf16=False`
 is the plain float64 network whose gradient is pinned by finite differences in the tests.
+The kernels' forward pass works on pre-activations scaled by log2(e) (csrc/mpm_mlp.h MLP_LOG2E):
+its BF16 weights are log2(e) W_0, W_1..W_3 and ln(2) W_4, its biases log2(e) b_0..b_3 and b_4, and
+its BF16 activations log2(e) silu(z); the backward pass recomputes unscaled activations
+bf16(silu(z)) from the saved pre-activations z = ln(2) z'. With emulate_bf16=True both are restated
+that way (the same function as the plain network up to the roundings).
 The projection the network learns is the Drucker-Prager cone return of HybridSolver.cpp:646-673
 (oracle/mpm_oracle.cpp dp_project).
 """
@@ -94,21 +99,39 @@ def _rb(x, emulate):
     return bf16(x).astype(np.float64) if emulate else np.asarray(x, dtype=np.float64)
 
 
+LOG2E, LN2 = np.float32(1.44269504088896341), np.float32(0.693147180559945309)
+
+
+def _f32(x):
+    return np.asarray(x, dtype=np.float32).astype(np.float64)
+
+
 def forward(w, inp, emulate_bf16=True):
-    """Normalised outputs y (n, 3) and the cache (a_0..a_4, z_1..z_4) for backprop."""
+    """Normalised outputs y (n, 3) and the cache (a_0..a_4, z_1..z_4) for backprop: with
+    emulate_bf16, the activations the kernels' backward pass uses and the pre-activations it saves."""
     layers = unpack(w)
     a = _rb(normalise(inp), emulate_bf16)
     acts, zs = [a], []
+    if not emulate_bf16:
+        for l in range(4):
+            W, b = layers[l]
+            z = a @ W.T + b
+            zs.append(z)
+            a = silu(z)
+            acts.append(a)
+        W, b = layers[4]
+        return (a @ W.T + b)[:, :OUT], (acts, zs)
     for l in range(4):
         W, b = layers[l]
-        z = a @ _rb(W, emulate_bf16).T + b
-        if emulate_bf16:
-            z = z.astype(np.float32).astype(np.float64)
+        Wf = W.astype(np.float32) * LOG2E if l == 0 else W
+        bs = _f32(b.astype(np.float32) * LOG2E)
+        zp = _f32(a @ _rb(Wf, True).T + bs)  # log2(e) z, accumulated in FP32
+        z = _f32(zp * LN2)                   # what the forward saves for the backward
         zs.append(z)
-        a = _rb(silu(z), emulate_bf16)
-        acts.append(a)
+        a = _rb(zp / (1.0 + np.exp2(-zp)), True)  # log2(e) silu(z), the next layer's operand
+        acts.append(_rb(silu(z), True))           # the backward's recomputed activation
     W, b = layers[4]
-    y = a @ _rb(W, emulate_bf16).T + b
+    y = a @ _rb(W.astype(np.float32) * LN2, True).T + b
     return y[:, :OUT], (acts, zs)
 
 
