@@ -152,7 +152,9 @@ struct SortBufs {
   int64_t ncells;
   SceneTable T;       // the counting sort also writes src and flags the occupied tiles
   int* tile_slot;
-  const int* pos;
+  const int* pid;     // caller id at each physical slot (the state's id plane)
+  int* tslot;         // [n] scratch: the physical slot of each binned caller id
+  int* pos;           // caller id -> physical slot: written by the radix mode's key conversion
   int* src;
   int* counters;      // zeroed by the cell scan (after the counting kernel has read them)
   ResetArgs reset;    // the counting sort's reset work, done by its counting kernel
@@ -204,12 +206,12 @@ struct G2PArgs {
   const int* perm;     // caller ids in sorted order
   const int* src;      // sorted position -> physical index in sin
   const uint32_t* ks;  // cell key of each sorted position (the last sort's; used only to place node boxes)
-  int* pos_out;        // caller id -> physical index in sout
+  int* pos_out;        // caller id -> physical index in sout (null: not maintained, no cloth)
   const float4* gvel;
   float dt;
   int write_stress;    // store V tau of the new state (stress at F^n)
   const float* thr;
-  uint32_t* keys;      // next cell key, indexed by caller id
+  uint32_t* keys;      // next cell key, indexed by physical slot in sout
   int* status;
   unsigned long long* fallback_svd;
   unsigned long long* fallback_apex;
@@ -274,7 +276,9 @@ struct SurrogateArgs {
 };
 
 // Particle-indexed launchers take the number of 256-particle blocks (SceneTable::pblock_off[S]).
-void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* st, int64_t cap, int n, const int* pos,
+// pos[id] = physical slot of caller id id, from the id plane pid of a state
+void launch_pos_of_ids(hipStream_t s, const int* pid, int n, int* pos);
+void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* st, int64_t cap, int n,
                  const float* thr, uint32_t* keys, int* status);
 void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* st, int64_t cap, int n,
                         unsigned long long* fallback);

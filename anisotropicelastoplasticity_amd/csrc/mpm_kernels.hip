@@ -128,17 +128,15 @@ __device__ __forceinline__ uint32_t cell_key(const DevParams& P, const float* th
 // keys[id] = cell key of caller particle id (physical index pos[id]); the radix sort consumes the
 // keys in caller order, so that equal keys keep caller order as in the reference.
 __global__ __launch_bounds__(256) void k_keys(SceneTable T, const float* __restrict__ st, int64_t cap, int n,
-                                              const int* __restrict__ pos, const float* __restrict__ thr,
-                                              uint32_t* __restrict__ keys, int* __restrict__ status) {
-  const PBlock b = particle_block(T);
+                                              const float* __restrict__ thr, uint32_t* __restrict__ keys,
+                                              int* __restrict__ status) {
+  const PBlock b = particle_block(T);  // physical slots are scene-major like the caller ids
   const DevParams& P = T.params[b.s];
-  const int i = b.j;
+  const int j = b.j;
   int oob = 0;
-  if (i < b.end) {
-    int pi = clamp_index(pos[i], n);
-    keys[i] = (uint32_t)T.node_off[b.s] +
-              cell_key(P, thr + T.thr_off[b.s], st[PX * cap + pi], st[(PX + 1) * cap + pi], st[(PX + 2) * cap + pi], oob);
-  }
+  if (j < b.end)
+    keys[j] = (uint32_t)T.node_off[b.s] +
+              cell_key(P, thr + T.thr_off[b.s], st[PX * cap + j], st[(PX + 1) * cap + j], st[(PX + 2) * cap + j], oob);
   if (__ballot(oob) && lane_id() == 0) atomicOr(status, MPM_STATUS_OUT_OF_DOMAIN);
 }
 
@@ -360,7 +358,11 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint32_t* __restrict__ 
 // The cell key itself is the single digit: count the particles of every cell (atomic ranks, in
 // no particular order), scan the counts into cell offsets, scatter the caller ids into their
 // cells, then order each cell by caller id (the rank of id i in its cell is the number of smaller
-// ids there). The result equals the LSD passes' stable sort bit for bit.
+// ids there). The result equals the LSD passes' stable sort bit for bit. The count and the scatter
+// run over physical slots, where the previous sort left the particles in cell order: the keys are
+// read in order, lanes of a wave share few distinct keys, and the scatter's writes land close to
+// where they are read; the scatter keeps each id's slot beside it, so that the rank pass finds the
+// particle's key and slot without gathering by caller id.
 //
 // Counting: the lanes of a wave that share a key are grouped first (leader = the lowest such lane);
 // then every leader adds its group's size with one atomic, all leaders in the same instruction, so
@@ -539,29 +541,47 @@ __global__ __launch_bounds__(256) void k_fill_bins(const uint32_t* __restrict__ 
   for (int64_t c = (int64_t)b * SCAN_TILE + tid; c < min((int64_t)(b + 1) * SCAN_TILE, ncells); c += 256) cell_start[c] = boff;
 }
 
-__global__ __launch_bounds__(256) void k_cell_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rank,
-                                                      int n, const int* __restrict__ cell_start, int* __restrict__ tmp,
-                                                      const Clock* clk) {
+// The caller id -> physical slot map from the state's id plane and, when kc is given, the keys by
+// caller id from the keys by physical slot (the input of the radix passes).
+__global__ __launch_bounds__(256) void k_caller_keys(const uint32_t* __restrict__ keys, const int* __restrict__ pid, int n,
+                                                     uint32_t* __restrict__ kc, int* __restrict__ pos, const Clock* clk) {
   if (clock_skip(clk)) return;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int j = cell_start[keys[i]] + (int)rank[i];
-  if ((unsigned)j < (unsigned)n) tmp[j] = i;  // always true for consistent counts
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int i = pid[j];
+  if ((unsigned)i < (unsigned)n) {  // always true for a consistent id plane
+    if (kc) kc[i] = keys[j];
+    pos[i] = j;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cell_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rank,
+                                                      int n, const int* __restrict__ cell_start,
+                                                      const int* __restrict__ pid, int* __restrict__ tmp,
+                                                      int* __restrict__ tslot, const Clock* clk) {
+  if (clock_skip(clk)) return;
+  const int j = blockIdx.x * 256 + threadIdx.x;  // physical slot
+  if (j >= n) return;
+  const int p = cell_start[keys[j]] + (int)rank[j];
+  if ((unsigned)p < (unsigned)n) {  // always true for consistent counts
+    tmp[p] = pid[j];
+    tslot[p] = j;
+  }
 }
 
 // Thread per sorted position: the caller id placed there by the scatter moves to its stable rank
 // within the cell (a cell of c particles costs c comparisons per particle). It also does the bin
-// pass's work (k_bounds) for its new position: src = pos of the caller id, and the cell's first
-// particle clears the cell's count and flags the cell's tile.
+// pass's work (k_bounds) for its new position: src = the particle's physical slot, and the cell's
+// first particle clears the cell's count and flags the cell's tile.
 // The workgroup first copies the ids of its 256 positions and RANK_HALO on either side into LDS (one
 // coalesced round trip): a cell of at most RANK_HALO particles around any of its positions lies in
 // that window, and its ids are compared there; a larger cell reads them from global memory.
 constexpr int RANK_HALO = 64;
 __global__ __launch_bounds__(256) void k_cell_rank(SceneTable T, const uint32_t* __restrict__ keys,
-                                                   const int* __restrict__ tmp, int n, const int* __restrict__ cell_start,
-                                                   uint32_t* __restrict__ ks, int* __restrict__ vs, int* __restrict__ cnt,
-                                                   int* __restrict__ tile_slot, const int* __restrict__ pos,
-                                                   int* __restrict__ src, const Clock* clk) {
+                                                   const int* __restrict__ tmp, const int* __restrict__ tslot, int n,
+                                                   const int* __restrict__ cell_start, uint32_t* __restrict__ ks,
+                                                   int* __restrict__ vs, int* __restrict__ cnt,
+                                                   int* __restrict__ tile_slot, int* __restrict__ src, const Clock* clk) {
   if (clock_skip(clk)) return;
   constexpr int WIN = 256 + 2 * RANK_HALO;
   __shared__ int s_ids[WIN];
@@ -574,8 +594,10 @@ __global__ __launch_bounds__(256) void k_cell_rank(SceneTable T, const uint32_t*
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
   const int i = clamp_index(s_ids[j - wlo], n);
-  const uint32_t k = keys[i];
-  const int pi = pos[i];
+  // the particle's physical slot (the scatter kept it beside the id) and, through it, its key: the
+  // slots of a cell's particles are nearly consecutive, so these reads are close to coalesced
+  const int pi = clamp_index(tslot[j], n);
+  const uint32_t k = keys[pi];
   const int s0 = cell_start[k], e0 = min(cell_start[k + 1], n);
   int r = 0;
   if (s0 >= wlo && e0 <= wlo + WIN) {
@@ -595,7 +617,7 @@ __global__ __launch_bounds__(256) void k_cell_rank(SceneTable T, const uint32_t*
   if ((unsigned)d >= (unsigned)n) return;  // never, for consistent counts
   vs[d] = i;
   ks[d] = k;
-  src[d] = clamp_index(pi, n);
+  src[d] = pi;
   if (r != 0) return;
   cnt[k] = 0;
   const int sc = scene_of(T.part_off, T.nscenes, i);
@@ -1867,8 +1889,8 @@ __global__ __launch_bounds__(256) void k_g2p(G2PArgs a) {
     so[PVOL * cap + j] = vol;
     so[PPHI * cap + j] = phi_p;
     reinterpret_cast<int*>(so + (size_t)PID * cap)[j] = pid;
-    a.pos_out[pid] = j;
-    a.keys[pid] = (uint32_t)noff + cell_key(P, a.thr + a.T.thr_off[pb.s], xn[0], xn[1], xn[2], oob);
+    if (a.pos_out) a.pos_out[pid] = j;
+    a.keys[j] = (uint32_t)noff + cell_key(P, a.thr + a.T.thr_off[pb.s], xn[0], xn[1], xn[2], oob);
     bad = !(isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]) && isfinite(vp[0]) && isfinite(vp[1]) &&
             isfinite(vp[2]) && isfinite(Fh[0]) && isfinite(Fh[4]) && isfinite(Fh[8]) && isfinite(plastic));
   }
@@ -2312,7 +2334,7 @@ __global__ __launch_bounds__(256) void k_cloth_elements(ClothArgs a) {
     cloth_return_map(P, D, fb);
 #pragma unroll
     for (int i = 0; i < 9; ++i) a.st[(PF + i) * a.cap + pe] = D[i];
-    a.keys[id] = cell_key(P, a.thr, xe[0], xe[1], xe[2], oob);
+    a.keys[pe] = cell_key(P, a.thr, xe[0], xe[1], xe[2], oob);
   }
   if (fb) atomicAdd(a.fallback, (unsigned long long)fb);
   if (oob) atomicOr(a.status, MPM_STATUS_OUT_OF_DOMAIN);
@@ -2433,9 +2455,13 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, int64_t n
 // ================================================================== launchers
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* st, int64_t cap, int n, const int* pos,
+void launch_pos_of_ids(hipStream_t s, const int* pid, int n, int* pos) {
+  if (n > 0) hipLaunchKernelGGL(k_caller_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, nullptr, pid, n, nullptr, pos, nullptr);
+}
+
+void launch_keys(hipStream_t s, const SceneTable& T, int nblocks, const float* st, int64_t cap, int n,
                  const float* thr, uint32_t* keys, int* status) {
-  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, pos, thr, keys, status);
+  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(nblocks), dim3(256), 0, s, T, st, cap, n, thr, keys, status);
 }
 
 void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* st, int64_t cap, int n,
@@ -2460,15 +2486,19 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
     int* vf = (passes & 1) ? b.v1 : b.v2;
     const uint32_t* rank = (passes & 1) ? b.k2 : b.k1;
     int* tmp = (passes & 1) ? b.v2 : b.v1;
-    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, rank, n, b.cell_start, tmp,
-                       b.clk);
-    hipLaunchKernelGGL(k_cell_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.T, keys_in, tmp, n, b.cell_start, kf,
-                       vf, b.cell_count, b.tile_slot, b.pos, b.src, b.clk);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, rank, n, b.cell_start, b.pid,
+                       tmp, b.tslot, b.clk);
+    hipLaunchKernelGGL(k_cell_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.T, keys_in, tmp, b.tslot, n,
+                       b.cell_start, kf, vf, b.cell_count, b.tile_slot, b.src, b.clk);
     return;
   }
+  // the digit passes are stable in their input order, which must be the caller order: the keys
+  // (by physical slot) go to caller order first, and the caller id -> slot map comes with them
+  uint32_t* kc = reinterpret_cast<uint32_t*>(b.tslot);
+  hipLaunchKernelGGL(k_caller_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys_in, b.pid, n, kc, b.pos, b.clk);
   const int nb = blocks_for(n, SORT_TILE);
-  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, keys_in, n, passes, b.hist, b.clk);
-  const uint32_t* kin = keys_in;
+  hipLaunchKernelGGL(k_sort_hist, dim3(blocks_for(n, HIST_TILE)), dim3(256), 0, s, kc, n, passes, b.hist, b.clk);
+  const uint32_t* kin = kc;
   const int* vin = nullptr;
   for (int p = 0; p < passes; ++p) {
     uint32_t* ko = (p & 1) ? b.k2 : b.k1;

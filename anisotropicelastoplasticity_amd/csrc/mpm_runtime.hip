@@ -114,6 +114,11 @@ struct mpm_ctx {
   int64_t sortws_words = 0;
   int nb_sort = 0;
   bool keys_valid = false;
+  // pos[cur] maps every caller id to its slot in state[cur]. G2P maintains it only for contexts with a
+  // cloth mesh (the cloth kernels address vertices and elements by caller id); otherwise it is rebuilt
+  // from the state's id plane when something needs it (ensure_pos).
+  bool pos_ok = false;
+  int* tslot = nullptr;  // counting-sort scratch: each binned id's physical slot (n)
   bool stress_valid = false;  // PK planes of state[cur] hold V tau of the current F
 
   // counters and statistics
@@ -371,11 +376,19 @@ static int64_t scan_blocks(const mpm_ctx* ctx) { return (ctx->ng + SCAN_TILE - 1
 static int64_t sortws_radix_status(const mpm_ctx* ctx) { return SORTWS_SCAN + scan_blocks(ctx) + 1; }
 
 // ------------------------------------------------------------------ substep stages
+// Rebuild pos[cur] from the id plane of state[cur] if G2P did not maintain it.
+static void ensure_pos(mpm_ctx* ctx) {
+  if (ctx->pos_ok || ctx->n <= 0) return;
+  launch_pos_of_ids(ctx->stream, reinterpret_cast<const int*>(ctx->state[ctx->cur] + (size_t)PID * ctx->cap), ctx->n,
+                    ctx->pos[ctx->cur]);
+  ctx->pos_ok = true;
+}
+
 static int stage_sort(mpm_ctx* ctx) {
   hipStream_t s = ctx->stream;
   float* st = ctx->state[ctx->cur];
-  if (!ctx->keys_valid)
-    launch_keys(s, ctx->T, ctx->nblocks_p, st, ctx->cap, ctx->n, ctx->pos[ctx->cur], ctx->thr, ctx->keys, ctx->status);
+  if (ctx->mesh.ne > 0 || ctx->mesh.nv > 0) ensure_pos(ctx);  // the cloth kernels address by caller id
+  if (!ctx->keys_valid) launch_keys(s, ctx->T, ctx->nblocks_p, st, ctx->cap, ctx->n, ctx->thr, ctx->keys, ctx->status);
   ResetArgs r;
   if (ctx->sort_mode == SORT_COUNTING) {  // nothing to zero: the scan totals are overwritten
     r.zero_words = ctx->sortws + SORTWS_TICKETS;
@@ -408,6 +421,8 @@ static int stage_sort(mpm_ctx* ctx) {
   b.ncells = ctx->ng;
   b.T = ctx->T;
   b.tile_slot = ctx->tile_slot;
+  b.pid = reinterpret_cast<const int*>(st + (size_t)PID * ctx->cap);
+  b.tslot = ctx->tslot;
   b.pos = ctx->pos[ctx->cur];
   b.src = ctx->src;
   b.counters = ctx->counters;
@@ -417,6 +432,7 @@ static int stage_sort(mpm_ctx* ctx) {
   ctx->occ_last = b.occ;
   ctx->occ_par = 1 - ctx->occ_par;
   launch_sort(s, b, ctx->keys, ctx->n, ctx->sort_passes, ctx->sort_mode);
+  if (ctx->sort_mode != SORT_COUNTING) ctx->pos_ok = true;  // the radix mode's key conversion wrote pos[cur]
   launch_bounds(s, ctx->T, ctx->nblocks_p, ctx->ks_sorted, ctx->n, ctx->ntiles, ctx->cell_count,
                 ctx->tile_slot, ctx->active_tiles, ctx->block_slot, ctx->active_blocks, ctx->counters, ctx->vs_sorted,
                 ctx->pos[ctx->cur], ctx->src, ctx->clk_active, ctx->sort_mode);
@@ -532,7 +548,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   a.perm = ctx->vs_sorted;
   a.src = ctx->src;
   a.ks = ctx->ks_sorted;
-  a.pos_out = ctx->pos[1 - ctx->cur];
+  a.pos_out = ctx->mesh.ne > 0 || ctx->mesh.nv > 0 ? ctx->pos[1 - ctx->cur] : nullptr;
   a.gvel = reinterpret_cast<const float4*>(ctx->grid + GRID_VEL * ctx->ng);
   a.dt = dt;
   a.write_stress = ctx->predicted ? 0 : 1;
@@ -621,6 +637,7 @@ static int stage_g2p(mpm_ctx* ctx, float dt) {
   ctx->cur = 1 - ctx->cur;
   ctx->keys_valid = true;
   ctx->stress_valid = !ctx->predicted;
+  ctx->pos_ok = a.pos_out != nullptr;
   ctx->substeps += 1;
   return MPM_OK;
 }
@@ -746,7 +763,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   ctx->cell_end = nullptr;
   dfree(ctx->tile_slot); dfree(ctx->active_tiles); dfree(ctx->block_slot); dfree(ctx->active_blocks);
   dfree(ctx->partials);
-  dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]);
+  dfree(ctx->state[0]); dfree(ctx->state[1]); dfree(ctx->keys); dfree(ctx->pos[0]); dfree(ctx->pos[1]); dfree(ctx->tslot);
   dfree(ctx->k1); dfree(ctx->k2); dfree(ctx->v1); dfree(ctx->v2); dfree(ctx->src); dfree(ctx->sortws); dfree(ctx->apex_slot); dfree(ctx->apex_idx); dfree(ctx->apex_usv);
   dfree(ctx->counters); dfree(ctx->status); dfree(ctx->fallbacks);
   dfree(ctx->clock);
@@ -956,6 +973,7 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
     if (!rc) rc = dalloc(ctx, ctx->keys, n);
     if (!rc) rc = dalloc(ctx, ctx->pos[0], n);
     if (!rc) rc = dalloc(ctx, ctx->pos[1], n);
+    if (!rc) rc = dalloc(ctx, ctx->tslot, n);
     if (!rc) rc = dalloc(ctx, ctx->k1, n);
     if (!rc) rc = dalloc(ctx, ctx->k2, n);
     if (!rc) rc = dalloc(ctx, ctx->v1, n);
@@ -988,6 +1006,7 @@ static int install_state(mpm_ctx* ctx, int n, const std::vector<float>& h) {
   CK(hipMemsetAsync(ctx->k1, 0, sizeof(uint32_t) * n, ctx->stream));
   CK(hipMemsetAsync(ctx->k2, 0, sizeof(uint32_t) * n, ctx->stream));
   launch_iota(ctx->stream, ctx->pos[0], n);
+  ctx->pos_ok = true;
   CK(hipStreamSynchronize(ctx->stream));
   ctx->stress_valid = false;
   ctx->cur = 0;
@@ -1286,6 +1305,7 @@ int mpm_set_stress(mpm_ctx* ctx, const float* K) {
     e = hipMemcpyAsync(tmp + (size_t)p * ctx->n, K + (size_t)p * ctx->n_api, sizeof(float) * ctx->n_api,
                        hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
+    ensure_pos(ctx);
     launch_permute_in(ctx->stream, ctx->state[ctx->cur], ctx->cap, ctx->n, PK, 6, tmp, ctx->pos[ctx->cur]);
     e = hipStreamSynchronize(ctx->stream);
   }
