@@ -118,3 +118,30 @@ def test_checkpoint_resumes_bitwise(tmp_path):
     lines = (tmp_path / "f.obj").read_text().splitlines()
     s2.close()
     assert len(lines) == sc.n and lines[0].startswith("v ")
+
+
+def test_set_stress_mid_run_round_trip_and_resume():
+    """mpm_set_stress after substeps in a context without a cloth mesh, where G2P does not keep the
+    caller id -> slot map and the setter rebuilds it from the state's id plane: a stress with a
+    different value per caller id reads back unchanged (the map sends every id to its own slot), and
+    putting the run's own stress back leaves the following substeps bit for bit as without it."""
+    sc = scenes.small_scene(_lib.SAND, 2, predicted=0, friction_model=_lib.FRICTION_COULOMB, seed=8,
+                            res=(32, 32, 32), cells=(6, 6, 6), lo=(0.3, 0.15, 0.3), hi=(0.55, 0.4, 0.55),
+                            v0=(0.0, -1.0, 0.0))
+    say(f"[set_stress mid-run] n={sc.n}")
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, 25)
+        a = s.get_particles()
+    with Solver() as s:
+        sc.load_into(s)
+        s.run(sc.dt, 13)  # particles have moved between slots by now
+        K = s.get_stress()
+        probe = (np.arange(6 * sc.n, dtype=np.float32).reshape(6, sc.n) + 1.0) * np.float32(1e-3)
+        s.set_stress(probe)
+        assert np.array_equal(s.get_stress().view(np.uint32), probe.view(np.uint32))
+        s.set_stress(K)
+        s.run(sc.dt, 12)
+        b = s.get_particles()
+    for k in ("x", "v", "B", "F", "plastic"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
