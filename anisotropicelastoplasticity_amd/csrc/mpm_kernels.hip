@@ -2184,7 +2184,7 @@ __global__ __launch_bounds__(MLP_FWD_THREADS) void k_surrogate(SurrogateArgs a) 
     }
     const float x[5] = {eps[0], eps[1], eps[2], phi, kmu};
     float y[3];
-    wave_forward64(sh, x, y, nullptr, 0, 0);
+    wave_forward64<MLP_FWD_TILES>(sh, x, y, nullptr, 0, 0);
     // the network's registers are free from here on: keep the update's loads below this point
     __builtin_amdgcn_sched_barrier(0);
     bool accepted = false;

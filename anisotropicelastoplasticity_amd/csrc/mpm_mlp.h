@@ -12,8 +12,13 @@ constexpr float MLP_EPS_SCALE = 0.02f;
 constexpr float MLP_PHI_MID = 32.5f, MLP_PHI_SPAN = 7.5f;
 constexpr float MLP_KMU_MID = 2.5f, MLP_KMU_SPAN = 1.0f;
 constexpr float MLP_CORR_SCALE = 1e-3f;
-// forward kernels: 8 waves of 64 rows share one LDS copy of the weights (one workgroup per CU)
-constexpr int MLP_FWD_THREADS = 512;
+// Forward kernels: 12 waves of 64 rows share one LDS copy of the weights (121 KB: one workgroup per
+// CU), 3 waves per SIMD; each wave takes its 64 rows through the network MLP_FWD_TILES 16-row tiles at
+// a time, which keeps it under the 168 VGPRs that 3 waves per SIMD allow (4 tiles at once need about
+// 235, 2 waves per SIMD). Measured on the surrogate kernel at config 4 (3.05M candidates): 8 waves
+// of 4 tiles 592 us, 12 waves of 2 tiles 584 us, 16 waves of 2 tiles (128 VGPRs, spilling) 654 us.
+constexpr int MLP_FWD_THREADS = 768;
+constexpr int MLP_FWD_TILES = 2;
 // The forward pass works on pre-activations scaled by log2(e): the forward copy of the first layer's
 // weights and the staged biases of layers 1-4 are multiplied by log2(e), so the MFMA yields
 // z' = log2(e) z and SiLU is z' / (1 + 2^-z') = log2(e) silu(z) with no multiply before the exp2;

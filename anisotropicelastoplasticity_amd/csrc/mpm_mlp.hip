@@ -40,7 +40,7 @@ __global__ __launch_bounds__(MLP_FWD_THREADS) void k_mlp_forward(const uint16_t*
     if (r < n)
       for (int c = 0; c < 5; ++c) x[c] = in[r * 5 + c];
     float y[3];
-    wave_forward64(s, x, y, zsave, (int64_t)g * 64, n);
+    wave_forward64<MLP_FWD_TILES>(s, x, y, zsave, (int64_t)g * 64, n);
     if (r < n)
       for (int c = 0; c < 3; ++c) out[r * 3 + c] = y[c];
   }

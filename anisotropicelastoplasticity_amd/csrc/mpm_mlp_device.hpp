@@ -92,26 +92,133 @@ __device__ __forceinline__ void wave_gemm_nt(const uint16_t* A, int lda, const u
   }
 }
 
+// One step of the layer pipeline over NT sample tiles (tiles T0 .. T0 + NT - 1 of the wave). Unit u
+// of a layer is its output feature tiles 2u and 2u + 1 (the next layer's K step u), accumulated over
+// the layer's KS K steps in ascending order into acc. The step issues those MFMAs and, between them,
+// the SiLU of the previous unit's pre-activations pacc (sched_barrier keeps the order), so that the
+// matrix cores and the vector ALUs of one wave are busy at the same time; the SiLU results go,
+// rounded to BF16, to nb[T][pu]. Without a previous unit (has_prev false) it only issues the MFMAs.
+// FEED: the previous unit is the last of the previous layer and completes this layer's operand
+// (nb is b, pu = KS - 1): its SiLU runs between the MFMAs of the first KS - 1 K steps and is stored
+// before the last K step reads it.
+template <int KS, int NT, bool FEED>
+__device__ __forceinline__ void fwd_step(const uint16_t* W, int ldw, const float* bias, int u, bf16x8 (&b)[NT][4],
+                                         f32x4 (&acc)[NT][2], const f32x4 (&pacc)[NT][2], bf16x8 (&nb)[NT][4], int pu,
+                                         bool has_prev, int c, int q) {
+  constexpr int NSP = FEED ? KS - 1 : KS;                 // K steps the SiLU is spread over
+  constexpr int VPP = (8 * NT + NSP * NT - 1) / (NSP * NT);  // SiLU values per pair of MFMAs
+  float sv[NT][2][4];
+  auto pack = [&]() {
+#pragma unroll
+    for (int T = 0; T < NT; ++T) {
+      bf16x8 o;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        o[j] = (__bf16)sv[T][0][j];
+        o[4 + j] = (__bf16)sv[T][1][j];
+      }
+      nb[T][pu] = o;
+    }
+  };
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const f32x4 bb = *reinterpret_cast<const f32x4*>(&bias[16 * (2 * u + p) + 4 * q]);
+#pragma unroll
+    for (int T = 0; T < NT; ++T) acc[T][p] = bb;
+  }
+  bf16x8 a[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) a[p] = *reinterpret_cast<const bf16x8*>(W + (16 * (2 * u + p) + c) * ldw + 8 * q);
+  int v0 = 0;
+#pragma unroll
+  for (int ss = 0; ss < KS; ++ss) {
+    bf16x8 an[2];
+    if (ss + 1 < KS) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+        an[p] = *reinterpret_cast<const bf16x8*>(W + (16 * (2 * u + p) + c) * ldw + 32 * (ss + 1) + 8 * q);
+    }
+#pragma unroll
+    for (int m = 0; m < 2 * NT; m += 2) {
+      // MFMAs (p, T) = (m / NT, m % NT) and (m / NT, m % NT + 1)
+      const int p = m / NT, T0 = m % NT;
+      acc[T0][p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], b[T0][ss], acc[T0][p], 0, 0, 0);
+      acc[T0 + 1][p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], b[T0 + 1][ss], acc[T0 + 1][p], 0, 0, 0);
+      if (has_prev && ss < NSP) {
+#pragma unroll
+        for (int v = v0; v < v0 + VPP && v < 8 * NT; ++v) {
+          const int T = v >> 3, pp = (v >> 2) & 1, j = v & 3;
+          sv[T][pp][j] = silu_log2(pacc[T][pp][j]);
+        }
+        v0 += VPP;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (ss + 1 < KS) {
+      a[0] = an[0];
+      a[1] = an[1];
+    }
+    if (FEED && ss == KS - 2) pack();
+  }
+  if (!FEED && has_prev) pack();
+}
+
+// The pre-activations of unit u of layer l for tiles T0 .. T0 + NT - 1, unscaled, into zsave (training).
+template <int NT>
+__device__ __forceinline__ void fwd_zsave(const f32x4 (&acc)[NT][2], int T0, int l, int u, int q,
+                                          float* __restrict__ zsave, int64_t row0, int64_t nrows, int c) {
+#pragma unroll
+  for (int T = 0; T < NT; ++T) {
+    const int64_t r = row0 + 16 * (T0 + T) + c;
+    if (r < nrows) {
+      float* zr = zsave + (((r >> 4) * 4 + l) * 16 + (r & 15)) * MLP_HIDDEN + 4 * q;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) *reinterpret_cast<f32x4*>(zr + 16 * (2 * u + p)) = acc[T][p] * MLP_LN2;
+    }
+  }
+}
+
+// SiLU and BF16 rounding of a unit on its own (the last unit of a layer before the next layer's first).
+template <int NT>
+__device__ __forceinline__ void fwd_silu(const f32x4 (&acc)[NT][2], bf16x8 (&nb)[NT][4], int u) {
+#pragma unroll
+  for (int T = 0; T < NT; ++T) {
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      o[j] = (__bf16)silu_log2(acc[T][0][j]);
+      o[4 + j] = (__bf16)silu_log2(acc[T][1][j]);
+    }
+    nb[T][u] = o;
+  }
+}
+
 // Forward pass of 64 samples held by one wave, one per lane: lane L passes the raw inputs x of
 // sample L (eps(3), phi, K/mu) and receives its 3 normalised outputs in y. Each layer computes
 // Z^T = W A^T on MFMA with the staged weights as the A operand (16 output features x 32 inputs)
 // and the activations as the B operand (32 inputs x 16 samples), for the wave's four 16-sample
-// tiles T: lane i of tile T holds output features 16 t + 4 (i / 16) + 0..3 of sample 16 T + i % 16,
+// tiles T, NT at a time (NT = 2: two passes, with half the registers of one pass over all four):
+// lane i of tile T holds output features 16 t + 4 (i / 16) + 0..3 of sample 16 T + i % 16,
 // which SiLU and BF16 rounding turn into the next layer's operand in place (pre-activations and
 // activations scaled by log2(e), see MLP_LOG2E). When zsave is not null, the FP32 pre-activations z
 // (unscaled) of sample r = row0 + L < nrows go to zsave[(r / 16) * 4 + layer][r % 16][128].
-static __device__ void wave_forward64(const MLPShared& s, const float x[5], float y[3], float* __restrict__ zsave,
-                                      int64_t row0, int64_t nrows) {
+// The layers run as a pipeline of units (fwd_step): the MFMAs of one unit are issued between the
+// SiLU instructions of the one before it.
+template <int NT>
+__device__ void wave_forward64(const MLPShared& s, const float x[5], float y[3], float* __restrict__ zsave,
+                               int64_t row0, int64_t nrows) {
   const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
-  bf16x8 b[4][4];
-  {
-    const float xn[5] = {x[0] * (1.0f / MLP_EPS_SCALE), x[1] * (1.0f / MLP_EPS_SCALE), x[2] * (1.0f / MLP_EPS_SCALE),
-                         (x[3] - MLP_PHI_MID) * (1.0f / MLP_PHI_SPAN), (x[4] - MLP_KMU_MID) * (1.0f / MLP_KMU_SPAN)};
+  const float xn[5] = {x[0] * (1.0f / MLP_EPS_SCALE), x[1] * (1.0f / MLP_EPS_SCALE), x[2] * (1.0f / MLP_EPS_SCALE),
+                       (x[3] - MLP_PHI_MID) * (1.0f / MLP_PHI_SPAN), (x[4] - MLP_KMU_MID) * (1.0f / MLP_KMU_SPAN)};
+  float yv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+  for (int T0 = 0; T0 < 4; T0 += NT) {
+    bf16x8 b[NT][4];
 #pragma unroll
-    for (int T = 0; T < 4; ++T) {
+    for (int T = 0; T < NT; ++T) {
       float v[5];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) v[k] = __shfl(xn[k], 16 * T + c);
+      for (int k = 0; k < 5; ++k) v[k] = __shfl(xn[k], 16 * (T0 + T) + c);
       // layer-1 slots: group 0 holds inputs 0..3 (slots 0..3), group 1 input 4 (slot 0)
       bf16x8 o;
 #pragma unroll
@@ -126,88 +233,59 @@ static __device__ void wave_forward64(const MLPShared& s, const float x[5], floa
 #pragma unroll
       for (int ss = 1; ss < 4; ++ss) b[T][ss] = o;  // not read by layer 1
     }
-  }
+    // Units (l, u) in order; unit (l, u) feeds K step u of layer l + 1. Step (l, u) issues the MFMAs
+    // of unit (l, u) and, between them, the SiLU of the unit before it. The last unit of a layer
+    // completes the next layer's operand (its K step 3): its SiLU runs under the first three K steps
+    // of the next layer's first unit (FEED), the last unit of the network's on its own.
+    f32x4 acc[2][NT][2];
+    bf16x8 nb[NT][4];
 #pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    const uint16_t* W = l == 0 ? s.w1 : s.wh[l - 1];
-    const int ldw = l == 0 ? LDW_IN : LDW_H;
-    const int KS = l == 0 ? 1 : 4;
-    bf16x8 nb[4][4];
-    // two halves of 4 output tiles each (64 accumulator registers live at a time): half h gives
-    // the next layer's K steps 2h and 2h + 1
+    for (int l = 0; l < 4; ++l) {
+      const uint16_t* W = l == 0 ? s.w1 : s.wh[l - 1];
+      const int ldw = l == 0 ? LDW_IN : LDW_H;
+      const float* bias = s.b + l * MLP_HIDDEN;
+      if (l > 0) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      f32x4 acc[4][4];
+        for (int T = 0; T < NT; ++T)
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt) {
-        const f32x4 bb = *reinterpret_cast<const f32x4*>(&s.b[l * MLP_HIDDEN + 16 * (4 * h + tt) + 4 * q]);
-#pragma unroll
-        for (int T = 0; T < 4; ++T) acc[T][tt] = bb;
+          for (int ss = 0; ss < 3; ++ss) b[T][ss] = nb[T][ss];
       }
 #pragma unroll
-      for (int ss = 0; ss < KS; ++ss) {
-#pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
-          const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * (4 * h + tt) + c) * ldw + 32 * ss + 8 * q);
-#pragma unroll
-          for (int T = 0; T < 4; ++T)
-            acc[T][tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[T][ss], acc[T][tt], 0, 0, 0);
-        }
-        // keep the scheduler from hoisting the weight loads of later K steps (registers)
-        __builtin_amdgcn_sched_barrier(0);
+      for (int u = 0; u < 4; ++u) {
+        if (l == 0) fwd_step<1, NT, false>(W, ldw, bias, u, b, acc[u & 1], acc[(u - 1) & 1], nb, u - 1, u > 0, c, q);
+        else if (u == 0) fwd_step<4, NT, true>(W, ldw, bias, 0, b, acc[0], acc[1], b, 3, true, c, q);
+        else fwd_step<4, NT, false>(W, ldw, bias, u, b, acc[u & 1], acc[(u - 1) & 1], nb, u - 1, true, c, q);
+        if (zsave) fwd_zsave<NT>(acc[u & 1], T0, l, u, q, zsave, row0, nrows, c);
       }
-      if (zsave) {
-#pragma unroll
-        for (int T = 0; T < 4; ++T) {
-          const int64_t r = row0 + 16 * T + c;
-          if (r < nrows) {
-            float* zr = zsave + (((r >> 4) * 4 + l) * 16 + (r & 15)) * MLP_HIDDEN + 64 * h + 4 * q;
-#pragma unroll
-            for (int tt = 0; tt < 4; ++tt) *reinterpret_cast<f32x4*>(zr + 16 * tt) = acc[T][tt] * MLP_LN2;
-          }
-        }
-      }
-#pragma unroll
-      for (int T = 0; T < 4; ++T)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          bf16x8 o;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            o[j] = (__bf16)silu_log2(acc[T][2 * k][j]);
-            o[4 + j] = (__bf16)silu_log2(acc[T][2 * k + 1][j]);
-          }
-          nb[T][2 * h + k] = o;
-        }
     }
+    fwd_silu<NT>(acc[1], nb, 3);
 #pragma unroll
-    for (int T = 0; T < 4; ++T)
+    for (int T = 0; T < NT; ++T)
 #pragma unroll
       for (int ss = 0; ss < 4; ++ss) b[T][ss] = nb[T][ss];
-  }
-  f32x4 out[4];
-  {
-    const f32x4 bb = *reinterpret_cast<const f32x4*>(&s.b[4 * MLP_HIDDEN + 4 * q]);
+    f32x4 out[NT];
+    {
+      const f32x4 bb = *reinterpret_cast<const f32x4*>(&s.b[4 * MLP_HIDDEN + 4 * q]);
 #pragma unroll
-    for (int T = 0; T < 4; ++T) out[T] = bb;
-  }
-#pragma unroll
-  for (int ss = 0; ss < 4; ++ss) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(s.w5 + c * LDW_H + 32 * ss + 8 * q);
-#pragma unroll
-    for (int T = 0; T < 4; ++T) out[T] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[T][ss], out[T], 0, 0, 0);
-  }
-  // outputs 0..2 of sample 16 T + c sit in lane c (group 0) of tile T
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float v = 0.0f;
-#pragma unroll
-    for (int T = 0; T < 4; ++T) {
-      const float u = __shfl(out[T][k], c);
-      if (q == T) v = u;
+      for (int T = 0; T < NT; ++T) out[T] = bb;
     }
-    y[k] = v;
+#pragma unroll
+    for (int ss = 0; ss < 4; ++ss) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(s.w5 + c * LDW_H + 32 * ss + 8 * q);
+#pragma unroll
+      for (int T = 0; T < NT; ++T) out[T] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[T][ss], out[T], 0, 0, 0);
+    }
+    // outputs 0..2 of sample 16 T + c sit in lane c (group 0) of tile T
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int T = 0; T < NT; ++T) {
+        const float u = __shfl(out[T][k], c);
+        if (q == T0 + T) yv[k] = u;
+      }
   }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) y[k] = yv[k];
 }
 
 }  // namespace mpm

@@ -219,6 +219,7 @@ def test_surrogate_trained_is_accepted_and_close():
         s.set_pair_logging(1 << 20)
         s.run(sc.dt, nsteps)
         pairs = s.get_pairs()
+        pairs = pairs[np.lexsort(pairs.T[::-1])]  # append order varies between runs
         say(f"[surrogate trained] pairs {len(pairs)}")
         s.surrogate_init(1)
         losses = s.surrogate_train(pairs[:, :5], pairs[:, 5:8], epochs=60, batch=512, lr=2e-3, seed=3)
@@ -243,6 +244,9 @@ def _trained_weights(sc, nsteps=30, hold_out=0.0, seed=3):
         s.set_pair_logging(1 << 20)
         s.run(sc.dt, nsteps)
         pairs = s.get_pairs()
+    # the log's row order is the order in which threads appended to it (an atomic counter): sorted
+    # rows make the split and the training data the same in every run
+    pairs = pairs[np.lexsort(pairs.T[::-1])]
     rng = np.random.default_rng(seed)
     idx = rng.permutation(len(pairs))
     k = int(round(len(pairs) * (1.0 - hold_out)))
