@@ -141,16 +141,42 @@ def advance(s, dt, k):
         s.run(dt, k)
 
 
+def pair_hash(pairs):
+    """A 64-bit hash of each (trial, projected) row's bytes."""
+    u = np.ascontiguousarray(pairs, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    h = np.zeros(len(u), np.uint64)
+    mult = (0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9, 0xD6E8FEB86659FD93,
+            0xFF51AFD7ED558CCD, 0xC4CEB9FE1A85EC53, 0x94D049BB133111EB, 0xBF58476D1CE4E5B9)
+    with np.errstate(over="ignore"):
+        for k in range(u.shape[1]):
+            h = (h ^ u[:, k]) * np.uint64(mult[k])
+            h ^= h >> np.uint64(29)
+    return h
+
+
+def canonical_order(pairs):
+    """The pairs sorted by their hash. The log's row order is the order in which GPU threads appended
+    to it, which varies from run to run; training on a fixed order (and a fixed set, see log_pairs)
+    makes the trained network, and so the measured leg, the same in every run (rows with equal hashes
+    are, but for a negligible chance, equal rows)."""
+    return pairs[np.argsort(pair_hash(pairs), kind="stable")]
+
+
 def log_pairs(s, dt, total, chunks, per_chunk):
     """Run `total` substeps, logging the exact path's (trial, projected) pairs in `chunks` substeps
-    spread over the run (at most per_chunk pairs each)."""
+    spread over the run. A logging substep records all of its pairs (a particle gives at most one);
+    of those, the per_chunk rows with the smallest hashes are kept, a choice that does not depend on
+    the order in which the GPU wrote them."""
     pairs = []
     every = max(1, total // chunks)
     done = 0
     while done < total:
-        s.set_pair_logging(per_chunk)
+        s.set_pair_logging(s.n)
         advance(s, dt, 1)
-        pairs.append(s.get_pairs())
+        p = s.get_pairs()
+        if len(p) > per_chunk:
+            p = p[np.argpartition(pair_hash(p), per_chunk)[:per_chunk]]
+        pairs.append(p)
         s.set_pair_logging(0)
         k = min(every - 1, total - done - 1)
         if k > 0:
@@ -178,7 +204,7 @@ def train_surrogate(target_pairs=10_000_000, schedule=LR_SCHEDULE, batch=4096):
     with Solver(0) as s:
         s.load_batch(bat)
         p_bat = log_pairs(s, bat[0].dt, 500, 20, target_pairs // 40)
-    pairs = np.concatenate([p_col, p_bat])
+    pairs = canonical_order(np.concatenate([p_col, p_bat]))
     log_s = time.perf_counter() - t0
     with Solver(0) as s:
         t1 = time.perf_counter()
