@@ -105,6 +105,8 @@ template <int KS, int NT, bool FEED>
 __device__ __forceinline__ void fwd_step(const uint16_t* W, int ldw, const float* bias, int u, bf16x8 (&b)[NT][4],
                                          f32x4 (&acc)[NT][2], const f32x4 (&pacc)[NT][2], bf16x8 (&nb)[NT][4], int pu,
                                          bool has_prev, int c, int q) {
+  static_assert(NT == 2 || NT == 4, "MFMAs go in pairs of sample tiles");
+  static_assert(!FEED || KS > 1, "a fed step spreads the SiLU over all but its last K step");
   constexpr int NSP = FEED ? KS - 1 : KS;                 // K steps the SiLU is spread over
   constexpr int VPP = (8 * NT + NSP * NT - 1) / (NSP * NT);  // SiLU values per pair of MFMAs
   float sv[NT][2][4];
