@@ -50,6 +50,7 @@ STAGES = ["sort", "p2g", "grid", "g2p"]
 ALG_PER_PARTICLE = {"sort": 36, "p2g": 104, "grid": 0, "g2p": 152}
 ALG_PER_NODE = {"sort": 0, "p2g": 16, "grid": 40, "g2p": 24}
 SUBSTEP_PER_PARTICLE, SUBSTEP_PER_NODE = 208, 56
+P2G_LAYER_NAME = "k_p2g_layer<2, P2G_MOM_STRESS>"  # the autotuner's other P2G schedule
 KERNEL_NAMES = {"p2g": "k_p2g<2, P2G_MOM_STRESS>", "g2p": "k_g2p<2>", "grid": "k_grid<GRID_FUSED>",
                 "sort": "k_cell_count + k_cell_scan + k_cell_scatter + k_cell_rank + k_claim",
                 "surrogate": "k_surrogate"}
@@ -226,15 +227,35 @@ def train_surrogate(target_pairs=10_000_000, schedule=LR_SCHEDULE, batch=4096):
     return w, info
 
 
-def timed_run(sc, total, steps, weights=None, gate_tol=1e-5, load_checkpoint=None, save_checkpoint=None):
+def tuning_for(sc, warm=300, reps=5):
+    """The autotuner's launch shapes (sort mode, P2G pass, ranks and workgroups, grid workgroups) for
+    sc, chosen on a copy of the scene advanced `warm` substeps, so that the particles' counts per cell
+    are those of the timed window rather than of the initial lattice. Results do not depend on the
+    choice (the kernels sum in the same order whatever the launch shape)."""
+    from anisotropicelastoplasticity_amd import Solver
+    with Solver(0) as s:
+        sc.load_into(s)
+        advance(s, sc.dt, warm)
+        return s.autotune(sc.dt, reps=reps)
+
+
+def apply_tuning(s, t):
+    s.set_tuning(t["p2g_ranks"], t["p2g_blocks"], t["grid_blocks"], t["sort_mode"], t["p2g_pass"])
+
+
+def timed_run(sc, total, steps, weights=None, gate_tol=1e-5, load_checkpoint=None, save_checkpoint=None,
+              tuning=None):
     """Run `total` substeps of sc; time the last `steps` of them. Returns wall, positions, stats and
     a per-kernel breakdown measured afterwards on eagerly launched substeps. A checkpoint can replace
-    the scene's initial state (load) or keep the state after the run (save)."""
+    the scene's initial state (load) or keep the state after the run (save); `tuning` replaces the
+    default launch shapes."""
     from anisotropicelastoplasticity_amd import Solver
     with Solver(0) as s:
         sc.load_into(s)
         if load_checkpoint:
             s.load_checkpoint(load_checkpoint)
+        if tuning is not None:
+            apply_tuning(s, tuning)
         if weights is not None:
             s.surrogate_set_weights(weights)
             s.set_surrogate(True, gate_tol)
@@ -261,7 +282,7 @@ def timed_run(sc, total, steps, weights=None, gate_tol=1e-5, load_checkpoint=Non
                 active_nodes=st_after["n_active_nodes"])
 
 
-def roofline_of(kernel_ms, n, nodes, sur, per_particle_phi, workload):
+def roofline_of(kernel_ms, n, nodes, sur, per_particle_phi, workload, tuning=None):
     """The dominant kernel's roofline: HBM-bound stencil kernels by algorithmic bytes, the network
     kernel by algorithmic flops."""
     cands = {k: kernel_ms[k] for k in ("p2g", "grid", "g2p")}
@@ -277,7 +298,8 @@ def roofline_of(kernel_ms, n, nodes, sur, per_particle_phi, workload):
                 "alg_flops_per_launch": flops, "avg_launch_ms": ms}
     alg = (ALG_PER_PARTICLE[dom] + (4 if dom == "g2p" and per_particle_phi else 0)) * n + ALG_PER_NODE[dom] * nodes
     ach = alg / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    name = P2G_LAYER_NAME if dom == "p2g" and tuning is not None and tuning["p2g_pass"] == 1 else KERNEL_NAMES[dom]
+    return {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": load_traffic(dom, workload), "alg_bytes_per_launch": alg,
             "avg_launch_ms": ms}
 
@@ -287,8 +309,9 @@ def headline(args, weights, train_info):
     from anisotropicelastoplasticity_amd import scenes
     sc = scenes.config4_sand_phi()
     total = max(COMPARE_SUBSTEPS, args.warmup + args.steps)
-    ex = timed_run(sc, total, args.steps)
-    su = timed_run(sc, total, args.steps, weights=weights)
+    tun = tuning_for(sc)
+    ex = timed_run(sc, total, args.steps, tuning=tun)
+    su = timed_run(sc, total, args.steps, weights=weights, tuning=tun)
     dx = su["x"].astype(np.float64) - ex["x"].astype(np.float64)
     h = (sc.gmax[0] - sc.gmin[0]) / sc.res[0]
     rel_err = float(np.linalg.norm(dx) / np.linalg.norm(ex["x"].astype(np.float64)))
@@ -302,7 +325,7 @@ def headline(args, weights, train_info):
         "value": value, "ms_per_step": 1e3 * su["wall"] / args.steps,
         "exact": {"value": exact_value, "ms_per_step": 1e3 * ex["wall"] / args.steps, "kernel_ms": ex["kernel_ms"],
                   "stage_ms": ex["stage_ms"],
-                  "roofline": roofline_of(ex["kernel_ms"], sc.n, ex["active_nodes"], None, True, "config4"),
+                  "roofline": roofline_of(ex["kernel_ms"], sc.n, ex["active_nodes"], None, True, "config4", tun),
                   "fp64_apex_count": ex["stats"]["fp64_apex_count"], "status": ex["stats"]["status"]},
         "speedup_vs_exact": value / exact_value,
         "rel_pos_err": rel_err, "max_dx_over_h": float(np.max(np.abs(dx)) / h), "error_bar_met": rel_err <= 1e-3,
@@ -316,16 +339,18 @@ def headline(args, weights, train_info):
                              "bf16_frac": (sur_flops / (sur["kernel_ms"] * 1e-3) / 1e12) / BF16_PEAK_TFLOPS
                              if sur["kernel_ms"] else None,
                              "traffic": load_traffic("surrogate", "config4")},
-        "roofline": roofline_of(su["kernel_ms"], sc.n, su["active_nodes"], sur, True, "config4"),
-        "active_nodes": su["active_nodes"], "status": st["status"], "train": train_info,
+        "roofline": roofline_of(su["kernel_ms"], sc.n, su["active_nodes"], sur, True, "config4", tun),
+        "active_nodes": su["active_nodes"], "status": st["status"], "train": train_info, "tuning": tun,
     }
 
 
-def config2_leg(steps, warmup):
-    """The 2^20-particle sand column, exact path, with the dominant kernel's roofline."""
+def config2_leg(steps, warmup, tun=None):
+    """The 2^20-particle sand column, exact path, with the dominant kernel's roofline; launch shapes
+    from the autotuner unless `tun` gives them."""
     from anisotropicelastoplasticity_amd import scenes
     sc = scenes.config2_sand_column()
-    r = timed_run(sc, warmup + steps, steps)
+    tun = tun if tun is not None else tuning_for(sc)
+    r = timed_run(sc, warmup + steps, steps, tuning=tun)
     n, nodes = sc.n, r["active_nodes"]
     b_alg = SUBSTEP_PER_PARTICLE * n + SUBSTEP_PER_NODE * nodes
     ms = 1e3 * r["wall"] / steps
@@ -333,7 +358,7 @@ def config2_leg(steps, warmup):
     return {"workload": "config2_sand_column", "particles": n, "grid": list(sc.res), "dt": sc.dt,
             "substeps": warmup + steps, "value": n * steps / r["wall"], "ms_per_step": ms,
             "kernel_ms": r["kernel_ms"], "stage_ms": r["stage_ms"], "active_nodes": nodes,
-            "roofline": roofline_of(r["kernel_ms"], n, nodes, None, False, "config2"),
+            "roofline": roofline_of(r["kernel_ms"], n, nodes, None, False, "config2", tun), "tuning": tun,
             "substep_roofline": {"achieved": sub_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": sub_ach / HBM_PEAK_GBS, "alg_bytes_per_substep": b_alg},
             "fp64_apex_count": r["stats"]["fp64_apex_count"], "status": r["stats"]["status"]}
@@ -456,6 +481,9 @@ def main():
     ap.add_argument("--save-weights", help="with --only config4-surrogate: save the trained weights (.npy)")
     ap.add_argument("--load-checkpoint", help="with --only config4/config4-surrogate: start from this state (.npz)")
     ap.add_argument("--save-checkpoint", help="with --only config4/config4-surrogate: save the state after the run")
+    ap.add_argument("--tuning", help="with --only config2/config4/config4-surrogate: launch shapes (.json written by "
+                                     "--save-tuning) instead of running the autotuner")
+    ap.add_argument("--save-tuning", help="with --only config2/config4/config4-surrogate: save the launch shapes used")
     args = ap.parse_args()
     global EAGER
     EAGER = args.eager
@@ -465,8 +493,14 @@ def main():
         sys.exit(2)
     from anisotropicelastoplasticity_amd import Solver, scenes
     if args.only:
+        tun = None
+        if args.tuning:
+            with open(args.tuning) as f:
+                tun = json.load(f)
         if args.only == "config2":
-            leg = config2_leg(args.steps, args.warmup)
+            sc2 = scenes.config2_sand_column()
+            tun = tun if tun is not None else tuning_for(sc2)
+            leg = config2_leg(args.steps, args.warmup, tun)
         elif args.only == "config3":
             leg = config3_leg(args.steps, args.warmup)
         elif args.only == "batched":
@@ -483,10 +517,14 @@ def main():
                     w, _ = train_surrogate(target_pairs=args.train_pairs)
                 if args.save_weights:
                     np.save(args.save_weights, w)
+            tun = tun if tun is not None else tuning_for(sc)
             r = timed_run(sc, args.warmup + args.steps, args.steps, weights=w, load_checkpoint=args.load_checkpoint,
-                          save_checkpoint=args.save_checkpoint)
+                          save_checkpoint=args.save_checkpoint, tuning=tun)
             leg = {"workload": args.only, "value": sc.n * args.steps / r["wall"], "kernel_ms": r["kernel_ms"],
-                   "surrogate_kernel": r["sur"], "status": r["stats"]["status"]}
+                   "surrogate_kernel": r["sur"], "status": r["stats"]["status"], "tuning": tun}
+        if args.save_tuning and tun is not None:
+            with open(args.save_tuning, "w") as f:
+                json.dump(tun, f)
         print(json.dumps(leg))
         return
 
@@ -537,6 +575,7 @@ def main():
         "active_nodes": h["active_nodes"],
         "status": h["status"],
         "train": h["train"],
+        "tuning": h["tuning"],
     }
     if c2 is not None:
         out["config2"] = c2
