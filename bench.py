@@ -12,6 +12,9 @@ substeps is `rel_pos_err`, and `error_bar_met` says whether it is within 1e-3. T
 trained on the GPU beforehand, on (trial, projected) pairs logged by the exact path in other runs:
 a sand column of config 2's shape with per-particle angles and a batch of 64 mixed scenes, each
 from its own seed. Inputs are synthetic (seeded jittered lattices), resident in HBM before timing.
+Every leg runs with the launch shapes that the library's autotuner (mpm_autotune) picks for its
+workload, on a copy of the scene advanced 300 substeps (the batched leg on its own batch); the
+results do not depend on them, only the times do (`tuning` on the line).
 
 Other objects on the same line: `config2` (the 2^20-particle sand column, exact path, with its
 dominant kernel's roofline), `batched` (64 independent snow/sand scenes in one context against the
