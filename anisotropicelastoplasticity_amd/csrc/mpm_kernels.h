@@ -284,7 +284,7 @@ void launch_stress_init(hipStream_t s, const SceneTable& T, int nblocks, float* 
                         unsigned long long* fallback);
 void launch_reset(hipStream_t s, const ResetArgs& r);
 // cell_start of the scan blocks without particles (only the getter of the bins needs them).
-void launch_fill_bins(hipStream_t s, const uint32_t* occ, const uint32_t* agg, int64_t ncells, int* cell_start);
+void launch_fill_bins(hipStream_t s, const uint32_t* ks, int n, int64_t ncells, int* cell_start);
 // Stable sort of the cell keys by caller id. Both modes count the particles of every cell and scan
 // the counts into cell_start. SORT_COUNTING then places each caller id at its stable rank inside
 // its cell; SORT_RADIX runs `passes` 8-bit LSD digit passes (1..4). Both give the same order. The

@@ -524,21 +524,22 @@ __global__ __launch_bounds__(256) void k_cell_scan(const int* __restrict__ cnt, 
   if (tid == 255 && !last) cell_start[(int64_t)(b + 1) * SCAN_TILE] = (int)run;  // the next block's first cell
 }
 
-// cell_start of the blocks without particles: every cell starts (and ends) at the block's offset.
-__global__ __launch_bounds__(256) void k_fill_bins(const uint32_t* __restrict__ occ, const uint32_t* __restrict__ agg,
-                                                   int64_t ncells, int* __restrict__ cell_start) {
-  const int b = blockIdx.x;
-  if (occ[b]) return;
-  __shared__ uint32_t s_boff[4];
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  uint32_t p = 0;
-  for (int q = tid; q < b; q += 256) p += agg[q];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
-  if (lane == 0) s_boff[w] = p;
-  __syncthreads();
-  const int boff = (int)(s_boff[0] + s_boff[1] + s_boff[2] + s_boff[3]);
-  for (int64_t c = (int64_t)b * SCAN_TILE + tid; c < min((int64_t)(b + 1) * SCAN_TILE, ncells); c += 256) cell_start[c] = boff;
+// The bins of the last sort for the host: cell_start[c] = the first sorted position whose key is at
+// least c (a binary search of the sorted keys), for every cell c in [0, ncells]. The sort's own scan
+// writes only the scan blocks that hold particles, which is all the substep reads; the keys, unlike
+// the block flags, belong to the last sort that ran, also when the device clock skipped a later
+// captured substep.
+__global__ __launch_bounds__(256) void k_fill_bins(const uint32_t* __restrict__ ks, int n, int64_t ncells,
+                                                   int* __restrict__ cell_start) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c > ncells) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if ((int64_t)ks[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  cell_start[c] = lo;
 }
 
 // The caller id -> physical slot map from the state's id plane and, when kc is given, the keys by
@@ -2510,9 +2511,9 @@ void launch_sort(hipStream_t s, const SortBufs& b, const uint32_t* keys_in, int 
   }
 }
 
-void launch_fill_bins(hipStream_t s, const uint32_t* occ, const uint32_t* agg, int64_t ncells, int* cell_start) {
+void launch_fill_bins(hipStream_t s, const uint32_t* ks, int n, int64_t ncells, int* cell_start) {
   if (ncells <= 0) return;
-  hipLaunchKernelGGL(k_fill_bins, dim3(blocks_for(ncells, SCAN_TILE)), dim3(256), 0, s, occ, agg, ncells, cell_start);
+  hipLaunchKernelGGL(k_fill_bins, dim3(blocks_for(ncells + 1, 256)), dim3(256), 0, s, ks, n, ncells, cell_start);
 }
 
 void launch_reset(hipStream_t s, const ResetArgs& r) {

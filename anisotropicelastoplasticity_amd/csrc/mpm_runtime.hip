@@ -1653,8 +1653,8 @@ int mpm_get_sorted(mpm_ctx* ctx, uint32_t* keys, int32_t* ids, int32_t* bins) {
   if (keys) CK(hipMemcpy(keys, ctx->ks_sorted, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
   if (ids) CK(hipMemcpy(ids, ctx->vs_sorted, sizeof(int) * n, hipMemcpyDeviceToHost));
   if (bins) {
-    // the scan skipped the blocks without particles: their cells start at the block's offset
-    launch_fill_bins(ctx->stream, ctx->occ_last, ctx->sortws + SORTWS_SCAN, ctx->ng, ctx->cell_start);
+    // every cell's start from the sorted keys (the scan wrote only the blocks holding particles)
+    launch_fill_bins(ctx->stream, ctx->ks_sorted, n, ctx->ng, ctx->cell_start);
     CK(hipStreamSynchronize(ctx->stream));
     CK(hipMemcpy(bins, ctx->cell_start, sizeof(int) * (ctx->ng + 1), hipMemcpyDeviceToHost));
   }
