@@ -145,3 +145,28 @@ def test_set_stress_mid_run_round_trip_and_resume():
         b = s.get_particles()
     for k in ("x", "v", "B", "F", "plastic"):
         assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+
+
+def test_sorted_bins_after_solve_match_keys():
+    """After mpm_solve the bins that mpm_get_sorted returns are those of the sorted keys it returns
+    (cell c starts at the first key >= c), also when the solve's last captured pair of substeps ran
+    only its first substep (an odd substep count): the device clock skips the second, whose set of
+    block flags is then stale, and the getter must not depend on it."""
+    sc = scenes.small_scene(_lib.SAND, 2, predicted=0, friction_model=_lib.FRICTION_COULOMB, seed=5,
+                            res=(24, 24, 24), cells=(6, 5, 6), lo=(0.3, 0.2, 0.35), hi=(0.55, 0.42, 0.6),
+                            v0=(0.3, -1.5, 0.2))
+    ncells = int(np.prod(sc.res))
+    counts = []
+    for maxt in (1.0 / 60.0, 1.5 / 60.0, 2.0 / 60.0, 2.5 / 60.0):
+        with Solver() as s:
+            sc.load_into(s)
+            s.solve(0.3, maxt)
+            keys, ids, bins = s.get_sorted()
+            subs = s.clock()["substeps"]
+        counts.append(subs)
+        say(f"[bins after solve] maxt {maxt:.4f}: {subs} substeps")
+        assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+        assert np.array_equal(np.sort(ids), np.arange(sc.n))
+        expect = np.searchsorted(keys.astype(np.int64), np.arange(ncells + 1), side="left")
+        assert np.array_equal(bins.astype(np.int64), expect)
+    assert any(c % 2 == 1 for c in counts), counts
