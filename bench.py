@@ -492,7 +492,7 @@ def main():
     EAGER = args.eager
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
+              "sharding over several GPUs is designed but not built", file=sys.stderr)
         sys.exit(2)
     from anisotropicelastoplasticity_amd import Solver, scenes
     if args.only:
